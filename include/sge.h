@@ -1,0 +1,209 @@
+/*
+ * sge.h -- C ABI of the MI355X subgraph-enumeration executor (libsge.so).
+ *
+ * This is the drop-in boundary for the reference's plan-to-join path:
+ *   main.cpp:64-69      optCostPlan / selectSymmetry / buildTraverseUnlabeled /
+ *                       refineIntersectionInfo  -> sge_plan_build
+ *   matching/join.cpp:812       sharedJoin (single thread)      -> sge_count
+ *   matching/parallel.cpp:1154  parSharedJoin (WCOJ, OpenMP)    -> sge_count
+ *   matching/parallel.cpp:1488  parSharedJoin (exploration)     -> sge_count (SGE_MODE_EXPLORE)
+ *   relation/graph.cpp:255      DataGraph::loadDataGraph        -> sge_graph_load
+ *   relation/graph.cpp:661      PatternGraph::loadPatternGraph  -> sge_pattern_load
+ *
+ * Conventions: every function returns 0 on success and a negative SGE_E* code
+ * on failure; sge_last_error() gives the message (thread-local).  Only POD types
+ * cross the boundary.  The caller owns every host buffer it passes; the library
+ * owns graph/batch/model handles and their device memory (freed by *_destroy).
+ * Counts are assigned, never accumulated, and are exact uint64 (the IEP
+ * closed form reproduces the reference's wrapping choosec, utility/utils.cpp:80-94).
+ * Handles are thread-compatible, not thread-safe.
+ */
+#ifndef SGE_H
+#define SGE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SGE_MAX_K 8 /* pattern vertices; the reference's benchmark sets use <= 7 */
+
+/* error codes */
+#define SGE_OK 0
+#define SGE_EINVAL (-1)   /* bad argument / unsupported shape */
+#define SGE_EIO (-2)      /* file open / parse failure (reference: exit(1), graph.cpp:260,296,665,683) */
+#define SGE_EDEVICE (-3)  /* HIP runtime failure or no device */
+#define SGE_ENOMEM (-4)   /* allocation failure */
+#define SGE_ERANGE (-5)   /* capacity exceeded (output buffer, sizes) */
+
+/* executor modes (reference flags: default WCOJ, -i exploration, -u IEP) */
+#define SGE_MODE_WCOJ 0u
+#define SGE_MODE_EXPLORE 1u
+#define SGE_MODE_IEP 2u
+
+/* sge_count flags */
+#define SGE_COUNT_BATCHED 1u  /* all plans in one persistent launch with a shared root queue */
+#define SGE_COUNT_GRAPH 2u    /* replay through a captured hipGraph (with BATCHED) */
+
+/* graph construction flags */
+#define SGE_GRAPH_DEGREE_ORDER 1u /* relabel vertices by ascending degree (ties by id) */
+
+/* ------------------------------------------------------------------ errors */
+const char *sge_last_error(void);
+int sge_version(void);
+
+/* ------------------------------------------------------------------ graphs */
+typedef struct sge_graph sge_graph;
+
+typedef struct sge_graph_stats {
+    uint32_t n;          /* vertices */
+    uint64_t m;          /* undirected edges after dedup / self-loop removal */
+    uint32_t max_degree;
+    uint32_t degree_ordered;
+    uint64_t csr_bytes;  /* offsets + neighbour bytes */
+    uint64_t triangles_hint; /* 0 unless computed */
+} sge_graph_stats;
+
+/* Text "n m" + m lines "u v", or the reference's binary "BINGRPH" + u32 n + u32 m
+ * + m x (u32,u32) (relation/graph.cpp:255-314, 357-419).  Duplicate edges and
+ * self-loops are dropped. */
+int sge_graph_load(const char *path, uint32_t flags, sge_graph **out);
+/* Undirected edge list, uv[2*i], uv[2*i+1]. */
+int sge_graph_from_edges(uint32_t n, const uint32_t *uv, uint64_t m, uint32_t flags, sge_graph **out);
+/* Symmetric CSR with ascending neighbour lists (the reference's Graph layout,
+ * relation/graph.h:29-102); nnz = 2m. */
+int sge_graph_from_csr(const uint32_t *offsets, const uint32_t *nbrs, uint32_t n, uint64_t nnz, uint32_t flags,
+                       sge_graph **out);
+/* Synthetic generators (seeded, deterministic, independent of thread count). */
+int sge_graph_chung_lu(uint32_t n, uint64_t m, double gamma, double degree_cap, uint64_t seed, uint32_t flags,
+                       sge_graph **out);
+int sge_graph_rmat(uint32_t scale, uint32_t edge_factor, double a, double b, double c, uint64_t seed,
+                   uint32_t flags, sge_graph **out);
+int sge_graph_stats_get(const sge_graph *g, sge_graph_stats *out);
+/* Copy the CSR (in the graph's own labelling) into caller buffers of n+1 / nnz entries. */
+int sge_graph_csr(const sge_graph *g, uint32_t *offsets, uint32_t *nbrs);
+/* perm[new_id] = original id (identity if not degree-ordered). */
+int sge_graph_perm(const sge_graph *g, uint32_t *perm);
+/* Write "n m" + "u v" (u<v, original ids) text, or BINGRPH when binary != 0. */
+int sge_graph_write(const sge_graph *g, const char *path, int binary);
+/* Upload the CSR to the current HIP device (idempotent). */
+int sge_graph_to_device(sge_graph *g);
+void sge_graph_destroy(sge_graph *g);
+
+/* ---------------------------------------------------------------- patterns */
+typedef struct sge_pattern {
+    uint32_t k;                 /* vertices, 1..SGE_MAX_K */
+    uint32_t adj[SGE_MAX_K];    /* adjacency bitmasks */
+} sge_pattern;
+
+typedef struct sge_pattern_info {
+    uint32_t k, m;
+    uint64_t aut_size;          /* |Aut(P)| */
+    uint64_t canon;             /* canonical code (min adjacency word over relabellings) */
+    uint32_t orbit[SGE_MAX_K];  /* orbit representative (smallest vertex) per vertex */
+    uint32_t core_size;         /* 2-core size (relation/graph.cpp:1231) */
+    uint32_t num_bags;          /* bags of the chosen minimal FHD incl. peripheral edge bags */
+    uint32_t single_bag;        /* 1: the reference prints embeddings, not subgraphs (decomposition.cpp:983) */
+    double fhw;                 /* fractional hypertree width of the core */
+    uint32_t connected;
+} sge_pattern_info;
+
+/* Pattern text file (relation/graph.cpp:661-772): "k m", m edges, optional orbit line (ignored). */
+int sge_pattern_load(const char *path, sge_pattern *out);
+int sge_pattern_from_edges(uint32_t k, const uint32_t *uv, uint32_t m, sge_pattern *out);
+int sge_pattern_analyze(const sge_pattern *p, sge_pattern_info *out);
+/* Connected k-vertex patterns with emin..emax edges: uniform edge count, uniform
+ * edge subset, reject disconnected and isomorphic duplicates (and codes listed in
+ * exclude).  Returns the number written (<= want) in *got. */
+int sge_pattern_sample(uint32_t k, uint32_t emin, uint32_t emax, uint32_t want, uint64_t seed,
+                       const uint64_t *exclude, uint32_t n_exclude, sge_pattern *out, uint32_t *got);
+
+/* Replace each embedding row (k images per pattern vertex) by its
+ * lexicographically smallest relabelling over Aut(P), then sort the rows:
+ * a plan-independent canonical embedding list. */
+int sge_canonicalize_embeddings(const sge_pattern *p, uint32_t *rows, uint64_t n);
+
+/* ------------------------------------------------------------------- plans */
+/* Flat plan descriptor: one matching order over the whole pattern, symmetry
+ * breaking as per-level clamps, last level(s) counted in closed form.  Level i
+ * matches pattern vertex order[i]; every mask is over LEVELS (bit j = level j). */
+typedef struct sge_plan {
+    uint32_t k;
+    uint32_t mode;                     /* SGE_MODE_* */
+    uint32_t order[SGE_MAX_K];         /* level -> pattern vertex */
+    uint32_t parent[SGE_MAX_K];        /* earlier levels adjacent in the pattern (backward neighbours) */
+    uint32_t lower[SGE_MAX_K];         /* image must be > image of these levels (setBeginPos, utils.cpp:581) */
+    uint32_t upper[SGE_MAX_K];         /* image must be < image of these levels (setEndPos, utils.cpp:599) */
+    uint32_t check[SGE_MAX_K];         /* image must differ from these levels' images (injectivity) */
+    int32_t reuse[SGE_MAX_K];          /* earlier level whose candidate set contains this one's (candidatesBefore), -1 */
+    uint32_t ready[SGE_MAX_K];         /* exploration: level after which this level's parent intersection is formed */
+    uint32_t tail;                     /* IEP: trailing symmetric group counted by choosec (0/1 = none) */
+    uint32_t num_bags;                 /* decomposition bag count (reporting / features) */
+    uint64_t multiplier;               /* |Aut| when the reference counts embeddings (single bag), else 1 */
+    uint64_t aut_size;
+    double cost;                       /* cost-model estimate of the order */
+} sge_plan;
+
+/* Build a plan.  order == NULL lets the cost model choose (plan/optimizer.cpp:441
+ * optCostOrder restated); g may be NULL (uniform-degree cost model). */
+int sge_plan_build(const sge_pattern *p, const sge_graph *g, uint32_t mode, const uint32_t *order, sge_plan *out);
+/* Cost-model estimate of one order (lower is better). */
+int sge_plan_cost(const sge_pattern *p, const sge_graph *g, const uint32_t *order, double *cost);
+/* All connected orders of the pattern (k! bounded), deduplicated up to automorphism; returns count in *n. */
+int sge_plan_orders(const sge_pattern *p, uint32_t *orders, uint32_t cap, uint32_t *n);
+/* Reference-style plan text ("num bags", symmetry rules, ...) into buf. */
+int sge_plan_text(const sge_plan *plan, char *buf, size_t cap);
+
+/* ---------------------------------------------------------------- executor */
+typedef struct sge_tuning {
+    uint32_t block_threads;   /* 64..1024, multiple of 64 (one DFS per wavefront) */
+    uint32_t chunk;           /* root tasks taken per queue fetch */
+    uint32_t stage_max;       /* stage a probed list window in LDS when <= this many ids */
+    uint32_t blocks_per_cu;   /* 0 = occupancy-derived */
+    uint32_t reserved[4];
+} sge_tuning;
+
+typedef struct sge_stats {
+    uint64_t intersections;   /* pairwise sorted-list intersections */
+    uint64_t bytes;           /* algorithmic bytes: 4(|A'|+|B'|) + 4|out| + 8 per CSR row */
+    uint64_t tasks;           /* root tasks processed */
+    uint64_t nodes;           /* materialised search-tree nodes */
+    double kernel_ms;         /* device time of the counting launch(es) */
+    double wall_ms;           /* host wall time of the call */
+} sge_stats;
+
+void sge_tuning_default(sge_tuning *t);
+
+/* Count embeddings of every plan over g.  counts[i] is assigned (not accumulated).
+ * flags: 0 = one launch per plan (plain executor); SGE_COUNT_BATCHED = one
+ * persistent launch over all plans with a shared root queue.
+ * stats may be NULL; per-plan stats when per_plan_stats != NULL (nplans entries). */
+int sge_count(sge_graph *g, const sge_plan *plans, uint32_t nplans, uint32_t flags, const sge_tuning *tune,
+              uint64_t *counts, sge_stats *stats, sge_stats *per_plan_stats);
+/* Count over the root-task subrange [task_begin, task_end) of one plan (counts are
+ * additive over disjoint ranges).  Used for sampled work logging. */
+int sge_count_range(sge_graph *g, const sge_plan *plan, uint64_t task_begin, uint64_t task_end,
+                    const sge_tuning *tune, uint64_t *count, sge_stats *stats);
+/* Number of root tasks (directed edges) of a plan over g. */
+int sge_num_tasks(const sge_graph *g, const sge_plan *plan, uint64_t *n);
+
+/* Write every counted embedding (one representative per subgraph, images in
+ * pattern-vertex order, original vertex ids) into out[cap*k]; *n_out gets the
+ * total (may exceed cap -> SGE_ERANGE, nothing beyond cap written). */
+int sge_enumerate(sge_graph *g, const sge_plan *plan, uint32_t *out, uint64_t cap, uint64_t *n_out);
+
+/* Batched executor captured once in a hipGraph, replayed per run. */
+typedef struct sge_batch sge_batch;
+int sge_batch_create(sge_graph *g, const sge_plan *plans, uint32_t nplans, const sge_tuning *tune, sge_batch **out);
+int sge_batch_run(sge_batch *b, uint64_t *counts, sge_stats *stats);
+void sge_batch_destroy(sge_batch *b);
+
+/* Device info for reporting. */
+int sge_device_info(int *num_cu, int *num_xcd_hint, char *name, size_t cap);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SGE_H */

@@ -1,0 +1,431 @@
+/*
+ * sge_oracle.c -- CPU restatement of the reference's WCOJ counting path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load this library, and only as the checker (or the timed
+ * CPU baseline); the product path never calls it.
+ *
+ * What each function restates (reference file:line):
+ *   oracle_set_begin_pos  utility/utils.cpp:581-597   (setBeginPos: first index with value > max)
+ *   oracle_set_end_pos    utility/utils.cpp:599-612   (setEndPos: first index with value >= min)
+ *   oracle_intersect      utility/compute_set_intersection.cpp:10-45, 89-147
+ *                         (ComputeCandidates: galloping when max/min > 50, merge otherwise)
+ *   oracle_choosec        utility/utils.cpp:80-94     (choosec, wrapping uint64 numerator)
+ *   count_last            relation/relation.cpp:301-338 (TrieLevel::numResults: |clamped range|
+ *                         minus matched pattern vertices whose image lies in the range)
+ *                         + matching/join.cpp:1532-1537 (IEP: choosec over the symmetric group)
+ *   dfs                   matching/join.cpp:137-178, 309-421 (generateCandidates + nodeJoin DFS
+ *                         with visited[] injectivity), parallel over root edges like
+ *                         matching/parallel.cpp:1414-1458 (schedule(dynamic), + reduction)
+ *   oracle_count_brute    no reference counterpart: all injective homomorphisms by plain
+ *                         backtracking, the independent check for tiny graphs.
+ *
+ * Plans are the flat sge_plan of include/sge.h: level i matches pattern vertex
+ * order[i]; parent/lower/upper/check masks are over levels.
+ *
+ * Parity pins (tests/test_oracle.py): golden counts written by the reference
+ * binary built from its own sources (tests/golden/*.json), K_n closed forms
+ * n!/(n-k)!/|Aut|, brute force on random small graphs, the diamond formula
+ * sum over edges of C(common neighbours, 2), mode/order/relabelling invariance,
+ * std-library set operations for the primitives, and a hand-computed wrap case
+ * for choosec.
+ */
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#include "../include/sge.h"
+
+#define BIN_THRESHOLD 32 /* BINARY_SEARCH_THRESHOLD, config/config.h:36 */
+
+uint32_t oracle_set_begin_pos(const uint32_t *a, uint32_t count, uint32_t max_compared) {
+    if (count <= BIN_THRESHOLD) {
+        uint32_t j;
+        for (j = 0; j < count; ++j)
+            if (a[j] > max_compared) return j;
+        return count;
+    }
+    uint32_t lo = 0, hi = count;
+    while (lo < hi) {
+        uint32_t mid = lo + (hi - lo) / 2;
+        if (a[mid] <= max_compared) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+uint32_t oracle_set_end_pos(const uint32_t *a, uint32_t count, uint32_t min_compared) {
+    if (count <= BIN_THRESHOLD) {
+        uint32_t j;
+        for (j = 0; j < count; ++j)
+            if (a[j] >= min_compared) return j;
+        return count;
+    }
+    uint32_t lo = 0, hi = count;
+    while (lo < hi) {
+        uint32_t mid = lo + (hi - lo) / 2;
+        if (a[mid] < min_compared) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+/* first index in [lo, hi) with a[idx] >= key */
+static uint32_t lower_bound_u32(const uint32_t *a, uint32_t lo, uint32_t hi, uint32_t key) {
+    while (lo < hi) {
+        uint32_t mid = lo + (hi - lo) / 2;
+        if (a[mid] < key) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+/* galloping from position lo: exponential probe then binary search */
+static uint32_t gallop(const uint32_t *a, uint32_t lo, uint32_t n, uint32_t key) {
+    uint32_t step = 1, hi = lo;
+    while (hi < n && a[hi] < key) {
+        lo = hi + 1;
+        hi += step;
+        step <<= 1;
+    }
+    if (hi > n) hi = n;
+    return lower_bound_u32(a, lo, hi, key);
+}
+
+uint32_t oracle_intersect(const uint32_t *a, uint32_t na, const uint32_t *b, uint32_t nb, uint32_t *out) {
+    uint32_t cnt = 0;
+    if (na == 0 || nb == 0) return 0;
+    if (na > nb) {
+        const uint32_t *t = a; a = b; b = t;
+        uint32_t tn = na; na = nb; nb = tn;
+    }
+    if ((double)nb / (double)na > 50.0) {
+        uint32_t pos = 0, i;
+        for (i = 0; i < na && pos < nb; ++i) {
+            pos = gallop(b, pos, nb, a[i]);
+            if (pos < nb && b[pos] == a[i]) {
+                if (out) out[cnt] = a[i];
+                ++cnt;
+                ++pos;
+            }
+        }
+        return cnt;
+    }
+    uint32_t i = 0, j = 0;
+    while (i < na && j < nb) {
+        if (a[i] < b[j]) ++i;
+        else if (a[i] > b[j]) ++j;
+        else {
+            if (out) out[cnt] = a[i];
+            ++cnt;
+            ++i;
+            ++j;
+        }
+    }
+    return cnt;
+}
+
+uint64_t oracle_choosec(uint64_t n, int k) {
+    if ((uint64_t)k > n) return 0;
+    if (k == 0) return 1;
+    uint64_t nom = n, denom = 1;
+    int i;
+    for (i = 1; i < k; ++i) {
+        nom *= (n - (uint64_t)i);
+        denom *= (uint64_t)(i + 1);
+    }
+    return nom / denom;
+}
+
+typedef struct {
+    const sge_plan *pl;
+    const uint32_t *off, *nbr;
+    uint32_t n, maxdeg;
+    uint32_t phi[SGE_MAX_K];
+    uint32_t *buf[SGE_MAX_K];  /* candidate set per level */
+    uint32_t len[SGE_MAX_K];
+    uint32_t *tmp, *tmp2;
+    uint32_t *emb;             /* enumeration output (may be NULL) */
+    uint64_t emb_cap, *emb_n;
+    uint64_t intersections;
+} ctx_t;
+
+/* clamp bounds of level d from the images of its lower/upper levels */
+static void bounds(const ctx_t *c, int d, uint32_t *lo_excl, int *has_lo, uint32_t *hi_excl, int *has_hi) {
+    const sge_plan *pl = c->pl;
+    uint32_t mx = 0, mn = 0xffffffffu;
+    int j;
+    *has_lo = 0;
+    *has_hi = 0;
+    for (j = 0; j < d; ++j) {
+        if ((pl->lower[d] >> j) & 1u) {
+            if (!*has_lo || c->phi[j] > mx) mx = c->phi[j];
+            *has_lo = 1;
+        }
+        if ((pl->upper[d] >> j) & 1u) {
+            if (!*has_hi || c->phi[j] < mn) mn = c->phi[j];
+            *has_hi = 1;
+        }
+    }
+    *lo_excl = mx;
+    *hi_excl = mn;
+}
+
+/* Candidate set of level d (ignoring the IEP group's internal chain when
+ * `group_base` >= 0: members of the group beyond group_base are not matched yet).
+ * Returns the size; elements in c->buf[d] when materialise != 0. */
+static uint32_t candidates(ctx_t *c, int d, int materialise, uint32_t lower_mask, uint32_t upper_mask,
+                           uint32_t check_mask) {
+    const sge_plan *pl = c->pl;
+    uint32_t lists = pl->parent[d];
+    const uint32_t *src = NULL;
+    uint32_t srclen = 0;
+    int j;
+    int r = pl->reuse[d];
+    uint32_t mx = 0, mn = 0xffffffffu;
+    int has_lo = 0, has_hi = 0;
+    for (j = 0; j < d; ++j) {
+        if ((lower_mask >> j) & 1u) { if (!has_lo || c->phi[j] > mx) mx = c->phi[j]; has_lo = 1; }
+        if ((upper_mask >> j) & 1u) { if (!has_hi || c->phi[j] < mn) mn = c->phi[j]; has_hi = 1; }
+    }
+    if (pl->mode != SGE_MODE_EXPLORE && r >= 0) {
+        src = c->buf[r];
+        srclen = c->len[r];
+        lists &= ~pl->parent[r];
+    }
+    /* generateCandidates: clamp every list, then chain pairwise intersections */
+    uint32_t *cur = c->tmp, *nxt = c->tmp2;
+    uint32_t curlen = 0;
+    int first = 1;
+    if (src) {
+        uint32_t b = has_lo ? oracle_set_begin_pos(src, srclen, mx) : 0;
+        uint32_t e = has_hi ? oracle_set_end_pos(src, srclen, mn) : srclen;
+        if (e < b) e = b;
+        memcpy(cur, src + b, (size_t)(e - b) * 4);
+        curlen = e - b;
+        first = 0;
+    }
+    for (j = 0; j < d; ++j) {
+        if (!((lists >> j) & 1u)) continue;
+        uint32_t v = c->phi[j];
+        const uint32_t *row = c->nbr + c->off[v];
+        uint32_t deg = c->off[v + 1] - c->off[v];
+        uint32_t b = has_lo ? oracle_set_begin_pos(row, deg, mx) : 0;
+        uint32_t e = has_hi ? oracle_set_end_pos(row, deg, mn) : deg;
+        if (e < b) e = b;
+        if (first) {
+            memcpy(cur, row + b, (size_t)(e - b) * 4);
+            curlen = e - b;
+            first = 0;
+        } else {
+            curlen = oracle_intersect(cur, curlen, row + b, e - b, nxt);
+            uint32_t *t = cur; cur = nxt; nxt = t;
+            c->intersections++;
+        }
+        if (curlen == 0) break;
+    }
+    /* injectivity: drop images of already matched, non-adjacent, unclamped levels */
+    uint32_t out = 0, i;
+    uint32_t *dst = materialise ? c->buf[d] : NULL;
+    for (i = 0; i < curlen; ++i) {
+        uint32_t x = cur[i];
+        int bad = 0;
+        for (j = 0; j < d && !bad; ++j)
+            if (((check_mask >> j) & 1u) && c->phi[j] == x) bad = 1;
+        if (bad) continue;
+        if (dst) dst[out] = x;
+        ++out;
+    }
+    if (materialise) c->len[d] = out;
+    return out;
+}
+
+static void emit(ctx_t *c, int upto) {
+    /* record phi (by pattern vertex) for levels < upto plus every candidate of level upto */
+    const sge_plan *pl = c->pl;
+    uint32_t i;
+    for (i = 0; i < c->len[upto]; ++i) {
+        uint64_t slot;
+#pragma omp atomic capture
+        slot = (*c->emb_n)++;
+        if (slot >= c->emb_cap) continue;
+        uint32_t *row = c->emb + slot * pl->k;
+        int j;
+        for (j = 0; j < upto; ++j) row[pl->order[j]] = c->phi[j];
+        row[pl->order[upto]] = c->buf[upto][i];
+    }
+}
+
+static uint64_t dfs(ctx_t *c, int d) {
+    const sge_plan *pl = c->pl;
+    int k = (int)pl->k;
+    int tail = pl->tail >= 2 ? (int)pl->tail : 0;
+    int last = tail ? k - tail : k - 1;
+    if (d == last) {
+        if (c->emb) {
+            candidates(c, d, 1, pl->lower[d], pl->upper[d], pl->check[d]);
+            emit(c, d);
+            return c->len[d];
+        }
+        if (tail) {
+            /* group members share one candidate set; their chain is counted by choosec */
+            uint32_t below = (1u << d) - 1;
+            uint32_t n = candidates(c, d, 0, pl->lower[d] & below, pl->upper[d] & below, pl->check[d] & below);
+            return oracle_choosec(n, tail);
+        }
+        return candidates(c, d, 0, pl->lower[d], pl->upper[d], pl->check[d]);
+    }
+    candidates(c, d, 1, pl->lower[d], pl->upper[d], pl->check[d]);
+    uint64_t sum = 0;
+    uint32_t i;
+    for (i = 0; i < c->len[d]; ++i) {
+        c->phi[d] = c->buf[d][i];
+        sum += dfs(c, d + 1);
+    }
+    return sum;
+}
+
+static int ctx_init(ctx_t *c, const sge_plan *pl, const uint32_t *off, const uint32_t *nbr, uint32_t n) {
+    uint32_t v, i;
+    memset(c, 0, sizeof(*c));
+    c->pl = pl;
+    c->off = off;
+    c->nbr = nbr;
+    c->n = n;
+    c->maxdeg = 0;
+    for (v = 0; v < n; ++v)
+        if (off[v + 1] - off[v] > c->maxdeg) c->maxdeg = off[v + 1] - off[v];
+    for (i = 0; i < SGE_MAX_K; ++i) {
+        c->buf[i] = (uint32_t *)malloc(((size_t)c->maxdeg + 1) * 4);
+        if (!c->buf[i]) return -1;
+    }
+    c->tmp = (uint32_t *)malloc(((size_t)c->maxdeg + 1) * 4);
+    c->tmp2 = (uint32_t *)malloc(((size_t)c->maxdeg + 1) * 4);
+    return (c->tmp && c->tmp2) ? 0 : -1;
+}
+
+static void ctx_free(ctx_t *c) {
+    int i;
+    for (i = 0; i < SGE_MAX_K; ++i) free(c->buf[i]);
+    free(c->tmp);
+    free(c->tmp2);
+}
+
+/* source vertex of CSR position e */
+static uint32_t src_of(const uint32_t *off, uint32_t n, uint64_t e) {
+    uint32_t lo = 0, hi = n;
+    while (hi - lo > 1) {
+        uint32_t mid = lo + (hi - lo) / 2;
+        if ((uint64_t)off[mid] <= e) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+/* Count over root tasks [tb, te) (tasks are CSR positions = directed edges).
+ * threads <= 0: OpenMP default.  Returns the count (times plan multiplier). */
+uint64_t oracle_count_range(const sge_plan *pl, const uint32_t *off, const uint32_t *nbr, uint32_t n,
+                            uint64_t tb, uint64_t te, int threads, uint64_t *intersections) {
+    uint64_t total = 0, inter = 0;
+    int k = (int)pl->k;
+    if (k == 1) return (uint64_t)n * pl->multiplier;
+#ifdef _OPENMP
+    if (threads > 0) omp_set_num_threads(threads);
+#endif
+#pragma omp parallel reduction(+ : total, inter)
+    {
+        ctx_t c;
+        ctx_init(&c, pl, off, nbr, n);
+        uint64_t t;
+#pragma omp for schedule(dynamic, 64)
+        for (t = tb; t < te; ++t) {
+            uint32_t v0 = src_of(off, n, t), v1 = nbr[t];
+            if (((pl->lower[1] & 1u) && !(v1 > v0)) || ((pl->upper[1] & 1u) && !(v1 < v0))) continue;
+            c.phi[0] = v0;
+            c.phi[1] = v1;
+            if (k == 2) {
+                total += 1;
+                continue;
+            }
+            total += dfs(&c, 2);
+        }
+        inter += c.intersections;
+        ctx_free(&c);
+    }
+    if (intersections) *intersections = inter;
+    return total * pl->multiplier;
+}
+
+uint64_t oracle_count(const sge_plan *pl, const uint32_t *off, const uint32_t *nbr, uint32_t n, int threads) {
+    return oracle_count_range(pl, off, nbr, n, 0, off[n], threads, NULL);
+}
+
+/* Enumerate one representative per counted subgraph (plan order clamps), images
+ * written per pattern vertex into emb[cap*k].  Returns the number found. */
+uint64_t oracle_enumerate(const sge_plan *pl, const uint32_t *off, const uint32_t *nbr, uint32_t n, uint32_t *emb,
+                          uint64_t cap) {
+    uint64_t found = 0;
+    sge_plan p2 = *pl;
+    p2.tail = 0; /* enumerate every member */
+    if (p2.k < 3) return 0;
+#pragma omp parallel
+    {
+        ctx_t c;
+        ctx_init(&c, &p2, off, nbr, n);
+        c.emb = emb;
+        c.emb_cap = cap;
+        c.emb_n = &found;
+        uint64_t t;
+#pragma omp for schedule(dynamic, 64)
+        for (t = 0; t < off[n]; ++t) {
+            uint32_t v0 = src_of(off, n, t), v1 = nbr[t];
+            if (((p2.lower[1] & 1u) && !(v1 > v0)) || ((p2.upper[1] & 1u) && !(v1 < v0))) continue;
+            c.phi[0] = v0;
+            c.phi[1] = v1;
+            dfs(&c, 2);
+        }
+        ctx_free(&c);
+    }
+    return found;
+}
+
+/* Brute force: every injective homomorphism of the k-vertex pattern (adjacency
+ * bitmasks) into the graph, by backtracking in vertex order 0..k-1 with a plain
+ * adjacency test.  No reference counterpart; independent check for small graphs. */
+static int has_edge(const uint32_t *off, const uint32_t *nbr, uint32_t a, uint32_t b) {
+    uint32_t lo = off[a], hi = off[a + 1];
+    while (lo < hi) {
+        uint32_t mid = lo + (hi - lo) / 2;
+        if (nbr[mid] < b) lo = mid + 1; else hi = mid;
+    }
+    return lo < off[a + 1] && nbr[lo] == b;
+}
+
+static uint64_t brute_rec(int i, int k, const uint32_t *adj, const uint32_t *off, const uint32_t *nbr, uint32_t n,
+                          uint32_t *img) {
+    if (i == k) return 1;
+    uint64_t s = 0;
+    uint32_t v;
+    for (v = 0; v < n; ++v) {
+        int ok = 1, j;
+        for (j = 0; j < i && ok; ++j) {
+            if (img[j] == v) ok = 0;
+            else if (((adj[i] >> j) & 1u) && !has_edge(off, nbr, img[j], v)) ok = 0;
+        }
+        if (!ok) continue;
+        img[i] = v;
+        s += brute_rec(i + 1, k, adj, off, nbr, n, img);
+    }
+    return s;
+}
+
+uint64_t oracle_count_brute(uint32_t k, const uint32_t *adj, const uint32_t *off, const uint32_t *nbr, uint32_t n) {
+    uint64_t total = 0;
+    uint32_t v0;
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : total)
+    for (v0 = 0; v0 < n; ++v0) {
+        uint32_t img[SGE_MAX_K];
+        img[0] = v0;
+        total += brute_rec(1, (int)k, adj, off, nbr, n, img);
+    }
+    return total;
+}

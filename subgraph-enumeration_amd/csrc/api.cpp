@@ -1,0 +1,187 @@
+// C-ABI entry points for graphs and errors (include/sge.h).
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+
+#include "common.h"
+
+namespace sge {
+static thread_local std::string g_err;
+void set_error(const std::string &msg) { g_err = msg; }
+}  // namespace sge
+
+using namespace sge;
+
+extern "C" void sge_graph_device_free(sge_graph *g);
+
+extern "C" const char *sge_last_error(void) { return g_err.c_str(); }
+extern "C" int sge_version(void) { return 10000; }
+
+static int finish_graph(uint32_t n, EdgeVec &edges, uint32_t flags, sge_graph **out) {
+    if (!out) {
+        set_error("null output handle");
+        return SGE_EINVAL;
+    }
+    sge_graph *g = new sge_graph();
+    int rc = build_graph(n, edges, flags, g->h);
+    if (rc) {
+        delete g;
+        return rc;
+    }
+    *out = g;
+    return SGE_OK;
+}
+
+extern "C" int sge_graph_load(const char *path, uint32_t flags, sge_graph **out) {
+    if (!path) {
+        set_error("null path");
+        return SGE_EINVAL;
+    }
+    uint32_t n = 0;
+    EdgeVec edges;
+    int rc = read_edge_file(path, n, edges);
+    if (rc) return rc;
+    return finish_graph(n, edges, flags, out);
+}
+
+extern "C" int sge_graph_from_edges(uint32_t n, const uint32_t *uv, uint64_t m, uint32_t flags, sge_graph **out) {
+    if (m && !uv) {
+        set_error("null edge list");
+        return SGE_EINVAL;
+    }
+    EdgeVec edges(m);
+    for (uint64_t i = 0; i < m; ++i) {
+        if (uv[2 * i] >= n || uv[2 * i + 1] >= n) {
+            set_error("edge endpoint out of range");
+            return SGE_EINVAL;
+        }
+        edges[i] = ((uint64_t)uv[2 * i] << 32) | uv[2 * i + 1];
+    }
+    return finish_graph(n, edges, flags, out);
+}
+
+extern "C" int sge_graph_from_csr(const uint32_t *off, const uint32_t *nbr, uint32_t n, uint64_t nnz, uint32_t flags,
+                                  sge_graph **out) {
+    if (!off || (nnz && !nbr)) {
+        set_error("null CSR array");
+        return SGE_EINVAL;
+    }
+    if (off[n] != nnz) {
+        set_error("offsets[n] != nnz");
+        return SGE_EINVAL;
+    }
+    EdgeVec edges;
+    edges.reserve(nnz / 2);
+    for (uint32_t v = 0; v < n; ++v) {
+        if (off[v + 1] < off[v]) {
+            set_error("offsets not monotone");
+            return SGE_EINVAL;
+        }
+        for (uint32_t e = off[v]; e < off[v + 1]; ++e) {
+            uint32_t u = nbr[e];
+            if (u >= n) {
+                set_error("neighbour id out of range");
+                return SGE_EINVAL;
+            }
+            if (v < u) edges.push_back(((uint64_t)v << 32) | u);
+        }
+    }
+    return finish_graph(n, edges, flags, out);
+}
+
+extern "C" int sge_graph_chung_lu(uint32_t n, uint64_t m, double gamma, double cap, uint64_t seed, uint32_t flags,
+                                  sge_graph **out) {
+    if (n < 2 || gamma <= 1.0 || m > (uint64_t)n * (n - 1) / 2) {
+        set_error("chung_lu: need n >= 2, gamma > 1, m <= n(n-1)/2");
+        return SGE_EINVAL;
+    }
+    EdgeVec edges;
+    chung_lu_edges(n, m, gamma, cap > 0 ? cap : 1e300, seed, edges);
+    return finish_graph(n, edges, flags, out);
+}
+
+extern "C" int sge_graph_rmat(uint32_t scale, uint32_t ef, double a, double b, double c, uint64_t seed, uint32_t flags,
+                              sge_graph **out) {
+    if (scale < 1 || scale > 30 || ef == 0 || a < 0 || b < 0 || c < 0 || a + b + c > 1.0) {
+        set_error("rmat: bad parameters");
+        return SGE_EINVAL;
+    }
+    EdgeVec edges;
+    rmat_edges(scale, ef, a, b, c, seed, edges);
+    return finish_graph(1u << scale, edges, flags, out);
+}
+
+extern "C" int sge_graph_stats_get(const sge_graph *g, sge_graph_stats *s) {
+    if (!g || !s) {
+        set_error("null argument");
+        return SGE_EINVAL;
+    }
+    std::memset(s, 0, sizeof(*s));
+    s->n = g->h.n;
+    s->m = g->h.nnz / 2;
+    s->max_degree = g->h.max_degree;
+    s->degree_ordered = g->h.degree_ordered;
+    s->csr_bytes = 4ull * ((uint64_t)g->h.n + 1) + 4ull * g->h.nnz;
+    return SGE_OK;
+}
+
+extern "C" int sge_graph_csr(const sge_graph *g, uint32_t *off, uint32_t *nbr) {
+    if (!g || !off || (g->h.nnz && !nbr)) {
+        set_error("null argument");
+        return SGE_EINVAL;
+    }
+    std::memcpy(off, g->h.off.data(), 4ull * ((uint64_t)g->h.n + 1));
+    if (g->h.nnz) std::memcpy(nbr, g->h.nbr.data(), 4ull * g->h.nnz);
+    return SGE_OK;
+}
+
+extern "C" int sge_graph_perm(const sge_graph *g, uint32_t *perm) {
+    if (!g || !perm) {
+        set_error("null argument");
+        return SGE_EINVAL;
+    }
+    std::memcpy(perm, g->h.perm.data(), 4ull * g->h.n);
+    return SGE_OK;
+}
+
+extern "C" int sge_graph_write(const sge_graph *g, const char *path, int binary) {
+    if (!g || !path) {
+        set_error("null argument");
+        return SGE_EINVAL;
+    }
+    const HostGraph &h = g->h;
+    std::vector<uint64_t> es;
+    es.reserve(h.nnz / 2);
+    for (uint32_t v = 0; v < h.n; ++v)
+        for (uint32_t e = h.off[v]; e < h.off[v + 1]; ++e) {
+            uint32_t a = h.perm[v], b = h.perm[h.nbr[e]];
+            if (a < b) es.push_back(((uint64_t)a << 32) | b);
+        }
+    std::sort(es.begin(), es.end());
+    FILE *f = std::fopen(path, binary ? "wb" : "w");
+    if (!f) {
+        set_error(std::string("Can not create file ") + path + "!");
+        return SGE_EIO;
+    }
+    if (binary) {
+        uint32_t hdr[2] = {h.n, (uint32_t)es.size()};
+        std::fwrite("BINGRPH", 1, 7, f);
+        std::fwrite(hdr, 4, 2, f);
+        for (uint64_t e : es) {
+            uint32_t uv[2] = {(uint32_t)(e >> 32), (uint32_t)e};
+            std::fwrite(uv, 4, 2, f);
+        }
+    } else {
+        std::fprintf(f, "%u %llu\n", h.n, (unsigned long long)es.size());
+        for (uint64_t e : es) std::fprintf(f, "%u %u\n", (uint32_t)(e >> 32), (uint32_t)e);
+    }
+    std::fclose(f);
+    return SGE_OK;
+}
+
+extern "C" void sge_graph_destroy(sge_graph *g) {
+    if (!g) return;
+    sge_graph_device_free(g);
+    delete g;
+}

@@ -1,0 +1,1167 @@
+// MI355X (gfx950) WCOJ enumeration executor.
+//
+// Replaces the reference's join executors -- sharedJoin (matching/join.cpp:812),
+// parSharedJoin WCOJ / exploration (matching/parallel.cpp:1154, 1488) and their
+// set-intersection kernels (utility/compute_set_intersection.cpp) -- with one
+// persistent kernel:
+//
+//  * one depth-first search per wavefront (64 lanes); the wave's per-level
+//    candidate buffers live in a per-wave global scratch slab, its DFS cursor
+//    (images, lengths, positions, list descriptors) in LDS;
+//  * root tasks are directed data edges (CSR positions), handed out in chunks by
+//    a global atomic queue (the OpenMP schedule(dynamic) root split of
+//    parallel.cpp:1254-1299 / 1414-1458); chunks run from the high-degree end of
+//    the degree-ordered CSR down, so heavy subtrees start first;
+//  * every level's candidate set is an intersection of sorted neighbour lists
+//    clamped by symmetry-breaking bounds (setBeginPos/setEndPos, utils.cpp:581-612)
+//    computed by lane-parallel binary search, the shortest list driving; survivors
+//    are compacted with a wave64 ballot + popcount prefix;
+//  * the last level is never enumerated: like TrieLevel::numResults
+//    (relation.cpp:301-338) it is counted in closed form, and all siblings of the
+//    second-to-last level are counted together with a flattened, load-balanced
+//    lane mapping (per-sibling work prefix-summed in LDS);
+//  * IEP mode counts a trailing symmetric group with choosec (utils.cpp:80-94,
+//    join.cpp:1532-1537), wrapping uint64 arithmetic included;
+//  * exploration mode forms each level's parent intersection as soon as its last
+//    parent is matched (attributesAfter, join.cpp:423-563) and only filters it later;
+//  * batched mode runs many plans in the same launch: queue item i is chunk i/P of
+//    plan i%P, so all patterns drain the shared queue together.
+// Counts are exact uint64.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "common.h"
+
+#define HIP_TRY(x)                                                                         \
+    do {                                                                                   \
+        hipError_t _e = (x);                                                               \
+        if (_e != hipSuccess) {                                                            \
+            sge::set_error(std::string("HIP error: ") + hipGetErrorString(_e) + " at " #x); \
+            return SGE_EDEVICE;                                                            \
+        }                                                                                  \
+    } while (0)
+
+namespace {
+
+constexpr int WAVE = 64;
+constexpr int MAXK = SGE_MAX_K;
+constexpr int SIB_BLOCK = 256;  // siblings per flattened last-level batch
+constexpr uint32_t NONE = 0xffffffffu;
+
+// Device copy of a plan (levels masks) plus derived fields.
+struct DPlan {
+    uint32_t k, tail, mode, last, sib;  // last: level counted in closed form; sib = last - 1
+    uint32_t parent[MAXK], lower[MAXK], upper[MAXK], check[MAXK];
+    int32_t reuse[MAXK];
+    uint32_t ready[MAXK];
+    // last-level decomposition (relative to the sibling level s)
+    uint32_t f_parent;     // parents of `last` other than s
+    uint32_t s_is_parent;  // s adjacent to last
+    uint32_t s_lower, s_upper, s_check;  // last's clamp/check against s
+    uint32_t f_ready;      // exploration: level after which F's intersection is formed
+    uint32_t order[MAXK];  // level -> pattern vertex (enumeration output)
+    uint32_t pad[2];
+};
+
+struct PlanOut {
+    unsigned long long count, inter, bytes, nodes, tasks;
+    unsigned long long pad[3];
+};
+
+struct KParams {
+    const uint32_t *off, *nbr, *src;
+    uint32_t n;
+    uint64_t nnz;
+    const DPlan *plans;
+    uint32_t nplans;
+    uint64_t task_begin, task_end;
+    uint32_t chunk;
+    uint64_t nitems;
+    unsigned long long *queue;
+    uint32_t *scratch;
+    uint64_t wave_stride;  // u32 per wave
+    uint32_t buf_stride;   // u32 per buffer
+    PlanOut *out;
+    uint32_t *emb;
+    uint64_t emb_cap;
+    unsigned long long *emb_n;
+};
+
+// list descriptor: [ptr + b, ptr + e)
+struct Src {
+    const uint32_t *ptr;
+    uint32_t b, e;
+};
+
+// Per-wave LDS state.  Uniform values are written by every lane (same value),
+// so no cross-lane ordering is needed for them.
+struct WaveLds {
+    DPlan pl;
+    uint32_t phi[MAXK];
+    uint32_t len[MAXK];
+    uint32_t pos[MAXK];
+    Src pre[MAXK + 1];  // exploration: pre-intersected parent lists (index MAXK = F)
+    // flattened sibling batch
+    uint32_t pre_sum[SIB_BLOCK + 1];
+    uint32_t drv[SIB_BLOCK];   // driver start (index into drv array base)
+    uint32_t tlo[SIB_BLOCK];   // target range
+    uint32_t thi[SIB_BLOCK];
+    uint32_t cnt[SIB_BLOCK];   // per-sibling counts (IEP)
+    uint32_t flag[SIB_BLOCK];  // bit0: driver is the CSR row (target F), else driver F (target row)
+};
+
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+__device__ __forceinline__ uint32_t uni(uint32_t x) { return __builtin_amdgcn_readfirstlane(x); }
+
+// first index in [lo, hi) with a[i] >= key
+__device__ __forceinline__ uint32_t lower_bound(const uint32_t *a, uint32_t lo, uint32_t hi, uint32_t key) {
+    while (lo < hi) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (a[mid] < key) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+// first index in [lo, hi) with a[i] > key
+__device__ __forceinline__ uint32_t upper_bound(const uint32_t *a, uint32_t lo, uint32_t hi, uint32_t key) {
+    while (lo < hi) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (a[mid] <= key) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ bool contains(const uint32_t *a, uint32_t lo, uint32_t hi, uint32_t key) {
+    uint32_t p = lower_bound(a, lo, hi, key);
+    return p < hi && a[p] == key;
+}
+
+// Wave-cooperative: clamp [b,e) of a sorted list to (lo, hi) exclusive bounds.
+// Executed redundantly by all lanes (uniform result).
+__device__ __forceinline__ void clamp_range(const uint32_t *a, uint32_t &b, uint32_t &e, bool has_lo, uint32_t lo,
+                                            bool has_hi, uint32_t hi) {
+    if (has_lo) b = upper_bound(a, b, e, lo);
+    if (has_hi) e = lower_bound(a, b, e, hi);
+    if (e < b) e = b;
+}
+
+__device__ __forceinline__ uint64_t choosec(uint64_t n, uint32_t k) {
+    if ((uint64_t)k > n) return 0;
+    if (k == 0) return 1;
+    uint64_t nom = n, den = 1;
+    for (uint32_t i = 1; i < k; ++i) {
+        nom *= (n - i);
+        den *= (i + 1);
+    }
+    return nom / den;
+}
+
+struct Acc {
+    unsigned long long count, inter, bytes, nodes, tasks;
+};
+
+// Bounds of level d from the images of the levels in the masks.
+__device__ __forceinline__ void level_bounds(const WaveLds &w, uint32_t lmask, uint32_t umask, bool &has_lo,
+                                             uint32_t &lo, bool &has_hi, uint32_t &hi) {
+    has_lo = lmask != 0;
+    has_hi = umask != 0;
+    lo = 0;
+    hi = 0xffffffffu;
+    for (uint32_t m = lmask; m; m &= m - 1) lo = max(lo, w.phi[__builtin_ctz(m)]);
+    for (uint32_t m = umask; m; m &= m - 1) hi = min(hi, w.phi[__builtin_ctz(m)]);
+}
+
+// Intersect the lists in srcs[0..ns) (already clamped), drop check images, write
+// survivors to out (if non-null).  Returns the (uniform) survivor count.
+__device__ uint32_t wave_intersect(Src *srcs, int ns, const uint32_t *chk, int nchk, uint32_t *out, Acc &acc) {
+    // shortest list drives
+    int drv = 0;
+    for (int i = 1; i < ns; ++i)
+        if (srcs[i].e - srcs[i].b < srcs[drv].e - srcs[drv].b) drv = i;
+    const uint32_t *D = srcs[drv].ptr;
+    uint32_t db = srcs[drv].b, de = srcs[drv].e;
+    uint64_t bytes = 0;
+    for (int i = 0; i < ns; ++i) bytes += 4ull * (srcs[i].e - srcs[i].b);
+    acc.bytes += bytes;
+    acc.inter += (ns > 1) ? (ns - 1) : 0;
+    uint32_t lo[MAXK];
+    for (int i = 0; i < ns; ++i) lo[i] = srcs[i].b;
+    uint32_t total = 0;
+    const int lane = lane_id();
+    if (db >= de) return 0;
+    for (uint32_t base = db; base < de; base += WAVE) {
+        uint32_t idx = base + lane;
+        bool ok = idx < de;
+        uint32_t x = ok ? D[idx] : 0xffffffffu;
+        for (int i = 0; i < ns; ++i) {
+            if (i == drv) continue;
+            uint32_t p = lower_bound(srcs[i].ptr, lo[i], srcs[i].e, x);
+            ok = ok && p < srcs[i].e && srcs[i].ptr[p] == x;
+            // next chunk starts beyond the last lane's key
+            uint32_t last = __shfl(p, WAVE - 1);
+            lo[i] = last;
+        }
+        for (int c = 0; c < nchk; ++c) ok = ok && (x != chk[c]);
+        uint64_t mask = __ballot(ok);
+        if (out) {
+            uint32_t pre = __popcll(mask & ((1ull << lane) - 1ull));
+            if (ok) out[total + pre] = x;
+        }
+        total += __popcll(mask);
+    }
+    if (out) {
+        acc.bytes += 4ull * total;
+        wave_sync();  // other lanes read the survivors next
+    }
+    return total;
+}
+
+__device__ __forceinline__ const uint32_t *buf_ptr(const KParams &P, uint32_t *wave_base, int i) {
+    return wave_base + (uint64_t)i * P.buf_stride;
+}
+
+// Build the list descriptors for the parent intersection of level d; apply the
+// level's bounds.  Returns the number of lists.
+__device__ int gather_sources(const KParams &P, WaveLds &w, uint32_t *wb, int d, uint32_t pmask, int reuse,
+                              bool use_pre, bool has_lo, uint32_t lo, bool has_hi, uint32_t hi, Src *srcs, Acc &acc) {
+    int ns = 0;
+    if (use_pre) {
+        Src s = w.pre[d];
+        clamp_range(s.ptr, s.b, s.e, has_lo, lo, has_hi, hi);
+        srcs[ns++] = s;
+        return ns;
+    }
+    if (reuse >= 0) {
+        Src s;
+        s.ptr = buf_ptr(P, wb, reuse);
+        s.b = 0;
+        s.e = w.len[reuse];
+        pmask &= ~w.pl.parent[reuse];
+        clamp_range(s.ptr, s.b, s.e, has_lo, lo, has_hi, hi);
+        srcs[ns++] = s;
+    }
+    for (uint32_t m = pmask; m; m &= m - 1) {
+        uint32_t v = w.phi[__builtin_ctz(m)];
+        Src s;
+        s.ptr = P.nbr;
+        s.b = P.off[v];
+        s.e = P.off[v + 1];
+        clamp_range(s.ptr, s.b, s.e, has_lo, lo, has_hi, hi);
+        srcs[ns++] = s;
+        acc.bytes += 8;
+    }
+    return ns;
+}
+
+// Materialise the candidate set of level d into buffer d.
+__device__ void materialize(const KParams &P, WaveLds &w, uint32_t *wb, int d, Acc &acc) {
+    const DPlan &pl = w.pl;
+    bool has_lo, has_hi;
+    uint32_t lo, hi;
+    level_bounds(w, pl.lower[d], pl.upper[d], has_lo, lo, has_hi, hi);
+    Src srcs[MAXK + 1];
+    bool use_pre = pl.mode == SGE_MODE_EXPLORE;
+    int ns = gather_sources(P, w, wb, d, pl.parent[d], use_pre ? -1 : pl.reuse[d], use_pre, has_lo, lo, has_hi, hi,
+                            srcs, acc);
+    uint32_t chk[MAXK];
+    int nchk = 0;
+    for (uint32_t m = pl.check[d]; m; m &= m - 1) chk[nchk++] = w.phi[__builtin_ctz(m)];
+    uint32_t n = wave_intersect(srcs, ns, chk, nchk, const_cast<uint32_t *>(buf_ptr(P, wb, d)), acc);
+    w.len[d] = n;
+    w.pos[d] = 0;
+    acc.nodes += 1;
+}
+
+// Exploration: pre-intersect the parent lists of every level whose last parent
+// is level d (the F list of the last level included).  Unclamped, unfiltered.
+__device__ void explore_after(const KParams &P, WaveLds &w, uint32_t *wb, int d, Acc &acc) {
+    const DPlan &pl = w.pl;
+    for (uint32_t j = d + 1; j <= pl.sib; ++j) {
+        if (pl.ready[j] != (uint32_t)d || j < 2) continue;
+        Src srcs[MAXK + 1];
+        int ns = gather_sources(P, w, wb, j, pl.parent[j], -1, false, false, 0, false, 0, srcs, acc);
+        if (ns == 1) {
+            w.pre[j] = srcs[0];
+        } else {
+            uint32_t *dst = const_cast<uint32_t *>(buf_ptr(P, wb, MAXK + j));
+            uint32_t n = wave_intersect(srcs, ns, nullptr, 0, dst, acc);
+            Src s;
+            s.ptr = dst;
+            s.b = 0;
+            s.e = n;
+            w.pre[j] = s;
+        }
+    }
+    if (pl.f_parent && pl.f_ready == (uint32_t)d) {
+        Src srcs[MAXK + 1];
+        int ns = gather_sources(P, w, wb, pl.last, pl.f_parent, -1, false, false, 0, false, 0, srcs, acc);
+        if (ns == 1) {
+            w.pre[MAXK] = srcs[0];
+        } else {
+            uint32_t *dst = const_cast<uint32_t *>(buf_ptr(P, wb, 2 * MAXK));
+            uint32_t n = wave_intersect(srcs, ns, nullptr, 0, dst, acc);
+            Src s;
+            s.ptr = dst;
+            s.b = 0;
+            s.e = n;
+            w.pre[MAXK] = s;
+        }
+    }
+}
+
+// Count the last level for every sibling in buffer `sib` (or the single sibling
+// phi[sib] when sib == 1).  F = the last level's candidates that do not depend on
+// the sibling, filtered by every bound/check from levels below the sibling.
+__device__ void count_last(const KParams &P, WaveLds &w, uint32_t *wb, Acc &acc) {
+    const DPlan &pl = w.pl;
+    const int s = pl.sib, L = pl.last;
+    const int lane = lane_id();
+    const uint32_t below = (1u << s) - 1u;  // levels < s
+    // ---- F: fixed part of the last level
+    bool has_lo, has_hi;
+    uint32_t lo, hi;
+    level_bounds(w, pl.lower[L] & below, pl.upper[L] & below, has_lo, lo, has_hi, hi);
+    uint32_t chk[MAXK];
+    int nchk = 0;
+    for (uint32_t m = pl.check[L] & below; m; m &= m - 1) chk[nchk++] = w.phi[__builtin_ctz(m)];
+    const bool universe = pl.f_parent == 0;
+    const uint32_t *F = nullptr;
+    uint32_t fn = 0;
+    if (!universe) {
+        Src srcs[MAXK + 1];
+        int ns;
+        if (pl.mode == SGE_MODE_EXPLORE) {
+            Src sp = w.pre[MAXK];
+            clamp_range(sp.ptr, sp.b, sp.e, has_lo, lo, has_hi, hi);
+            srcs[0] = sp;
+            ns = 1;
+        } else {
+            int reuse = pl.reuse[L];
+            if (reuse >= 0 && ((uint32_t)reuse >= (uint32_t)s || (pl.parent[reuse] & ~pl.f_parent))) reuse = -1;
+            ns = gather_sources(P, w, wb, L, pl.f_parent, reuse, false, has_lo, lo, has_hi, hi, srcs, acc);
+        }
+        uint32_t *dst = const_cast<uint32_t *>(buf_ptr(P, wb, 2 * MAXK + 1));
+        fn = wave_intersect(srcs, ns, chk, nchk, dst, acc);
+        F = dst;
+        acc.nodes += 1;
+    }
+    // ---- siblings
+    const uint32_t *sibs;
+    uint32_t nsib;
+    uint32_t one = w.phi[s];
+    if (s == 1) {
+        sibs = nullptr;
+        nsib = 1;
+    } else {
+        sibs = buf_ptr(P, wb, s);
+        nsib = w.len[s];
+    }
+    const bool s_par = pl.s_is_parent != 0;
+    const bool iep = pl.tail >= 2;
+    unsigned long long local = 0;  // per-lane partial (non-IEP)
+    for (uint32_t sb = 0; sb < nsib; sb += SIB_BLOCK) {
+        uint32_t nb = min((uint32_t)SIB_BLOCK, nsib - sb);
+        // phase 1: per-sibling ranges and work (lane per sibling)
+        uint32_t carry = 0;
+        for (uint32_t r = 0; r < nb; r += WAVE) {
+            uint32_t i = r + lane;
+            uint32_t work = 0;
+            if (i < nb) {
+                uint32_t c = sibs ? sibs[sb + i] : one;
+                bool clo = has_lo, chi = has_hi;
+                uint32_t clo_v = lo, chi_v = hi;
+                if (pl.s_lower) { clo_v = clo ? max(clo_v, c) : c; clo = true; }
+                if (pl.s_upper) { chi_v = chi ? min(chi_v, c) : c; chi = true; }
+                uint32_t cntv = 0;
+                uint32_t flag = 0, dstart = 0, tl = 0, th = 0;
+                if (s_par) {
+                    uint32_t rb = P.off[c], re = P.off[c + 1];
+                    clamp_range(P.nbr, rb, re, clo, clo_v, chi, chi_v);
+                    if (universe) {
+                        cntv = re - rb;
+                        for (int q = 0; q < nchk; ++q)
+                            if (contains(P.nbr, rb, re, chk[q])) --cntv;
+                    } else {
+                        uint32_t fb = 0, fe = fn;
+                        // only the sibling-dependent bound is left to apply to F
+                        clamp_range(F, fb, fe, pl.s_lower != 0, c, pl.s_upper != 0, c);
+                        if (re - rb <= fe - fb) {
+                            flag = 1;
+                            dstart = rb;
+                            tl = fb;
+                            th = fe;
+                            work = re - rb;
+                        } else {
+                            flag = 0;
+                            dstart = fb;
+                            tl = rb;
+                            th = re;
+                            work = fe - fb;
+                        }
+                    }
+                } else {
+                    uint32_t fb = 0, fe = fn;
+                    clamp_range(F, fb, fe, pl.s_lower != 0, c, pl.s_upper != 0, c);
+                    cntv = fe - fb;
+                    if (pl.s_check && contains(F, fb, fe, c)) --cntv;
+                }
+                w.cnt[i] = cntv;
+                w.flag[i] = flag;
+                w.drv[i] = dstart;
+                w.tlo[i] = tl;
+                w.thi[i] = th;
+            }
+            // inclusive scan of work within the wave
+            uint32_t incl = work;
+            for (int off = 1; off < WAVE; off <<= 1) {
+                uint32_t t = __shfl_up(incl, off);
+                if (lane >= off) incl += t;
+            }
+            if (i < nb) w.pre_sum[i + 1] = carry + incl;
+            carry += __shfl(incl, WAVE - 1);
+        }
+        if (lane == 0) w.pre_sum[0] = 0;
+        wave_sync();
+        uint32_t W = w.pre_sum[nb];
+        acc.inter += nb;
+        acc.bytes += 8ull * nb;
+        // phase 2: flattened probes
+        for (uint32_t u0 = 0; u0 < W; u0 += WAVE) {
+            uint32_t u = u0 + lane;
+            if (u < W) {
+                // sibling index: last i with pre_sum[i] <= u
+                uint32_t a = 0, b = nb;
+                while (b - a > 1) {
+                    uint32_t m = (a + b) >> 1;
+                    if (w.pre_sum[m] <= u) a = m; else b = m;
+                }
+                uint32_t off = u - w.pre_sum[a];
+                bool hit;
+                if (w.flag[a]) {
+                    uint32_t x = P.nbr[w.drv[a] + off];
+                    hit = contains(F, w.tlo[a], w.thi[a], x);
+                } else {
+                    uint32_t x = F[w.drv[a] + off];
+                    hit = contains(P.nbr, w.tlo[a], w.thi[a], x);
+                }
+                if (hit) {
+                    if (iep) atomicAdd(&w.cnt[a], 1u);
+                    else local += 1;
+                }
+            }
+        }
+        acc.bytes += 4ull * W * 2;
+        wave_sync();
+        // phase 3: closed-form contributions
+        for (uint32_t r = 0; r < nb; r += WAVE) {
+            uint32_t i = r + lane;
+            if (i < nb) {
+                uint32_t cv = w.cnt[i];
+                local += iep ? choosec(cv, pl.tail) : cv;
+            }
+        }
+        wave_sync();
+    }
+    // wave reduction of the per-lane partial
+    for (int off = WAVE / 2; off > 0; off >>= 1) local += __shfl_xor(local, off);
+    acc.count += local;
+}
+
+__device__ void flush(const KParams &P, uint32_t plan, Acc &acc) {
+    if (lane_id() == 0) {
+        PlanOut *o = P.out + plan;
+        if (acc.count) atomicAdd(&o->count, acc.count);
+        if (acc.inter) atomicAdd(&o->inter, acc.inter);
+        if (acc.bytes) atomicAdd(&o->bytes, acc.bytes);
+        if (acc.nodes) atomicAdd(&o->nodes, acc.nodes);
+        if (acc.tasks) atomicAdd(&o->tasks, acc.tasks);
+    }
+    acc = Acc{0, 0, 0, 0, 0};
+}
+
+// Enumeration: materialise the last level and write one row per embedding,
+// images stored at their pattern vertex (row[order[j]] = phi[j]).
+__device__ void emit_last(const KParams &P, WaveLds &w, uint32_t *wb, Acc &acc) {
+    const DPlan &pl = w.pl;
+    const int L = pl.k - 1;
+    materialize(P, w, wb, L, acc);
+    uint32_t n = w.len[L];
+    if (n == 0) return;
+    unsigned long long base = 0;
+    if (lane_id() == 0) base = atomicAdd(P.emb_n, (unsigned long long)n);
+    base = __shfl(base, 0);
+    const uint32_t *c = buf_ptr(P, wb, L);
+    for (uint32_t i = lane_id(); i < n; i += WAVE) {
+        unsigned long long slot = base + i;
+        if (slot >= P.emb_cap) continue;
+        uint32_t *row = P.emb + slot * pl.k;
+        for (int j = 0; j < L; ++j) row[pl.order[j]] = w.phi[j];
+        row[pl.order[L]] = c[i];
+    }
+    acc.count += n;
+}
+
+template <bool ENUM>
+__global__ void __launch_bounds__(256) wcoj_kernel(KParams P) {
+    extern __shared__ unsigned char lds_raw[];
+    const int lane = lane_id();
+    const int wid = threadIdx.x / WAVE;
+    WaveLds &w = reinterpret_cast<WaveLds *>(lds_raw)[wid];
+    const uint64_t gwave = (uint64_t)blockIdx.x * (blockDim.x / WAVE) + wid;
+    uint32_t *wb = P.scratch + gwave * P.wave_stride;
+    Acc acc{0, 0, 0, 0, 0};
+    uint32_t cur_plan = NONE;
+    const uint64_t ntask = P.task_end - P.task_begin;
+    const uint64_t chunks = (ntask + P.chunk - 1) / P.chunk;
+    while (true) {
+        unsigned long long item = 0;
+        if (lane == 0) item = atomicAdd(P.queue, 1ull);
+        item = __shfl(item, 0);
+        if (item >= P.nitems) break;
+        uint32_t plan = (uint32_t)(item % P.nplans);
+        uint64_t ch = item / P.nplans;
+        if (ch >= chunks) continue;
+        if (plan != cur_plan) {
+            if (cur_plan != NONE) flush(P, cur_plan, acc);
+            cur_plan = plan;
+            // load the plan into LDS, one dword per lane
+            const uint32_t *srcw = reinterpret_cast<const uint32_t *>(P.plans + plan);
+            uint32_t *dstw = reinterpret_cast<uint32_t *>(&w.pl);
+            for (int q = lane; q < (int)(sizeof(DPlan) / 4); q += WAVE) dstw[q] = srcw[q];
+            wave_sync();
+        }
+        const DPlan &pl = w.pl;
+        // heavy-first: chunk 0 covers the highest CSR positions
+        uint64_t hi_t = P.task_end - ch * P.chunk;
+        uint64_t lo_t = (hi_t - P.task_begin > P.chunk) ? hi_t - P.chunk : P.task_begin;
+        for (uint64_t t = lo_t; t < hi_t; ++t) {
+            uint32_t v0 = P.src[t], v1 = P.nbr[t];
+            if ((pl.lower[1] & 1u) && !(v1 > v0)) continue;
+            if ((pl.upper[1] & 1u) && !(v1 < v0)) continue;
+            acc.tasks += 1;
+            w.phi[0] = v0;
+            w.phi[1] = v1;
+            if (pl.k == 2) {
+                acc.count += 1;
+                continue;
+            }
+            if (pl.mode == SGE_MODE_EXPLORE && !ENUM) {
+                explore_after(P, w, wb, 0, acc);
+                explore_after(P, w, wb, 1, acc);
+            }
+            if (!ENUM && pl.sib == 1) {
+                count_last(P, w, wb, acc);
+                continue;
+            }
+            if (ENUM && pl.k == 3) {
+                emit_last(P, w, wb, acc);
+                continue;
+            }
+            const int stop = ENUM ? (int)pl.k - 2 : (int)pl.sib;  // level whose buffer feeds the leaf step
+            materialize(P, w, wb, 2, acc);
+            int d = 2;
+            while (d >= 2) {
+                if (d == stop) {
+                    if (ENUM) {
+                        const uint32_t *b = buf_ptr(P, wb, d);
+                        for (uint32_t i = 0; i < w.len[d]; ++i) {
+                            w.phi[d] = b[i];
+                            emit_last(P, w, wb, acc);
+                        }
+                    } else {
+                        count_last(P, w, wb, acc);
+                    }
+                    --d;
+                    continue;
+                }
+                uint32_t p = w.pos[d];
+                if (p < w.len[d]) {
+                    w.pos[d] = p + 1;
+                    w.phi[d] = buf_ptr(P, wb, d)[p];
+                    if (pl.mode == SGE_MODE_EXPLORE && !ENUM) explore_after(P, w, wb, d, acc);
+                    materialize(P, w, wb, d + 1, acc);
+                    ++d;
+                } else {
+                    --d;
+                }
+            }
+        }
+    }
+    if (cur_plan != NONE) flush(P, cur_plan, acc);
+}
+
+}  // namespace
+
+// ============================================================================
+// Host side
+// ============================================================================
+namespace {
+
+struct DeviceCtx {
+    bool init = false;
+    int dev = 0;
+    int num_cu = 0;
+    hipStream_t stream = nullptr;
+    uint32_t *scratch = nullptr;
+    size_t scratch_bytes = 0;
+    unsigned long long *queue = nullptr;
+    PlanOut *out = nullptr;
+    size_t out_cap = 0;
+    DPlan *plans = nullptr;
+    size_t plans_cap = 0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+DeviceCtx &dctx() {
+    static DeviceCtx c;
+    return c;
+}
+
+std::mutex &dmutex() {
+    static std::mutex m;
+    return m;
+}
+
+int ensure_ctx() {
+    DeviceCtx &c = dctx();
+    if (c.init) return SGE_OK;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+        sge::set_error("no HIP device available: the GPU executor requires an MI355X (gfx950)");
+        return SGE_EDEVICE;
+    }
+    HIP_TRY(hipGetDevice(&c.dev));
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, c.dev));
+    c.num_cu = prop.multiProcessorCount;
+    HIP_TRY(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+    HIP_TRY(hipMalloc(&c.queue, sizeof(unsigned long long) * 4));
+    HIP_TRY(hipEventCreate(&c.ev0));
+    HIP_TRY(hipEventCreate(&c.ev1));
+    c.init = true;
+    return SGE_OK;
+}
+
+int ensure_buffer(void **p, size_t &cap, size_t need) {
+    if (need <= cap && *p) return SGE_OK;
+    if (*p) HIP_TRY(hipFree(*p));
+    *p = nullptr;
+    size_t n = std::max(need, cap * 2);
+    HIP_TRY(hipMalloc(p, n));
+    cap = n;
+    return SGE_OK;
+}
+
+DPlan to_dplan(const sge_plan &p) {
+    DPlan d;
+    std::memset(&d, 0, sizeof(d));
+    d.k = p.k;
+    d.mode = p.mode;
+    d.tail = (p.mode == SGE_MODE_IEP && p.tail >= 2) ? p.tail : 0;
+    d.last = d.tail ? p.k - d.tail : (p.k >= 1 ? p.k - 1 : 0);
+    d.sib = d.last ? d.last - 1 : 0;
+    for (int i = 0; i < MAXK; ++i) {
+        d.parent[i] = p.parent[i];
+        d.lower[i] = p.lower[i];
+        d.upper[i] = p.upper[i];
+        d.check[i] = p.check[i];
+        d.reuse[i] = p.reuse[i];
+        d.ready[i] = p.ready[i];
+        d.order[i] = p.order[i];
+    }
+    uint32_t L = d.last, s = d.sib;
+    d.f_parent = d.parent[L] & ~(1u << s);
+    d.s_is_parent = (d.parent[L] >> s) & 1u;
+    d.s_lower = (d.lower[L] >> s) & 1u;
+    d.s_upper = (d.upper[L] >> s) & 1u;
+    d.s_check = (d.check[L] >> s) & 1u;
+    d.f_ready = d.f_parent ? 31 - __builtin_clz(d.f_parent) : 0;
+    return d;
+}
+
+int validate_plan(const sge_plan &p) {
+    if (p.k < 1 || p.k > MAXK) {
+        sge::set_error("plan: k must be 1..8");
+        return SGE_EINVAL;
+    }
+    if (p.mode > SGE_MODE_IEP) {
+        sge::set_error("plan: unknown mode");
+        return SGE_EINVAL;
+    }
+    uint32_t seen = 0;
+    for (uint32_t i = 0; i < p.k; ++i) {
+        if (p.order[i] >= p.k || ((seen >> p.order[i]) & 1u)) {
+            sge::set_error("plan: order is not a permutation");
+            return SGE_EINVAL;
+        }
+        seen |= 1u << p.order[i];
+        uint32_t earlier = (1u << i) - 1u;
+        if ((p.parent[i] | p.lower[i] | p.upper[i] | p.check[i]) & ~earlier) {
+            sge::set_error("plan: masks may only name earlier levels");
+            return SGE_EINVAL;
+        }
+        if (i > 0 && p.parent[i] == 0) {
+            sge::set_error("plan: every level after the first needs an earlier neighbour");
+            return SGE_EINVAL;
+        }
+        if (p.reuse[i] >= (int32_t)i || p.reuse[i] < -1 || (p.reuse[i] >= 0 && p.reuse[i] < 2)) {
+            sge::set_error("plan: bad reuse level");
+            return SGE_EINVAL;
+        }
+        if (p.ready[i] >= (i ? i : 1)) {
+            sge::set_error("plan: bad ready level");
+            return SGE_EINVAL;
+        }
+    }
+    if (p.mode == SGE_MODE_IEP && p.tail >= 2 && (p.tail >= p.k || p.k - p.tail < 2)) {
+        sge::set_error("plan: IEP tail must leave at least two matched levels");
+        return SGE_EINVAL;
+    }
+    return SGE_OK;
+}
+
+struct LaunchCfg {
+    int block = 256;
+    int grid = 0;
+    size_t lds = 0;
+    uint32_t chunk = 64;
+};
+
+int launch_cfg(const sge_tuning *t, bool enumerate, LaunchCfg &cfg) {
+    DeviceCtx &c = dctx();
+    sge_tuning def;
+    sge_tuning_default(&def);
+    if (!t) t = &def;
+    cfg.block = (int)t->block_threads;
+    if (cfg.block < 64 || cfg.block > 256 || cfg.block % 64) {
+        sge::set_error("tuning: block_threads must be 64, 128, 192 or 256");
+        return SGE_EINVAL;
+    }
+    cfg.chunk = t->chunk ? t->chunk : 64;
+    cfg.lds = (size_t)(cfg.block / WAVE) * sizeof(WaveLds);
+    int per_cu = (int)t->blocks_per_cu;
+    if (per_cu <= 0) {
+        int occ = 0;
+        if (enumerate)
+            HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, wcoj_kernel<true>, cfg.block, cfg.lds));
+        else
+            HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, wcoj_kernel<false>, cfg.block, cfg.lds));
+        per_cu = std::max(1, occ);
+    }
+    cfg.grid = c.num_cu * per_cu;
+    return SGE_OK;
+}
+
+int ensure_scratch(const sge_graph *g, const LaunchCfg &cfg, KParams &kp) {
+    DeviceCtx &c = dctx();
+    uint32_t stride = ((g->h.max_degree + 64 + 63) / 64) * 64;
+    kp.buf_stride = stride;
+    kp.wave_stride = (uint64_t)stride * (2 * MAXK + 2);
+    size_t waves = (size_t)cfg.grid * (cfg.block / WAVE);
+    size_t need = waves * kp.wave_stride * sizeof(uint32_t);
+    int rc = ensure_buffer((void **)&c.scratch, c.scratch_bytes, need);
+    if (rc) return rc;
+    kp.scratch = c.scratch;
+    return SGE_OK;
+}
+
+__global__ void reset_kernel(PlanOut *out, uint32_t n, unsigned long long *queue, unsigned long long *emb_n) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = PlanOut{0, 0, 0, 0, 0, {0, 0, 0}};
+    if (i == 0) {
+        queue[0] = 0;
+        if (emb_n) *emb_n = 0;
+    }
+}
+
+// One counting launch over `np` device plans (already uploaded at `dplans`).
+int run_launch(sge_graph *g, const DPlan *dplans, uint32_t np, PlanOut *dout, uint64_t tb, uint64_t te,
+               const LaunchCfg &cfg, bool enumerate, uint32_t *emb, uint64_t emb_cap, unsigned long long *emb_n,
+               float *ms) {
+    DeviceCtx &c = dctx();
+    KParams kp;
+    std::memset(&kp, 0, sizeof(kp));
+    kp.off = (const uint32_t *)g->d_off;
+    kp.nbr = (const uint32_t *)g->d_nbr;
+    kp.src = (const uint32_t *)g->d_src;
+    kp.n = g->h.n;
+    kp.nnz = g->h.nnz;
+    kp.plans = dplans;
+    kp.nplans = np;
+    kp.task_begin = tb;
+    kp.task_end = te;
+    kp.chunk = cfg.chunk;
+    uint64_t chunks = (te - tb + cfg.chunk - 1) / cfg.chunk;
+    kp.nitems = chunks * np;
+    kp.queue = c.queue;
+    kp.out = dout;
+    kp.emb = emb;
+    kp.emb_cap = emb_cap;
+    kp.emb_n = emb_n;
+    int rc = ensure_scratch(g, cfg, kp);
+    if (rc) return rc;
+    reset_kernel<<<(np + 255) / 256, 256, 0, c.stream>>>(dout, np, c.queue, emb_n);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(c.ev0, c.stream));
+    if (kp.nitems > 0) {
+        if (enumerate)
+            wcoj_kernel<true><<<cfg.grid, cfg.block, cfg.lds, c.stream>>>(kp);
+        else
+            wcoj_kernel<false><<<cfg.grid, cfg.block, cfg.lds, c.stream>>>(kp);
+        HIP_TRY(hipGetLastError());
+    }
+    HIP_TRY(hipEventRecord(c.ev1, c.stream));
+    HIP_TRY(hipStreamSynchronize(c.stream));
+    float t = 0;
+    HIP_TRY(hipEventElapsedTime(&t, c.ev0, c.ev1));
+    if (ms) *ms = t;
+    return SGE_OK;
+}
+
+int upload_plans(const sge_plan *plans, uint32_t n, bool enumerate) {
+    DeviceCtx &c = dctx();
+    std::vector<DPlan> dp(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        int rc = validate_plan(plans[i]);
+        if (rc) return rc;
+        sge_plan p = plans[i];
+        if (enumerate) {
+            p.mode = SGE_MODE_WCOJ;
+            p.tail = 0;
+        }
+        dp[i] = to_dplan(p);
+    }
+    int rc = ensure_buffer((void **)&c.plans, c.plans_cap, sizeof(DPlan) * std::max<uint32_t>(n, 1));
+    if (rc) return rc;
+    rc = ensure_buffer((void **)&c.out, c.out_cap, sizeof(PlanOut) * std::max<uint32_t>(n, 1));
+    if (rc) return rc;
+    HIP_TRY(hipMemcpy(c.plans, dp.data(), sizeof(DPlan) * n, hipMemcpyHostToDevice));
+    return SGE_OK;
+}
+
+void fill_stats(const PlanOut &o, double ms, sge_stats *s) {
+    if (!s) return;
+    s->intersections += o.inter;
+    s->bytes += o.bytes;
+    s->tasks += o.tasks;
+    s->nodes += o.nodes;
+    s->kernel_ms += ms;
+}
+
+}  // namespace
+
+extern "C" void sge_tuning_default(sge_tuning *t) {
+    if (!t) return;
+    std::memset(t, 0, sizeof(*t));
+    t->block_threads = 256;
+    t->chunk = 32;
+    t->stage_max = 0;
+    t->blocks_per_cu = 0;
+}
+
+extern "C" int sge_graph_to_device(sge_graph *g) {
+    if (!g) {
+        sge::set_error("sge_graph_to_device: null graph");
+        return SGE_EINVAL;
+    }
+    std::lock_guard<std::mutex> lk(dmutex());
+    int rc = ensure_ctx();
+    if (rc) return rc;
+    if (g->d_off) return SGE_OK;
+    const sge::HostGraph &h = g->h;
+    std::vector<uint32_t> src(h.nnz);
+    for (uint32_t v = 0; v < h.n; ++v)
+        for (uint32_t e = h.off[v]; e < h.off[v + 1]; ++e) src[e] = v;
+    HIP_TRY(hipMalloc(&g->d_off, sizeof(uint32_t) * ((size_t)h.n + 1)));
+    HIP_TRY(hipMalloc(&g->d_nbr, sizeof(uint32_t) * std::max<uint64_t>(h.nnz, 1)));
+    HIP_TRY(hipMalloc(&g->d_src, sizeof(uint32_t) * std::max<uint64_t>(h.nnz, 1)));
+    HIP_TRY(hipMemcpy(g->d_off, h.off.data(), sizeof(uint32_t) * ((size_t)h.n + 1), hipMemcpyHostToDevice));
+    if (h.nnz) {
+        HIP_TRY(hipMemcpy(g->d_nbr, h.nbr.data(), sizeof(uint32_t) * h.nnz, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(g->d_src, src.data(), sizeof(uint32_t) * h.nnz, hipMemcpyHostToDevice));
+    }
+    g->device = dctx().dev;
+    return SGE_OK;
+}
+
+extern "C" void sge_graph_device_free(sge_graph *g) {
+    if (!g) return;
+    if (g->d_off) (void)hipFree(g->d_off);
+    if (g->d_nbr) (void)hipFree(g->d_nbr);
+    if (g->d_src) (void)hipFree(g->d_src);
+    g->d_off = g->d_nbr = g->d_src = nullptr;
+}
+
+static int count_impl(sge_graph *g, const sge_plan *plans, uint32_t nplans, uint32_t flags, const sge_tuning *tune,
+                      uint64_t tb, uint64_t te, uint64_t *counts, sge_stats *stats, sge_stats *per_plan) {
+    auto t0 = std::chrono::steady_clock::now();
+    if (!g || !plans || !counts || nplans == 0) {
+        sge::set_error("sge_count: null argument or no plans");
+        return SGE_EINVAL;
+    }
+    int rc = sge_graph_to_device(g);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lk(dmutex());
+    DeviceCtx &c = dctx();
+    if (te > g->h.nnz) te = g->h.nnz;
+    if (tb > te) tb = te;
+    rc = upload_plans(plans, nplans, false);
+    if (rc) return rc;
+    LaunchCfg cfg;
+    rc = launch_cfg(tune, false, cfg);
+    if (rc) return rc;
+    if (stats) std::memset(stats, 0, sizeof(*stats));
+    if (per_plan) std::memset(per_plan, 0, sizeof(sge_stats) * nplans);
+    std::vector<PlanOut> host(nplans);
+    std::vector<float> ms(nplans, 0.f);
+    if (flags & SGE_COUNT_BATCHED) {
+        float t = 0;
+        rc = run_launch(g, c.plans, nplans, c.out, tb, te, cfg, false, nullptr, 0, nullptr, &t);
+        if (rc) return rc;
+        HIP_TRY(hipMemcpy(host.data(), c.out, sizeof(PlanOut) * nplans, hipMemcpyDeviceToHost));
+        if (stats) stats->kernel_ms = t;
+        for (uint32_t i = 0; i < nplans; ++i) {
+            fill_stats(host[i], 0.0, stats);
+            if (per_plan) fill_stats(host[i], 0.0, per_plan + i);
+        }
+    } else {
+        for (uint32_t i = 0; i < nplans; ++i) {
+            rc = run_launch(g, c.plans + i, 1, c.out + i, tb, te, cfg, false, nullptr, 0, nullptr, &ms[i]);
+            if (rc) return rc;
+        }
+        HIP_TRY(hipMemcpy(host.data(), c.out, sizeof(PlanOut) * nplans, hipMemcpyDeviceToHost));
+        for (uint32_t i = 0; i < nplans; ++i) {
+            fill_stats(host[i], ms[i], stats);
+            if (per_plan) fill_stats(host[i], ms[i], per_plan + i);
+        }
+    }
+    for (uint32_t i = 0; i < nplans; ++i) {
+        uint64_t cnt = host[i].count;
+        if (plans[i].k == 1) cnt = g->h.n;
+        counts[i] = cnt * (plans[i].multiplier ? plans[i].multiplier : 1);
+    }
+    if (stats) stats->wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return SGE_OK;
+}
+
+extern "C" int sge_count(sge_graph *g, const sge_plan *plans, uint32_t nplans, uint32_t flags, const sge_tuning *tune,
+                         uint64_t *counts, sge_stats *stats, sge_stats *per_plan) {
+    if (!g) {
+        sge::set_error("sge_count: null graph");
+        return SGE_EINVAL;
+    }
+    return count_impl(g, plans, nplans, flags, tune, 0, g->h.nnz, counts, stats, per_plan);
+}
+
+extern "C" int sge_count_range(sge_graph *g, const sge_plan *plan, uint64_t task_begin, uint64_t task_end,
+                               const sge_tuning *tune, uint64_t *count, sge_stats *stats) {
+    if (plan && plan->k == 1) {
+        sge::set_error("sge_count_range: single-vertex patterns have no edge tasks");
+        return SGE_EINVAL;
+    }
+    return count_impl(g, plan, 1, 0, tune, task_begin, task_end, count, stats, nullptr);
+}
+
+extern "C" int sge_num_tasks(const sge_graph *g, const sge_plan *plan, uint64_t *n) {
+    if (!g || !n) {
+        sge::set_error("sge_num_tasks: null argument");
+        return SGE_EINVAL;
+    }
+    (void)plan;
+    *n = g->h.nnz;
+    return SGE_OK;
+}
+
+extern "C" int sge_enumerate(sge_graph *g, const sge_plan *plan, uint32_t *out, uint64_t cap, uint64_t *n_out) {
+    if (!g || !plan || !n_out || (cap && !out)) {
+        sge::set_error("sge_enumerate: bad arguments");
+        return SGE_EINVAL;
+    }
+    if (plan->k < 3) {
+        sge::set_error("sge_enumerate: patterns need at least 3 vertices");
+        return SGE_EINVAL;
+    }
+    int rc = sge_graph_to_device(g);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lk(dmutex());
+    DeviceCtx &c = dctx();
+    rc = upload_plans(plan, 1, true);
+    if (rc) return rc;
+    LaunchCfg cfg;
+    rc = launch_cfg(nullptr, true, cfg);
+    if (rc) return rc;
+    uint32_t *d_emb = nullptr;
+    unsigned long long *d_n = nullptr;
+    size_t bytes = (size_t)std::max<uint64_t>(cap, 1) * plan->k * sizeof(uint32_t);
+    HIP_TRY(hipMalloc(&d_emb, bytes));
+    HIP_TRY(hipMalloc(&d_n, sizeof(unsigned long long)));
+    float t = 0;
+    rc = run_launch(g, c.plans, 1, c.out, 0, g->h.nnz, cfg, true, d_emb, cap, d_n, &t);
+    unsigned long long n = 0;
+    if (!rc) {
+        HIP_TRY(hipMemcpy(&n, d_n, sizeof(n), hipMemcpyDeviceToHost));
+        uint64_t w = std::min<uint64_t>(n, cap);
+        if (w) HIP_TRY(hipMemcpy(out, d_emb, w * plan->k * sizeof(uint32_t), hipMemcpyDeviceToHost));
+        for (uint64_t i = 0; i < w * plan->k; ++i) out[i] = g->h.perm[out[i]];
+    }
+    (void)hipFree(d_emb);
+    (void)hipFree(d_n);
+    if (rc) return rc;
+    *n_out = n;
+    if (n > cap) {
+        sge::set_error("sge_enumerate: output capacity exceeded");
+        return SGE_ERANGE;
+    }
+    return SGE_OK;
+}
+
+// ---------------------------------------------------------------- batches
+struct sge_batch {
+    sge_graph *g = nullptr;
+    std::vector<sge_plan> plans;
+    DPlan *dplans = nullptr;
+    PlanOut *dout = nullptr;
+    unsigned long long *queue = nullptr;
+    uint32_t *scratch = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+extern "C" int sge_batch_create(sge_graph *g, const sge_plan *plans, uint32_t nplans, const sge_tuning *tune,
+                                sge_batch **out) {
+    if (!g || !plans || !nplans || !out) {
+        sge::set_error("sge_batch_create: bad arguments");
+        return SGE_EINVAL;
+    }
+    int rc = sge_graph_to_device(g);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lk(dmutex());
+    DeviceCtx &c = dctx();
+    std::vector<DPlan> dp(nplans);
+    for (uint32_t i = 0; i < nplans; ++i) {
+        rc = validate_plan(plans[i]);
+        if (rc) return rc;
+        if (plans[i].k < 2) {
+            sge::set_error("sge_batch_create: patterns need at least 2 vertices");
+            return SGE_EINVAL;
+        }
+        dp[i] = to_dplan(plans[i]);
+    }
+    LaunchCfg cfg;
+    rc = launch_cfg(tune, false, cfg);
+    if (rc) return rc;
+    sge_batch *b = new sge_batch();
+    b->g = g;
+    b->plans.assign(plans, plans + nplans);
+    KParams kp;
+    std::memset(&kp, 0, sizeof(kp));
+    uint32_t stride = ((g->h.max_degree + 64 + 63) / 64) * 64;
+    kp.buf_stride = stride;
+    kp.wave_stride = (uint64_t)stride * (2 * MAXK + 2);
+    size_t waves = (size_t)cfg.grid * (cfg.block / WAVE);
+    auto fail = [&](hipError_t e, const char *what) {
+        sge::set_error(std::string("HIP error: ") + hipGetErrorString(e) + " at " + what);
+        sge_batch_destroy(b);
+        return SGE_EDEVICE;
+    };
+    hipError_t e;
+    if ((e = hipMalloc(&b->dplans, sizeof(DPlan) * nplans)) != hipSuccess) return fail(e, "plans");
+    if ((e = hipMalloc(&b->dout, sizeof(PlanOut) * nplans)) != hipSuccess) return fail(e, "out");
+    if ((e = hipMalloc(&b->queue, sizeof(unsigned long long) * 2)) != hipSuccess) return fail(e, "queue");
+    if ((e = hipMalloc(&b->scratch, waves * kp.wave_stride * sizeof(uint32_t))) != hipSuccess) return fail(e, "scratch");
+    if ((e = hipMemcpy(b->dplans, dp.data(), sizeof(DPlan) * nplans, hipMemcpyHostToDevice)) != hipSuccess)
+        return fail(e, "plan upload");
+    if ((e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking)) != hipSuccess) return fail(e, "stream");
+    if ((e = hipEventCreate(&b->ev0)) != hipSuccess) return fail(e, "event");
+    if ((e = hipEventCreate(&b->ev1)) != hipSuccess) return fail(e, "event");
+    kp.off = (const uint32_t *)g->d_off;
+    kp.nbr = (const uint32_t *)g->d_nbr;
+    kp.src = (const uint32_t *)g->d_src;
+    kp.n = g->h.n;
+    kp.nnz = g->h.nnz;
+    kp.plans = b->dplans;
+    kp.nplans = nplans;
+    kp.task_begin = 0;
+    kp.task_end = g->h.nnz;
+    kp.chunk = cfg.chunk;
+    kp.nitems = ((g->h.nnz + cfg.chunk - 1) / cfg.chunk) * nplans;
+    kp.queue = b->queue;
+    kp.scratch = b->scratch;
+    kp.out = b->dout;
+    // capture: reset (a fill kernel, no memset node) + the persistent count kernel
+    if ((e = hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal)) != hipSuccess) return fail(e, "capture");
+    reset_kernel<<<(nplans + 255) / 256, 256, 0, b->stream>>>(b->dout, nplans, b->queue, nullptr);
+    if (kp.nitems) wcoj_kernel<false><<<cfg.grid, cfg.block, cfg.lds, b->stream>>>(kp);
+    if ((e = hipStreamEndCapture(b->stream, &b->graph)) != hipSuccess) return fail(e, "end capture");
+    if ((e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0)) != hipSuccess) return fail(e, "instantiate");
+    (void)c;
+    *out = b;
+    return SGE_OK;
+}
+
+extern "C" int sge_batch_run(sge_batch *b, uint64_t *counts, sge_stats *stats) {
+    if (!b || !counts) {
+        sge::set_error("sge_batch_run: bad arguments");
+        return SGE_EINVAL;
+    }
+    auto t0 = std::chrono::steady_clock::now();
+    uint32_t n = (uint32_t)b->plans.size();
+    HIP_TRY(hipEventRecord(b->ev0, b->stream));
+    HIP_TRY(hipGraphLaunch(b->exec, b->stream));
+    HIP_TRY(hipEventRecord(b->ev1, b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    std::vector<PlanOut> host(n);
+    HIP_TRY(hipMemcpy(host.data(), b->dout, sizeof(PlanOut) * n, hipMemcpyDeviceToHost));
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+    if (stats) {
+        std::memset(stats, 0, sizeof(*stats));
+        for (auto &o : host) fill_stats(o, 0.0, stats);
+        stats->kernel_ms = ms;
+    }
+    for (uint32_t i = 0; i < n; ++i) counts[i] = host[i].count * (b->plans[i].multiplier ? b->plans[i].multiplier : 1);
+    if (stats) stats->wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return SGE_OK;
+}
+
+extern "C" void sge_batch_destroy(sge_batch *b) {
+    if (!b) return;
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    if (b->ev0) (void)hipEventDestroy(b->ev0);
+    if (b->ev1) (void)hipEventDestroy(b->ev1);
+    if (b->dplans) (void)hipFree(b->dplans);
+    if (b->dout) (void)hipFree(b->dout);
+    if (b->queue) (void)hipFree(b->queue);
+    if (b->scratch) (void)hipFree(b->scratch);
+    delete b;
+}
+
+extern "C" int sge_device_info(int *num_cu, int *num_xcd_hint, char *name, size_t cap) {
+    std::lock_guard<std::mutex> lk(dmutex());
+    int rc = ensure_ctx();
+    if (rc) return rc;
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, dctx().dev));
+    if (num_cu) *num_cu = prop.multiProcessorCount;
+    if (num_xcd_hint) *num_xcd_hint = 8;
+    if (name && cap) {
+        std::snprintf(name, cap, "%s", prop.gcnArchName);
+    }
+    return SGE_OK;
+}

@@ -1,0 +1,362 @@
+// Host planner: matching order, symmetry breaking and the flat plan descriptor.
+//
+// Restates, from scratch, the planning steps the reference runs before the join
+// (main.cpp:64-69):
+//  * symmetry breaking (PatternGraph::computeCandidateRules graph.cpp:910,
+//    HyperTree::selectSymmetry/setSymmetry decomposition.cpp:621-868): here the
+//    textbook stabiliser-chain construction along the matching order, so every
+//    subgraph instance satisfies exactly one image ordering;
+//  * the clamps (largerAttrs/smallerAttrs -> setBeginPos/setEndPos,
+//    join.cpp:137-178, utils.cpp:581-612) become per-level lower/upper masks;
+//  * injectivity (visited[], join.cpp:369-373, and attributesToCheck in
+//    TrieLevel::numResults, relation.cpp:301-338) becomes the per-level check mask;
+//  * candidate reuse (HyperNode::candidatesBefore) becomes reuse[];
+//  * the IEP last-level group (buildTraverseUnlabeled symmLastLevel,
+//    decomposition.cpp:1022-1047, counted by choosec in join.cpp:1532-1537)
+//    becomes `tail`;
+//  * the single-bag embedding-count convention (decomposition.cpp:983-986)
+//    becomes `multiplier` = |Aut|.
+// Order choice restates the idea of optCostOrder/computeOrderCost
+// (optimizer.cpp:319-577): estimated partial-match cardinality per level times
+// the list work of the next level, with data-graph degree moments and a sampled
+// wedge-closure rate standing in for the reference's sampled cardinalities.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <functional>
+#include <numeric>
+#include <set>
+
+#include "common.h"
+
+namespace sge {
+
+GraphSummary summarize(const HostGraph *g) {
+    GraphSummary s;
+    if (!g || g->n == 0) {
+        s.n = 1000;
+        s.m = 10000;
+        s.avg_deg = 20;
+        s.d2 = 40;
+        s.tri_per_wedge = 0.05;
+        s.valid = false;
+        return s;
+    }
+    s.n = g->n;
+    s.m = g->nnz / 2.0;
+    double sd = 0, sd2 = 0;
+    for (uint32_t v = 0; v < g->n; ++v) {
+        double d = g->deg(v);
+        sd += d;
+        sd2 += d * d;
+    }
+    s.avg_deg = sd / std::max(1.0, (double)g->n);
+    s.d2 = sd > 0 ? sd2 / sd : 0;
+    // wedge closure: sample wedges (a - c - b) centred on a vertex chosen with
+    // probability ~ deg^2 (via a random edge endpoint), test edge a-b
+    Rng rng(12345);
+    uint64_t closed = 0, total = 0;
+    for (int it = 0; it < 20000 && g->nnz > 0; ++it) {
+        uint64_t e = rng.below(g->nnz);
+        uint32_t c = (uint32_t)(std::upper_bound(g->off.begin(), g->off.end(), (uint32_t)e) - g->off.begin()) - 1;
+        uint32_t d = g->deg(c);
+        if (d < 2) continue;
+        uint32_t a = g->nbr[g->off[c] + rng.below(d)], b = g->nbr[g->off[c] + rng.below(d)];
+        if (a == b) continue;
+        ++total;
+        const uint32_t *row = g->nbr.data() + g->off[a];
+        if (std::binary_search(row, row + g->deg(a), b)) ++closed;
+    }
+    s.tri_per_wedge = total ? std::max(1e-4, (double)closed / total) : 0.01;
+    s.valid = true;
+    return s;
+}
+
+// Stabiliser chain along `order`: for each level i, every other vertex in the
+// orbit of order[i] under the pointwise stabiliser of order[0..i-1] must map to
+// a larger data vertex.  Returns lower masks (bit i set in lower[level(w)]).
+static void symmetry_masks(const Pattern &p, const int *order, const std::vector<std::vector<int>> &aut,
+                           uint32_t *lower) {
+    int k = p.k;
+    int level_of[SGE_MAX_K];
+    for (int i = 0; i < k; ++i) level_of[order[i]] = i;
+    std::vector<const std::vector<int> *> G;
+    for (auto &g : aut) G.push_back(&g);
+    for (int i = 0; i < k; ++i) lower[i] = 0;
+    for (int i = 0; i < k; ++i) {
+        int u = order[i];
+        uint32_t orbit = 0;
+        for (auto *g : G) orbit |= 1u << (*g)[u];
+        for (int w = 0; w < k; ++w)
+            if (w != u && ((orbit >> w) & 1u)) lower[level_of[w]] |= 1u << i;
+        std::vector<const std::vector<int> *> H;
+        for (auto *g : G)
+            if ((*g)[u] == u) H.push_back(g);
+        G.swap(H);
+    }
+}
+
+double order_cost(const Pattern &p, const GraphSummary &s, const int *order, const std::vector<std::vector<int>> &aut) {
+    int k = p.k;
+    uint32_t lower[SGE_MAX_K];
+    symmetry_masks(p, order, aut, lower);
+    int level_of[SGE_MAX_K];
+    for (int i = 0; i < k; ++i) level_of[order[i]] = i;
+    double q = std::min(0.9, std::max(1e-4, s.tri_per_wedge));
+    double card = s.n;  // partial matches after level 0
+    double cost = 0;
+    for (int i = 1; i < k; ++i) {
+        uint32_t par = 0;
+        for (int j = 0; j < i; ++j)
+            if (p.edge(order[i], order[j])) par |= 1u << j;
+        int np = __builtin_popcount(par);
+        if (np == 0) return 1e300;  // disconnected prefix
+        double list = (i == 1) ? s.avg_deg : s.d2;
+        // each parent contributes its list to the intersection work
+        double work = (np == 1) ? std::log2(list + 2.0) : np * list;
+        if (i < k - 1 || np > 1) cost += card * work;
+        double size = list * std::pow(q, np - 1);
+        if (lower[i]) size *= 0.5;
+        card *= std::max(size, 1e-9);
+        // level i's materialisation writes its candidates
+        if (i < k - 1) cost += card * 0.25;
+    }
+    return cost;
+}
+
+std::vector<std::vector<int>> connected_orders(const Pattern &p, const std::vector<std::vector<int>> &aut) {
+    std::vector<std::vector<int>> out;
+    std::set<std::vector<int>> seen;
+    std::vector<int> o(p.k);
+    std::iota(o.begin(), o.end(), 0);
+    do {
+        bool ok = true;
+        uint32_t placed = 1u << o[0];
+        for (int i = 1; i < p.k && ok; ++i) {
+            ok = (p.adj[o[i]] & placed) != 0;
+            placed |= 1u << o[i];
+        }
+        if (!ok) continue;
+        std::vector<int> best = o;
+        for (auto &g : aut) {
+            std::vector<int> t(p.k);
+            for (int i = 0; i < p.k; ++i) t[i] = g[o[i]];
+            best = std::min(best, t);
+        }
+        if (seen.insert(best).second) out.push_back(best);
+    } while (std::next_permutation(o.begin(), o.end()));
+    return out;
+}
+
+// Largest set of pairwise non-adjacent vertices with identical neighbourhoods.
+static uint32_t largest_twin_group(const Pattern &p) {
+    uint32_t best = 0;
+    for (int a = 0; a < p.k; ++a) {
+        uint32_t grp = 0;
+        for (int b = 0; b < p.k; ++b)
+            if (p.adj[a] == p.adj[b] && !p.edge(a, b)) grp |= 1u << b;
+        if (__builtin_popcount(grp) > __builtin_popcount(best)) best = grp;
+    }
+    return __builtin_popcount(best) >= 2 ? best : 0;
+}
+
+static bool valid_tail(const sge_plan &pl, int r) {
+    int k = pl.k, b = k - r;
+    if (r < 2 || b < 2) return false;  // the root edge (levels 0, 1) is never part of the group
+    uint32_t below = (1u << b) - 1;
+    for (int i = b; i < k; ++i) {
+        if ((pl.parent[i] & ~below) != 0) return false;  // group members non-adjacent
+        if ((pl.parent[i] & below) != (pl.parent[b] & below)) return false;
+        if ((pl.lower[i] & below) != (pl.lower[b] & below)) return false;
+        if ((pl.upper[i] & below) != (pl.upper[b] & below)) return false;
+        if ((pl.check[i] & below) != (pl.check[b] & below)) return false;
+        uint32_t chain = ((1u << i) - 1) & ~below;  // all earlier group members
+        if ((pl.lower[i] & ~below) != chain) return false;
+        if ((pl.upper[i] & ~below) != 0) return false;
+    }
+    return true;
+}
+
+static void fill_masks(const Pattern &p, const int *order, const std::vector<std::vector<int>> &aut, uint32_t mode,
+                       sge_plan &pl) {
+    int k = p.k;
+    pl.k = k;
+    pl.mode = mode;
+    for (int i = 0; i < SGE_MAX_K; ++i) {
+        pl.order[i] = i < k ? order[i] : 0;
+        pl.parent[i] = pl.lower[i] = pl.upper[i] = pl.check[i] = 0;
+        pl.reuse[i] = -1;
+        pl.ready[i] = 0;
+    }
+    symmetry_masks(p, order, aut, pl.lower);
+    for (int i = 0; i < k; ++i) {
+        for (int j = 0; j < i; ++j)
+            if (p.edge(order[i], order[j])) pl.parent[i] |= 1u << j;
+        uint32_t earlier = (1u << i) - 1;
+        pl.check[i] = earlier & ~(pl.parent[i] | pl.lower[i] | pl.upper[i]);
+        pl.ready[i] = pl.parent[i] ? (31 - __builtin_clz(pl.parent[i])) : 0;
+    }
+    // candidate reuse: an earlier level whose filters are all implied by this
+    // level's filters and whose parents are a subset of this level's parents
+    for (int j = 3; j < k; ++j) {
+        int best = -1, bestp = 1;
+        for (int i = 2; i < j; ++i) {
+            if ((pl.parent[i] & pl.parent[j]) != pl.parent[i]) continue;
+            int np = __builtin_popcount(pl.parent[i]);
+            if (np < 2) continue;
+            if ((pl.lower[i] & ~pl.lower[j]) || (pl.upper[i] & ~pl.upper[j])) continue;
+            uint32_t excl_j = pl.check[j] | pl.lower[j] | pl.upper[j] | pl.parent[j];
+            if (pl.check[i] & ~excl_j) continue;
+            if (np > bestp) {
+                best = i;
+                bestp = np;
+            }
+        }
+        pl.reuse[j] = best;
+    }
+    pl.tail = 0;
+    if (mode == SGE_MODE_IEP) {
+        for (int r = k - 1; r >= 2; --r)
+            if (valid_tail(pl, r)) {
+                pl.tail = r;
+                break;
+            }
+    }
+}
+
+int build_plan(const Pattern &p, const HostGraph *g, uint32_t mode, const uint32_t *order_in, sge_plan &out) {
+    if (p.k < 1 || p.k > SGE_MAX_K) {
+        set_error("pattern must have 1..8 vertices");
+        return SGE_EINVAL;
+    }
+    if (!p.connected()) {
+        set_error("pattern must be connected");
+        return SGE_EINVAL;
+    }
+    if (mode > SGE_MODE_IEP) {
+        set_error("unknown mode");
+        return SGE_EINVAL;
+    }
+    std::memset(&out, 0, sizeof(out));
+    auto aut = automorphisms(p);
+    Decomposition d = decompose(p, aut);
+    GraphSummary s = summarize(g);
+    int order[SGE_MAX_K];
+    double cost = 0;
+    if (order_in) {
+        uint32_t seen = 0;
+        for (int i = 0; i < p.k; ++i) {
+            if (order_in[i] >= (uint32_t)p.k || ((seen >> order_in[i]) & 1u)) {
+                set_error("order is not a permutation of the pattern vertices");
+                return SGE_EINVAL;
+            }
+            seen |= 1u << order_in[i];
+            order[i] = (int)order_in[i];
+        }
+        for (int i = 1; i < p.k; ++i) {
+            uint32_t placed = 0;
+            for (int j = 0; j < i; ++j) placed |= 1u << order[j];
+            if (!(p.adj[order[i]] & placed)) {
+                set_error("order is not connected (a level has no earlier neighbour)");
+                return SGE_EINVAL;
+            }
+        }
+        cost = order_cost(p, s, order, aut);
+    } else {
+        auto orders = connected_orders(p, aut);
+        uint32_t twins = (mode == SGE_MODE_IEP && !d.single_bag) ? largest_twin_group(p) : 0;
+        double best = 1e308;
+        int besti = -1;
+        for (int pass = 0; pass < 2 && besti < 0; ++pass) {
+            for (size_t i = 0; i < orders.size(); ++i) {
+                const auto &o = orders[i];
+                if (twins && pass == 0) {
+                    int r = __builtin_popcount(twins);
+                    bool ok = true;
+                    for (int j = p.k - r; j < p.k; ++j) ok = ok && ((twins >> o[j]) & 1u);
+                    if (!ok) continue;
+                }
+                double c = order_cost(p, s, o.data(), aut);
+                if (c < best) {
+                    best = c;
+                    besti = (int)i;
+                }
+            }
+        }
+        for (int i = 0; i < p.k; ++i) order[i] = orders[besti][i];
+        cost = best;
+    }
+    fill_masks(p, order, aut, mode, out);
+    out.num_bags = d.num_bags;
+    out.aut_size = aut.size();
+    out.multiplier = d.single_bag ? aut.size() : 1;
+    out.cost = cost;
+    return SGE_OK;
+}
+
+}  // namespace sge
+
+using namespace sge;
+
+extern "C" int sge_plan_build(const sge_pattern *p, const sge_graph *g, uint32_t mode, const uint32_t *order,
+                              sge_plan *out) {
+    if (!p || !out) {
+        set_error("sge_plan_build: null argument");
+        return SGE_EINVAL;
+    }
+    return build_plan(from_c(*p), g ? &g->h : nullptr, mode, order, *out);
+}
+
+extern "C" int sge_plan_cost(const sge_pattern *cp, const sge_graph *g, const uint32_t *order, double *cost) {
+    if (!cp || !order || !cost) {
+        set_error("sge_plan_cost: null argument");
+        return SGE_EINVAL;
+    }
+    Pattern p = from_c(*cp);
+    auto aut = automorphisms(p);
+    int o[SGE_MAX_K];
+    for (int i = 0; i < p.k; ++i) o[i] = (int)order[i];
+    *cost = order_cost(p, summarize(g ? &g->h : nullptr), o, aut);
+    return SGE_OK;
+}
+
+extern "C" int sge_plan_orders(const sge_pattern *cp, uint32_t *orders, uint32_t cap, uint32_t *n) {
+    if (!cp || !n) {
+        set_error("sge_plan_orders: null argument");
+        return SGE_EINVAL;
+    }
+    Pattern p = from_c(*cp);
+    auto aut = automorphisms(p);
+    auto os = connected_orders(p, aut);
+    *n = (uint32_t)os.size();
+    if (orders)
+        for (uint32_t i = 0; i < std::min<uint32_t>(cap, *n); ++i)
+            for (int j = 0; j < p.k; ++j) orders[i * p.k + j] = os[i][j];
+    return SGE_OK;
+}
+
+// Plan text in the reference's -r / stdout style (decomposition.cpp:537-553
+// prints "num bags" and bags; symmetry rules are printed as "u < a,b,").
+extern "C" int sge_plan_text(const sge_plan *pl, char *buf, size_t cap) {
+    if (!pl || !buf || cap == 0) {
+        set_error("sge_plan_text: bad arguments");
+        return SGE_EINVAL;
+    }
+    std::string s = "num bags: " + std::to_string(pl->num_bags) + "\n";
+    s += "order:";
+    for (uint32_t i = 0; i < pl->k; ++i) s += " " + std::to_string(pl->order[i]);
+    s += "\n";
+    for (uint32_t i = 0; i < pl->k; ++i) {
+        std::string larger;
+        for (uint32_t j = 0; j < pl->k; ++j)
+            if ((pl->lower[j] >> i) & 1u) larger += std::to_string(pl->order[j]) + ",";
+        if (!larger.empty()) s += std::to_string(pl->order[i]) + " < " + larger + "\n";
+    }
+    if (s.size() + 1 > cap) {
+        set_error("sge_plan_text: buffer too small");
+        return SGE_ERANGE;
+    }
+    std::memcpy(buf, s.c_str(), s.size() + 1);
+    return SGE_OK;
+}

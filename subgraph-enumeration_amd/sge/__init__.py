@@ -1,0 +1,389 @@
+"""Python host for the MI355X subgraph-enumeration executor (libsge.so).
+
+Mirrors the reference's plan-to-join interface (main.cpp:12-112 drives
+optCostPlan -> sharedJoin / parSharedJoin): load a pattern and a data graph,
+build a plan (cost-model order, symmetry clamps, WCOJ / exploration / IEP mode),
+count on the GPU.  All compute goes through the C ABI of include/sge.h; there is
+no CPU fallback -- if libsge.so is missing or no GPU is present the calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+from typing import Iterable, List, Optional, Sequence
+
+import numpy as np
+
+MAX_K = 8
+MODE_WCOJ, MODE_EXPLORE, MODE_IEP = 0, 1, 2
+COUNT_BATCHED = 1
+GRAPH_DEGREE_ORDER = 1
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "libsge.so")
+CLI_PATH = os.path.join(os.path.dirname(_HERE), "graph_mining_system")
+
+
+class SgeError(RuntimeError):
+    pass
+
+
+class GraphStats(C.Structure):
+    _fields_ = [("n", C.c_uint32), ("m", C.c_uint64), ("max_degree", C.c_uint32),
+                ("degree_ordered", C.c_uint32), ("csr_bytes", C.c_uint64), ("triangles_hint", C.c_uint64)]
+
+
+class CPattern(C.Structure):
+    _fields_ = [("k", C.c_uint32), ("adj", C.c_uint32 * MAX_K)]
+
+
+class PatternInfo(C.Structure):
+    _fields_ = [("k", C.c_uint32), ("m", C.c_uint32), ("aut_size", C.c_uint64), ("canon", C.c_uint64),
+                ("orbit", C.c_uint32 * MAX_K), ("core_size", C.c_uint32), ("num_bags", C.c_uint32),
+                ("single_bag", C.c_uint32), ("fhw", C.c_double), ("connected", C.c_uint32)]
+
+
+class Plan(C.Structure):
+    _fields_ = [("k", C.c_uint32), ("mode", C.c_uint32), ("order", C.c_uint32 * MAX_K),
+                ("parent", C.c_uint32 * MAX_K), ("lower", C.c_uint32 * MAX_K), ("upper", C.c_uint32 * MAX_K),
+                ("check", C.c_uint32 * MAX_K), ("reuse", C.c_int32 * MAX_K), ("ready", C.c_uint32 * MAX_K),
+                ("tail", C.c_uint32), ("num_bags", C.c_uint32), ("multiplier", C.c_uint64),
+                ("aut_size", C.c_uint64), ("cost", C.c_double)]
+
+    def as_dict(self):
+        k = self.k
+        return {"k": k, "mode": self.mode, "order": list(self.order)[:k], "parent": list(self.parent)[:k],
+                "lower": list(self.lower)[:k], "upper": list(self.upper)[:k], "check": list(self.check)[:k],
+                "reuse": list(self.reuse)[:k], "ready": list(self.ready)[:k], "tail": self.tail,
+                "num_bags": self.num_bags, "multiplier": self.multiplier, "aut_size": self.aut_size,
+                "cost": self.cost}
+
+
+class Tuning(C.Structure):
+    _fields_ = [("block_threads", C.c_uint32), ("chunk", C.c_uint32), ("stage_max", C.c_uint32),
+                ("blocks_per_cu", C.c_uint32), ("reserved", C.c_uint32 * 4)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("intersections", C.c_uint64), ("bytes", C.c_uint64), ("tasks", C.c_uint64),
+                ("nodes", C.c_uint64), ("kernel_ms", C.c_double), ("wall_ms", C.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+_lib = None
+
+_SIGS = {
+    "sge_last_error": (C.c_char_p, []),
+    "sge_version": (C.c_int, []),
+    "sge_graph_load": (C.c_int, [C.c_char_p, C.c_uint32, C.POINTER(C.c_void_p)]),
+    "sge_graph_from_edges": (C.c_int, [C.c_uint32, C.c_void_p, C.c_uint64, C.c_uint32, C.POINTER(C.c_void_p)]),
+    "sge_graph_from_csr": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint64, C.c_uint32,
+                                     C.POINTER(C.c_void_p)]),
+    "sge_graph_chung_lu": (C.c_int, [C.c_uint32, C.c_uint64, C.c_double, C.c_double, C.c_uint64, C.c_uint32,
+                                     C.POINTER(C.c_void_p)]),
+    "sge_graph_rmat": (C.c_int, [C.c_uint32, C.c_uint32, C.c_double, C.c_double, C.c_double, C.c_uint64,
+                                 C.c_uint32, C.POINTER(C.c_void_p)]),
+    "sge_graph_stats_get": (C.c_int, [C.c_void_p, C.POINTER(GraphStats)]),
+    "sge_graph_csr": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "sge_graph_perm": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "sge_graph_write": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int]),
+    "sge_graph_to_device": (C.c_int, [C.c_void_p]),
+    "sge_graph_destroy": (None, [C.c_void_p]),
+    "sge_pattern_load": (C.c_int, [C.c_char_p, C.POINTER(CPattern)]),
+    "sge_pattern_from_edges": (C.c_int, [C.c_uint32, C.c_void_p, C.c_uint32, C.POINTER(CPattern)]),
+    "sge_pattern_analyze": (C.c_int, [C.POINTER(CPattern), C.POINTER(PatternInfo)]),
+    "sge_pattern_sample": (C.c_int, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64, C.c_void_p,
+                                     C.c_uint32, C.POINTER(CPattern), C.POINTER(C.c_uint32)]),
+    "sge_canonicalize_embeddings": (C.c_int, [C.POINTER(CPattern), C.c_void_p, C.c_uint64]),
+    "sge_plan_build": (C.c_int, [C.POINTER(CPattern), C.c_void_p, C.c_uint32, C.c_void_p, C.POINTER(Plan)]),
+    "sge_plan_cost": (C.c_int, [C.POINTER(CPattern), C.c_void_p, C.c_void_p, C.POINTER(C.c_double)]),
+    "sge_plan_orders": (C.c_int, [C.POINTER(CPattern), C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32)]),
+    "sge_plan_text": (C.c_int, [C.POINTER(Plan), C.c_char_p, C.c_size_t]),
+    "sge_tuning_default": (None, [C.POINTER(Tuning)]),
+    "sge_count": (C.c_int, [C.c_void_p, C.POINTER(Plan), C.c_uint32, C.c_uint32, C.POINTER(Tuning), C.c_void_p,
+                            C.POINTER(Stats), C.POINTER(Stats)]),
+    "sge_count_range": (C.c_int, [C.c_void_p, C.POINTER(Plan), C.c_uint64, C.c_uint64, C.POINTER(Tuning),
+                                  C.POINTER(C.c_uint64), C.POINTER(Stats)]),
+    "sge_num_tasks": (C.c_int, [C.c_void_p, C.POINTER(Plan), C.POINTER(C.c_uint64)]),
+    "sge_enumerate": (C.c_int, [C.c_void_p, C.POINTER(Plan), C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64)]),
+    "sge_batch_create": (C.c_int, [C.c_void_p, C.POINTER(Plan), C.c_uint32, C.POINTER(Tuning),
+                                   C.POINTER(C.c_void_p)]),
+    "sge_batch_run": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Stats)]),
+    "sge_batch_destroy": (None, [C.c_void_p]),
+    "sge_device_info": (C.c_int, [C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_char_p, C.c_size_t]),
+}
+
+
+def lib():
+    """Load libsge.so (built in-tree by `make`); raise if it is missing."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise SgeError(f"libsge.so not built at {LIB_PATH}; run __graft_entry__.build() or make")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc: int, what: str):
+    if rc != 0:
+        msg = lib().sge_last_error().decode(errors="replace")
+        raise SgeError(f"{what} failed ({rc}): {msg}")
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class Graph:
+    """Data graph: symmetric CSR (u32), optionally relabelled by ascending degree."""
+
+    def __init__(self, handle):
+        self._h = C.c_void_p(handle) if not isinstance(handle, C.c_void_p) else handle
+
+    @classmethod
+    def load(cls, path: str, degree_order: bool = True) -> "Graph":
+        h = C.c_void_p()
+        _check(lib().sge_graph_load(path.encode(), GRAPH_DEGREE_ORDER if degree_order else 0, C.byref(h)),
+               "sge_graph_load")
+        return cls(h)
+
+    @classmethod
+    def from_edges(cls, n: int, edges, degree_order: bool = True) -> "Graph":
+        e = np.ascontiguousarray(np.asarray(edges, dtype=np.uint32).reshape(-1, 2))
+        h = C.c_void_p()
+        _check(lib().sge_graph_from_edges(n, _ptr(e), e.shape[0], GRAPH_DEGREE_ORDER if degree_order else 0,
+                                          C.byref(h)), "sge_graph_from_edges")
+        return cls(h)
+
+    @classmethod
+    def chung_lu(cls, n, m, gamma, cap=0.0, seed=0, degree_order=True) -> "Graph":
+        h = C.c_void_p()
+        _check(lib().sge_graph_chung_lu(n, m, gamma, cap, seed, GRAPH_DEGREE_ORDER if degree_order else 0,
+                                        C.byref(h)), "sge_graph_chung_lu")
+        return cls(h)
+
+    @classmethod
+    def rmat(cls, scale, edge_factor=16, a=0.57, b=0.19, c=0.19, seed=0, degree_order=True) -> "Graph":
+        h = C.c_void_p()
+        _check(lib().sge_graph_rmat(scale, edge_factor, a, b, c, seed, GRAPH_DEGREE_ORDER if degree_order else 0,
+                                    C.byref(h)), "sge_graph_rmat")
+        return cls(h)
+
+    @property
+    def stats(self) -> GraphStats:
+        s = GraphStats()
+        _check(lib().sge_graph_stats_get(self._h, C.byref(s)), "sge_graph_stats_get")
+        return s
+
+    @property
+    def n(self) -> int:
+        return self.stats.n
+
+    @property
+    def m(self) -> int:
+        return self.stats.m
+
+    def csr(self):
+        s = self.stats
+        off = np.zeros(s.n + 1, dtype=np.uint32)
+        nbr = np.zeros(max(2 * s.m, 1), dtype=np.uint32)
+        _check(lib().sge_graph_csr(self._h, _ptr(off), _ptr(nbr)), "sge_graph_csr")
+        return off, nbr[: 2 * s.m]
+
+    def perm(self):
+        p = np.zeros(max(self.n, 1), dtype=np.uint32)
+        _check(lib().sge_graph_perm(self._h, _ptr(p)), "sge_graph_perm")
+        return p[: self.n]
+
+    def write(self, path: str, binary: bool = False):
+        _check(lib().sge_graph_write(self._h, path.encode(), 1 if binary else 0), "sge_graph_write")
+
+    def to_device(self):
+        _check(lib().sge_graph_to_device(self._h), "sge_graph_to_device")
+
+    def close(self):
+        if self._h and self._h.value:
+            lib().sge_graph_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Pattern:
+    def __init__(self, c: CPattern):
+        self.c = c
+
+    @classmethod
+    def load(cls, path: str) -> "Pattern":
+        c = CPattern()
+        _check(lib().sge_pattern_load(path.encode(), C.byref(c)), "sge_pattern_load")
+        return cls(c)
+
+    @classmethod
+    def from_edges(cls, k: int, edges) -> "Pattern":
+        e = np.ascontiguousarray(np.asarray(edges, dtype=np.uint32).reshape(-1, 2))
+        c = CPattern()
+        _check(lib().sge_pattern_from_edges(k, _ptr(e), e.shape[0], C.byref(c)), "sge_pattern_from_edges")
+        return cls(c)
+
+    @property
+    def k(self) -> int:
+        return self.c.k
+
+    def edges(self) -> List[tuple]:
+        return [(i, j) for i in range(self.k) for j in range(i + 1, self.k) if (self.c.adj[i] >> j) & 1]
+
+    def info(self) -> PatternInfo:
+        i = PatternInfo()
+        _check(lib().sge_pattern_analyze(C.byref(self.c), C.byref(i)), "sge_pattern_analyze")
+        return i
+
+    def orders(self) -> List[List[int]]:
+        n = C.c_uint32()
+        _check(lib().sge_plan_orders(C.byref(self.c), None, 0, C.byref(n)), "sge_plan_orders")
+        buf = np.zeros(max(n.value * self.k, 1), dtype=np.uint32)
+        _check(lib().sge_plan_orders(C.byref(self.c), _ptr(buf), n.value, C.byref(n)), "sge_plan_orders")
+        return buf[: n.value * self.k].reshape(-1, self.k).tolist()
+
+    def __repr__(self):
+        return f"Pattern(k={self.k}, edges={self.edges()})"
+
+
+def sample_patterns(k, emin, emax, want, seed, exclude: Sequence[int] = ()) -> List[Pattern]:
+    out = (CPattern * want)()
+    got = C.c_uint32()
+    ex = np.ascontiguousarray(np.asarray(list(exclude), dtype=np.uint64))
+    _check(lib().sge_pattern_sample(k, emin, emax, want, seed, _ptr(ex) if len(ex) else None, len(ex), out,
+                                    C.byref(got)), "sge_pattern_sample")
+    return [Pattern(out[i]) for i in range(got.value)]
+
+
+def build_plan(pattern: Pattern, graph: Optional[Graph] = None, mode: int = MODE_WCOJ,
+               order: Optional[Sequence[int]] = None) -> Plan:
+    p = Plan()
+    o = None
+    if order is not None:
+        o = np.ascontiguousarray(np.asarray(order, dtype=np.uint32))
+    _check(lib().sge_plan_build(C.byref(pattern.c), graph._h if graph else None, mode,
+                                _ptr(o) if o is not None else None, C.byref(p)), "sge_plan_build")
+    return p
+
+
+def order_cost(pattern: Pattern, graph: Optional[Graph], order: Sequence[int]) -> float:
+    o = np.ascontiguousarray(np.asarray(order, dtype=np.uint32))
+    c = C.c_double()
+    _check(lib().sge_plan_cost(C.byref(pattern.c), graph._h if graph else None, _ptr(o), C.byref(c)),
+           "sge_plan_cost")
+    return c.value
+
+
+def plan_text(plan: Plan) -> str:
+    buf = C.create_string_buffer(8192)
+    _check(lib().sge_plan_text(C.byref(plan), buf, 8192), "sge_plan_text")
+    return buf.value.decode()
+
+
+def default_tuning() -> Tuning:
+    t = Tuning()
+    lib().sge_tuning_default(C.byref(t))
+    return t
+
+
+def count(graph: Graph, plans, batched: bool = False, tuning: Optional[Tuning] = None, per_plan: bool = False):
+    """Exact counts for each plan (list of int), plus Stats (and per-plan Stats)."""
+    if isinstance(plans, Plan):
+        plans = [plans]
+    n = len(plans)
+    arr = (Plan * n)(*plans)
+    counts = np.zeros(n, dtype=np.uint64)
+    st = Stats()
+    pp = (Stats * n)() if per_plan else None
+    _check(lib().sge_count(graph._h, arr, n, COUNT_BATCHED if batched else 0,
+                           C.byref(tuning) if tuning else None, _ptr(counts), C.byref(st), pp), "sge_count")
+    res = [int(x) for x in counts]
+    if per_plan:
+        return res, st, [pp[i] for i in range(n)]
+    return res, st
+
+
+def count_range(graph: Graph, plan: Plan, begin: int, end: int, tuning: Optional[Tuning] = None):
+    c = C.c_uint64()
+    st = Stats()
+    _check(lib().sge_count_range(graph._h, C.byref(plan), begin, end, C.byref(tuning) if tuning else None,
+                                 C.byref(c), C.byref(st)), "sge_count_range")
+    return int(c.value), st
+
+
+def num_tasks(graph: Graph, plan: Plan) -> int:
+    n = C.c_uint64()
+    _check(lib().sge_num_tasks(graph._h, C.byref(plan), C.byref(n)), "sge_num_tasks")
+    return int(n.value)
+
+
+def enumerate_embeddings(graph: Graph, plan: Plan, pattern: Pattern, cap: int, canonical: bool = True):
+    """One row per counted subgraph (images per pattern vertex, original ids)."""
+    out = np.zeros(max(cap, 1) * plan.k, dtype=np.uint32)
+    n = C.c_uint64()
+    _check(lib().sge_enumerate(graph._h, C.byref(plan), _ptr(out), cap, C.byref(n)), "sge_enumerate")
+    rows = out[: n.value * plan.k].reshape(-1, plan.k).copy()
+    if canonical:
+        canonicalize(pattern, rows)
+    return rows
+
+
+def canonicalize(pattern: Pattern, rows: np.ndarray) -> np.ndarray:
+    assert rows.dtype == np.uint32 and rows.flags.c_contiguous
+    _check(lib().sge_canonicalize_embeddings(C.byref(pattern.c), _ptr(rows), rows.shape[0]),
+           "sge_canonicalize_embeddings")
+    return rows
+
+
+class Batch:
+    """Batched executor: all plans in one persistent launch, captured in a hipGraph."""
+
+    def __init__(self, graph: Graph, plans: Sequence[Plan], tuning: Optional[Tuning] = None):
+        self.graph = graph
+        self.n = len(plans)
+        self._plans = (Plan * self.n)(*plans)
+        self._h = C.c_void_p()
+        _check(lib().sge_batch_create(graph._h, self._plans, self.n, C.byref(tuning) if tuning else None,
+                                      C.byref(self._h)), "sge_batch_create")
+
+    def run(self):
+        counts = np.zeros(self.n, dtype=np.uint64)
+        st = Stats()
+        _check(lib().sge_batch_run(self._h, _ptr(counts), C.byref(st)), "sge_batch_run")
+        return [int(x) for x in counts], st
+
+    def close(self):
+        if self._h and self._h.value:
+            lib().sge_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def device_info():
+    cu, xcd = C.c_int(), C.c_int()
+    name = C.create_string_buffer(128)
+    _check(lib().sge_device_info(C.byref(cu), C.byref(xcd), name, 128), "sge_device_info")
+    return {"num_cu": cu.value, "xcd": xcd.value, "arch": name.value.decode()}
+
+
+EXPORTED_SYMBOLS = sorted(_SIGS)

@@ -1,0 +1,132 @@
+"""Writes golden counts by running the reference's graph_mining_system, built
+from its own sources by oracle/ref.mk into oracle/_ref/, on this repo's
+synthetic graphs.  Run in a container that has /root/reference and the built
+binary:  python tests/golden/make_golden.py [section ...]
+
+Sections: c1, c2, c3, rmat7, dataset.  Output: tests/golden/<section>.json
+(inputs are named by generator + parameters; counts are the reference's
+"Number of matches" lines; "bags" its "num bags" line).
+"""
+import json
+import os
+import re
+import subprocess
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, os.path.join(ROOT, "subgraph-enumeration_amd"))
+import sge  # noqa: E402
+import sge.workloads as W  # noqa: E402
+
+REF = os.path.join(ROOT, "oracle", "_ref", "graph_mining_system")
+REF_DATA = "/root/reference/dataset"
+OUT = os.path.dirname(os.path.abspath(__file__))
+TMP = tempfile.mkdtemp(prefix="sge_golden_")
+
+
+def write_pattern(k, edges, path):
+    with open(path, "w") as f:
+        f.write(f"{k} {len(edges)}\n")
+        for a, b in edges:
+            f.write(f"{a} {b}\n")
+
+
+def run_ref(pattern_path, graph_path, flags, threads):
+    cmd = [REF, "-q", pattern_path, "-d", graph_path, "-t", str(threads)] + flags
+    t0 = time.time()
+    out = subprocess.run(cmd, capture_output=True, text=True, timeout=3600)
+    wall = time.time() - t0
+    if out.returncode != 0:
+        return {"error": out.returncode, "wall_s": wall}
+    m = re.search(r"Number of matches: (\d+)", out.stdout)
+    b = re.search(r"num bags: (\d+)", out.stdout)
+    e = re.search(r"Execution Time: ([0-9.e+-]+)", out.stdout)
+    p = re.search(r"Planning time \(s\): ([0-9.e+-]+)", out.stdout)
+    return {"count": int(m.group(1)), "bags": int(b.group(1)) if b else None,
+            "exec_s": float(e.group(1)) if e else None, "plan_s": float(p.group(1)) if p else None,
+            "wall_s": wall, "threads": threads, "flags": " ".join(flags)}
+
+
+def graph_file(name, g):
+    path = os.path.join(TMP, name + ".txt")
+    g.write(path)
+    return path
+
+
+def section_c1():
+    g = W.graph_c1(False)
+    gp = graph_file("c1", g)
+    res = {"graph": "chung_lu(n=379, m=914, gamma=2.5, cap=none, seed=0)", "n": g.n, "m": g.m, "patterns": {}}
+    for name in ["triangle", "diamond", "house", "near5clique", "k4", "k5"]:
+        k, e = W.NAMED[name]
+        pp = os.path.join(TMP, name + ".txt")
+        write_pattern(k, e, pp)
+        res["patterns"][name] = {"edges": e, "k": k, "wcoj": run_ref(pp, gp, [], 1)}
+    return res
+
+
+def section_c2():
+    g = W.graph_c2(False)
+    gp = graph_file("c2", g)
+    res = {"graph": "chung_lu(n=7115, m=100762, gamma=2.1, cap=1100, seed=1)", "n": g.n, "m": g.m,
+           "cores": os.cpu_count(), "patterns": {}}
+    for name in ["near5clique", "house"]:
+        k, e = W.NAMED[name]
+        pp = os.path.join(TMP, name + ".txt")
+        write_pattern(k, e, pp)
+        res["patterns"][name] = {"edges": e, "k": k,
+                                 "wcoj": run_ref(pp, gp, [], 8), "iep": run_ref(pp, gp, ["-u"], 8),
+                                 "explore": run_ref(pp, gp, ["-i"], 8)}
+        print(name, res["patterns"][name], flush=True)
+    return res
+
+
+def section_c3():
+    g = W.graph_c2(False)
+    gp = graph_file("c2", g)
+    res = {"graph": "chung_lu(n=7115, m=100762, gamma=2.1, cap=1100, seed=1)", "mode": "-u -t 8",
+           "cores": os.cpu_count(), "patterns": []}
+    for i, p in enumerate(W.six_vertex_set()):
+        pp = os.path.join(TMP, f"six{i}.txt")
+        write_pattern(p.k, p.edges(), pp)
+        r = run_ref(pp, gp, ["-u"], 8)
+        res["patterns"].append({"index": i, "edges": p.edges(), "canon": p.info().canon, "iep": r})
+        print(i, r, flush=True)
+    return res
+
+
+def section_rmat7():
+    g = W.graph_rmat(10, 7, False)
+    gp = graph_file("rmat10", g)
+    res = {"graph": "rmat(scale=10, ef=16, a=.57, b=.19, c=.19, seed=7)", "n": g.n, "m": g.m, "patterns": []}
+    for i, p in enumerate(W.seven_vertex_set()):
+        pp = os.path.join(TMP, f"seven{i}.txt")
+        write_pattern(p.k, p.edges(), pp)
+        r = run_ref(pp, gp, [], 8)
+        res["patterns"].append({"index": i, "edges": p.edges(), "canon": p.info().canon, "wcoj": r})
+        print(i, r, flush=True)
+    return res
+
+
+def section_dataset():
+    """Canonical codes of the reference's benchmark pattern files (data only)."""
+    res = {}
+    for d in ["6vertex", "7vertex", "light_query", "huge_query"]:
+        items = []
+        for f in sorted(os.listdir(os.path.join(REF_DATA, d))):
+            p = sge.Pattern.load(os.path.join(REF_DATA, d, f))
+            info = p.info()
+            items.append({"file": f, "k": p.k, "edges": p.edges(), "canon": info.canon, "aut": info.aut_size})
+        res[d] = items
+    return res
+
+
+if __name__ == "__main__":
+    sections = sys.argv[1:] or ["c1", "c2", "dataset", "rmat7", "c3"]
+    for s in sections:
+        data = globals()["section_" + s]()
+        with open(os.path.join(OUT, s + ".json"), "w") as f:
+            json.dump(data, f, indent=1)
+        print("wrote", s, flush=True)

@@ -1,0 +1,204 @@
+"""GPU parity: the HIP executor (through the C ABI) against the oracle, the
+reference's golden counts and brute force.  Bit-exact uint64 counts required."""
+import itertools
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import sge
+import sge.workloads as W
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+MODES = (sge.MODE_WCOJ, sge.MODE_EXPLORE, sge.MODE_IEP)
+
+
+def say(*a):
+    print("[gpu-case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def golden(name):
+    with open(os.path.join(GOLDEN, name + ".json")) as f:
+        return json.load(f)
+
+
+def random_graph(n, p, seed):
+    rng = np.random.default_rng(seed)
+    edges = [(i, j) for i in range(n) for j in range(i + 1, n) if rng.random() < p]
+    return sge.Graph.from_edges(n, edges if edges else [(0, 1)])
+
+
+def small_patterns():
+    return [W.named(n) for n in ["triangle", "diamond", "house", "near5clique", "k4", "k5"]] + [
+        sge.Pattern.from_edges(3, [(0, 1), (1, 2)]),
+        sge.Pattern.from_edges(4, [(0, 1), (1, 2), (2, 3), (3, 0)]),
+        sge.Pattern.from_edges(4, [(0, 1), (0, 2), (0, 3)]),
+        sge.Pattern.from_edges(5, [(0, 1), (0, 2), (0, 3), (0, 4)]),
+        sge.Pattern.from_edges(5, [(0, 1), (1, 2), (2, 3), (3, 4), (4, 0)]),
+        sge.Pattern.from_edges(6, [(0, 1), (1, 2), (2, 0), (0, 3), (1, 4), (2, 5)]),
+        sge.Pattern.from_edges(2, [(0, 1)]),
+    ]
+
+
+def test_c1_all_patterns_all_modes():
+    g = W.graph_c1()
+    off, nbr = g.csr()
+    gold = golden("c1")["patterns"]
+    for p in small_patterns() + W.six_vertex_set()[:6]:
+        for mode in MODES:
+            say("c1", p.edges(), "mode", mode)
+            plan = sge.build_plan(p, g, mode)
+            got = sge.count(g, plan)[0][0]
+            assert got == O.count(plan, off, nbr), (p, mode)
+    for name, rec in gold.items():
+        for mode in MODES:
+            assert sge.count(g, sge.build_plan(W.named(name), g, mode))[0][0] == rec["wcoj"]["count"]
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_random_graphs_vs_brute_force(seed):
+    g = random_graph(40, 0.3, seed)
+    off, nbr = g.csr()
+    for p in small_patterns():
+        if p.k > 5:
+            continue
+        info = p.info()
+        brute = O.count_brute(p.k, p.c.adj, off, nbr)
+        want = brute if info.single_bag else brute // info.aut_size
+        for mode in MODES:
+            say("random", seed, p.edges(), mode)
+            assert sge.count(g, sge.build_plan(p, g, mode))[0][0] == want
+
+
+def test_every_order_same_count():
+    g = W.graph_c1()
+    for p in [W.named("house"), W.named("diamond")]:
+        seen = set()
+        for o in p.orders():
+            say("order", o)
+            seen.add(sge.count(g, sge.build_plan(p, g, sge.MODE_WCOJ, o))[0][0])
+        assert len(seen) == 1
+
+
+def test_c2_golden_all_modes():
+    g = W.graph_c2()
+    gold = golden("c2")["patterns"]
+    for name in ["near5clique", "house"]:
+        for mode, key in ((sge.MODE_WCOJ, "wcoj"), (sge.MODE_EXPLORE, "explore"), (sge.MODE_IEP, "iep")):
+            say("c2", name, key)
+            got, st = sge.count(g, sge.build_plan(W.named(name), g, mode))
+            assert got[0] == gold[name][key]["count"], (name, key)
+
+
+def test_c2_house_root_ranges_vs_oracle():
+    g = W.graph_c2()
+    off, nbr = g.csr()
+    plan = sge.build_plan(W.named("house"), g)
+    nnz = int(off[-1])
+    for a in [0, nnz // 3, nnz - 3000]:
+        say("c2 house range", a)
+        got, _ = sge.count_range(g, plan, a, a + 1500)
+        assert got == O.count(plan, off, nbr, begin=a, end=a + 1500)
+
+
+def test_batched_equals_plain_and_oracle():
+    g = W.graph_c1()
+    off, nbr = g.csr()
+    pats = W.six_vertex_set()
+    for mode in MODES:
+        plans = [sge.build_plan(p, g, mode) for p in pats]
+        say("batched six-vertex mode", mode)
+        plain, _ = sge.count(g, plans, batched=False)
+        batched, _ = sge.count(g, plans, batched=True)
+        assert plain == batched
+        assert plain == [O.count(pl, off, nbr) for pl in plans]
+
+
+def test_hipgraph_batch_replays():
+    g = W.graph_c2()
+    pats = W.six_vertex_set()[:5]
+    plans = [sge.build_plan(p, g, sge.MODE_IEP) for p in pats]
+    want, _ = sge.count(g, plans)
+    b = sge.Batch(g, plans)
+    for rep in range(4):
+        say("hipgraph replay", rep)
+        got, st = b.run()
+        assert got == want
+        _ = sge.count(g, plans[:1])  # eager work between replays
+    b.close()
+
+
+def test_tuning_variants_same_counts():
+    g = W.graph_c1()
+    plans = [sge.build_plan(p, g) for p in W.six_vertex_set()[:4]]
+    want, _ = sge.count(g, plans)
+    for bt, ch in [(64, 1), (128, 7), (192, 64), (256, 1000)]:
+        t = sge.default_tuning()
+        t.block_threads, t.chunk = bt, ch
+        say("tuning", bt, ch)
+        assert sge.count(g, plans, batched=True, tuning=t)[0] == want
+
+
+def test_enumeration_c1_diamond_matches_brute_force():
+    g = W.graph_c1()
+    p = W.named("diamond")
+    plan = sge.build_plan(p, g)
+    cnt = sge.count(g, plan)[0][0]
+    say("enumerate diamond")
+    rows = sge.enumerate_embeddings(g, plan, p, cnt + 10)
+    assert rows.shape[0] == cnt
+    # brute force canonical set in original ids
+    g0 = W.graph_c1(False)
+    off, nbr = g0.csr()
+    adj = [set(nbr[off[v]:off[v + 1]].tolist()) for v in range(g0.n)]
+    auts = [perm for perm in itertools.permutations(range(4))
+            if all(((perm[a], perm[b]) in p.edges() or (perm[b], perm[a]) in p.edges()) for a, b in p.edges())]
+    want = set()
+    for a in range(g0.n):
+        for b in adj[a]:
+            for c in adj[a] & adj[b]:
+                for d in adj[a] & adj[c]:
+                    if len({a, b, c, d}) < 4:
+                        continue
+                    img = (a, b, c, d)
+                    if all(img[y] in adj[img[x]] for x, y in p.edges()):
+                        want.add(min(tuple(img[q[i]] for i in range(4)) for q in auts))
+    got = set(map(tuple, rows.tolist()))
+    assert got == want
+    assert rows.tolist() == sorted(rows.tolist())
+
+
+def test_iep_reproduces_choosec_wrap():
+    # star graph: hub 0 with 70000 leaves.  K_{1,5} in IEP mode counts the last
+    # four leaves with choosec(n, 4), whose numerator wraps mod 2^64 at this size.
+    n = 70001
+    g = sge.Graph.from_edges(n, [(0, i) for i in range(1, n)])
+    off, nbr = g.csr()
+    star5 = sge.Pattern.from_edges(6, [(0, i) for i in range(1, 6)])
+    iep = sge.build_plan(star5, g, sge.MODE_IEP)
+    assert iep.tail == 4
+    say("iep wrap")
+    got = sge.count(g, iep)[0][0]
+    assert got == O.count(iep, off, nbr)
+    exact = sge.count(g, sge.build_plan(star5, g, sge.MODE_WCOJ))[0][0]
+    import math
+    assert exact == math.comb(n - 1, 5)
+    assert got != exact
+
+
+def test_edge_cases():
+    say("edge cases")
+    empty = sge.Graph.from_edges(10, [(0, 1)])
+    assert sge.count(empty, sge.build_plan(W.named("triangle"), empty))[0][0] == 0
+    g = W.graph_c1()
+    edge = sge.Pattern.from_edges(2, [(0, 1)])
+    pl = sge.build_plan(edge, g)
+    assert edge.info().single_bag == 1  # the reference prints 2m for the edge pattern
+    assert sge.count(g, pl)[0][0] == 2 * g.m
+    single = sge.Pattern.from_edges(1, [])
+    assert sge.count(g, sge.build_plan(single, g))[0][0] == g.n

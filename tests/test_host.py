@@ -1,0 +1,178 @@
+"""Host-side logic: pattern analysis, planner invariants, graph I/O, the C ABI."""
+import hashlib
+import json
+import math
+import os
+import re
+
+import numpy as np
+import pytest
+
+import sge
+import sge.workloads as W
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def test_library_exports_every_declared_symbol():
+    hdr = open(os.path.join(ROOT, "include", "sge.h")).read()
+    names = set(re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(sge_\w+)\s*\(", hdr, re.M))
+    assert len(names) > 30
+    L = sge.lib()
+    missing = [n for n in sorted(names) if not hasattr(L, n)]
+    assert not missing, missing
+    assert set(sge.EXPORTED_SYMBOLS) <= names
+
+
+@pytest.mark.parametrize("k,edges,aut", [
+    (3, W.TRIANGLE, 6), (4, W.DIAMOND, 4), (5, W.HOUSE, 2), (5, W.NEAR_5_CLIQUE, 12),
+    (5, [(0, 1), (0, 2), (0, 3), (0, 4)], 24), (5, [(0, 1), (1, 2), (2, 3), (3, 4), (4, 0)], 10)])
+def test_automorphism_group_sizes(k, edges, aut):
+    assert sge.Pattern.from_edges(k, edges).info().aut_size == aut
+
+
+def test_fractional_widths_closed_form():
+    cases = [(3, W.TRIANGLE, 1.5), (4, [(0, 1), (1, 2), (2, 3), (3, 0)], 2.0), (4, W.clique(4), 2.0),
+             (5, [(0, 1), (1, 2), (2, 3), (3, 4), (4, 0)], 2.0),  # bags {0,1,4},{1,2,3,4}; rho*(C5)=2.5
+             (5, W.clique(5), 2.5)]
+    for k, e, w in cases:
+        info = sge.Pattern.from_edges(k, e).info()
+        assert info.fhw == w, (k, e)
+
+
+def test_single_bag_rule():
+    assert sge.Pattern.from_edges(4, W.clique(4)).info().single_bag == 1
+    assert sge.Pattern.from_edges(5, W.clique(5)).info().single_bag == 1
+    assert W.named("diamond").info().single_bag == 0
+    assert W.named("house").info().single_bag == 0
+
+
+def test_canonical_code_is_isomorphism_invariant():
+    rng = np.random.default_rng(0)
+    for p in W.six_vertex_set()[:6]:
+        c = p.info().canon
+        for _ in range(5):
+            perm = rng.permutation(p.k)
+            q = sge.Pattern.from_edges(p.k, [(int(perm[a]), int(perm[b])) for a, b in p.edges()])
+            assert q.info().canon == c
+
+
+def test_sampled_sets_are_the_reference_benchmark_classes():
+    path = os.path.join(GOLDEN, "dataset.json")
+    if not os.path.exists(path):
+        pytest.skip("dataset.json not generated")
+    data = json.load(open(path))
+    six = {p.info().canon for p in W.six_vertex_set()}
+    seven = {p.info().canon for p in W.seven_vertex_set()}
+    assert len(six) == 16 and len(seven) == 78
+    assert six == {d["canon"] for d in data["6vertex"]}
+    assert seven == {d["canon"] for d in data["7vertex"]}
+    train = {p.info().canon for p in W.training_set(50)}
+    assert not (train & seven)
+
+
+def test_bag_counts_match_reference_dataset_runs():
+    path = os.path.join(GOLDEN, "rmat7.json")
+    if not os.path.exists(path):
+        pytest.skip("rmat7.json not generated")
+    pats = W.seven_vertex_set()
+    for rec in json.load(open(path))["patterns"]:
+        if rec["wcoj"].get("bags") is None:
+            continue
+        info = pats[rec["index"]].info()
+        assert info.single_bag == (rec["wcoj"]["bags"] == 1)
+
+
+def test_plan_masks_are_consistent():
+    g = W.graph_c1()
+    for p in W.six_vertex_set():
+        for mode in (0, 1, 2):
+            pl = sge.build_plan(p, g, mode)
+            d = pl.as_dict()
+            k = d["k"]
+            assert sorted(d["order"]) == list(range(k))
+            for i in range(k):
+                earlier = (1 << i) - 1
+                for key in ("parent", "lower", "upper", "check"):
+                    assert d[key][i] & ~earlier == 0
+                assert (d["parent"][i] | d["lower"][i] | d["upper"][i] | d["check"][i]) == earlier
+                if i:
+                    assert d["parent"][i]
+
+
+def test_iep_tail_on_star():
+    star = sge.Pattern.from_edges(5, [(0, 1), (0, 2), (0, 3), (0, 4)])
+    pl = sge.build_plan(star, None, sge.MODE_IEP)
+    assert pl.tail == 3  # four leaves: root edge fixes one, the other three are a choosec group
+    assert sge.build_plan(star, None, sge.MODE_WCOJ).tail == 0
+
+
+def test_graph_text_and_binary_roundtrip(tmp_path):
+    g = W.graph_c1(False)
+    t, b = str(tmp_path / "g.txt"), str(tmp_path / "g.bin")
+    g.write(t)
+    g.write(b, binary=True)
+    assert open(b, "rb").read(7) == b"BINGRPH"
+    for path in (t, b):
+        h = sge.Graph.load(path, degree_order=False)
+        o1, n1 = g.csr()
+        o2, n2 = h.csr()
+        assert np.array_equal(o1, o2) and np.array_equal(n1, n2)
+
+
+def test_degree_order_is_a_relabelling():
+    g0 = W.graph_c1(False)
+    g1 = W.graph_c1(True)
+    perm = g1.perm()
+    o0, n0 = g0.csr()
+    o1, n1 = g1.csr()
+    deg1 = np.diff(o1)
+    assert np.all(np.diff(deg1.astype(np.int64)) >= 0)
+    e0 = {(min(a, b), max(a, b)) for v in range(g0.n) for a, b in [(v, u) for u in n0[o0[v]:o0[v + 1]]]}
+    e1 = {(min(perm[v], perm[u]), max(perm[v], perm[u])) for v in range(g1.n) for u in n1[o1[v]:o1[v + 1]]}
+    assert e0 == e1
+
+
+def test_generators_are_deterministic():
+    a, b = W.graph_c2(), W.graph_c2()
+    oa, na = a.csr()
+    ob, nb = b.csr()
+    assert np.array_equal(na, nb)
+    assert (a.n, a.m) == (7115, 100762)
+    r = W.graph_rmat(12, 3)
+    o, n = r.csr()
+    h = hashlib.sha256(o.tobytes() + n.tobytes()).hexdigest()
+    r2 = W.graph_rmat(12, 3)
+    o2, n2 = r2.csr()
+    assert h == hashlib.sha256(o2.tobytes() + n2.tobytes()).hexdigest()
+
+
+def test_bad_inputs_raise(tmp_path):
+    bad = tmp_path / "bad.txt"
+    bad.write_text("3 5\n0 1\n")
+    with pytest.raises(sge.SgeError):
+        sge.Graph.load(str(bad))
+    with pytest.raises(sge.SgeError):
+        sge.Pattern.load(str(tmp_path / "missing.txt"))
+    with pytest.raises(sge.SgeError):
+        sge.Pattern.from_edges(9, [(0, 1)])
+    disc = sge.Pattern.from_edges(4, [(0, 1), (2, 3)])
+    with pytest.raises(sge.SgeError):
+        sge.build_plan(disc)
+    with pytest.raises(sge.SgeError):
+        sge.build_plan(W.named("house"), None, 0, [0, 4, 1, 2, 3])  # level 1 not adjacent to level 0
+
+
+def test_count_without_gpu_fails_loudly():
+    import ctypes
+    try:
+        import torch
+        has_gpu = torch.cuda.is_available()
+    except Exception:
+        has_gpu = False
+    if has_gpu:
+        pytest.skip("GPU present")
+    g = W.graph_c1()
+    with pytest.raises(sge.SgeError):
+        sge.count(g, sge.build_plan(W.named("diamond"), g))

@@ -1,0 +1,186 @@
+"""Pins the CPU oracle (oracle/sge_oracle.c) before anything is compared with it.
+
+Checks: primitives against numpy's sorted-set routines; choosec against exact
+integer arithmetic and a hand-computed wrapping case; whole counts against brute
+force (all injective homomorphisms / |Aut|), K_n closed forms, the diamond
+formula, and the reference binary's golden counts (tests/golden/*.json).
+"""
+import itertools
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+import sge
+import sge.workloads as W
+from oracle import oracle as O
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def load_golden(name):
+    path = os.path.join(GOLDEN, name + ".json")
+    if not os.path.exists(path):
+        pytest.skip(f"golden fixture {name}.json not generated")
+    with open(path) as f:
+        return json.load(f)
+
+
+# ---------------------------------------------------------------- primitives
+@pytest.mark.parametrize("seed", range(6))
+def test_clamps_match_searchsorted(seed):
+    rng = np.random.default_rng(seed)
+    for n in [0, 1, 5, 31, 32, 33, 100, 1000]:
+        a = np.unique(rng.integers(0, 3 * n + 3, size=n)).astype(np.uint32)
+        for key in list(rng.integers(0, 3 * n + 4, size=8)) + [0, 2**32 - 1]:
+            key = int(key)
+            assert O.set_begin_pos(a, key) == np.searchsorted(a, key, side="right")
+            assert O.set_end_pos(a, key) == np.searchsorted(a, key, side="left")
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_intersect_matches_numpy(seed):
+    rng = np.random.default_rng(100 + seed)
+    for na, nb in [(0, 5), (5, 0), (7, 7), (10, 600), (600, 10), (64, 64), (1, 5000), (300, 301)]:
+        a = np.unique(rng.integers(0, 2000, size=na)).astype(np.uint32)
+        b = np.unique(rng.integers(0, 2000, size=nb)).astype(np.uint32)
+        assert np.array_equal(O.intersect(a, b), np.intersect1d(a, b))
+    a = np.arange(0, 1000, dtype=np.uint32)
+    assert np.array_equal(O.intersect(a, a), a)
+    assert len(O.intersect(np.arange(0, 100, 2, dtype=np.uint32), np.arange(1, 100, 2, dtype=np.uint32))) == 0
+
+
+def test_choosec_exact_and_wrap():
+    for n in range(0, 40):
+        for k in range(0, 7):
+            assert O.choosec(n, k) == (math.comb(n, k) if k <= n else 0)
+    # reference arithmetic: nom = n(n-1)...(n-k+1) mod 2^64, then / k!
+    n, k = 200000, 4
+    nom = 1
+    for i in range(k):
+        nom = (nom * (n - i)) % 2**64
+    assert O.choosec(n, k) == nom // 24
+    assert O.choosec(n, k) != math.comb(n, k)  # the wrap is real at this size
+
+
+# ---------------------------------------------------------------- whole counts
+def random_graph(n, p, seed):
+    rng = np.random.default_rng(seed)
+    edges = [(i, j) for i in range(n) for j in range(i + 1, n) if rng.random() < p]
+    return sge.Graph.from_edges(n, edges if edges else [(0, 1)])
+
+
+def small_patterns():
+    pats = []
+    for name in ["triangle", "diamond", "house", "near5clique", "k4"]:
+        pats.append(W.named(name))
+    pats += [sge.Pattern.from_edges(3, [(0, 1), (1, 2)]),                   # wedge
+             sge.Pattern.from_edges(4, [(0, 1), (1, 2), (2, 3), (3, 0)]),   # 4-cycle
+             sge.Pattern.from_edges(4, [(0, 1), (0, 2), (0, 3)]),           # star
+             sge.Pattern.from_edges(5, [(0, 1), (0, 2), (0, 3), (0, 4)]),   # 4-star (twin group)
+             sge.Pattern.from_edges(5, [(0, 1), (1, 2), (2, 3), (3, 4), (4, 0)]),  # 5-cycle
+             sge.Pattern.from_edges(5, [(0, 1), (0, 2), (1, 2), (2, 3), (2, 4)])]  # bowtie-ish
+    return pats
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_oracle_vs_brute_force(seed):
+    g = random_graph(14, 0.45, seed)
+    off, nbr = g.csr()
+    for p in small_patterns():
+        info = p.info()
+        brute = O.count_brute(p.k, p.c.adj, off, nbr)
+        for mode in (sge.MODE_WCOJ, sge.MODE_EXPLORE, sge.MODE_IEP):
+            plan = sge.build_plan(p, g, mode)
+            want = brute if info.single_bag else brute // info.aut_size
+            assert O.count(plan, off, nbr) == want, (p, mode)
+
+
+def test_oracle_every_order_agrees():
+    g = random_graph(16, 0.4, 9)
+    off, nbr = g.csr()
+    for p in [W.named("house"), W.named("diamond"), sge.Pattern.from_edges(5, [(0, 1), (0, 2), (0, 3), (0, 4)])]:
+        counts = set()
+        for o in p.orders():
+            plan = sge.build_plan(p, g, sge.MODE_WCOJ, o)
+            counts.add(O.count(plan, off, nbr))
+        assert len(counts) == 1
+
+
+@pytest.mark.parametrize("n", [6, 7, 8])
+def test_closed_form_on_cliques(n):
+    kn = sge.Graph.from_edges(n, [(i, j) for i in range(n) for j in range(i + 1, n)])
+    off, nbr = kn.csr()
+    for p in W.six_vertex_set()[:8] + W.seven_vertex_set()[:6] + small_patterns():
+        if p.k > n:
+            continue
+        info = p.info()
+        emb = math.perm(n, p.k)
+        want = emb if info.single_bag else emb // info.aut_size
+        plan = sge.build_plan(p, kn, sge.MODE_IEP)
+        assert O.count(plan, off, nbr) == want
+
+
+def test_diamond_formula():
+    g = W.graph_c1()
+    off, nbr = g.csr()
+    total = 0
+    for v in range(g.n):
+        for e in range(off[v], off[v + 1]):
+            u = nbr[e]
+            if u > v:
+                c = len(np.intersect1d(nbr[off[v]:off[v + 1]], nbr[off[u]:off[u + 1]]))
+                total += c * (c - 1) // 2
+    plan = sge.build_plan(W.named("diamond"), g)
+    assert O.count(plan, off, nbr) == total
+
+
+def test_root_ranges_are_additive():
+    g = W.graph_c1()
+    off, nbr = g.csr()
+    plan = sge.build_plan(W.named("house"), g)
+    whole = O.count(plan, off, nbr)
+    cuts = [0, 100, 555, 1200, int(off[-1])]
+    parts = sum(O.count(plan, off, nbr, begin=a, end=b) for a, b in zip(cuts, cuts[1:]))
+    assert parts == whole
+
+
+# ---------------------------------------------------------------- reference goldens
+def test_golden_c1():
+    gold = load_golden("c1")
+    g = W.graph_c1()
+    assert (g.n, g.m) == (gold["n"], gold["m"])
+    off, nbr = g.csr()
+    for name, rec in gold["patterns"].items():
+        p = W.named(name)
+        for mode in (sge.MODE_WCOJ, sge.MODE_EXPLORE, sge.MODE_IEP):
+            plan = sge.build_plan(p, g, mode)
+            assert O.count(plan, off, nbr) == rec["wcoj"]["count"], (name, mode)
+        assert p.info().num_bags == rec["wcoj"]["bags"], name
+
+
+def test_golden_c2_near5clique():
+    gold = load_golden("c2")
+    g = W.graph_c2()
+    assert (g.n, g.m) == (gold["n"], gold["m"])
+    off, nbr = g.csr()
+    rec = gold["patterns"]["near5clique"]
+    p = W.named("near5clique")
+    for mode, key in ((sge.MODE_WCOJ, "wcoj"), (sge.MODE_EXPLORE, "explore"), (sge.MODE_IEP, "iep")):
+        plan = sge.build_plan(p, g, mode)
+        assert O.count(plan, off, nbr) == rec[key]["count"]
+
+
+def test_golden_rmat7_sample():
+    gold = load_golden("rmat7")
+    g = W.graph_rmat(10, 7)
+    assert (g.n, g.m) == (gold["n"], gold["m"])
+    off, nbr = g.csr()
+    pats = W.seven_vertex_set()
+    for rec in gold["patterns"][::9]:
+        p = pats[rec["index"]]
+        assert p.edges() == [tuple(e) for e in rec["edges"]]
+        plan = sge.build_plan(p, g, sge.MODE_IEP)
+        assert O.count(plan, off, nbr) == rec["wcoj"]["count"], rec["index"]
