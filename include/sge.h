@@ -200,6 +200,31 @@ int sge_batch_create(sge_graph *g, const sge_plan *plans, uint32_t nplans, const
 int sge_batch_run(sge_batch *b, uint64_t *counts, sge_stats *stats);
 void sge_batch_destroy(sge_batch *b);
 
+/* ------------------------------------------------------ learned cost model
+ * 64 -> 128 -> 128 -> 1 ReLU MLP; hidden layers on BF16 MFMA with FP32
+ * accumulation, FP32 master weights trained by Adam.  No reference counterpart:
+ * it replaces the order choice of plan/optimizer.cpp:441-577 (optCostOrder),
+ * which remains the fallback.  Flat FP32 parameter layout:
+ * W1[64][128] | b1[128] | W2[128][128] | b2[128] | w3[128] | b3. */
+typedef struct sge_mlp sge_mlp;
+int sge_mlp_num_params(void);
+int sge_mlp_input_dim(void);
+int sge_mlp_create(uint64_t seed, sge_mlp **out);
+int sge_mlp_set_params(sge_mlp *m, const float *params, size_t n);
+int sge_mlp_get_params(sge_mlp *m, float *params_out, size_t n);
+/* y[N] = f(X[N][64]) */
+int sge_mlp_forward(sge_mlp *m, const float *X, uint32_t N, float *y);
+/* MSE loss gradients w.r.t. the flat parameters (no update). */
+int sge_mlp_gradients(sge_mlp *m, const float *X, const float *target, uint32_t N, float *grads, float *loss);
+/* forward + backward + Adam step */
+int sge_mlp_train_step(sge_mlp *m, const float *X, const float *target, uint32_t N, float lr, float *loss);
+void sge_mlp_destroy(sge_mlp *m);
+/* The model's 64 input features for one (pattern, order): pattern shape, plan
+ * structure per level, graph moments, analytic cost, and a Knuth random-walk
+ * estimate (nsamples random root edges, seeded) of nodes and list work per level. */
+int sge_order_features(const sge_pattern *p, const sge_graph *g, const uint32_t *order, uint32_t mode,
+                       uint32_t nsamples, uint64_t seed, float *feat64);
+
 /* Device info for reporting. */
 int sge_device_info(int *num_cu, int *num_xcd_hint, char *name, size_t cap);
 

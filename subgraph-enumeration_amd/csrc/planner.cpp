@@ -25,6 +25,7 @@
 #include <cstdio>
 #include <cstring>
 #include <functional>
+#include <iterator>
 #include <numeric>
 #include <set>
 
@@ -97,31 +98,50 @@ static void symmetry_masks(const Pattern &p, const int *order, const std::vector
     }
 }
 
+// Analytic cost of one order, following the executor's structure: root tasks
+// are directed edges; levels 2..s are materialised per parent node (work = the
+// clamped parent lists); the last level is counted for all siblings at once,
+// with its sibling-independent part F formed once per node of level s-1.
+// Sizes use degree moments (first list of a node reached through an edge has
+// the size-biased degree E[d^2]/E[d]) and a sampled wedge-closure rate q for
+// every further list in an intersection; a symmetry clamp halves a set.
 double order_cost(const Pattern &p, const GraphSummary &s, const int *order, const std::vector<std::vector<int>> &aut) {
     int k = p.k;
+    if (k <= 2) return 1.0;
     uint32_t lower[SGE_MAX_K];
     symmetry_masks(p, order, aut, lower);
-    int level_of[SGE_MAX_K];
-    for (int i = 0; i < k; ++i) level_of[order[i]] = i;
-    double q = std::min(0.9, std::max(1e-4, s.tri_per_wedge));
-    double card = s.n;  // partial matches after level 0
-    double cost = 0;
-    for (int i = 1; i < k; ++i) {
-        uint32_t par = 0;
+    uint32_t par[SGE_MAX_K] = {0};
+    for (int i = 1; i < k; ++i)
         for (int j = 0; j < i; ++j)
-            if (p.edge(order[i], order[j])) par |= 1u << j;
-        int np = __builtin_popcount(par);
-        if (np == 0) return 1e300;  // disconnected prefix
-        double list = (i == 1) ? s.avg_deg : s.d2;
-        // each parent contributes its list to the intersection work
-        double work = (np == 1) ? std::log2(list + 2.0) : np * list;
-        if (i < k - 1 || np > 1) cost += card * work;
-        double size = list * std::pow(q, np - 1);
-        if (lower[i]) size *= 0.5;
-        card *= std::max(size, 1e-9);
-        // level i's materialisation writes its candidates
-        if (i < k - 1) cost += card * 0.25;
+            if (p.edge(order[i], order[j])) par[i] |= 1u << j;
+    for (int i = 1; i < k; ++i)
+        if (!par[i]) return 1e300;
+    double q = std::min(0.9, std::max(1e-4, s.tri_per_wedge));
+    double d2 = std::max(1.0, s.d2);
+    auto set_size = [&](int i) {
+        int np = __builtin_popcount(par[i]);
+        double sz = d2 * std::pow(q, np - 1);
+        if (lower[i]) sz *= 0.5;
+        return std::max(sz, 1e-6);
+    };
+    double card = 2.0 * s.m * ((lower[1] & 1u) ? 0.5 : 1.0);  // nodes at level 1
+    double cost = card * 2.0;                                 // root task overhead
+    int last = k - 1, sib = k - 2;
+    for (int i = 2; i <= sib; ++i) {
+        int np = __builtin_popcount(par[i]);
+        cost += card * (np * d2 * (lower[i] ? 0.6 : 1.0) + 8.0);  // materialise level i
+        card *= set_size(i);
     }
+    // last level: F once per node of level sib-1, then one probe loop per sibling
+    uint32_t fpar = par[last] & ~(1u << sib);
+    bool s_par = (par[last] >> sib) & 1u;
+    double parent_nodes = card / std::max(set_size(sib), 1e-6);
+    int nf = __builtin_popcount(fpar);
+    double fsize = nf ? d2 * std::pow(q, nf - 1) : 0.0;
+    if (nf) cost += parent_nodes * nf * d2;
+    double per_sib = s_par ? (nf ? std::min(fsize, d2) * std::log2(std::max(fsize, d2) + 2.0) : std::log2(d2 + 2.0))
+                           : std::log2(fsize + 2.0);
+    cost += card * (per_sib + 4.0);
     return cost;
 }
 
@@ -359,4 +379,137 @@ extern "C" int sge_plan_text(const sge_plan *pl, char *buf, size_t cap) {
     }
     std::memcpy(buf, s.c_str(), s.size() + 1);
     return SGE_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Features of one (pattern, order) for the learned cost model.
+//
+// 64 floats: pattern shape (k, m, |Aut|, fhw, bag count, degree sequence),
+// per-level plan structure (parents, clamps, checks, reuse, analytic size),
+// graph moments, the analytic cost, and sampled statistics: a Knuth random-walk
+// estimate over `nsamples` random root edges of the number of search-tree nodes
+// and of the list work at every level (a seeded restatement of the reference's
+// sampled cardinalities, plan/estimator.cpp:562-661).
+namespace sge {
+
+static void knuth_estimate(const HostGraph &g, const sge_plan &pl, uint32_t nsamples, uint64_t seed, double *card,
+                           double *work) {
+    int k = pl.k;
+    for (int i = 0; i < SGE_MAX_K; ++i) card[i] = work[i] = 0;
+    if (g.nnz == 0 || k < 3) return;
+    Rng rng(seed);
+    std::vector<uint32_t> cur, nxt;
+    cur.reserve(g.max_degree + 1);
+    nxt.reserve(g.max_degree + 1);
+    for (uint32_t s = 0; s < nsamples; ++s) {
+        uint64_t e = rng.below(g.nnz);
+        uint32_t v0 = (uint32_t)(std::upper_bound(g.off.begin(), g.off.end(), (uint32_t)e) - g.off.begin()) - 1;
+        uint32_t v1 = g.nbr[e];
+        if ((pl.lower[1] & 1u) && !(v1 > v0)) continue;
+        uint32_t phi[SGE_MAX_K] = {v0, v1};
+        double w = (double)g.nnz;
+        card[1] += w;
+        for (int d = 2; d < k; ++d) {
+            uint32_t lo = 0, hi = 0xffffffffu;
+            bool has_lo = false, has_hi = false;
+            for (int j = 0; j < d; ++j) {
+                if ((pl.lower[d] >> j) & 1u) { lo = std::max(lo, phi[j]); has_lo = true; }
+                if ((pl.upper[d] >> j) & 1u) { hi = std::min(hi, phi[j]); has_hi = true; }
+            }
+            bool first = true;
+            double wk = 0;
+            for (int j = 0; j < d; ++j) {
+                if (!((pl.parent[d] >> j) & 1u)) continue;
+                const uint32_t *row = g.nbr.data() + g.off[phi[j]];
+                uint32_t n = g.deg(phi[j]);
+                const uint32_t *b = has_lo ? std::upper_bound(row, row + n, lo) : row;
+                const uint32_t *en = has_hi ? std::lower_bound(b, row + n, hi) : row + n;
+                if (en < b) en = b;
+                wk += (double)(en - b);
+                if (first) {
+                    cur.assign(b, en);
+                    first = false;
+                } else {
+                    nxt.clear();
+                    std::set_intersection(cur.begin(), cur.end(), b, en, std::back_inserter(nxt));
+                    cur.swap(nxt);
+                }
+            }
+            // injectivity
+            size_t wpos = 0;
+            for (size_t i = 0; i < cur.size(); ++i) {
+                bool bad = false;
+                for (int j = 0; j < d; ++j)
+                    if (((pl.check[d] >> j) & 1u) && phi[j] == cur[i]) bad = true;
+                if (!bad) cur[wpos++] = cur[i];
+            }
+            cur.resize(wpos);
+            work[d] += w * wk;
+            card[d] += w * (double)cur.size();
+            if (cur.empty() || d == k - 1) break;
+            w *= (double)cur.size();
+            phi[d] = cur[rng.below(cur.size())];
+        }
+    }
+    for (int i = 0; i < SGE_MAX_K; ++i) {
+        card[i] /= std::max(1u, nsamples);
+        work[i] /= std::max(1u, nsamples);
+    }
+}
+
+int order_features(const Pattern &p, const HostGraph *g, const uint32_t *order, uint32_t mode, uint32_t nsamples,
+                   uint64_t seed, float *f) {
+    sge_plan pl;
+    int rc = build_plan(p, g, mode, order, pl);
+    if (rc) return rc;
+    for (int i = 0; i < 64; ++i) f[i] = 0.f;
+    auto aut = automorphisms(p);
+    Decomposition d = decompose(p, aut);
+    GraphSummary s = summarize(g);
+    int k = p.k;
+    f[0] = k / 8.f;
+    f[1] = p.num_edges() / 28.f;
+    f[2] = (float)(std::log2((double)aut.size()) / 16.0);
+    f[3] = (float)(d.fhw / 4.0);
+    f[4] = d.num_bags / 8.f;
+    std::vector<int> deg;
+    for (int v = 0; v < k; ++v) deg.push_back(__builtin_popcount(p.adj[v]));
+    std::sort(deg.rbegin(), deg.rend());
+    for (int i = 0; i < k && i < 7; ++i) f[5 + i] = deg[i] / 7.f;
+    double q = std::min(0.9, std::max(1e-4, s.tri_per_wedge));
+    for (int i = 1; i < k && i < 8; ++i) {
+        int base = 12 + (i - 1) * 5;
+        int np = __builtin_popcount(pl.parent[i]);
+        f[base + 0] = np / 7.f;
+        f[base + 1] = pl.lower[i] ? 1.f : 0.f;
+        f[base + 2] = __builtin_popcount(pl.check[i]) / 7.f;
+        f[base + 3] = pl.reuse[i] >= 0 ? 1.f : 0.f;
+        double sz = (i == 1 ? s.avg_deg : s.d2) * std::pow(q, np - 1) * (pl.lower[i] ? 0.5 : 1.0);
+        f[base + 4] = (float)(std::log2(std::max(sz, 1e-6)) / 20.0);
+    }
+    double card[SGE_MAX_K], work[SGE_MAX_K];
+    if (g) knuth_estimate(*g, pl, nsamples, seed, card, work);
+    else for (int i = 0; i < SGE_MAX_K; ++i) card[i] = work[i] = 0;
+    double total = 0;
+    for (int i = 1; i < 8; ++i) {
+        f[46 + i] = (float)(std::log2(1.0 + card[i]) / 48.0);
+        f[53 + i] = (float)(std::log2(1.0 + work[i]) / 48.0);
+        total += work[i] + card[i];
+    }
+    f[61] = (float)(std::log2(1.0 + total) / 48.0);
+    f[62] = (float)(std::log2(1.0 + std::max(0.0, pl.cost)) / 64.0);
+    f[63] = (float)(std::log2(1.0 + s.m) / 32.0);
+    f[46] = pl.tail / 8.f;
+    return SGE_OK;
+}
+
+}  // namespace sge
+
+extern "C" int sge_order_features(const sge_pattern *cp, const sge_graph *g, const uint32_t *order, uint32_t mode,
+                                  uint32_t nsamples, uint64_t seed, float *feat) {
+    if (!cp || !order || !feat) {
+        set_error("sge_order_features: null argument");
+        return SGE_EINVAL;
+    }
+    return order_features(from_c(*cp), g ? &g->h : nullptr, order, mode, nsamples, seed, feat);
 }

@@ -98,9 +98,9 @@ def section_c3():
 
 
 def section_rmat7():
-    g = W.graph_rmat(10, 7, False)
-    gp = graph_file("rmat10", g)
-    res = {"graph": "rmat(scale=10, ef=16, a=.57, b=.19, c=.19, seed=7)", "n": g.n, "m": g.m, "patterns": []}
+    g = W.graph_rmat(8, 7, False)
+    gp = graph_file("rmat8", g)
+    res = {"graph": "rmat(scale=8, ef=16, a=.57, b=.19, c=.19, seed=7)", "n": g.n, "m": g.m, "patterns": []}
     for i, p in enumerate(W.seven_vertex_set()):
         pp = os.path.join(TMP, f"seven{i}.txt")
         write_pattern(p.k, p.edges(), pp)

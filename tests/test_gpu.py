@@ -120,7 +120,7 @@ def test_batched_equals_plain_and_oracle():
 
 
 def test_hipgraph_batch_replays():
-    g = W.graph_c2()
+    g = W.graph_c1()
     pats = W.six_vertex_set()[:5]
     plans = [sge.build_plan(p, g, sge.MODE_IEP) for p in pats]
     want, _ = sge.count(g, plans)

@@ -175,7 +175,7 @@ def test_golden_c2_near5clique():
 
 def test_golden_rmat7_sample():
     gold = load_golden("rmat7")
-    g = W.graph_rmat(10, 7)
+    g = W.graph_rmat(8, 7)
     assert (g.n, g.m) == (gold["n"], gold["m"])
     off, nbr = g.csr()
     pats = W.seven_vertex_set()
