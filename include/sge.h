@@ -186,6 +186,11 @@ int sge_count(sge_graph *g, const sge_plan *plans, uint32_t nplans, uint32_t fla
  * additive over disjoint ranges).  Used for sampled work logging. */
 int sge_count_range(sge_graph *g, const sge_plan *plan, uint64_t task_begin, uint64_t task_end,
                     const sge_tuning *tune, uint64_t *count, sge_stats *stats);
+/* Count over the strided root-task sample {offset, offset+stride, ...} of each
+ * plan, one launch per plan (the plain executor); per-plan stats.  Used to log
+ * (order, measured work) pairs for the learned cost model. */
+int sge_count_sampled(sge_graph *g, const sge_plan *plans, uint32_t nplans, uint64_t stride, uint64_t offset,
+                      const sge_tuning *tune, uint64_t *counts, sge_stats *per_plan);
 /* Number of root tasks (directed edges) of a plan over g. */
 int sge_num_tasks(const sge_graph *g, const sge_plan *plan, uint64_t *n);
 

@@ -81,7 +81,7 @@ struct KParams {
     uint64_t nnz;
     const DPlan *plans;
     uint32_t nplans;
-    uint64_t task_begin, task_end;
+    uint64_t task_begin, task_end, stride;  // tasks are CSR positions task_begin + i*stride < task_end
     uint32_t chunk;
     uint64_t nitems;
     unsigned long long *queue;
@@ -101,20 +101,26 @@ struct Src {
 };
 
 // Per-wave LDS state.  Uniform values are written by every lane (same value),
-// so no cross-lane ordering is needed for them.
+// so no cross-lane ordering is needed for them.  Candidate sets are
+// descriptors: a clamped CSR row range when a level has one source list (no
+// copy; injectivity checks are then applied when the set is read), otherwise a
+// range of the wave's scratch buffer for that level.
 struct WaveLds {
     DPlan pl;
     uint32_t phi[MAXK];
-    uint32_t len[MAXK];
     uint32_t pos[MAXK];
-    Src pre[MAXK + 1];  // exploration: pre-intersected parent lists (index MAXK = F)
+    Src cand[MAXK];      // candidate set per level
+    Src pre[MAXK + 1];   // exploration: parent intersections formed early ([MAXK] = F)
+    Src srcs[MAXK + 2];  // sources of the intersection being formed
+    uint32_t cur[MAXK + 2];  // per-source cursor of the running merge
+    uint32_t chk[MAXK];  // check images of the intersection being formed
     // flattened sibling batch
     uint32_t pre_sum[SIB_BLOCK + 1];
-    uint32_t drv[SIB_BLOCK];   // driver start (index into drv array base)
+    uint32_t drv[SIB_BLOCK];   // driver start
     uint32_t tlo[SIB_BLOCK];   // target range
     uint32_t thi[SIB_BLOCK];
-    uint32_t cnt[SIB_BLOCK];   // per-sibling counts (IEP)
-    uint32_t flag[SIB_BLOCK];  // bit0: driver is the CSR row (target F), else driver F (target row)
+    uint32_t cnt[SIB_BLOCK];   // per-sibling counts
+    uint32_t flag[SIB_BLOCK];  // 1: driver is the sibling's CSR row (target F), 0: driver F (target row)
 };
 
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
@@ -124,12 +130,10 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_wave_barrier();
 }
 
-__device__ __forceinline__ uint32_t uni(uint32_t x) { return __builtin_amdgcn_readfirstlane(x); }
-
 // first index in [lo, hi) with a[i] >= key
 __device__ __forceinline__ uint32_t lower_bound(const uint32_t *a, uint32_t lo, uint32_t hi, uint32_t key) {
     while (lo < hi) {
-        uint32_t mid = (lo + hi) >> 1;
+        uint32_t mid = lo + ((hi - lo) >> 1);
         if (a[mid] < key) lo = mid + 1; else hi = mid;
     }
     return lo;
@@ -137,7 +141,7 @@ __device__ __forceinline__ uint32_t lower_bound(const uint32_t *a, uint32_t lo, 
 // first index in [lo, hi) with a[i] > key
 __device__ __forceinline__ uint32_t upper_bound(const uint32_t *a, uint32_t lo, uint32_t hi, uint32_t key) {
     while (lo < hi) {
-        uint32_t mid = (lo + hi) >> 1;
+        uint32_t mid = lo + ((hi - lo) >> 1);
         if (a[mid] <= key) lo = mid + 1; else hi = mid;
     }
     return lo;
@@ -148,8 +152,8 @@ __device__ __forceinline__ bool contains(const uint32_t *a, uint32_t lo, uint32_
     return p < hi && a[p] == key;
 }
 
-// Wave-cooperative: clamp [b,e) of a sorted list to (lo, hi) exclusive bounds.
-// Executed redundantly by all lanes (uniform result).
+// Clamp [b,e) of a sorted list to the open interval (lo, hi) (setBeginPos /
+// setEndPos).  Every lane computes the same result.
 __device__ __forceinline__ void clamp_range(const uint32_t *a, uint32_t &b, uint32_t &e, bool has_lo, uint32_t lo,
                                             bool has_hi, uint32_t hi) {
     if (has_lo) b = upper_bound(a, b, e, lo);
@@ -172,7 +176,7 @@ struct Acc {
     unsigned long long count, inter, bytes, nodes, tasks;
 };
 
-// Bounds of level d from the images of the levels in the masks.
+// Bounds from the images of the levels in the masks.
 __device__ __forceinline__ void level_bounds(const WaveLds &w, uint32_t lmask, uint32_t umask, bool &has_lo,
                                              uint32_t &lo, bool &has_hi, uint32_t &hi) {
     has_lo = lmask != 0;
@@ -183,74 +187,83 @@ __device__ __forceinline__ void level_bounds(const WaveLds &w, uint32_t lmask, u
     for (uint32_t m = umask; m; m &= m - 1) hi = min(hi, w.phi[__builtin_ctz(m)]);
 }
 
-// Intersect the lists in srcs[0..ns) (already clamped), drop check images, write
-// survivors to out (if non-null).  Returns the (uniform) survivor count.
-__device__ uint32_t wave_intersect(Src *srcs, int ns, const uint32_t *chk, int nchk, uint32_t *out, Acc &acc) {
-    // shortest list drives
+// Load the check images of `mask` into w.chk; returns their number.
+__device__ __forceinline__ int load_checks(WaveLds &w, uint32_t mask) {
+    int n = 0;
+    for (uint32_t m = mask; m; m &= m - 1) w.chk[n++] = w.phi[__builtin_ctz(m)];
+    return n;
+}
+
+__device__ __forceinline__ bool is_checked(const WaveLds &w, uint32_t mask, uint32_t x) {
+    bool hit = false;
+    for (uint32_t m = mask; m; m &= m - 1) hit |= (w.phi[__builtin_ctz(m)] == x);
+    return hit;
+}
+
+// Intersect the ns sources in w.srcs (already clamped), drop the nchk check
+// images in w.chk, and write survivors to out.  The shortest source drives; for
+// every other source each lane binary-searches its element within a window that
+// starts at the previous chunk's last position.  Survivors are compacted with a
+// wave ballot + popcount prefix.  Returns the survivor count (uniform).
+__device__ uint32_t wave_intersect(WaveLds &w, int ns, int nchk, uint32_t *out, Acc &acc) {
+    if (ns <= 0) return 0;
     int drv = 0;
-    for (int i = 1; i < ns; ++i)
-        if (srcs[i].e - srcs[i].b < srcs[drv].e - srcs[drv].b) drv = i;
-    const uint32_t *D = srcs[drv].ptr;
-    uint32_t db = srcs[drv].b, de = srcs[drv].e;
     uint64_t bytes = 0;
-    for (int i = 0; i < ns; ++i) bytes += 4ull * (srcs[i].e - srcs[i].b);
+    for (int i = 0; i < ns; ++i) {
+        uint32_t len = w.srcs[i].e - w.srcs[i].b;
+        bytes += 4ull * len;
+        if (len < w.srcs[drv].e - w.srcs[drv].b) drv = i;
+        w.cur[i] = w.srcs[i].b;
+    }
     acc.bytes += bytes;
     acc.inter += (ns > 1) ? (ns - 1) : 0;
-    uint32_t lo[MAXK];
-    for (int i = 0; i < ns; ++i) lo[i] = srcs[i].b;
-    uint32_t total = 0;
+    const uint32_t *D = w.srcs[drv].ptr;
+    const uint32_t db = w.srcs[drv].b, de = w.srcs[drv].e;
     const int lane = lane_id();
-    if (db >= de) return 0;
+    uint32_t total = 0;
     for (uint32_t base = db; base < de; base += WAVE) {
         uint32_t idx = base + lane;
         bool ok = idx < de;
         uint32_t x = ok ? D[idx] : 0xffffffffu;
         for (int i = 0; i < ns; ++i) {
             if (i == drv) continue;
-            uint32_t p = lower_bound(srcs[i].ptr, lo[i], srcs[i].e, x);
-            ok = ok && p < srcs[i].e && srcs[i].ptr[p] == x;
-            // next chunk starts beyond the last lane's key
+            const uint32_t *a = w.srcs[i].ptr;
+            uint32_t e = w.srcs[i].e;
+            uint32_t p = lower_bound(a, w.cur[i], e, x);
+            ok = ok && p < e && a[p] == x;
             uint32_t last = __shfl(p, WAVE - 1);
-            lo[i] = last;
+            w.cur[i] = last;
         }
-        for (int c = 0; c < nchk; ++c) ok = ok && (x != chk[c]);
+        for (int c = 0; c < nchk; ++c) ok = ok && (x != w.chk[c]);
         uint64_t mask = __ballot(ok);
-        if (out) {
-            uint32_t pre = __popcll(mask & ((1ull << lane) - 1ull));
-            if (ok) out[total + pre] = x;
-        }
+        uint32_t pre = __popcll(mask & ((1ull << lane) - 1ull));
+        if (ok) out[total + pre] = x;
         total += __popcll(mask);
     }
-    if (out) {
-        acc.bytes += 4ull * total;
-        wave_sync();  // other lanes read the survivors next
-    }
+    acc.bytes += 4ull * total;
+    wave_sync();  // other lanes read the survivors next
     return total;
 }
 
-__device__ __forceinline__ const uint32_t *buf_ptr(const KParams &P, uint32_t *wave_base, int i) {
+__device__ __forceinline__ uint32_t *buf_ptr(const KParams &P, uint32_t *wave_base, int i) {
     return wave_base + (uint64_t)i * P.buf_stride;
 }
 
-// Build the list descriptors for the parent intersection of level d; apply the
-// level's bounds.  Returns the number of lists.
-__device__ int gather_sources(const KParams &P, WaveLds &w, uint32_t *wb, int d, uint32_t pmask, int reuse,
-                              bool use_pre, bool has_lo, uint32_t lo, bool has_hi, uint32_t hi, Src *srcs, Acc &acc) {
+// Sources of the parent intersection of level d into w.srcs, bounds applied.
+__device__ int gather_sources(const KParams &P, WaveLds &w, int d, uint32_t pmask, int reuse, bool use_pre,
+                              bool has_lo, uint32_t lo, bool has_hi, uint32_t hi, Acc &acc) {
     int ns = 0;
     if (use_pre) {
         Src s = w.pre[d];
         clamp_range(s.ptr, s.b, s.e, has_lo, lo, has_hi, hi);
-        srcs[ns++] = s;
+        w.srcs[ns++] = s;
         return ns;
     }
     if (reuse >= 0) {
-        Src s;
-        s.ptr = buf_ptr(P, wb, reuse);
-        s.b = 0;
-        s.e = w.len[reuse];
+        Src s = w.cand[reuse];
         pmask &= ~w.pl.parent[reuse];
         clamp_range(s.ptr, s.b, s.e, has_lo, lo, has_hi, hi);
-        srcs[ns++] = s;
+        w.srcs[ns++] = s;
     }
     for (uint32_t m = pmask; m; m &= m - 1) {
         uint32_t v = w.phi[__builtin_ctz(m)];
@@ -259,71 +272,65 @@ __device__ int gather_sources(const KParams &P, WaveLds &w, uint32_t *wb, int d,
         s.b = P.off[v];
         s.e = P.off[v + 1];
         clamp_range(s.ptr, s.b, s.e, has_lo, lo, has_hi, hi);
-        srcs[ns++] = s;
+        w.srcs[ns++] = s;
         acc.bytes += 8;
     }
     return ns;
 }
 
-// Materialise the candidate set of level d into buffer d.
+// Form a set from w.srcs: alias the single source when allowed, else
+// intersect into the scratch buffer `dst`.
+__device__ Src form_set(WaveLds &w, int ns, int nchk, bool may_alias, uint32_t *dst, Acc &acc) {
+    if (ns <= 0) {
+        Src z;
+        z.ptr = dst;
+        z.b = z.e = 0;
+        return z;
+    }
+    if (ns == 1 && may_alias) {
+        acc.bytes += 4ull * (w.srcs[0].e - w.srcs[0].b);
+        return w.srcs[0];
+    }
+    Src r;
+    r.ptr = dst;
+    r.b = 0;
+    r.e = wave_intersect(w, ns, nchk, dst, acc);
+    return r;
+}
+
+// Candidate set of level d (checks of level d applied now when the set is
+// materialised, or when it is read if it aliases a CSR row).
 __device__ void materialize(const KParams &P, WaveLds &w, uint32_t *wb, int d, Acc &acc) {
     const DPlan &pl = w.pl;
     bool has_lo, has_hi;
     uint32_t lo, hi;
     level_bounds(w, pl.lower[d], pl.upper[d], has_lo, lo, has_hi, hi);
-    Src srcs[MAXK + 1];
     bool use_pre = pl.mode == SGE_MODE_EXPLORE;
-    int ns = gather_sources(P, w, wb, d, pl.parent[d], use_pre ? -1 : pl.reuse[d], use_pre, has_lo, lo, has_hi, hi,
-                            srcs, acc);
-    uint32_t chk[MAXK];
-    int nchk = 0;
-    for (uint32_t m = pl.check[d]; m; m &= m - 1) chk[nchk++] = w.phi[__builtin_ctz(m)];
-    uint32_t n = wave_intersect(srcs, ns, chk, nchk, const_cast<uint32_t *>(buf_ptr(P, wb, d)), acc);
-    w.len[d] = n;
+    int ns = gather_sources(P, w, d, pl.parent[d], use_pre ? -1 : pl.reuse[d], use_pre, has_lo, lo, has_hi, hi, acc);
+    int nchk = load_checks(w, pl.check[d]);
+    w.cand[d] = form_set(w, ns, nchk, true, buf_ptr(P, wb, d), acc);
     w.pos[d] = 0;
     acc.nodes += 1;
 }
 
-// Exploration: pre-intersect the parent lists of every level whose last parent
-// is level d (the F list of the last level included).  Unclamped, unfiltered.
+// Exploration: form the parent intersection of every level whose last parent
+// is level d (and of the last level's sibling-independent part F), unclamped.
 __device__ void explore_after(const KParams &P, WaveLds &w, uint32_t *wb, int d, Acc &acc) {
     const DPlan &pl = w.pl;
     for (uint32_t j = d + 1; j <= pl.sib; ++j) {
         if (pl.ready[j] != (uint32_t)d || j < 2) continue;
-        Src srcs[MAXK + 1];
-        int ns = gather_sources(P, w, wb, j, pl.parent[j], -1, false, false, 0, false, 0, srcs, acc);
-        if (ns == 1) {
-            w.pre[j] = srcs[0];
-        } else {
-            uint32_t *dst = const_cast<uint32_t *>(buf_ptr(P, wb, MAXK + j));
-            uint32_t n = wave_intersect(srcs, ns, nullptr, 0, dst, acc);
-            Src s;
-            s.ptr = dst;
-            s.b = 0;
-            s.e = n;
-            w.pre[j] = s;
-        }
+        int ns = gather_sources(P, w, j, pl.parent[j], -1, false, false, 0, false, 0, acc);
+        w.pre[j] = form_set(w, ns, 0, true, buf_ptr(P, wb, MAXK + j), acc);
     }
     if (pl.f_parent && pl.f_ready == (uint32_t)d) {
-        Src srcs[MAXK + 1];
-        int ns = gather_sources(P, w, wb, pl.last, pl.f_parent, -1, false, false, 0, false, 0, srcs, acc);
-        if (ns == 1) {
-            w.pre[MAXK] = srcs[0];
-        } else {
-            uint32_t *dst = const_cast<uint32_t *>(buf_ptr(P, wb, 2 * MAXK));
-            uint32_t n = wave_intersect(srcs, ns, nullptr, 0, dst, acc);
-            Src s;
-            s.ptr = dst;
-            s.b = 0;
-            s.e = n;
-            w.pre[MAXK] = s;
-        }
+        int ns = gather_sources(P, w, pl.last, pl.f_parent, -1, false, false, 0, false, 0, acc);
+        w.pre[MAXK] = form_set(w, ns, 0, true, buf_ptr(P, wb, 2 * MAXK), acc);
     }
 }
 
-// Count the last level for every sibling in buffer `sib` (or the single sibling
-// phi[sib] when sib == 1).  F = the last level's candidates that do not depend on
-// the sibling, filtered by every bound/check from levels below the sibling.
+// Count the last level for every sibling in cand[sib] (or the single sibling
+// phi[1] when sib == 1).  F = the last level's candidates that do not depend on
+// the sibling, with every bound/check from levels below the sibling applied.
 __device__ void count_last(const KParams &P, WaveLds &w, uint32_t *wb, Acc &acc) {
     const DPlan &pl = w.pl;
     const int s = pl.sib, L = pl.last;
@@ -333,44 +340,44 @@ __device__ void count_last(const KParams &P, WaveLds &w, uint32_t *wb, Acc &acc)
     bool has_lo, has_hi;
     uint32_t lo, hi;
     level_bounds(w, pl.lower[L] & below, pl.upper[L] & below, has_lo, lo, has_hi, hi);
-    uint32_t chk[MAXK];
-    int nchk = 0;
-    for (uint32_t m = pl.check[L] & below; m; m &= m - 1) chk[nchk++] = w.phi[__builtin_ctz(m)];
+    const uint32_t fchk_mask = pl.check[L] & below;
     const bool universe = pl.f_parent == 0;
-    const uint32_t *F = nullptr;
-    uint32_t fn = 0;
+    Src F;
+    F.ptr = nullptr;
+    F.b = F.e = 0;
     if (!universe) {
-        Src srcs[MAXK + 1];
         int ns;
         if (pl.mode == SGE_MODE_EXPLORE) {
             Src sp = w.pre[MAXK];
             clamp_range(sp.ptr, sp.b, sp.e, has_lo, lo, has_hi, hi);
-            srcs[0] = sp;
+            w.srcs[0] = sp;
             ns = 1;
         } else {
             int reuse = pl.reuse[L];
             if (reuse >= 0 && ((uint32_t)reuse >= (uint32_t)s || (pl.parent[reuse] & ~pl.f_parent))) reuse = -1;
-            ns = gather_sources(P, w, wb, L, pl.f_parent, reuse, false, has_lo, lo, has_hi, hi, srcs, acc);
+            ns = gather_sources(P, w, L, pl.f_parent, reuse, false, has_lo, lo, has_hi, hi, acc);
         }
-        uint32_t *dst = const_cast<uint32_t *>(buf_ptr(P, wb, 2 * MAXK + 1));
-        fn = wave_intersect(srcs, ns, chk, nchk, dst, acc);
-        F = dst;
+        int nchk = load_checks(w, fchk_mask);
+        F = form_set(w, ns, nchk, nchk == 0, buf_ptr(P, wb, 2 * MAXK + 1), acc);
         acc.nodes += 1;
     }
+    const uint32_t *Fp = F.ptr;
     // ---- siblings
-    const uint32_t *sibs;
+    Src sibs;
     uint32_t nsib;
-    uint32_t one = w.phi[s];
+    const uint32_t one = w.phi[s];
     if (s == 1) {
-        sibs = nullptr;
+        sibs.ptr = nullptr;
+        sibs.b = sibs.e = 0;
         nsib = 1;
     } else {
-        sibs = buf_ptr(P, wb, s);
-        nsib = w.len[s];
+        sibs = w.cand[s];
+        nsib = sibs.e - sibs.b;
     }
+    const uint32_t schk_mask = pl.check[s];
     const bool s_par = pl.s_is_parent != 0;
     const bool iep = pl.tail >= 2;
-    unsigned long long local = 0;  // per-lane partial (non-IEP)
+    unsigned long long local = 0;  // per-lane partial
     for (uint32_t sb = 0; sb < nsib; sb += SIB_BLOCK) {
         uint32_t nb = min((uint32_t)SIB_BLOCK, nsib - sb);
         // phase 1: per-sibling ranges and work (lane per sibling)
@@ -379,43 +386,46 @@ __device__ void count_last(const KParams &P, WaveLds &w, uint32_t *wb, Acc &acc)
             uint32_t i = r + lane;
             uint32_t work = 0;
             if (i < nb) {
-                uint32_t c = sibs ? sibs[sb + i] : one;
-                bool clo = has_lo, chi = has_hi;
-                uint32_t clo_v = lo, chi_v = hi;
-                if (pl.s_lower) { clo_v = clo ? max(clo_v, c) : c; clo = true; }
-                if (pl.s_upper) { chi_v = chi ? min(chi_v, c) : c; chi = true; }
+                uint32_t c = sibs.ptr ? sibs.ptr[sibs.b + sb + i] : one;
                 uint32_t cntv = 0;
                 uint32_t flag = 0, dstart = 0, tl = 0, th = 0;
-                if (s_par) {
-                    uint32_t rb = P.off[c], re = P.off[c + 1];
-                    clamp_range(P.nbr, rb, re, clo, clo_v, chi, chi_v);
-                    if (universe) {
-                        cntv = re - rb;
-                        for (int q = 0; q < nchk; ++q)
-                            if (contains(P.nbr, rb, re, chk[q])) --cntv;
-                    } else {
-                        uint32_t fb = 0, fe = fn;
-                        // only the sibling-dependent bound is left to apply to F
-                        clamp_range(F, fb, fe, pl.s_lower != 0, c, pl.s_upper != 0, c);
-                        if (re - rb <= fe - fb) {
-                            flag = 1;
-                            dstart = rb;
-                            tl = fb;
-                            th = fe;
-                            work = re - rb;
+                // a sibling aliasing a CSR row may still equal a checked image
+                bool valid = (s == 1) || !is_checked(w, schk_mask, c);
+                if (valid) {
+                    bool clo = has_lo, chi = has_hi;
+                    uint32_t clo_v = lo, chi_v = hi;
+                    if (pl.s_lower) { clo_v = clo ? max(clo_v, c) : c; clo = true; }
+                    if (pl.s_upper) { chi_v = chi ? min(chi_v, c) : c; chi = true; }
+                    if (s_par) {
+                        uint32_t rb = P.off[c], re = P.off[c + 1];
+                        clamp_range(P.nbr, rb, re, clo, clo_v, chi, chi_v);
+                        if (universe) {
+                            cntv = re - rb;
+                            for (uint32_t m = fchk_mask; m; m &= m - 1)
+                                if (contains(P.nbr, rb, re, w.phi[__builtin_ctz(m)])) --cntv;
                         } else {
-                            flag = 0;
-                            dstart = fb;
-                            tl = rb;
-                            th = re;
-                            work = fe - fb;
+                            uint32_t fb = F.b, fe = F.e;
+                            clamp_range(Fp, fb, fe, pl.s_lower != 0, c, pl.s_upper != 0, c);
+                            if (re - rb <= fe - fb) {
+                                flag = 1;
+                                dstart = rb;
+                                tl = fb;
+                                th = fe;
+                                work = re - rb;
+                            } else {
+                                flag = 0;
+                                dstart = fb;
+                                tl = rb;
+                                th = re;
+                                work = fe - fb;
+                            }
                         }
+                    } else {
+                        uint32_t fb = F.b, fe = F.e;
+                        clamp_range(Fp, fb, fe, pl.s_lower != 0, c, pl.s_upper != 0, c);
+                        cntv = fe - fb;
+                        if (pl.s_check && contains(Fp, fb, fe, c)) --cntv;
                     }
-                } else {
-                    uint32_t fb = 0, fe = fn;
-                    clamp_range(F, fb, fe, pl.s_lower != 0, c, pl.s_upper != 0, c);
-                    cntv = fe - fb;
-                    if (pl.s_check && contains(F, fb, fe, c)) --cntv;
                 }
                 w.cnt[i] = cntv;
                 w.flag[i] = flag;
@@ -434,7 +444,7 @@ __device__ void count_last(const KParams &P, WaveLds &w, uint32_t *wb, Acc &acc)
         }
         if (lane == 0) w.pre_sum[0] = 0;
         wave_sync();
-        uint32_t W = w.pre_sum[nb];
+        const uint32_t W = w.pre_sum[nb];
         acc.inter += nb;
         acc.bytes += 8ull * nb;
         // phase 2: flattened probes
@@ -451,9 +461,9 @@ __device__ void count_last(const KParams &P, WaveLds &w, uint32_t *wb, Acc &acc)
                 bool hit;
                 if (w.flag[a]) {
                     uint32_t x = P.nbr[w.drv[a] + off];
-                    hit = contains(F, w.tlo[a], w.thi[a], x);
+                    hit = contains(Fp, w.tlo[a], w.thi[a], x);
                 } else {
-                    uint32_t x = F[w.drv[a] + off];
+                    uint32_t x = Fp[w.drv[a] + off];
                     hit = contains(P.nbr, w.tlo[a], w.thi[a], x);
                 }
                 if (hit) {
@@ -474,7 +484,6 @@ __device__ void count_last(const KParams &P, WaveLds &w, uint32_t *wb, Acc &acc)
         }
         wave_sync();
     }
-    // wave reduction of the per-lane partial
     for (int off = WAVE / 2; off > 0; off >>= 1) local += __shfl_xor(local, off);
     acc.count += local;
 }
@@ -491,26 +500,36 @@ __device__ void flush(const KParams &P, uint32_t plan, Acc &acc) {
     acc = Acc{0, 0, 0, 0, 0};
 }
 
-// Enumeration: materialise the last level and write one row per embedding,
-// images stored at their pattern vertex (row[order[j]] = phi[j]).
+// Enumeration: form the last level and write one row per embedding, images
+// stored at their pattern vertex (row[order[j]] = phi[j]).
 __device__ void emit_last(const KParams &P, WaveLds &w, uint32_t *wb, Acc &acc) {
     const DPlan &pl = w.pl;
     const int L = pl.k - 1;
     materialize(P, w, wb, L, acc);
-    uint32_t n = w.len[L];
-    if (n == 0) return;
-    unsigned long long base = 0;
-    if (lane_id() == 0) base = atomicAdd(P.emb_n, (unsigned long long)n);
-    base = __shfl(base, 0);
-    const uint32_t *c = buf_ptr(P, wb, L);
-    for (uint32_t i = lane_id(); i < n; i += WAVE) {
-        unsigned long long slot = base + i;
-        if (slot >= P.emb_cap) continue;
-        uint32_t *row = P.emb + slot * pl.k;
-        for (int j = 0; j < L; ++j) row[pl.order[j]] = w.phi[j];
-        row[pl.order[L]] = c[i];
+    const Src c = w.cand[L];
+    const uint32_t cmask = pl.check[L];
+    const int lane = lane_id();
+    for (uint32_t base = c.b; base < c.e; base += WAVE) {
+        uint32_t idx = base + lane;
+        bool ok = idx < c.e;
+        uint32_t x = ok ? c.ptr[idx] : 0;
+        ok = ok && !is_checked(w, cmask, x);
+        uint64_t mask = __ballot(ok);
+        uint32_t n = __popcll(mask);
+        if (n == 0) continue;
+        unsigned long long slot0 = 0;
+        if (lane == 0) slot0 = atomicAdd(P.emb_n, (unsigned long long)n);
+        slot0 = __shfl(slot0, 0);
+        if (ok) {
+            unsigned long long slot = slot0 + __popcll(mask & ((1ull << lane) - 1ull));
+            if (slot < P.emb_cap) {
+                uint32_t *row = P.emb + slot * pl.k;
+                for (int j = 0; j < L; ++j) row[pl.order[j]] = w.phi[j];
+                row[pl.order[L]] = x;
+            }
+        }
+        acc.count += n;
     }
-    acc.count += n;
 }
 
 template <bool ENUM>
@@ -523,7 +542,7 @@ __global__ void __launch_bounds__(256) wcoj_kernel(KParams P) {
     uint32_t *wb = P.scratch + gwave * P.wave_stride;
     Acc acc{0, 0, 0, 0, 0};
     uint32_t cur_plan = NONE;
-    const uint64_t ntask = P.task_end - P.task_begin;
+    const uint64_t ntask = (P.task_end - P.task_begin + P.stride - 1) / P.stride;
     const uint64_t chunks = (ntask + P.chunk - 1) / P.chunk;
     while (true) {
         unsigned long long item = 0;
@@ -543,18 +562,20 @@ __global__ void __launch_bounds__(256) wcoj_kernel(KParams P) {
             wave_sync();
         }
         const DPlan &pl = w.pl;
+        if (pl.k < 2) continue;  // never launched by the host; no edge tasks exist for it
         // heavy-first: chunk 0 covers the highest CSR positions
-        uint64_t hi_t = P.task_end - ch * P.chunk;
-        uint64_t lo_t = (hi_t - P.task_begin > P.chunk) ? hi_t - P.chunk : P.task_begin;
-        for (uint64_t t = lo_t; t < hi_t; ++t) {
+        uint64_t hi_i = ntask - ch * P.chunk;
+        uint64_t lo_i = hi_i > P.chunk ? hi_i - P.chunk : 0;
+        for (uint64_t ti = lo_i; ti < hi_i; ++ti) {
+            const uint64_t t = P.task_begin + ti * P.stride;
             uint32_t v0 = P.src[t], v1 = P.nbr[t];
             if ((pl.lower[1] & 1u) && !(v1 > v0)) continue;
             if ((pl.upper[1] & 1u) && !(v1 < v0)) continue;
             acc.tasks += 1;
             w.phi[0] = v0;
             w.phi[1] = v1;
-            if (pl.k == 2) {
-                acc.count += 1;
+            if (pl.k < 3) {  // an edge pattern counts its root tasks; k = 1 is counted on the host
+                if (pl.k == 2) acc.count += 1;
                 continue;
             }
             if (pl.mode == SGE_MODE_EXPLORE && !ENUM) {
@@ -569,15 +590,17 @@ __global__ void __launch_bounds__(256) wcoj_kernel(KParams P) {
                 emit_last(P, w, wb, acc);
                 continue;
             }
-            const int stop = ENUM ? (int)pl.k - 2 : (int)pl.sib;  // level whose buffer feeds the leaf step
+            const int stop = ENUM ? (int)pl.k - 2 : (int)pl.sib;  // level whose set feeds the leaf step
             materialize(P, w, wb, 2, acc);
             int d = 2;
             while (d >= 2) {
                 if (d == stop) {
                     if (ENUM) {
-                        const uint32_t *b = buf_ptr(P, wb, d);
-                        for (uint32_t i = 0; i < w.len[d]; ++i) {
-                            w.phi[d] = b[i];
+                        const Src c = w.cand[d];
+                        for (uint32_t i = c.b; i < c.e; ++i) {
+                            uint32_t x = c.ptr[i];
+                            if (is_checked(w, pl.check[d], x)) continue;
+                            w.phi[d] = x;
                             emit_last(P, w, wb, acc);
                         }
                     } else {
@@ -586,10 +609,22 @@ __global__ void __launch_bounds__(256) wcoj_kernel(KParams P) {
                     --d;
                     continue;
                 }
+                const Src c = w.cand[d];
                 uint32_t p = w.pos[d];
-                if (p < w.len[d]) {
-                    w.pos[d] = p + 1;
-                    w.phi[d] = buf_ptr(P, wb, d)[p];
+                // next candidate, skipping images excluded by injectivity checks
+                uint32_t x = 0;
+                bool found = false;
+                while (c.b + p < c.e) {
+                    x = c.ptr[c.b + p];
+                    ++p;
+                    if (!is_checked(w, pl.check[d], x)) {
+                        found = true;
+                        break;
+                    }
+                }
+                w.pos[d] = p;
+                if (found) {
+                    w.phi[d] = x;
                     if (pl.mode == SGE_MODE_EXPLORE && !ENUM) explore_after(P, w, wb, d, acc);
                     materialize(P, w, wb, d + 1, acc);
                     ++d;
@@ -788,7 +823,7 @@ __global__ void reset_kernel(PlanOut *out, uint32_t n, unsigned long long *queue
 
 // One counting launch over `np` device plans (already uploaded at `dplans`).
 int run_launch(sge_graph *g, const DPlan *dplans, uint32_t np, PlanOut *dout, uint64_t tb, uint64_t te,
-               const LaunchCfg &cfg, bool enumerate, uint32_t *emb, uint64_t emb_cap, unsigned long long *emb_n,
+               uint64_t stride, const LaunchCfg &cfg, bool enumerate, uint32_t *emb, uint64_t emb_cap, unsigned long long *emb_n,
                float *ms) {
     DeviceCtx &c = dctx();
     KParams kp;
@@ -802,8 +837,10 @@ int run_launch(sge_graph *g, const DPlan *dplans, uint32_t np, PlanOut *dout, ui
     kp.nplans = np;
     kp.task_begin = tb;
     kp.task_end = te;
+    kp.stride = stride ? stride : 1;
     kp.chunk = cfg.chunk;
-    uint64_t chunks = (te - tb + cfg.chunk - 1) / cfg.chunk;
+    uint64_t ntask = (te - tb + kp.stride - 1) / kp.stride;
+    uint64_t chunks = (ntask + cfg.chunk - 1) / cfg.chunk;
     kp.nitems = chunks * np;
     kp.queue = c.queue;
     kp.out = dout;
@@ -905,7 +942,8 @@ extern "C" void sge_graph_device_free(sge_graph *g) {
 }
 
 static int count_impl(sge_graph *g, const sge_plan *plans, uint32_t nplans, uint32_t flags, const sge_tuning *tune,
-                      uint64_t tb, uint64_t te, uint64_t *counts, sge_stats *stats, sge_stats *per_plan) {
+                      uint64_t tb, uint64_t te, uint64_t stride, uint64_t *counts, sge_stats *stats,
+                      sge_stats *per_plan) {
     auto t0 = std::chrono::steady_clock::now();
     if (!g || !plans || !counts || nplans == 0) {
         sge::set_error("sge_count: null argument or no plans");
@@ -926,9 +964,11 @@ static int count_impl(sge_graph *g, const sge_plan *plans, uint32_t nplans, uint
     if (per_plan) std::memset(per_plan, 0, sizeof(sge_stats) * nplans);
     std::vector<PlanOut> host(nplans);
     std::vector<float> ms(nplans, 0.f);
-    if (flags & SGE_COUNT_BATCHED) {
+    bool any_k1 = false;
+    for (uint32_t i = 0; i < nplans; ++i) any_k1 = any_k1 || plans[i].k < 2;
+    if ((flags & SGE_COUNT_BATCHED) && !any_k1) {
         float t = 0;
-        rc = run_launch(g, c.plans, nplans, c.out, tb, te, cfg, false, nullptr, 0, nullptr, &t);
+        rc = run_launch(g, c.plans, nplans, c.out, tb, te, stride, cfg, false, nullptr, 0, nullptr, &t);
         if (rc) return rc;
         HIP_TRY(hipMemcpy(host.data(), c.out, sizeof(PlanOut) * nplans, hipMemcpyDeviceToHost));
         if (stats) stats->kernel_ms = t;
@@ -938,9 +978,14 @@ static int count_impl(sge_graph *g, const sge_plan *plans, uint32_t nplans, uint
         }
     } else {
         for (uint32_t i = 0; i < nplans; ++i) {
-            rc = run_launch(g, c.plans + i, 1, c.out + i, tb, te, cfg, false, nullptr, 0, nullptr, &ms[i]);
+            if (plans[i].k < 2) {  // single vertex: n embeddings, no device work
+                HIP_TRY(hipMemsetAsync(c.out + i, 0, sizeof(PlanOut), c.stream));
+                continue;
+            }
+            rc = run_launch(g, c.plans + i, 1, c.out + i, tb, te, stride, cfg, false, nullptr, 0, nullptr, &ms[i]);
             if (rc) return rc;
         }
+        HIP_TRY(hipStreamSynchronize(c.stream));
         HIP_TRY(hipMemcpy(host.data(), c.out, sizeof(PlanOut) * nplans, hipMemcpyDeviceToHost));
         for (uint32_t i = 0; i < nplans; ++i) {
             fill_stats(host[i], ms[i], stats);
@@ -962,7 +1007,7 @@ extern "C" int sge_count(sge_graph *g, const sge_plan *plans, uint32_t nplans, u
         sge::set_error("sge_count: null graph");
         return SGE_EINVAL;
     }
-    return count_impl(g, plans, nplans, flags, tune, 0, g->h.nnz, counts, stats, per_plan);
+    return count_impl(g, plans, nplans, flags, tune, 0, g->h.nnz, 1, counts, stats, per_plan);
 }
 
 extern "C" int sge_count_range(sge_graph *g, const sge_plan *plan, uint64_t task_begin, uint64_t task_end,
@@ -971,7 +1016,21 @@ extern "C" int sge_count_range(sge_graph *g, const sge_plan *plan, uint64_t task
         sge::set_error("sge_count_range: single-vertex patterns have no edge tasks");
         return SGE_EINVAL;
     }
-    return count_impl(g, plan, 1, 0, tune, task_begin, task_end, count, stats, nullptr);
+    return count_impl(g, plan, 1, 0, tune, task_begin, task_end, 1, count, stats, nullptr);
+}
+
+extern "C" int sge_count_sampled(sge_graph *g, const sge_plan *plans, uint32_t nplans, uint64_t stride,
+                                 uint64_t offset, const sge_tuning *tune, uint64_t *counts, sge_stats *per_plan) {
+    if (!g || stride == 0) {
+        sge::set_error("sge_count_sampled: null graph or zero stride");
+        return SGE_EINVAL;
+    }
+    for (uint32_t i = 0; i < nplans; ++i)
+        if (plans && plans[i].k == 1) {
+            sge::set_error("sge_count_sampled: single-vertex patterns have no edge tasks");
+            return SGE_EINVAL;
+        }
+    return count_impl(g, plans, nplans, 0, tune, offset, g->h.nnz, stride, counts, nullptr, per_plan);
 }
 
 extern "C" int sge_num_tasks(const sge_graph *g, const sge_plan *plan, uint64_t *n) {
@@ -1008,7 +1067,7 @@ extern "C" int sge_enumerate(sge_graph *g, const sge_plan *plan, uint32_t *out, 
     HIP_TRY(hipMalloc(&d_emb, bytes));
     HIP_TRY(hipMalloc(&d_n, sizeof(unsigned long long)));
     float t = 0;
-    rc = run_launch(g, c.plans, 1, c.out, 0, g->h.nnz, cfg, true, d_emb, cap, d_n, &t);
+    rc = run_launch(g, c.plans, 1, c.out, 0, g->h.nnz, 1, cfg, true, d_emb, cap, d_n, &t);
     unsigned long long n = 0;
     if (!rc) {
         HIP_TRY(hipMemcpy(&n, d_n, sizeof(n), hipMemcpyDeviceToHost));
@@ -1097,6 +1156,7 @@ extern "C" int sge_batch_create(sge_graph *g, const sge_plan *plans, uint32_t np
     kp.nplans = nplans;
     kp.task_begin = 0;
     kp.task_end = g->h.nnz;
+    kp.stride = 1;
     kp.chunk = cfg.chunk;
     kp.nitems = ((g->h.nnz + cfg.chunk - 1) / cfg.chunk) * nplans;
     kp.queue = b->queue;

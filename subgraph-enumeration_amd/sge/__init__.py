@@ -108,6 +108,21 @@ _SIGS = {
     "sge_count_range": (C.c_int, [C.c_void_p, C.POINTER(Plan), C.c_uint64, C.c_uint64, C.POINTER(Tuning),
                                   C.POINTER(C.c_uint64), C.POINTER(Stats)]),
     "sge_num_tasks": (C.c_int, [C.c_void_p, C.POINTER(Plan), C.POINTER(C.c_uint64)]),
+    "sge_count_sampled": (C.c_int, [C.c_void_p, C.POINTER(Plan), C.c_uint32, C.c_uint64, C.c_uint64,
+                                    C.POINTER(Tuning), C.c_void_p, C.POINTER(Stats)]),
+    "sge_order_features": (C.c_int, [C.POINTER(CPattern), C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32,
+                                     C.c_uint64, C.c_void_p]),
+    "sge_mlp_num_params": (C.c_int, []),
+    "sge_mlp_input_dim": (C.c_int, []),
+    "sge_mlp_create": (C.c_int, [C.c_uint64, C.POINTER(C.c_void_p)]),
+    "sge_mlp_set_params": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    "sge_mlp_get_params": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    "sge_mlp_forward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]),
+    "sge_mlp_gradients": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p,
+                                    C.POINTER(C.c_float)]),
+    "sge_mlp_train_step": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_float,
+                                     C.POINTER(C.c_float)]),
+    "sge_mlp_destroy": (None, [C.c_void_p]),
     "sge_enumerate": (C.c_int, [C.c_void_p, C.POINTER(Plan), C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64)]),
     "sge_batch_create": (C.c_int, [C.c_void_p, C.POINTER(Plan), C.c_uint32, C.POINTER(Tuning),
                                    C.POINTER(C.c_void_p)]),
@@ -324,6 +339,81 @@ def count_range(graph: Graph, plan: Plan, begin: int, end: int, tuning: Optional
     _check(lib().sge_count_range(graph._h, C.byref(plan), begin, end, C.byref(tuning) if tuning else None,
                                  C.byref(c), C.byref(st)), "sge_count_range")
     return int(c.value), st
+
+
+def count_sampled(graph: Graph, plans, stride: int, offset: int = 0, tuning: Optional[Tuning] = None):
+    """Plain executor over the strided root sample; returns counts and per-plan Stats."""
+    if isinstance(plans, Plan):
+        plans = [plans]
+    n = len(plans)
+    arr = (Plan * n)(*plans)
+    counts = np.zeros(n, dtype=np.uint64)
+    pp = (Stats * n)()
+    _check(lib().sge_count_sampled(graph._h, arr, n, stride, offset, C.byref(tuning) if tuning else None,
+                                   _ptr(counts), pp), "sge_count_sampled")
+    return [int(x) for x in counts], [pp[i] for i in range(n)]
+
+
+def order_features(pattern: Pattern, graph: Optional[Graph], order, mode: int, nsamples: int = 1024,
+                   seed: int = 0) -> np.ndarray:
+    o = np.ascontiguousarray(np.asarray(order, dtype=np.uint32))
+    f = np.zeros(64, dtype=np.float32)
+    _check(lib().sge_order_features(C.byref(pattern.c), graph._h if graph else None, _ptr(o), mode, nsamples, seed,
+                                    _ptr(f)), "sge_order_features")
+    return f
+
+
+class MLP:
+    """The learned cost model's network on the GPU (include/sge.h sge_mlp_*)."""
+
+    def __init__(self, seed: int = 0):
+        self._h = C.c_void_p()
+        _check(lib().sge_mlp_create(seed, C.byref(self._h)), "sge_mlp_create")
+        self.nparam = lib().sge_mlp_num_params()
+        self.in_dim = lib().sge_mlp_input_dim()
+
+    def params(self) -> np.ndarray:
+        p = np.zeros(self.nparam, dtype=np.float32)
+        _check(lib().sge_mlp_get_params(self._h, _ptr(p), self.nparam), "sge_mlp_get_params")
+        return p
+
+    def set_params(self, p: np.ndarray):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        _check(lib().sge_mlp_set_params(self._h, _ptr(p), self.nparam), "sge_mlp_set_params")
+
+    def forward(self, X: np.ndarray) -> np.ndarray:
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        y = np.zeros(X.shape[0], dtype=np.float32)
+        _check(lib().sge_mlp_forward(self._h, _ptr(X), X.shape[0], _ptr(y)), "sge_mlp_forward")
+        return y
+
+    def gradients(self, X: np.ndarray, t: np.ndarray):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        g = np.zeros(self.nparam, dtype=np.float32)
+        loss = C.c_float()
+        _check(lib().sge_mlp_gradients(self._h, _ptr(X), _ptr(t), X.shape[0], _ptr(g), C.byref(loss)),
+               "sge_mlp_gradients")
+        return g, loss.value
+
+    def train_step(self, X: np.ndarray, t: np.ndarray, lr: float) -> float:
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        loss = C.c_float()
+        _check(lib().sge_mlp_train_step(self._h, _ptr(X), _ptr(t), X.shape[0], lr, C.byref(loss)),
+               "sge_mlp_train_step")
+        return loss.value
+
+    def close(self):
+        if self._h and self._h.value:
+            lib().sge_mlp_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def num_tasks(graph: Graph, plan: Plan) -> int:

@@ -1,0 +1,162 @@
+"""Learned attribute-order cost model.
+
+A 64 -> 128 -> 128 -> 1 MLP (BF16 MFMA hidden layers, FP32 accumulation, Adam;
+all on the GPU through sge_mlp_*) predicts log2 of the plain GPU executor's
+measured device time for a (pattern, order) from the order's features
+(sge_order_features: plan structure, degree moments, analytic cost, Knuth
+random-walk estimates over sampled root edges).  It only reorders: counts never
+change.  The analytic cost model (plan/optimizer.cpp:441-577 optCostOrder,
+restated in the planner) stays the fallback: when the network's choice and the
+cost model's choice disagree by more than `threshold` bits on the calibrated
+scale, the cost-model order is kept.
+
+Training data are (order, measured work) pairs logged by the plain executor on
+a strided root-edge sample (sge_count_sampled), for patterns disjoint from the
+ones the model is used on.
+"""
+from __future__ import annotations
+
+import json
+import os
+import time
+from typing import Dict, List, Optional, Sequence
+
+import numpy as np
+
+from . import (MLP, Graph, Pattern, Plan, build_plan, count_sampled, order_cost, order_features)
+
+FEATURE_SAMPLES = 512
+
+
+def candidate_orders(pattern: Pattern, graph: Optional[Graph], top: int = 16, rng=None) -> List[List[int]]:
+    """Connected orders (deduplicated up to automorphism), the `top` cheapest by
+    the analytic cost model first; with rng, the rest filled by random others."""
+    orders = pattern.orders()
+    costs = np.array([order_cost(pattern, graph, o) for o in orders])
+    idx = list(np.argsort(costs, kind="stable"))
+    pick = idx[:top]
+    if rng is not None and len(idx) > top:
+        extra = rng.choice(idx[top:], size=min(top, len(idx) - top), replace=False)
+        pick += list(extra)
+    return [orders[i] for i in pick]
+
+
+def log_work(graph: Graph, pattern: Pattern, orders: Sequence[Sequence[int]], mode: int, stride: int,
+             reps: int = 2):
+    """Measured work of each order: plain executor on the strided root sample.
+    Returns (device ms [median of reps], algorithmic bytes, counts)."""
+    plans = [build_plan(pattern, graph, mode, o) for o in orders]
+    times = []
+    for _ in range(reps):
+        counts, st = count_sampled(graph, plans, stride)
+        times.append([s.kernel_ms for s in st])
+    ms = np.median(np.array(times), axis=0)
+    return ms, np.array([s.bytes for s in st], dtype=np.float64), counts
+
+
+class LearnedOrder:
+    def __init__(self, mlp: Optional[MLP] = None, seed: int = 0):
+        self.mlp = mlp or MLP(seed)
+        self.mu = np.zeros(64, np.float32)
+        self.sd = np.ones(64, np.float32)
+        self.t_mu = 0.0
+        self.t_sd = 1.0
+        self.calib = (1.0, 0.0)  # log2 analytic cost -> log2 ms
+        self.threshold = 3.0
+        self.history: Dict[str, list] = {}
+
+    # ---- data
+    def _norm(self, X):
+        return (X - self.mu) / self.sd
+
+    def fit(self, X: np.ndarray, y_log2: np.ndarray, cost_log2: np.ndarray, epochs: int = 300, lr: float = 2e-3,
+            batch: int = 256, seed: int = 0):
+        self.mu = X.mean(0).astype(np.float32)
+        self.sd = (X.std(0) + 1e-3).astype(np.float32)
+        self.t_mu, self.t_sd = float(y_log2.mean()), float(y_log2.std() + 1e-6)
+        a, b = np.polyfit(cost_log2, y_log2, 1)
+        self.calib = (float(a), float(b))
+        Xn = self._norm(X).astype(np.float32)
+        tn = ((y_log2 - self.t_mu) / self.t_sd).astype(np.float32)
+        rng = np.random.default_rng(seed)
+        losses = []
+        n = len(Xn)
+        for ep in range(epochs):
+            perm = rng.permutation(n)
+            el = []
+            for i in range(0, n, batch):
+                j = perm[i:i + batch]
+                el.append(self.mlp.train_step(Xn[j], tn[j], lr))
+            losses.append(float(np.mean(el)))
+        self.history["loss"] = losses
+        return losses
+
+    def predict_log2(self, X: np.ndarray) -> np.ndarray:
+        y = self.mlp.forward(self._norm(np.asarray(X, np.float32)).astype(np.float32))
+        return y * self.t_sd + self.t_mu
+
+    # ---- persistence
+    def save(self, path: str):
+        np.savez(path, params=self.mlp.params(), mu=self.mu, sd=self.sd,
+                 meta=np.array(json.dumps({"t_mu": self.t_mu, "t_sd": self.t_sd, "calib": self.calib,
+                                           "threshold": self.threshold})))
+
+    @classmethod
+    def load(cls, path: str) -> "LearnedOrder":
+        d = np.load(path, allow_pickle=False)
+        m = cls()
+        m.mlp.set_params(d["params"])
+        m.mu, m.sd = d["mu"], d["sd"]
+        meta = json.loads(str(d["meta"]))
+        m.t_mu, m.t_sd = meta["t_mu"], meta["t_sd"]
+        m.calib = tuple(meta["calib"])
+        m.threshold = meta["threshold"]
+        return m
+
+
+def load(path: str) -> LearnedOrder:
+    return LearnedOrder.load(path)
+
+
+def choose_plan(model: LearnedOrder, pattern: Pattern, graph: Graph, mode: int, fallback: Optional[Plan] = None,
+                top: int = 16):
+    """Order with the lowest predicted work among the cost model's `top`
+    candidates; the cost-model plan is kept when the two disagree by more than
+    the model's threshold (bits, calibrated scale)."""
+    if fallback is None:
+        fallback = build_plan(pattern, graph, mode)
+    cands = candidate_orders(pattern, graph, top)
+    X = np.stack([order_features(pattern, graph, o, mode, FEATURE_SAMPLES, 11) for o in cands])
+    pred = model.predict_log2(X)
+    best = int(np.argmin(pred))
+    cost_order = list(fallback.order)[: pattern.k]
+    a, b = model.calib
+    cost_pred = a * np.log2(max(fallback.cost, 1.0)) + b
+    info = {"pred_log2": float(pred[best]), "cost_model_log2": float(cost_pred),
+            "cost_order": cost_order, "learned_order": list(cands[best]), "fallback": False}
+    if abs(float(pred[best]) - float(cost_pred)) > model.threshold and cands[best] != cost_order:
+        info["fallback"] = True
+        return fallback, info
+    return build_plan(pattern, graph, mode, cands[best]), info
+
+
+def train(graph: Graph, patterns: Sequence[Pattern], mode: int, stride: int = 64, orders_per_pattern: int = 8,
+          epochs: int = 300, seed: int = 0, log=print):
+    """Log (order, measured work) pairs with the plain executor and fit the MLP."""
+    rng = np.random.default_rng(seed)
+    X, y, c = [], [], []
+    t0 = time.time()
+    for i, p in enumerate(patterns):
+        orders = candidate_orders(p, graph, orders_per_pattern // 2, rng)[:orders_per_pattern]
+        ms, by, _ = log_work(graph, p, orders, mode, stride)
+        for o, t in zip(orders, ms):
+            X.append(order_features(p, graph, o, mode, FEATURE_SAMPLES, 11))
+            y.append(np.log2(max(t, 1e-3)))
+            c.append(np.log2(max(order_cost(p, graph, o), 1.0)))
+        if i % 25 == 0:
+            log(f"[learned] logged {i + 1}/{len(patterns)} patterns, {len(X)} orders, {time.time() - t0:.0f}s")
+    X, y, c = np.array(X, np.float32), np.array(y, np.float64), np.array(c, np.float64)
+    model = LearnedOrder(seed=seed)
+    losses = model.fit(X, y, c, epochs=epochs, seed=seed)
+    log(f"[learned] trained on {len(X)} orders, loss {losses[0]:.3f} -> {losses[-1]:.3f}")
+    return model, (X, y, c)

@@ -1,0 +1,103 @@
+"""Learned cost model: the NumPy reference network is pinned by finite
+differences; the GPU network (BF16 MFMA forward/backward, Adam) is checked
+against it; training must reduce the loss; the learned order must not lose to
+the cost model's order on held-out patterns."""
+import numpy as np
+import pytest
+
+import sge
+import sge.workloads as W
+from oracle import mlp_ref as R
+
+
+def rand_params(seed):
+    rng = np.random.default_rng(seed)
+    p = np.zeros(64 * 128 + 128 + 128 * 128 + 128 + 128 + 1, np.float32)
+    o = 0
+    for n, fan in ((64 * 128, 64), (128, 0), (128 * 128, 128), (128, 0), (128, 128), (1, 0)):
+        p[o:o + n] = rng.uniform(-1, 1, n) * (np.sqrt(6 / fan) if fan else 0.1)
+        o += n
+    return p
+
+
+def test_reference_gradients_match_finite_differences():
+    R.ROUND = False
+    try:
+        rng = np.random.default_rng(1)
+        p = rand_params(1).astype(np.float64)
+        X = rng.normal(size=(24, 64))
+        t = rng.normal(size=24)
+        g, loss = R.gradients(p, X, t)
+        idx = list(rng.choice(len(p), 40, replace=False)) + [len(p) - 1, 64 * 128 + 3]
+        for i in idx:
+            e = np.zeros_like(p)
+            e[i] = 1e-6
+            lp = np.mean((R.forward(p + e, X)[0] - t) ** 2)
+            lm = np.mean((R.forward(p - e, X)[0] - t) ** 2)
+            num = (lp - lm) / 2e-6
+            assert abs(num - g[i]) <= 1e-5 + 1e-4 * abs(num), i
+    finally:
+        R.ROUND = True
+
+
+@pytest.mark.gpu
+def test_gpu_forward_and_gradients_match_reference():
+    rng = np.random.default_rng(2)
+    m = sge.MLP(seed=3)
+    p = rand_params(4)
+    m.set_params(p)
+    for N in (1, 17, 64, 300):
+        X = rng.normal(size=(N, 64)).astype(np.float32)
+        t = rng.normal(size=N).astype(np.float32)
+        y = m.forward(X)
+        yr, _ = R.forward(p, X)
+        # BF16 operands, FP32 accumulation: compare at BF16-level tolerance
+        assert np.allclose(y, yr, rtol=2e-2, atol=2e-2 * np.abs(yr).max()), N
+        g, loss = m.gradients(X, t)
+        gr, lr = R.gradients(p, X, t)
+        assert abs(loss - lr) <= 2e-2 * max(1.0, lr)
+        scale = np.abs(gr).max()
+        assert np.abs(g - gr).max() <= 3e-2 * scale, N
+        # per-block relative check
+        o = 0
+        for n in (64 * 128, 128, 128 * 128, 128, 128, 1):
+            blk, rb = g[o:o + n], gr[o:o + n]
+            assert np.linalg.norm(blk - rb) <= 5e-2 * np.linalg.norm(rb) + 1e-6, (N, o)
+            o += n
+
+
+@pytest.mark.gpu
+def test_gpu_training_reduces_loss():
+    rng = np.random.default_rng(5)
+    X = rng.normal(size=(512, 64)).astype(np.float32)
+    w = rng.normal(size=64)
+    t = (np.tanh(X @ w / 8) * 2 + 0.1 * rng.normal(size=512)).astype(np.float32)
+    m = sge.MLP(seed=0)
+    first = m.train_step(X, t, 1e-3)
+    for _ in range(300):
+        last = m.train_step(X, t, 1e-3)
+    assert last < 0.2 * first
+
+
+@pytest.mark.gpu
+def test_learned_order_gain_over_cost_model():
+    """Train on patterns disjoint from the evaluated ones; the learned choice
+    (with the cost-model fallback) must not be slower in total than the
+    cost-model choice, measured by the same executor on the same root sample."""
+    from sge import learned as L
+
+    g = W.graph_c2()
+    train_p = W.training_set(40)
+    model, _ = L.train(g, train_p, sge.MODE_WCOJ, stride=8, orders_per_pattern=6, epochs=150, log=print)
+    test_p = W.six_vertex_set()
+    t_cost = t_learn = 0.0
+    for p in test_p:
+        cost_plan = sge.build_plan(p, g, sge.MODE_WCOJ)
+        plan, info = L.choose_plan(model, p, g, sge.MODE_WCOJ, fallback=cost_plan)
+        c1, s1 = sge.count_sampled(g, [cost_plan, plan], 8)
+        c2, s2 = sge.count_sampled(g, [cost_plan, plan], 8)
+        assert c1[0] == c1[1]
+        t_cost += min(s1[0].kernel_ms, s2[0].kernel_ms)
+        t_learn += min(s1[1].kernel_ms, s2[1].kernel_ms)
+    print(f"cost-model orders {t_cost:.2f} ms, learned orders {t_learn:.2f} ms")
+    assert t_learn <= 1.05 * t_cost
