@@ -139,7 +139,9 @@ typedef struct sge_plan {
     uint32_t check[SGE_MAX_K];         /* image must differ from these levels' images (injectivity) */
     int32_t reuse[SGE_MAX_K];          /* earlier level whose candidate set contains this one's (candidatesBefore), -1 */
     uint32_t ready[SGE_MAX_K];         /* exploration: level after which this level's parent intersection is formed */
-    uint32_t tail;                     /* IEP: trailing symmetric group counted by choosec (0/1 = none) */
+    uint32_t tail;                     /* IEP: trailing levels counted in closed form (0 = none) */
+    uint32_t tail_kind;                /* 1: symmetric twin group, choosec (join.cpp:1532-1537);
+                                          2: independent set, inclusion-exclusion over set partitions */
     uint32_t num_bags;                 /* decomposition bag count (reporting / features) */
     uint64_t multiplier;               /* |Aut| when the reference counts embeddings (single bag), else 1 */
     uint64_t aut_size;
@@ -150,7 +152,7 @@ typedef struct sge_plan {
  * optCostOrder restated); g may be NULL (uniform-degree cost model). */
 int sge_plan_build(const sge_pattern *p, const sge_graph *g, uint32_t mode, const uint32_t *order, sge_plan *out);
 /* Cost-model estimate of one order (lower is better). */
-int sge_plan_cost(const sge_pattern *p, const sge_graph *g, const uint32_t *order, double *cost);
+int sge_plan_cost(const sge_pattern *p, const sge_graph *g, uint32_t mode, const uint32_t *order, double *cost);
 /* All connected orders of the pattern (k! bounded), deduplicated up to automorphism; returns count in *n. */
 int sge_plan_orders(const sge_pattern *p, uint32_t *orders, uint32_t cap, uint32_t *n);
 /* Reference-style plan text ("num bags", symmetry rules, ...) into buf. */
@@ -229,6 +231,10 @@ void sge_mlp_destroy(sge_mlp *m);
  * estimate (nsamples random root edges, seeded) of nodes and list work per level. */
 int sge_order_features(const sge_pattern *p, const sge_graph *g, const uint32_t *order, uint32_t mode,
                        uint32_t nsamples, uint64_t seed, float *feat64);
+
+/* Streaming copy of `bytes` (read once, written once) timed on the device: the
+ * known-size kernel that scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings. */
+int sge_calibration_copy(uint64_t bytes, double *ms);
 
 /* Device info for reporting. */
 int sge_device_info(int *num_cu, int *num_xcd_hint, char *name, size_t cap);

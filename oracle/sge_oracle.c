@@ -14,6 +14,8 @@
  *   count_last            relation/relation.cpp:301-338 (TrieLevel::numResults: |clamped range|
  *                         minus matched pattern vertices whose image lies in the range)
  *                         + matching/join.cpp:1532-1537 (IEP: choosec over the symmetric group)
+ *                         + independent tails (no reference counterpart): inclusion-exclusion
+ *                         over set partitions from explicitly intersected member sets
  *   dfs                   matching/join.cpp:137-178, 309-421 (generateCandidates + nodeJoin DFS
  *                         with visited[] injectivity), parallel over root edges like
  *                         matching/parallel.cpp:1414-1458 (schedule(dynamic), + reduction)
@@ -148,6 +150,8 @@ typedef struct {
     uint32_t *emb;             /* enumeration output (may be NULL) */
     uint64_t emb_cap, *emb_n;
     uint64_t intersections;
+    int no_reuse;
+    uint32_t *ie[3];           /* independent-tail member sets */
 } ctx_t;
 
 /* clamp bounds of level d from the images of its lower/upper levels */
@@ -181,7 +185,7 @@ static uint32_t candidates(ctx_t *c, int d, int materialise, uint32_t lower_mask
     const uint32_t *src = NULL;
     uint32_t srclen = 0;
     int j;
-    int r = pl->reuse[d];
+    int r = c->no_reuse ? -1 : pl->reuse[d];
     uint32_t mx = 0, mn = 0xffffffffu;
     int has_lo = 0, has_hi = 0;
     for (j = 0; j < d; ++j) {
@@ -267,6 +271,30 @@ static uint64_t dfs(ctx_t *c, int d) {
             emit(c, d);
             return c->len[d];
         }
+        if (tail && pl->tail_kind == 2) {
+            /* independent tail: materialise each member's set (bounds/checks from
+             * the levels below the tail), then inclusion-exclusion over set
+             * partitions of the tail, from explicit intersections */
+            uint32_t below = (1u << d) - 1, nm[3], t;
+            for (t = 0; t < (uint32_t)tail; ++t) {
+                int lev = d + (int)t;
+                c->no_reuse = 1;
+                uint32_t n = candidates(c, lev, 1, pl->lower[lev] & below, pl->upper[lev] & below,
+                                        pl->check[lev] & below);
+                c->no_reuse = 0;
+                memcpy(c->ie[t], c->buf[lev], (size_t)n * 4);
+                nm[t] = n;
+            }
+            if (tail == 2) {
+                uint64_t n01 = oracle_intersect(c->ie[0], nm[0], c->ie[1], nm[1], NULL);
+                return (uint64_t)nm[0] * nm[1] - n01;
+            }
+            uint64_t n01 = oracle_intersect(c->ie[0], nm[0], c->ie[1], nm[1], c->tmp);
+            uint64_t n012 = oracle_intersect(c->tmp, (uint32_t)n01, c->ie[2], nm[2], NULL);
+            uint64_t n02 = oracle_intersect(c->ie[0], nm[0], c->ie[2], nm[2], NULL);
+            uint64_t n12 = oracle_intersect(c->ie[1], nm[1], c->ie[2], nm[2], NULL);
+            return (uint64_t)nm[0] * nm[1] * nm[2] - n01 * nm[2] - n02 * nm[1] - n12 * nm[0] + 2 * n012;
+        }
         if (tail) {
             /* group members share one candidate set; their chain is counted by choosec */
             uint32_t below = (1u << d) - 1;
@@ -301,12 +329,17 @@ static int ctx_init(ctx_t *c, const sge_plan *pl, const uint32_t *off, const uin
     }
     c->tmp = (uint32_t *)malloc(((size_t)c->maxdeg + 1) * 4);
     c->tmp2 = (uint32_t *)malloc(((size_t)c->maxdeg + 1) * 4);
+    for (i = 0; i < 3; ++i) {
+        c->ie[i] = (uint32_t *)malloc(((size_t)c->maxdeg + 1) * 4);
+        if (!c->ie[i]) return -1;
+    }
     return (c->tmp && c->tmp2) ? 0 : -1;
 }
 
 static void ctx_free(ctx_t *c) {
     int i;
     for (i = 0; i < SGE_MAX_K; ++i) free(c->buf[i]);
+    for (i = 0; i < 3; ++i) free(c->ie[i]);
     free(c->tmp);
     free(c->tmp2);
 }

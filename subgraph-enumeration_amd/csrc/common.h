@@ -85,7 +85,8 @@ struct GraphSummary {
     bool valid = false;
 };
 GraphSummary summarize(const HostGraph *g);
-double order_cost(const Pattern &p, const GraphSummary &s, const int *order, const std::vector<std::vector<int>> &aut);
+double order_cost(const Pattern &p, const GraphSummary &s, const int *order, const std::vector<std::vector<int>> &aut,
+                  uint32_t mode);
 int build_plan(const Pattern &p, const HostGraph *g, uint32_t mode, const uint32_t *order, sge_plan &out);
 std::vector<std::vector<int>> connected_orders(const Pattern &p, const std::vector<std::vector<int>> &aut);
 

@@ -52,7 +52,9 @@ namespace {
 
 constexpr int WAVE = 64;
 constexpr int MAXK = SGE_MAX_K;
-constexpr int SIB_BLOCK = 256;  // siblings per flattened last-level batch
+constexpr int SIB_BLOCK = 256;  // (sibling, term) entries per flattened last-level batch
+constexpr int MAX_TERMS = 7;    // inclusion-exclusion terms of a 3-level independent tail
+constexpr int NBUF = 2 * MAXK + 1 + MAX_TERMS;  // scratch buffers per wave
 constexpr uint32_t NONE = 0xffffffffu;
 
 // Device copy of a plan (levels masks) plus derived fields.
@@ -67,7 +69,13 @@ struct DPlan {
     uint32_t s_lower, s_upper, s_check;  // last's clamp/check against s
     uint32_t f_ready;      // exploration: level after which F's intersection is formed
     uint32_t order[MAXK];  // level -> pattern vertex (enumeration output)
-    uint32_t pad[2];
+    // closed-form terms counted per sibling: kind 0/1 use one term (the first
+    // closed-form level); kind 2 uses one term per non-empty subset B of the
+    // independent tail (index = bitmask(B) - 1).  Masks are over levels below
+    // the sibling level s; sflags bit0: s is a parent, bit1: lower bound from s,
+    // bit2: upper bound from s, bit3: must differ from s's image.
+    uint32_t tail_kind, nterms;
+    uint32_t t_par[MAX_TERMS], t_lower[MAX_TERMS], t_upper[MAX_TERMS], t_check[MAX_TERMS], t_sflags[MAX_TERMS];
 };
 
 struct PlanOut {
@@ -328,40 +336,68 @@ __device__ void explore_after(const KParams &P, WaveLds &w, uint32_t *wb, int d,
     }
 }
 
-// Count the last level for every sibling in cand[sib] (or the single sibling
-// phi[1] when sib == 1).  F = the last level's candidates that do not depend on
-// the sibling, with every bound/check from levels below the sibling applied.
-__device__ void count_last(const KParams &P, WaveLds &w, uint32_t *wb, Acc &acc) {
+// Per-term setup kept in LDS for a sibling batch.
+struct TermLds {
+    Src F;            // sibling-independent part (bounds/checks below s applied)
+    uint32_t lo, hi;  // fixed bounds (has_lo / has_hi in flags)
+    uint32_t flags;   // bit0 has_lo, bit1 has_hi, bit2 universe
+    uint32_t sflags;  // as DPlan::t_sflags
+    uint32_t chk;     // check mask below s (needed in the universe case)
+};
+
+// Inclusion-exclusion over set partitions of the tail: number of injective
+// assignments from per-subset intersection sizes n[B-1] (u64 arithmetic is
+// exact modulo 2^64, and the true count is below 2^64).
+__device__ __forceinline__ uint64_t ie_combine(const uint64_t *n, uint32_t r) {
+    if (r == 2) return n[0] * n[1] - n[2];
+    // r == 3: a=1 b=2 ab=3 c=4 ac=5 bc=6 abc=7
+    return n[0] * n[1] * n[3] - n[2] * n[3] - n[4] * n[1] - n[5] * n[0] + 2ull * n[6];
+}
+
+// Count the closed-form level(s) for every sibling in cand[sib] (or the single
+// sibling phi[1] when sib == 1).  Each term's F = the candidates that do not
+// depend on the sibling, with every bound/check from levels below the sibling
+// applied; per (sibling, term) the count is |F ∩ N(c) ∩ range(c)| (or a range
+// size), computed with the flattened probe loop.
+__device__ void count_last(const KParams &P, WaveLds &w, TermLds *terms, uint32_t *wb, Acc &acc) {
     const DPlan &pl = w.pl;
-    const int s = pl.sib, L = pl.last;
+    const int s = pl.sib;
     const int lane = lane_id();
-    const uint32_t below = (1u << s) - 1u;  // levels < s
-    // ---- F: fixed part of the last level
-    bool has_lo, has_hi;
-    uint32_t lo, hi;
-    level_bounds(w, pl.lower[L] & below, pl.upper[L] & below, has_lo, lo, has_hi, hi);
-    const uint32_t fchk_mask = pl.check[L] & below;
-    const bool universe = pl.f_parent == 0;
-    Src F;
-    F.ptr = nullptr;
-    F.b = F.e = 0;
-    if (!universe) {
-        int ns;
-        if (pl.mode == SGE_MODE_EXPLORE) {
-            Src sp = w.pre[MAXK];
-            clamp_range(sp.ptr, sp.b, sp.e, has_lo, lo, has_hi, hi);
-            w.srcs[0] = sp;
-            ns = 1;
+    const uint32_t K = pl.nterms;
+    // ---- per-term fixed parts
+    for (uint32_t j = 0; j < K; ++j) {
+        bool has_lo, has_hi;
+        uint32_t lo, hi;
+        level_bounds(w, pl.t_lower[j], pl.t_upper[j], has_lo, lo, has_hi, hi);
+        TermLds t;
+        t.lo = lo;
+        t.hi = hi;
+        t.sflags = pl.t_sflags[j];
+        t.chk = pl.t_check[j];
+        t.flags = (has_lo ? 1u : 0u) | (has_hi ? 2u : 0u);
+        t.F.ptr = nullptr;
+        t.F.b = t.F.e = 0;
+        if (pl.t_par[j] == 0) {
+            t.flags |= 4u;  // universe: only the sibling's row constrains it
         } else {
-            int reuse = pl.reuse[L];
-            if (reuse >= 0 && ((uint32_t)reuse >= (uint32_t)s || (pl.parent[reuse] & ~pl.f_parent))) reuse = -1;
-            ns = gather_sources(P, w, L, pl.f_parent, reuse, false, has_lo, lo, has_hi, hi, acc);
+            int ns;
+            if (pl.mode == SGE_MODE_EXPLORE && K == 1) {
+                Src sp = w.pre[MAXK];
+                clamp_range(sp.ptr, sp.b, sp.e, has_lo, lo, has_hi, hi);
+                w.srcs[0] = sp;
+                ns = 1;
+            } else {
+                int reuse = (K == 1) ? pl.reuse[pl.last] : -1;
+                if (reuse >= 0 && ((uint32_t)reuse >= (uint32_t)s || (pl.parent[reuse] & ~pl.t_par[j]))) reuse = -1;
+                ns = gather_sources(P, w, pl.last, pl.t_par[j], reuse, false, has_lo, lo, has_hi, hi, acc);
+            }
+            int nchk = load_checks(w, pl.t_check[j]);
+            t.F = form_set(w, ns, nchk, nchk == 0, buf_ptr(P, wb, 2 * MAXK + 1 + j), acc);
+            acc.nodes += 1;
         }
-        int nchk = load_checks(w, fchk_mask);
-        F = form_set(w, ns, nchk, nchk == 0, buf_ptr(P, wb, 2 * MAXK + 1), acc);
-        acc.nodes += 1;
+        terms[j] = t;
     }
-    const uint32_t *Fp = F.ptr;
+    wave_sync();
     // ---- siblings
     Src sibs;
     uint32_t nsib;
@@ -375,37 +411,42 @@ __device__ void count_last(const KParams &P, WaveLds &w, uint32_t *wb, Acc &acc)
         nsib = sibs.e - sibs.b;
     }
     const uint32_t schk_mask = pl.check[s];
-    const bool s_par = pl.s_is_parent != 0;
-    const bool iep = pl.tail >= 2;
-    unsigned long long local = 0;  // per-lane partial
-    for (uint32_t sb = 0; sb < nsib; sb += SIB_BLOCK) {
-        uint32_t nb = min((uint32_t)SIB_BLOCK, nsib - sb);
-        // phase 1: per-sibling ranges and work (lane per sibling)
+    const uint32_t kind = pl.tail_kind;
+    const bool per_entry = (kind != 0) || K > 1;  // counts needed per (sibling, term)
+    const uint32_t per_block = SIB_BLOCK / K;     // siblings per batch
+    unsigned long long local = 0;                 // per-lane partial
+    for (uint32_t sb = 0; sb < nsib; sb += per_block) {
+        const uint32_t nb = min(per_block, nsib - sb);
+        const uint32_t ne = nb * K;  // entries: sibling i, term j -> i*K + j
+        // phase 1: per-entry ranges and work (lane per entry)
         uint32_t carry = 0;
-        for (uint32_t r = 0; r < nb; r += WAVE) {
-            uint32_t i = r + lane;
+        for (uint32_t r0 = 0; r0 < ne; r0 += WAVE) {
+            uint32_t ei = r0 + lane;
             uint32_t work = 0;
-            if (i < nb) {
+            if (ei < ne) {
+                const uint32_t i = ei / K, j = ei - i * K;
+                const TermLds &t = terms[j];
                 uint32_t c = sibs.ptr ? sibs.ptr[sibs.b + sb + i] : one;
                 uint32_t cntv = 0;
                 uint32_t flag = 0, dstart = 0, tl = 0, th = 0;
                 // a sibling aliasing a CSR row may still equal a checked image
                 bool valid = (s == 1) || !is_checked(w, schk_mask, c);
                 if (valid) {
-                    bool clo = has_lo, chi = has_hi;
-                    uint32_t clo_v = lo, chi_v = hi;
-                    if (pl.s_lower) { clo_v = clo ? max(clo_v, c) : c; clo = true; }
-                    if (pl.s_upper) { chi_v = chi ? min(chi_v, c) : c; chi = true; }
-                    if (s_par) {
+                    bool clo = t.flags & 1u, chi = t.flags & 2u;
+                    uint32_t clo_v = t.lo, chi_v = t.hi;
+                    const bool s_lower = t.sflags & 2u, s_upper = t.sflags & 4u;
+                    if (s_lower) { clo_v = clo ? max(clo_v, c) : c; clo = true; }
+                    if (s_upper) { chi_v = chi ? min(chi_v, c) : c; chi = true; }
+                    if (t.sflags & 1u) {  // the sibling is a parent: intersect with its row
                         uint32_t rb = P.off[c], re = P.off[c + 1];
                         clamp_range(P.nbr, rb, re, clo, clo_v, chi, chi_v);
-                        if (universe) {
+                        if (t.flags & 4u) {
                             cntv = re - rb;
-                            for (uint32_t m = fchk_mask; m; m &= m - 1)
+                            for (uint32_t m = t.chk; m; m &= m - 1)
                                 if (contains(P.nbr, rb, re, w.phi[__builtin_ctz(m)])) --cntv;
                         } else {
-                            uint32_t fb = F.b, fe = F.e;
-                            clamp_range(Fp, fb, fe, pl.s_lower != 0, c, pl.s_upper != 0, c);
+                            uint32_t fb = t.F.b, fe = t.F.e;
+                            clamp_range(t.F.ptr, fb, fe, s_lower, c, s_upper, c);
                             if (re - rb <= fe - fb) {
                                 flag = 1;
                                 dstart = rb;
@@ -421,45 +462,47 @@ __device__ void count_last(const KParams &P, WaveLds &w, uint32_t *wb, Acc &acc)
                             }
                         }
                     } else {
-                        uint32_t fb = F.b, fe = F.e;
-                        clamp_range(Fp, fb, fe, pl.s_lower != 0, c, pl.s_upper != 0, c);
+                        uint32_t fb = t.F.b, fe = t.F.e;
+                        clamp_range(t.F.ptr, fb, fe, s_lower, c, s_upper, c);
                         cntv = fe - fb;
-                        if (pl.s_check && contains(Fp, fb, fe, c)) --cntv;
+                        if ((t.sflags & 8u) && contains(t.F.ptr, fb, fe, c)) --cntv;
                     }
                 }
-                w.cnt[i] = cntv;
-                w.flag[i] = flag;
-                w.drv[i] = dstart;
-                w.tlo[i] = tl;
-                w.thi[i] = th;
+                w.cnt[ei] = cntv;
+                w.flag[ei] = flag | (j << 1);
+                w.drv[ei] = dstart;
+                w.tlo[ei] = tl;
+                w.thi[ei] = th;
             }
             // inclusive scan of work within the wave
             uint32_t incl = work;
             for (int off = 1; off < WAVE; off <<= 1) {
-                uint32_t t = __shfl_up(incl, off);
-                if (lane >= off) incl += t;
+                uint32_t tt = __shfl_up(incl, off);
+                if (lane >= off) incl += tt;
             }
-            if (i < nb) w.pre_sum[i + 1] = carry + incl;
+            if (ei < ne) w.pre_sum[ei + 1] = carry + incl;
             carry += __shfl(incl, WAVE - 1);
         }
         if (lane == 0) w.pre_sum[0] = 0;
         wave_sync();
-        const uint32_t W = w.pre_sum[nb];
-        acc.inter += nb;
+        const uint32_t W = w.pre_sum[ne];
+        acc.inter += ne;
         acc.bytes += 8ull * nb;
         // phase 2: flattened probes
         for (uint32_t u0 = 0; u0 < W; u0 += WAVE) {
             uint32_t u = u0 + lane;
             if (u < W) {
-                // sibling index: last i with pre_sum[i] <= u
-                uint32_t a = 0, b = nb;
+                // entry index: last e with pre_sum[e] <= u
+                uint32_t a = 0, b = ne;
                 while (b - a > 1) {
                     uint32_t m = (a + b) >> 1;
                     if (w.pre_sum[m] <= u) a = m; else b = m;
                 }
                 uint32_t off = u - w.pre_sum[a];
+                const uint32_t fl = w.flag[a];
+                const uint32_t *Fp = terms[fl >> 1].F.ptr;
                 bool hit;
-                if (w.flag[a]) {
+                if (fl & 1u) {
                     uint32_t x = P.nbr[w.drv[a] + off];
                     hit = contains(Fp, w.tlo[a], w.thi[a], x);
                 } else {
@@ -467,19 +510,26 @@ __device__ void count_last(const KParams &P, WaveLds &w, uint32_t *wb, Acc &acc)
                     hit = contains(P.nbr, w.tlo[a], w.thi[a], x);
                 }
                 if (hit) {
-                    if (iep) atomicAdd(&w.cnt[a], 1u);
+                    if (per_entry) atomicAdd(&w.cnt[a], 1u);
                     else local += 1;
                 }
             }
         }
         acc.bytes += 4ull * W * 2;
         wave_sync();
-        // phase 3: closed-form contributions
-        for (uint32_t r = 0; r < nb; r += WAVE) {
-            uint32_t i = r + lane;
+        // phase 3: closed-form contributions (lane per sibling)
+        for (uint32_t r0 = 0; r0 < nb; r0 += WAVE) {
+            uint32_t i = r0 + lane;
             if (i < nb) {
-                uint32_t cv = w.cnt[i];
-                local += iep ? choosec(cv, pl.tail) : cv;
+                if (kind == 2) {
+                    uint64_t n[MAX_TERMS];
+                    for (uint32_t j = 0; j < K; ++j) n[j] = w.cnt[i * K + j];
+                    local += ie_combine(n, pl.tail);
+                } else if (kind == 1) {
+                    local += choosec(w.cnt[i], pl.tail);
+                } else {
+                    local += w.cnt[i];
+                }
             }
         }
         wave_sync();
@@ -537,7 +587,9 @@ __global__ void __launch_bounds__(256) wcoj_kernel(KParams P) {
     extern __shared__ unsigned char lds_raw[];
     const int lane = lane_id();
     const int wid = threadIdx.x / WAVE;
+    const int nwaves = blockDim.x / WAVE;
     WaveLds &w = reinterpret_cast<WaveLds *>(lds_raw)[wid];
+    TermLds *terms = reinterpret_cast<TermLds *>(lds_raw + sizeof(WaveLds) * nwaves) + wid * MAX_TERMS;
     const uint64_t gwave = (uint64_t)blockIdx.x * (blockDim.x / WAVE) + wid;
     uint32_t *wb = P.scratch + gwave * P.wave_stride;
     Acc acc{0, 0, 0, 0, 0};
@@ -583,7 +635,7 @@ __global__ void __launch_bounds__(256) wcoj_kernel(KParams P) {
                 explore_after(P, w, wb, 1, acc);
             }
             if (!ENUM && pl.sib == 1) {
-                count_last(P, w, wb, acc);
+                count_last(P, w, terms, wb, acc);
                 continue;
             }
             if (ENUM && pl.k == 3) {
@@ -604,7 +656,7 @@ __global__ void __launch_bounds__(256) wcoj_kernel(KParams P) {
                             emit_last(P, w, wb, acc);
                         }
                     } else {
-                        count_last(P, w, wb, acc);
+                        count_last(P, w, terms, wb, acc);
                     }
                     --d;
                     continue;
@@ -717,6 +769,35 @@ DPlan to_dplan(const sge_plan &p) {
         d.order[i] = p.order[i];
     }
     uint32_t L = d.last, s = d.sib;
+    d.tail_kind = d.tail ? p.tail_kind : 0;
+    const uint32_t below = (1u << s) - 1u;
+    auto term = [&](uint32_t j, uint32_t members) {
+        uint32_t par = 0, lo = 0, up = 0, ck = 0;
+        for (uint32_t m = members; m; m &= m - 1) {
+            uint32_t lev = __builtin_ctz(m);
+            par |= d.parent[lev];
+            lo |= d.lower[lev];
+            up |= d.upper[lev];
+            ck |= d.check[lev];
+        }
+        d.t_par[j] = par & below;
+        d.t_lower[j] = lo & below;
+        d.t_upper[j] = up & below;
+        d.t_check[j] = ck & below;
+        d.t_sflags[j] = ((par >> s) & 1u) | (((lo >> s) & 1u) << 1) | (((up >> s) & 1u) << 2) | (((ck >> s) & 1u) << 3);
+    };
+    if (d.tail_kind == 2) {
+        d.nterms = (1u << d.tail) - 1u;
+        for (uint32_t B = 1; B <= d.nterms; ++B) {
+            uint32_t members = 0;
+            for (uint32_t j = 0; j < d.tail; ++j)
+                if ((B >> j) & 1u) members |= 1u << (L + j);
+            term(B - 1, members);
+        }
+    } else {
+        d.nterms = 1;
+        term(0, 1u << L);
+    }
     d.f_parent = d.parent[L] & ~(1u << s);
     d.s_is_parent = (d.parent[L] >> s) & 1u;
     d.s_lower = (d.lower[L] >> s) & 1u;
@@ -760,9 +841,37 @@ int validate_plan(const sge_plan &p) {
             return SGE_EINVAL;
         }
     }
-    if (p.mode == SGE_MODE_IEP && p.tail >= 2 && (p.tail >= p.k || p.k - p.tail < 2)) {
-        sge::set_error("plan: IEP tail must leave at least two matched levels");
-        return SGE_EINVAL;
+    if (p.mode == SGE_MODE_IEP && p.tail >= 2) {
+        if (p.tail >= p.k || p.k - p.tail < 2) {
+            sge::set_error("plan: IEP tail must leave at least two matched levels");
+            return SGE_EINVAL;
+        }
+        uint32_t b = p.k - p.tail, tailmask = ((1u << p.k) - 1u) & ~((1u << b) - 1u);
+        if (p.tail_kind == 2) {
+            if (p.tail > 3) {
+                sge::set_error("plan: inclusion-exclusion tails hold at most 3 levels");
+                return SGE_EINVAL;
+            }
+            for (uint32_t i = b; i < p.k; ++i)
+                if ((p.parent[i] | p.lower[i] | p.upper[i]) & tailmask) {
+                    sge::set_error("plan: an inclusion-exclusion tail must be independent and unbounded internally");
+                    return SGE_EINVAL;
+                }
+        } else if (p.tail_kind == 1) {
+            for (uint32_t i = b; i < p.k; ++i) {
+                uint32_t below = (1u << b) - 1u;
+                if ((p.parent[i] & ~below) || (p.parent[i] & below) != (p.parent[b] & below) ||
+                    (p.lower[i] & below) != (p.lower[b] & below) || (p.upper[i] & below) != (p.upper[b] & below) ||
+                    (p.check[i] & below) != (p.check[b] & below) || (p.upper[i] & ~below) ||
+                    (p.lower[i] & ~below) != (((1u << i) - 1u) & ~below)) {
+                    sge::set_error("plan: a choosec tail must be a chained group of twins");
+                    return SGE_EINVAL;
+                }
+            }
+        } else {
+            sge::set_error("plan: tail_kind must be 1 or 2 when tail >= 2");
+            return SGE_EINVAL;
+        }
     }
     return SGE_OK;
 }
@@ -785,7 +894,7 @@ int launch_cfg(const sge_tuning *t, bool enumerate, LaunchCfg &cfg) {
         return SGE_EINVAL;
     }
     cfg.chunk = t->chunk ? t->chunk : 64;
-    cfg.lds = (size_t)(cfg.block / WAVE) * sizeof(WaveLds);
+    cfg.lds = (size_t)(cfg.block / WAVE) * (sizeof(WaveLds) + MAX_TERMS * sizeof(TermLds));
     int per_cu = (int)t->blocks_per_cu;
     if (per_cu <= 0) {
         int occ = 0;
@@ -803,7 +912,7 @@ int ensure_scratch(const sge_graph *g, const LaunchCfg &cfg, KParams &kp) {
     DeviceCtx &c = dctx();
     uint32_t stride = ((g->h.max_degree + 64 + 63) / 64) * 64;
     kp.buf_stride = stride;
-    kp.wave_stride = (uint64_t)stride * (2 * MAXK + 2);
+    kp.wave_stride = (uint64_t)stride * NBUF;
     size_t waves = (size_t)cfg.grid * (cfg.block / WAVE);
     size_t need = waves * kp.wave_stride * sizeof(uint32_t);
     int rc = ensure_buffer((void **)&c.scratch, c.scratch_bytes, need);
@@ -1130,7 +1239,7 @@ extern "C" int sge_batch_create(sge_graph *g, const sge_plan *plans, uint32_t np
     std::memset(&kp, 0, sizeof(kp));
     uint32_t stride = ((g->h.max_degree + 64 + 63) / 64) * 64;
     kp.buf_stride = stride;
-    kp.wave_stride = (uint64_t)stride * (2 * MAXK + 2);
+    kp.wave_stride = (uint64_t)stride * NBUF;
     size_t waves = (size_t)cfg.grid * (cfg.block / WAVE);
     auto fail = [&](hipError_t e, const char *what) {
         sge::set_error(std::string("HIP error: ") + hipGetErrorString(e) + " at " + what);
@@ -1223,5 +1332,44 @@ extern "C" int sge_device_info(int *num_cu, int *num_xcd_hint, char *name, size_
     if (name && cap) {
         std::snprintf(name, cap, "%s", prop.gcnArchName);
     }
+    return SGE_OK;
+}
+
+// ------------------------------------------------------------ calibration
+// A plain streaming copy of a known number of bytes, used to scale the
+// FETCH_SIZE / WRITE_SIZE counters read for the enumeration kernel.
+namespace {
+__global__ void __launch_bounds__(256) calib_copy_kernel(const uint4 *__restrict__ src, uint4 *__restrict__ dst,
+                                                         size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) dst[i] = src[i];
+}
+}  // namespace
+
+extern "C" int sge_calibration_copy(uint64_t bytes, double *ms) {
+    if (bytes < 16 || bytes % 16) {
+        sge::set_error("sge_calibration_copy: bytes must be a positive multiple of 16");
+        return SGE_EINVAL;
+    }
+    std::lock_guard<std::mutex> lk(dmutex());
+    int rc = ensure_ctx();
+    if (rc) return rc;
+    DeviceCtx &c = dctx();
+    uint4 *src = nullptr, *dst = nullptr;
+    HIP_TRY(hipMalloc(&src, bytes));
+    HIP_TRY(hipMalloc(&dst, bytes));
+    size_t n = bytes / 16;
+    int grid = c.num_cu * 8;
+    calib_copy_kernel<<<grid, 256, 0, c.stream>>>(src, dst, n);  // warm (page-in)
+    HIP_TRY(hipEventRecord(c.ev0, c.stream));
+    calib_copy_kernel<<<grid, 256, 0, c.stream>>>(src, dst, n);
+    HIP_TRY(hipEventRecord(c.ev1, c.stream));
+    HIP_TRY(hipStreamSynchronize(c.stream));
+    float t = 0;
+    HIP_TRY(hipEventElapsedTime(&t, c.ev0, c.ev1));
+    if (ms) *ms = t;
+    (void)hipFree(src);
+    (void)hipFree(dst);
     return SGE_OK;
 }

@@ -27,13 +27,37 @@
 #include <functional>
 #include <iterator>
 #include <numeric>
+#include <map>
+#include <mutex>
 #include <set>
 
 #include "common.h"
 
 namespace sge {
 
+static GraphSummary summarize_uncached(const HostGraph *g);
+
+// Degree moments and the sampled closure rate are computed once per graph
+// (keyed by address and a content fingerprint) and reused for every order.
 GraphSummary summarize(const HostGraph *g) {
+    static std::mutex mu;
+    static std::map<const HostGraph *, std::pair<uint64_t, GraphSummary>> cache;
+    if (!g || g->n == 0) return summarize_uncached(g);
+    uint64_t fp = ((uint64_t)g->n << 32) ^ g->nnz ^ ((uint64_t)g->off[g->n / 2] << 17) ^
+                  (g->nnz ? (uint64_t)g->nbr[g->nnz / 2] << 40 : 0);
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        auto it = cache.find(g);
+        if (it != cache.end() && it->second.first == fp) return it->second.second;
+    }
+    GraphSummary s = summarize_uncached(g);
+    std::lock_guard<std::mutex> lk(mu);
+    if (cache.size() > 64) cache.clear();
+    cache[g] = {fp, s};
+    return s;
+}
+
+static GraphSummary summarize_uncached(const HostGraph *g) {
     GraphSummary s;
     if (!g || g->n == 0) {
         s.n = 1000;
@@ -98,53 +122,6 @@ static void symmetry_masks(const Pattern &p, const int *order, const std::vector
     }
 }
 
-// Analytic cost of one order, following the executor's structure: root tasks
-// are directed edges; levels 2..s are materialised per parent node (work = the
-// clamped parent lists); the last level is counted for all siblings at once,
-// with its sibling-independent part F formed once per node of level s-1.
-// Sizes use degree moments (first list of a node reached through an edge has
-// the size-biased degree E[d^2]/E[d]) and a sampled wedge-closure rate q for
-// every further list in an intersection; a symmetry clamp halves a set.
-double order_cost(const Pattern &p, const GraphSummary &s, const int *order, const std::vector<std::vector<int>> &aut) {
-    int k = p.k;
-    if (k <= 2) return 1.0;
-    uint32_t lower[SGE_MAX_K];
-    symmetry_masks(p, order, aut, lower);
-    uint32_t par[SGE_MAX_K] = {0};
-    for (int i = 1; i < k; ++i)
-        for (int j = 0; j < i; ++j)
-            if (p.edge(order[i], order[j])) par[i] |= 1u << j;
-    for (int i = 1; i < k; ++i)
-        if (!par[i]) return 1e300;
-    double q = std::min(0.9, std::max(1e-4, s.tri_per_wedge));
-    double d2 = std::max(1.0, s.d2);
-    auto set_size = [&](int i) {
-        int np = __builtin_popcount(par[i]);
-        double sz = d2 * std::pow(q, np - 1);
-        if (lower[i]) sz *= 0.5;
-        return std::max(sz, 1e-6);
-    };
-    double card = 2.0 * s.m * ((lower[1] & 1u) ? 0.5 : 1.0);  // nodes at level 1
-    double cost = card * 2.0;                                 // root task overhead
-    int last = k - 1, sib = k - 2;
-    for (int i = 2; i <= sib; ++i) {
-        int np = __builtin_popcount(par[i]);
-        cost += card * (np * d2 * (lower[i] ? 0.6 : 1.0) + 8.0);  // materialise level i
-        card *= set_size(i);
-    }
-    // last level: F once per node of level sib-1, then one probe loop per sibling
-    uint32_t fpar = par[last] & ~(1u << sib);
-    bool s_par = (par[last] >> sib) & 1u;
-    double parent_nodes = card / std::max(set_size(sib), 1e-6);
-    int nf = __builtin_popcount(fpar);
-    double fsize = nf ? d2 * std::pow(q, nf - 1) : 0.0;
-    if (nf) cost += parent_nodes * nf * d2;
-    double per_sib = s_par ? (nf ? std::min(fsize, d2) * std::log2(std::max(fsize, d2) + 2.0) : std::log2(d2 + 2.0))
-                           : std::log2(fsize + 2.0);
-    cost += card * (per_sib + 4.0);
-    return cost;
-}
-
 std::vector<std::vector<int>> connected_orders(const Pattern &p, const std::vector<std::vector<int>> &aut) {
     std::vector<std::vector<int>> out;
     std::set<std::vector<int>> seen;
@@ -181,7 +158,11 @@ static uint32_t largest_twin_group(const Pattern &p) {
     return __builtin_popcount(best) >= 2 ? best : 0;
 }
 
-static bool valid_tail(const sge_plan &pl, int r) {
+// Trailing symmetric group (kind 1): the last r levels are pairwise
+// non-adjacent twins that share every parent, bound and check below the group
+// and are chained among themselves; choosec(|C|, r) counts them
+// (decomposition.cpp:1022-1047, join.cpp:1532-1537).
+static bool valid_sym_tail(const sge_plan &pl, int r) {
     int k = pl.k, b = k - r;
     if (r < 2 || b < 2) return false;  // the root edge (levels 0, 1) is never part of the group
     uint32_t below = (1u << b) - 1;
@@ -196,6 +177,39 @@ static bool valid_tail(const sge_plan &pl, int r) {
         if ((pl.upper[i] & ~below) != 0) return false;
     }
     return true;
+}
+
+// Independent tail (kind 2): the last r levels are pairwise non-adjacent and
+// carry no symmetry bound among themselves, so for fixed images of the levels
+// below, the number of injective assignments is an inclusion-exclusion sum
+// over set partitions of the tail of products of intersection sizes.
+static bool valid_ie_tail(const sge_plan &pl, int r) {
+    int k = pl.k, b = k - r;
+    if (r < 2 || r > 3 || b < 2) return false;
+    uint32_t below = (1u << b) - 1;
+    for (int i = b; i < k; ++i) {
+        if ((pl.parent[i] | pl.lower[i] | pl.upper[i]) & ~below) return false;
+    }
+    return true;
+}
+
+static void choose_tail(sge_plan &pl) {
+    pl.tail = 0;
+    pl.tail_kind = 0;
+    if (pl.mode != SGE_MODE_IEP) return;
+    int k = pl.k;
+    int sym = 0, ie = 0;
+    for (int r = k - 1; r >= 2 && !sym; --r)
+        if (valid_sym_tail(pl, r)) sym = r;
+    for (int r = 3; r >= 2 && !ie; --r)
+        if (valid_ie_tail(pl, r)) ie = r;
+    if (sym && sym >= ie) {
+        pl.tail = sym;
+        pl.tail_kind = 1;
+    } else if (ie) {
+        pl.tail = ie;
+        pl.tail_kind = 2;
+    }
 }
 
 static void fill_masks(const Pattern &p, const int *order, const std::vector<std::vector<int>> &aut, uint32_t mode,
@@ -235,14 +249,73 @@ static void fill_masks(const Pattern &p, const int *order, const std::vector<std
         }
         pl.reuse[j] = best;
     }
-    pl.tail = 0;
-    if (mode == SGE_MODE_IEP) {
-        for (int r = k - 1; r >= 2; --r)
-            if (valid_tail(pl, r)) {
-                pl.tail = r;
-                break;
-            }
+    choose_tail(pl);
+}
+
+// Analytic cost of a plan, following the executor's structure: root tasks are
+// directed edges; levels 2..s are materialised per parent node (work = the
+// clamped parent lists); the closed-form level(s) are counted for all
+// siblings of level s at once, their sibling-independent parts formed once per
+// node of level s-1 (one per inclusion-exclusion term).  Sizes use degree
+// moments (a list reached through an edge has the size-biased degree
+// E[d^2]/E[d]) and a sampled wedge-closure rate q per extra list; a symmetry
+// bound halves a set.
+static double plan_cost(const sge_plan &pl, const GraphSummary &s) {
+    int k = pl.k;
+    if (k <= 2) return 1.0;
+    double q = std::min(0.9, std::max(1e-4, s.tri_per_wedge));
+    double d2 = std::max(1.0, s.d2);
+    auto set_size = [&](uint32_t par, bool clamped) {
+        int np = __builtin_popcount(par);
+        double sz = d2 * std::pow(q, std::max(np - 1, 0));
+        if (clamped) sz *= 0.5;
+        return std::max(sz, 1e-6);
+    };
+    int r = pl.tail >= 2 ? (int)pl.tail : 1;
+    int first = k - r;  // first closed-form level
+    int sib = first - 1;
+    double card = 2.0 * s.m * ((pl.lower[1] & 1u) ? 0.5 : 1.0);  // nodes at level 1
+    double cost = card * 2.0;
+    for (int i = 2; i <= sib; ++i) {
+        int np = __builtin_popcount(pl.parent[i]);
+        cost += card * (np * d2 * (pl.lower[i] ? 0.6 : 1.0) + 8.0);
+        card *= set_size(pl.parent[i], pl.lower[i] != 0);
     }
+    double parent_nodes = card / set_size(pl.parent[sib], pl.lower[sib] != 0);
+    // terms: one per non-empty subset of an independent tail, else one
+    int nterms = (pl.tail_kind == 2) ? (1 << r) - 1 : 1;
+    for (int t = 1; t <= nterms; ++t) {
+        uint32_t par = 0;
+        if (pl.tail_kind == 2) {
+            for (int j = 0; j < r; ++j)
+                if ((t >> j) & 1) par |= pl.parent[first + j];
+        } else {
+            par = pl.parent[first];
+        }
+        uint32_t fpar = par & ~(1u << sib);
+        bool s_par = (par >> sib) & 1u;
+        int nf = __builtin_popcount(fpar);
+        double fsize = nf ? d2 * std::pow(q, nf - 1) : 0.0;
+        if (nf) cost += parent_nodes * nf * d2;
+        double per_sib = s_par ? (nf ? std::min(fsize, d2) * std::log2(std::max(fsize, d2) + 2.0)
+                                     : std::log2(d2 + 2.0))
+                               : std::log2(fsize + 2.0);
+        cost += card * (per_sib + 4.0);
+    }
+    return cost;
+}
+
+double order_cost(const Pattern &p, const GraphSummary &s, const int *order, const std::vector<std::vector<int>> &aut,
+                  uint32_t mode) {
+    for (int i = 1; i < p.k; ++i) {
+        uint32_t placed = 0;
+        for (int j = 0; j < i; ++j) placed |= 1u << order[j];
+        if (!(p.adj[order[i]] & placed)) return 1e300;
+    }
+    sge_plan pl;
+    std::memset(&pl, 0, sizeof(pl));
+    fill_masks(p, order, aut, mode, pl);
+    return plan_cost(pl, s);
 }
 
 int build_plan(const Pattern &p, const HostGraph *g, uint32_t mode, const uint32_t *order_in, sge_plan &out) {
@@ -282,26 +355,23 @@ int build_plan(const Pattern &p, const HostGraph *g, uint32_t mode, const uint32
                 return SGE_EINVAL;
             }
         }
-        cost = order_cost(p, s, order, aut);
+        cost = order_cost(p, s, order, aut, mode);
     } else {
         auto orders = connected_orders(p, aut);
-        uint32_t twins = (mode == SGE_MODE_IEP && !d.single_bag) ? largest_twin_group(p) : 0;
         double best = 1e308;
-        int besti = -1;
-        for (int pass = 0; pass < 2 && besti < 0; ++pass) {
-            for (size_t i = 0; i < orders.size(); ++i) {
-                const auto &o = orders[i];
-                if (twins && pass == 0) {
-                    int r = __builtin_popcount(twins);
-                    bool ok = true;
-                    for (int j = p.k - r; j < p.k; ++j) ok = ok && ((twins >> o[j]) & 1u);
-                    if (!ok) continue;
-                }
-                double c = order_cost(p, s, o.data(), aut);
-                if (c < best) {
-                    best = c;
-                    besti = (int)i;
-                }
+        int besti = 0, best_tail = -1;
+        for (size_t i = 0; i < orders.size(); ++i) {
+            sge_plan trial;
+            std::memset(&trial, 0, sizeof(trial));
+            fill_masks(p, orders[i].data(), aut, mode, trial);
+            double c = plan_cost(trial, s);
+            // IEP: a closed-form tail removes whole enumeration levels, so the
+            // longest tail wins and the cost model breaks ties
+            int tl = (int)trial.tail;
+            if (tl > best_tail || (tl == best_tail && c < best)) {
+                best = c;
+                besti = (int)i;
+                best_tail = tl;
             }
         }
         for (int i = 0; i < p.k; ++i) order[i] = orders[besti][i];
@@ -328,7 +398,8 @@ extern "C" int sge_plan_build(const sge_pattern *p, const sge_graph *g, uint32_t
     return build_plan(from_c(*p), g ? &g->h : nullptr, mode, order, *out);
 }
 
-extern "C" int sge_plan_cost(const sge_pattern *cp, const sge_graph *g, const uint32_t *order, double *cost) {
+extern "C" int sge_plan_cost(const sge_pattern *cp, const sge_graph *g, uint32_t mode, const uint32_t *order,
+                             double *cost) {
     if (!cp || !order || !cost) {
         set_error("sge_plan_cost: null argument");
         return SGE_EINVAL;
@@ -337,7 +408,7 @@ extern "C" int sge_plan_cost(const sge_pattern *cp, const sge_graph *g, const ui
     auto aut = automorphisms(p);
     int o[SGE_MAX_K];
     for (int i = 0; i < p.k; ++i) o[i] = (int)order[i];
-    *cost = order_cost(p, summarize(g ? &g->h : nullptr), o, aut);
+    *cost = order_cost(p, summarize(g ? &g->h : nullptr), o, aut, mode);
     return SGE_OK;
 }
 

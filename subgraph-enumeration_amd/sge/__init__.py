@@ -48,7 +48,8 @@ class Plan(C.Structure):
     _fields_ = [("k", C.c_uint32), ("mode", C.c_uint32), ("order", C.c_uint32 * MAX_K),
                 ("parent", C.c_uint32 * MAX_K), ("lower", C.c_uint32 * MAX_K), ("upper", C.c_uint32 * MAX_K),
                 ("check", C.c_uint32 * MAX_K), ("reuse", C.c_int32 * MAX_K), ("ready", C.c_uint32 * MAX_K),
-                ("tail", C.c_uint32), ("num_bags", C.c_uint32), ("multiplier", C.c_uint64),
+                ("tail", C.c_uint32), ("tail_kind", C.c_uint32), ("num_bags", C.c_uint32),
+                ("multiplier", C.c_uint64),
                 ("aut_size", C.c_uint64), ("cost", C.c_double)]
 
     def as_dict(self):
@@ -56,6 +57,7 @@ class Plan(C.Structure):
         return {"k": k, "mode": self.mode, "order": list(self.order)[:k], "parent": list(self.parent)[:k],
                 "lower": list(self.lower)[:k], "upper": list(self.upper)[:k], "check": list(self.check)[:k],
                 "reuse": list(self.reuse)[:k], "ready": list(self.ready)[:k], "tail": self.tail,
+                "tail_kind": self.tail_kind,
                 "num_bags": self.num_bags, "multiplier": self.multiplier, "aut_size": self.aut_size,
                 "cost": self.cost}
 
@@ -99,7 +101,7 @@ _SIGS = {
                                      C.c_uint32, C.POINTER(CPattern), C.POINTER(C.c_uint32)]),
     "sge_canonicalize_embeddings": (C.c_int, [C.POINTER(CPattern), C.c_void_p, C.c_uint64]),
     "sge_plan_build": (C.c_int, [C.POINTER(CPattern), C.c_void_p, C.c_uint32, C.c_void_p, C.POINTER(Plan)]),
-    "sge_plan_cost": (C.c_int, [C.POINTER(CPattern), C.c_void_p, C.c_void_p, C.POINTER(C.c_double)]),
+    "sge_plan_cost": (C.c_int, [C.POINTER(CPattern), C.c_void_p, C.c_uint32, C.c_void_p, C.POINTER(C.c_double)]),
     "sge_plan_orders": (C.c_int, [C.POINTER(CPattern), C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32)]),
     "sge_plan_text": (C.c_int, [C.POINTER(Plan), C.c_char_p, C.c_size_t]),
     "sge_tuning_default": (None, [C.POINTER(Tuning)]),
@@ -129,6 +131,7 @@ _SIGS = {
     "sge_batch_run": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Stats)]),
     "sge_batch_destroy": (None, [C.c_void_p]),
     "sge_device_info": (C.c_int, [C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_char_p, C.c_size_t]),
+    "sge_calibration_copy": (C.c_int, [C.c_uint64, C.POINTER(C.c_double)]),
 }
 
 
@@ -296,10 +299,10 @@ def build_plan(pattern: Pattern, graph: Optional[Graph] = None, mode: int = MODE
     return p
 
 
-def order_cost(pattern: Pattern, graph: Optional[Graph], order: Sequence[int]) -> float:
+def order_cost(pattern: Pattern, graph: Optional[Graph], order: Sequence[int], mode: int = MODE_WCOJ) -> float:
     o = np.ascontiguousarray(np.asarray(order, dtype=np.uint32))
     c = C.c_double()
-    _check(lib().sge_plan_cost(C.byref(pattern.c), graph._h if graph else None, _ptr(o), C.byref(c)),
+    _check(lib().sge_plan_cost(C.byref(pattern.c), graph._h if graph else None, mode, _ptr(o), C.byref(c)),
            "sge_plan_cost")
     return c.value
 
@@ -467,6 +470,12 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+def calibration_copy(nbytes: int) -> float:
+    ms = C.c_double()
+    _check(lib().sge_calibration_copy(nbytes, C.byref(ms)), "sge_calibration_copy")
+    return ms.value
 
 
 def device_info():
