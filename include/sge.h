@@ -164,7 +164,9 @@ typedef struct sge_tuning {
     uint32_t chunk;           /* root tasks taken per queue fetch */
     uint32_t stage_max;       /* stage a probed list window in LDS when <= this many ids */
     uint32_t blocks_per_cu;   /* 0 = occupancy-derived */
-    uint32_t reserved[4];
+    uint32_t split_span;      /* a root edge whose level-2 set may exceed this is split into
+                                 ceil(bound / split_span) subtasks (0 = default 64, 1 << 30 = off) */
+    uint32_t reserved[3];
 } sge_tuning;
 
 typedef struct sge_stats {
@@ -204,6 +206,9 @@ int sge_enumerate(sge_graph *g, const sge_plan *plan, uint32_t *out, uint64_t ca
 /* Batched executor captured once in a hipGraph, replayed per run. */
 typedef struct sge_batch sge_batch;
 int sge_batch_create(sge_graph *g, const sge_plan *plans, uint32_t nplans, const sge_tuning *tune, sge_batch **out);
+/* Same over the strided root-edge sample {offset, offset+stride, ...}. */
+int sge_batch_create_sampled(sge_graph *g, const sge_plan *plans, uint32_t nplans, uint64_t stride, uint64_t offset,
+                             const sge_tuning *tune, sge_batch **out);
 int sge_batch_run(sge_batch *b, uint64_t *counts, sge_stats *stats);
 void sge_batch_destroy(sge_batch *b);
 

@@ -32,6 +32,9 @@ def lib():
         L.oracle_count_range.restype = C.c_uint64
         L.oracle_count_range.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint64, C.c_uint64,
                                          C.c_int, C.POINTER(C.c_uint64)]
+        L.oracle_count_strided.restype = C.c_uint64
+        L.oracle_count_strided.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint64, C.c_uint64,
+                                           C.c_int]
         L.oracle_count.restype = C.c_uint64
         L.oracle_count.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_int]
         L.oracle_enumerate.restype = C.c_uint64
@@ -78,6 +81,11 @@ def count(plan, off, nbr, threads=0, begin=0, end=None):
         end = int(off[-1])
     inter = C.c_uint64()
     return int(lib().oracle_count_range(C.byref(plan), _p(off), _p(nbr), n, begin, end, threads, C.byref(inter)))
+
+
+def count_strided(plan, off, nbr, stride, offset=0, threads=0):
+    off, nbr = _u32(off), _u32(nbr)
+    return int(lib().oracle_count_strided(C.byref(plan), _p(off), _p(nbr), len(off) - 1, stride, offset, threads))
 
 
 def enumerate_embeddings(plan, off, nbr, cap):

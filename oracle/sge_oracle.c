@@ -388,6 +388,33 @@ uint64_t oracle_count_range(const sge_plan *pl, const uint32_t *off, const uint3
     return total * pl->multiplier;
 }
 
+/* Count over the strided root sample {offset, offset+stride, ...}. */
+uint64_t oracle_count_strided(const sge_plan *pl, const uint32_t *off, const uint32_t *nbr, uint32_t n,
+                              uint64_t stride, uint64_t offset, int threads) {
+    uint64_t total = 0, nnz = off[n], ntask = offset < nnz ? (nnz - offset + stride - 1) / stride : 0, i;
+    int k = (int)pl->k;
+    if (k < 2) return 0;
+#ifdef _OPENMP
+    if (threads > 0) omp_set_num_threads(threads);
+#endif
+#pragma omp parallel reduction(+ : total)
+    {
+        ctx_t c;
+        ctx_init(&c, pl, off, nbr, n);
+#pragma omp for schedule(dynamic, 16)
+        for (i = 0; i < ntask; ++i) {
+            uint64_t t = offset + i * stride;
+            uint32_t v0 = src_of(off, n, t), v1 = nbr[t];
+            if (((pl->lower[1] & 1u) && !(v1 > v0)) || ((pl->upper[1] & 1u) && !(v1 < v0))) continue;
+            c.phi[0] = v0;
+            c.phi[1] = v1;
+            total += (k == 2) ? 1 : dfs(&c, 2);
+        }
+        ctx_free(&c);
+    }
+    return total * pl->multiplier;
+}
+
 uint64_t oracle_count(const sge_plan *pl, const uint32_t *off, const uint32_t *nbr, uint32_t n, int threads) {
     return oracle_count_range(pl, off, nbr, n, 0, off[n], threads, NULL);
 }

@@ -97,6 +97,7 @@ struct sge_graph {
     // device mirror (executor.hip)
     void *d_off = nullptr, *d_nbr = nullptr, *d_src = nullptr;
     int device = -1;
+    void *split_cache = nullptr;  // executor-owned: per-shape subtask prefix arrays on the device
     sge::GraphSummary summary;
     bool summary_ready = false;
 };

@@ -33,7 +33,9 @@
 #include <chrono>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <mutex>
+#include <tuple>
 #include <string>
 #include <vector>
 
@@ -76,6 +78,11 @@ struct DPlan {
     // bit2: upper bound from s, bit3: must differ from s's image.
     uint32_t tail_kind, nterms;
     uint32_t t_par[MAX_TERMS], t_lower[MAX_TERMS], t_upper[MAX_TERMS], t_check[MAX_TERMS], t_sflags[MAX_TERMS];
+    // root-task split: split[i] = first subtask of sampled root edge i (prefix
+    // over ntask + 1 entries); nsub = total subtasks.  Subtask (i, ci) of
+    // split[i+1] - split[i] processes slice ci of the level-2 candidates.
+    const unsigned long long *split;
+    unsigned long long nsub;
 };
 
 struct PlanOut {
@@ -595,7 +602,6 @@ __global__ void __launch_bounds__(256) wcoj_kernel(KParams P) {
     Acc acc{0, 0, 0, 0, 0};
     uint32_t cur_plan = NONE;
     const uint64_t ntask = (P.task_end - P.task_begin + P.stride - 1) / P.stride;
-    const uint64_t chunks = (ntask + P.chunk - 1) / P.chunk;
     while (true) {
         unsigned long long item = 0;
         if (lane == 0) item = atomicAdd(P.queue, 1ull);
@@ -603,7 +609,6 @@ __global__ void __launch_bounds__(256) wcoj_kernel(KParams P) {
         if (item >= P.nitems) break;
         uint32_t plan = (uint32_t)(item % P.nplans);
         uint64_t ch = item / P.nplans;
-        if (ch >= chunks) continue;
         if (plan != cur_plan) {
             if (cur_plan != NONE) flush(P, cur_plan, acc);
             cur_plan = plan;
@@ -615,15 +620,29 @@ __global__ void __launch_bounds__(256) wcoj_kernel(KParams P) {
         }
         const DPlan &pl = w.pl;
         if (pl.k < 2) continue;  // never launched by the host; no edge tasks exist for it
+        const uint64_t nsub = pl.split ? pl.nsub : ntask;
+        if (ch * P.chunk >= nsub) continue;
         // heavy-first: chunk 0 covers the highest CSR positions
-        uint64_t hi_i = ntask - ch * P.chunk;
-        uint64_t lo_i = hi_i > P.chunk ? hi_i - P.chunk : 0;
-        for (uint64_t ti = lo_i; ti < hi_i; ++ti) {
+        uint64_t hi_u = nsub - ch * P.chunk;
+        uint64_t lo_u = hi_u > P.chunk ? hi_u - P.chunk : 0;
+        for (uint64_t u = lo_u; u < hi_u; ++u) {
+            uint64_t ti = u, ci = 0, nch = 1;
+            if (pl.split) {
+                // sampled root edge owning subtask u: last i with split[i] <= u
+                uint64_t a = 0, b = ntask;
+                while (b - a > 1) {
+                    uint64_t m = (a + b) >> 1;
+                    if (pl.split[m] <= u) a = m; else b = m;
+                }
+                ti = a;
+                ci = u - pl.split[a];
+                nch = pl.split[a + 1] - pl.split[a];
+            }
             const uint64_t t = P.task_begin + ti * P.stride;
             uint32_t v0 = P.src[t], v1 = P.nbr[t];
             if ((pl.lower[1] & 1u) && !(v1 > v0)) continue;
             if ((pl.upper[1] & 1u) && !(v1 < v0)) continue;
-            acc.tasks += 1;
+            if (ci == 0) acc.tasks += 1;
             w.phi[0] = v0;
             w.phi[1] = v1;
             if (pl.k < 3) {  // an edge pattern counts its root tasks; k = 1 is counted on the host
@@ -644,6 +663,14 @@ __global__ void __launch_bounds__(256) wcoj_kernel(KParams P) {
             }
             const int stop = ENUM ? (int)pl.k - 2 : (int)pl.sib;  // level whose set feeds the leaf step
             materialize(P, w, wb, 2, acc);
+            if (nch > 1) {  // this subtask owns slice ci of the level-2 candidates
+                Src c2 = w.cand[2];
+                uint64_t len = c2.e - c2.b;
+                uint32_t sb = c2.b + (uint32_t)(len * ci / nch), se = c2.b + (uint32_t)(len * (ci + 1) / nch);
+                c2.b = sb;
+                c2.e = se;
+                w.cand[2] = c2;
+            }
             int d = 2;
             while (d >= 2) {
                 if (d == stop) {
@@ -932,7 +959,7 @@ __global__ void reset_kernel(PlanOut *out, uint32_t n, unsigned long long *queue
 
 // One counting launch over `np` device plans (already uploaded at `dplans`).
 int run_launch(sge_graph *g, const DPlan *dplans, uint32_t np, PlanOut *dout, uint64_t tb, uint64_t te,
-               uint64_t stride, const LaunchCfg &cfg, bool enumerate, uint32_t *emb, uint64_t emb_cap, unsigned long long *emb_n,
+               uint64_t stride, uint64_t max_sub, const LaunchCfg &cfg, bool enumerate, uint32_t *emb, uint64_t emb_cap, unsigned long long *emb_n,
                float *ms) {
     DeviceCtx &c = dctx();
     KParams kp;
@@ -948,8 +975,7 @@ int run_launch(sge_graph *g, const DPlan *dplans, uint32_t np, PlanOut *dout, ui
     kp.task_end = te;
     kp.stride = stride ? stride : 1;
     kp.chunk = cfg.chunk;
-    uint64_t ntask = (te - tb + kp.stride - 1) / kp.stride;
-    uint64_t chunks = (ntask + cfg.chunk - 1) / cfg.chunk;
+    uint64_t chunks = (max_sub + cfg.chunk - 1) / cfg.chunk;
     kp.nitems = chunks * np;
     kp.queue = c.queue;
     kp.out = dout;
@@ -976,9 +1002,85 @@ int run_launch(sge_graph *g, const DPlan *dplans, uint32_t np, PlanOut *dout, ui
     return SGE_OK;
 }
 
-int upload_plans(const sge_plan *plans, uint32_t n, bool enumerate) {
+// ---- root-task split prefixes (cached per graph)
+struct SplitKey {
+    uint32_t shape;  // parent mask of level 2 (1, 2, 3), or 0: no level-2 iteration
+    uint32_t l1;     // level-1 bound bits: 1 lower, 2 upper
+    uint64_t tb, te, stride;
+    uint32_t span;
+    bool operator<(const SplitKey &o) const {
+        return std::tie(shape, l1, tb, te, stride, span) < std::tie(o.shape, o.l1, o.tb, o.te, o.stride, o.span);
+    }
+};
+struct SplitArr {
+    unsigned long long *d = nullptr;
+    unsigned long long nsub = 0;
+};
+using SplitCache = std::map<SplitKey, SplitArr>;
+
+void free_split_cache(sge_graph *g) {
+    if (!g->split_cache) return;
+    auto *c = static_cast<SplitCache *>(g->split_cache);
+    for (auto &kv : *c)
+        if (kv.second.d) (void)hipFree(kv.second.d);
+    delete c;
+    g->split_cache = nullptr;
+}
+
+// Subtasks per sampled root edge: 0 when the edge fails the level-1 bound,
+// else ceil(bound / span) clamped to [1, 4096], where bound is the size bound of
+// the level-2 candidate set given by the endpoint degrees.
+int get_split(sge_graph *g, const sge_plan &p, uint64_t tb, uint64_t te, uint64_t stride, uint32_t span,
+              const unsigned long long **out, unsigned long long *nsub) {
+    *out = nullptr;
+    *nsub = 0;
+    SplitKey key;
+    key.shape = (p.k >= 4 && !(p.mode == SGE_MODE_IEP && p.tail >= 2 && p.k - p.tail <= 2)) ? (p.parent[2] & 3u) : 0u;
+    key.l1 = ((p.lower[1] & 1u) ? 1u : 0u) | ((p.upper[1] & 1u) ? 2u : 0u);
+    key.tb = tb;
+    key.te = te;
+    key.stride = stride;
+    key.span = span;
+    if (key.shape == 0 || span >= (1u << 30)) return SGE_OK;  // no level-2 iteration: no split
+    if (!g->split_cache) g->split_cache = new SplitCache();
+    auto *cache = static_cast<SplitCache *>(g->split_cache);
+    auto it = cache->find(key);
+    if (it == cache->end()) {
+        const sge::HostGraph &h = g->h;
+        uint64_t ntask = (te - tb + stride - 1) / stride;
+        std::vector<unsigned long long> pre(ntask + 1, 0);
+        uint32_t v = 0;
+        for (uint64_t i = 0; i < ntask; ++i) {
+            uint64_t t = tb + i * stride;
+            while (h.off[v + 1] <= t) ++v;
+            uint32_t v0 = v, v1 = h.nbr[t];
+            uint64_t nch;
+            if (((key.l1 & 1u) && !(v1 > v0)) || ((key.l1 & 2u) && !(v1 < v0))) {
+                nch = 0;
+            } else {
+                uint32_t d0 = h.deg(v0), d1 = h.deg(v1);
+                uint32_t bound = key.shape == 3 ? std::min(d0, d1) : (key.shape == 1 ? d0 : d1);
+                nch = std::min<uint64_t>(4096, std::max<uint64_t>(1, (bound + span - 1) / span));
+            }
+            pre[i + 1] = pre[i] + nch;
+        }
+        SplitArr a;
+        a.nsub = pre[ntask];
+        HIP_TRY(hipMalloc(&a.d, sizeof(unsigned long long) * (ntask + 1)));
+        HIP_TRY(hipMemcpy(a.d, pre.data(), sizeof(unsigned long long) * (ntask + 1), hipMemcpyHostToDevice));
+        it = cache->emplace(key, a).first;
+    }
+    *out = it->second.d;
+    *nsub = it->second.nsub;
+    return SGE_OK;
+}
+
+int upload_plans(sge_graph *g, const sge_plan *plans, uint32_t n, bool enumerate, uint64_t tb, uint64_t te,
+                 uint64_t stride, uint32_t span, std::vector<uint64_t> &nsub) {
     DeviceCtx &c = dctx();
     std::vector<DPlan> dp(n);
+    nsub.assign(n, 0);
+    uint64_t ntask = (te - tb + stride - 1) / stride;
     for (uint32_t i = 0; i < n; ++i) {
         int rc = validate_plan(plans[i]);
         if (rc) return rc;
@@ -986,8 +1088,18 @@ int upload_plans(const sge_plan *plans, uint32_t n, bool enumerate) {
         if (enumerate) {
             p.mode = SGE_MODE_WCOJ;
             p.tail = 0;
+            p.tail_kind = 0;
         }
         dp[i] = to_dplan(p);
+        const unsigned long long *sp = nullptr;
+        unsigned long long ns = 0;
+        if (p.k >= 2 && ntask) {
+            rc = get_split(g, p, tb, te, stride, span, &sp, &ns);
+            if (rc) return rc;
+        }
+        dp[i].split = sp;
+        dp[i].nsub = ns;
+        nsub[i] = sp ? ns : ntask;
     }
     int rc = ensure_buffer((void **)&c.plans, c.plans_cap, sizeof(DPlan) * std::max<uint32_t>(n, 1));
     if (rc) return rc;
@@ -996,6 +1108,8 @@ int upload_plans(const sge_plan *plans, uint32_t n, bool enumerate) {
     HIP_TRY(hipMemcpy(c.plans, dp.data(), sizeof(DPlan) * n, hipMemcpyHostToDevice));
     return SGE_OK;
 }
+
+uint32_t span_of(const sge_tuning *t) { return (t && t->split_span) ? t->split_span : 64u; }
 
 void fill_stats(const PlanOut &o, double ms, sge_stats *s) {
     if (!s) return;
@@ -1048,6 +1162,7 @@ extern "C" void sge_graph_device_free(sge_graph *g) {
     if (g->d_nbr) (void)hipFree(g->d_nbr);
     if (g->d_src) (void)hipFree(g->d_src);
     g->d_off = g->d_nbr = g->d_src = nullptr;
+    free_split_cache(g);
 }
 
 static int count_impl(sge_graph *g, const sge_plan *plans, uint32_t nplans, uint32_t flags, const sge_tuning *tune,
@@ -1064,11 +1179,15 @@ static int count_impl(sge_graph *g, const sge_plan *plans, uint32_t nplans, uint
     DeviceCtx &c = dctx();
     if (te > g->h.nnz) te = g->h.nnz;
     if (tb > te) tb = te;
-    rc = upload_plans(plans, nplans, false);
+    if (stride == 0) stride = 1;
+    std::vector<uint64_t> nsub;
+    rc = upload_plans(g, plans, nplans, false, tb, te, stride, span_of(tune), nsub);
     if (rc) return rc;
     LaunchCfg cfg;
     rc = launch_cfg(tune, false, cfg);
     if (rc) return rc;
+    uint64_t max_sub = 0;
+    for (uint64_t x : nsub) max_sub = std::max(max_sub, x);
     if (stats) std::memset(stats, 0, sizeof(*stats));
     if (per_plan) std::memset(per_plan, 0, sizeof(sge_stats) * nplans);
     std::vector<PlanOut> host(nplans);
@@ -1077,7 +1196,7 @@ static int count_impl(sge_graph *g, const sge_plan *plans, uint32_t nplans, uint
     for (uint32_t i = 0; i < nplans; ++i) any_k1 = any_k1 || plans[i].k < 2;
     if ((flags & SGE_COUNT_BATCHED) && !any_k1) {
         float t = 0;
-        rc = run_launch(g, c.plans, nplans, c.out, tb, te, stride, cfg, false, nullptr, 0, nullptr, &t);
+        rc = run_launch(g, c.plans, nplans, c.out, tb, te, stride, max_sub, cfg, false, nullptr, 0, nullptr, &t);
         if (rc) return rc;
         HIP_TRY(hipMemcpy(host.data(), c.out, sizeof(PlanOut) * nplans, hipMemcpyDeviceToHost));
         if (stats) stats->kernel_ms = t;
@@ -1091,7 +1210,8 @@ static int count_impl(sge_graph *g, const sge_plan *plans, uint32_t nplans, uint
                 HIP_TRY(hipMemsetAsync(c.out + i, 0, sizeof(PlanOut), c.stream));
                 continue;
             }
-            rc = run_launch(g, c.plans + i, 1, c.out + i, tb, te, stride, cfg, false, nullptr, 0, nullptr, &ms[i]);
+            rc = run_launch(g, c.plans + i, 1, c.out + i, tb, te, stride, nsub[i], cfg, false, nullptr, 0, nullptr,
+                            &ms[i]);
             if (rc) return rc;
         }
         HIP_TRY(hipStreamSynchronize(c.stream));
@@ -1165,7 +1285,8 @@ extern "C" int sge_enumerate(sge_graph *g, const sge_plan *plan, uint32_t *out, 
     if (rc) return rc;
     std::lock_guard<std::mutex> lk(dmutex());
     DeviceCtx &c = dctx();
-    rc = upload_plans(plan, 1, true);
+    std::vector<uint64_t> nsub;
+    rc = upload_plans(g, plan, 1, true, 0, g->h.nnz, 1, span_of(nullptr), nsub);
     if (rc) return rc;
     LaunchCfg cfg;
     rc = launch_cfg(nullptr, true, cfg);
@@ -1176,7 +1297,7 @@ extern "C" int sge_enumerate(sge_graph *g, const sge_plan *plan, uint32_t *out, 
     HIP_TRY(hipMalloc(&d_emb, bytes));
     HIP_TRY(hipMalloc(&d_n, sizeof(unsigned long long)));
     float t = 0;
-    rc = run_launch(g, c.plans, 1, c.out, 0, g->h.nnz, 1, cfg, true, d_emb, cap, d_n, &t);
+    rc = run_launch(g, c.plans, 1, c.out, 0, g->h.nnz, 1, nsub[0], cfg, true, d_emb, cap, d_n, &t);
     unsigned long long n = 0;
     if (!rc) {
         HIP_TRY(hipMemcpy(&n, d_n, sizeof(n), hipMemcpyDeviceToHost));
@@ -1209,8 +1330,17 @@ struct sge_batch {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
 
+extern "C" int sge_batch_create_sampled(sge_graph *g, const sge_plan *plans, uint32_t nplans, uint64_t stride,
+                                        uint64_t offset, const sge_tuning *tune, sge_batch **out);
+
 extern "C" int sge_batch_create(sge_graph *g, const sge_plan *plans, uint32_t nplans, const sge_tuning *tune,
                                 sge_batch **out) {
+    return sge_batch_create_sampled(g, plans, nplans, 1, 0, tune, out);
+}
+
+extern "C" int sge_batch_create_sampled(sge_graph *g, const sge_plan *plans, uint32_t nplans, uint64_t stride,
+                                        uint64_t offset, const sge_tuning *tune, sge_batch **out) {
+    if (stride == 0) stride = 1;
     if (!g || !plans || !nplans || !out) {
         sge::set_error("sge_batch_create: bad arguments");
         return SGE_EINVAL;
@@ -1232,14 +1362,27 @@ extern "C" int sge_batch_create(sge_graph *g, const sge_plan *plans, uint32_t np
     LaunchCfg cfg;
     rc = launch_cfg(tune, false, cfg);
     if (rc) return rc;
+    uint64_t max_sub = 0;
+    for (uint32_t i = 0; i < nplans; ++i) {
+        const unsigned long long *sp = nullptr;
+        unsigned long long ns = 0;
+        if (offset < g->h.nnz) {
+            rc = get_split(g, plans[i], offset, g->h.nnz, stride, span_of(tune), &sp, &ns);
+            if (rc) return rc;
+        }
+        dp[i].split = sp;
+        dp[i].nsub = ns;
+        uint64_t ntask = offset < g->h.nnz ? (g->h.nnz - offset + stride - 1) / stride : 0;
+        max_sub = std::max<uint64_t>(max_sub, sp ? ns : ntask);
+    }
     sge_batch *b = new sge_batch();
     b->g = g;
     b->plans.assign(plans, plans + nplans);
     KParams kp;
     std::memset(&kp, 0, sizeof(kp));
-    uint32_t stride = ((g->h.max_degree + 64 + 63) / 64) * 64;
-    kp.buf_stride = stride;
-    kp.wave_stride = (uint64_t)stride * NBUF;
+    uint32_t bstride = ((g->h.max_degree + 64 + 63) / 64) * 64;
+    kp.buf_stride = bstride;
+    kp.wave_stride = (uint64_t)bstride * NBUF;
     size_t waves = (size_t)cfg.grid * (cfg.block / WAVE);
     auto fail = [&](hipError_t e, const char *what) {
         sge::set_error(std::string("HIP error: ") + hipGetErrorString(e) + " at " + what);
@@ -1263,11 +1406,11 @@ extern "C" int sge_batch_create(sge_graph *g, const sge_plan *plans, uint32_t np
     kp.nnz = g->h.nnz;
     kp.plans = b->dplans;
     kp.nplans = nplans;
-    kp.task_begin = 0;
+    kp.task_begin = std::min<uint64_t>(offset, g->h.nnz);
     kp.task_end = g->h.nnz;
-    kp.stride = 1;
+    kp.stride = stride;
     kp.chunk = cfg.chunk;
-    kp.nitems = ((g->h.nnz + cfg.chunk - 1) / cfg.chunk) * nplans;
+    kp.nitems = ((max_sub + cfg.chunk - 1) / cfg.chunk) * nplans;
     kp.queue = b->queue;
     kp.scratch = b->scratch;
     kp.out = b->dout;

@@ -64,7 +64,7 @@ class Plan(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("block_threads", C.c_uint32), ("chunk", C.c_uint32), ("stage_max", C.c_uint32),
-                ("blocks_per_cu", C.c_uint32), ("reserved", C.c_uint32 * 4)]
+                ("blocks_per_cu", C.c_uint32), ("split_span", C.c_uint32), ("reserved", C.c_uint32 * 3)]
 
 
 class Stats(C.Structure):
@@ -128,6 +128,8 @@ _SIGS = {
     "sge_enumerate": (C.c_int, [C.c_void_p, C.POINTER(Plan), C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64)]),
     "sge_batch_create": (C.c_int, [C.c_void_p, C.POINTER(Plan), C.c_uint32, C.POINTER(Tuning),
                                    C.POINTER(C.c_void_p)]),
+    "sge_batch_create_sampled": (C.c_int, [C.c_void_p, C.POINTER(Plan), C.c_uint32, C.c_uint64, C.c_uint64,
+                                           C.POINTER(Tuning), C.POINTER(C.c_void_p)]),
     "sge_batch_run": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Stats)]),
     "sge_batch_destroy": (None, [C.c_void_p]),
     "sge_device_info": (C.c_int, [C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_char_p, C.c_size_t]),
@@ -446,13 +448,15 @@ def canonicalize(pattern: Pattern, rows: np.ndarray) -> np.ndarray:
 class Batch:
     """Batched executor: all plans in one persistent launch, captured in a hipGraph."""
 
-    def __init__(self, graph: Graph, plans: Sequence[Plan], tuning: Optional[Tuning] = None):
+    def __init__(self, graph: Graph, plans: Sequence[Plan], tuning: Optional[Tuning] = None, stride: int = 1,
+                 offset: int = 0):
         self.graph = graph
         self.n = len(plans)
         self._plans = (Plan * self.n)(*plans)
         self._h = C.c_void_p()
-        _check(lib().sge_batch_create(graph._h, self._plans, self.n, C.byref(tuning) if tuning else None,
-                                      C.byref(self._h)), "sge_batch_create")
+        _check(lib().sge_batch_create_sampled(graph._h, self._plans, self.n, stride, offset,
+                                              C.byref(tuning) if tuning else None, C.byref(self._h)),
+               "sge_batch_create_sampled")
 
     def run(self):
         counts = np.zeros(self.n, dtype=np.uint64)

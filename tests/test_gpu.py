@@ -144,6 +144,27 @@ def test_tuning_variants_same_counts():
         assert sge.count(g, plans, batched=True, tuning=t)[0] == want
 
 
+def test_root_edge_split_same_counts():
+    """Heavy root edges split into level-2 slices must not change any count."""
+    g = W.graph_c2()
+    plans = [sge.build_plan(p, g, m) for p in W.six_vertex_set()[:3] for m in MODES]
+    base = None
+    for span in (1 << 30, 64, 4, 1):
+        t = sge.default_tuning()
+        t.split_span = span
+        say("split span", span)
+        got = [sge.count_sampled(g, [pl], 97, 5, tuning=t)[0][0] for pl in plans]
+        if base is None:
+            base = got
+        assert got == base
+    off, nbr = g.csr()
+    for pl, c in zip(plans, base):
+        assert c == O.count_strided(pl, off, nbr, 97, 5)
+    b = sge.Batch(g, plans, stride=97, offset=5)
+    assert b.run()[0] == base
+    b.close()
+
+
 def test_enumeration_c1_diamond_matches_brute_force():
     g = W.graph_c1()
     p = W.named("diamond")
