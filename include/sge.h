@@ -166,7 +166,9 @@ typedef struct sge_tuning {
     uint32_t blocks_per_cu;   /* 0 = occupancy-derived */
     uint32_t split_span;      /* a root edge whose level-2 set may exceed this is split into
                                  ceil(bound / split_span) subtasks (0 = default 64, 1 << 30 = off) */
-    uint32_t reserved[3];
+    uint32_t donate_budget;   /* search-tree nodes one work item may expand before its open
+                                 candidate ranges go to the next pass (0 = default 4096, 1 << 30 = off) */
+    uint32_t reserved[2];
 } sge_tuning;
 
 typedef struct sge_stats {

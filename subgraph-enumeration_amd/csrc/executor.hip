@@ -107,7 +107,20 @@ struct KParams {
     uint32_t *emb;
     uint64_t emb_cap;
     unsigned long long *emb_n;
+    // work donation between passes: a wave that expands more than `budget`
+    // search-tree nodes inside one item writes the untouched candidate ranges
+    // of its open levels as continuation items for the next pass
+    const uint32_t *in_items;          // this pass's input items (null: root pass)
+    const unsigned long long *in_count;
+    uint32_t *out_items;
+    unsigned long long *out_count;
+    uint64_t item_cap;                 // capacity of each item buffer
+    uint32_t budget;                   // 0: never donate
 };
+
+// continuation item: plan, level l, absolute candidate range [b, e) of level l,
+// images of levels 0..l-1
+constexpr int ITEM_WORDS = 4 + MAXK;
 
 // list descriptor: [ptr + b, ptr + e)
 struct Src {
@@ -123,8 +136,9 @@ struct Src {
 struct WaveLds {
     DPlan pl;
     uint32_t phi[MAXK];
-    uint32_t pos[MAXK];
-    Src cand[MAXK];      // candidate set per level
+    uint32_t pos[MAXK];  // next candidate position (absolute index into cand[d].ptr)
+    uint32_t iend[MAXK]; // end of this wave's iteration range at each level
+    Src cand[MAXK];      // candidate set per level (whole set: reuse reads it)
     Src pre[MAXK + 1];   // exploration: parent intersections formed early ([MAXK] = F)
     Src srcs[MAXK + 2];  // sources of the intersection being formed
     uint32_t cur[MAXK + 2];  // per-source cursor of the running merge
@@ -220,7 +234,7 @@ __device__ __forceinline__ bool is_checked(const WaveLds &w, uint32_t mask, uint
 // every other source each lane binary-searches its element within a window that
 // starts at the previous chunk's last position.  Survivors are compacted with a
 // wave ballot + popcount prefix.  Returns the survivor count (uniform).
-__device__ uint32_t wave_intersect(WaveLds &w, int ns, int nchk, uint32_t *out, Acc &acc) {
+__device__ __forceinline__ uint32_t wave_intersect(WaveLds &w, int ns, int nchk, uint32_t *out, Acc &acc) {
     if (ns <= 0) return 0;
     int drv = 0;
     uint64_t bytes = 0;
@@ -315,7 +329,7 @@ __device__ Src form_set(WaveLds &w, int ns, int nchk, bool may_alias, uint32_t *
 
 // Candidate set of level d (checks of level d applied now when the set is
 // materialised, or when it is read if it aliases a CSR row).
-__device__ void materialize(const KParams &P, WaveLds &w, uint32_t *wb, int d, Acc &acc) {
+__device__ __forceinline__ void materialize(const KParams &P, WaveLds &w, uint32_t *wb, int d, Acc &acc) {
     const DPlan &pl = w.pl;
     bool has_lo, has_hi;
     uint32_t lo, hi;
@@ -323,14 +337,16 @@ __device__ void materialize(const KParams &P, WaveLds &w, uint32_t *wb, int d, A
     bool use_pre = pl.mode == SGE_MODE_EXPLORE;
     int ns = gather_sources(P, w, d, pl.parent[d], use_pre ? -1 : pl.reuse[d], use_pre, has_lo, lo, has_hi, hi, acc);
     int nchk = load_checks(w, pl.check[d]);
-    w.cand[d] = form_set(w, ns, nchk, true, buf_ptr(P, wb, d), acc);
-    w.pos[d] = 0;
+    const Src c = form_set(w, ns, nchk, true, buf_ptr(P, wb, d), acc);
+    w.cand[d] = c;
+    w.pos[d] = c.b;
+    w.iend[d] = c.e;
     acc.nodes += 1;
 }
 
 // Exploration: form the parent intersection of every level whose last parent
 // is level d (and of the last level's sibling-independent part F), unclamped.
-__device__ void explore_after(const KParams &P, WaveLds &w, uint32_t *wb, int d, Acc &acc) {
+__device__ __forceinline__ void explore_after(const KParams &P, WaveLds &w, uint32_t *wb, int d, Acc &acc) {
     const DPlan &pl = w.pl;
     for (uint32_t j = d + 1; j <= pl.sib; ++j) {
         if (pl.ready[j] != (uint32_t)d || j < 2) continue;
@@ -352,21 +368,18 @@ struct TermLds {
     uint32_t chk;     // check mask below s (needed in the universe case)
 };
 
-// Inclusion-exclusion over set partitions of the tail: number of injective
-// assignments from per-subset intersection sizes n[B-1] (u64 arithmetic is
-// exact modulo 2^64, and the true count is below 2^64).
-__device__ __forceinline__ uint64_t ie_combine(const uint64_t *n, uint32_t r) {
-    if (r == 2) return n[0] * n[1] - n[2];
-    // r == 3: a=1 b=2 ab=3 c=4 ac=5 bc=6 abc=7
-    return n[0] * n[1] * n[3] - n[2] * n[3] - n[4] * n[1] - n[5] * n[0] + 2ull * n[6];
-}
+// Inclusion-exclusion over set partitions of the tail (phase 3 of count_last):
+// injective assignments = sum over partitions pi of prod_{B in pi} (-1)^{|B|-1}
+// (|B|-1)! |cap_{t in B} C_t|; r = 2: ab - |ab|; r = 3: abc - |ab|c - |ac|b -
+// |bc|a + 2|abc|.  u64 arithmetic is exact modulo 2^64 and the true count is
+// below 2^64.
 
 // Count the closed-form level(s) for every sibling in cand[sib] (or the single
 // sibling phi[1] when sib == 1).  Each term's F = the candidates that do not
 // depend on the sibling, with every bound/check from levels below the sibling
 // applied; per (sibling, term) the count is |F ∩ N(c) ∩ range(c)| (or a range
 // size), computed with the flattened probe loop.
-__device__ void count_last(const KParams &P, WaveLds &w, TermLds *terms, uint32_t *wb, Acc &acc) {
+__device__ __forceinline__ void count_last(const KParams &P, WaveLds &w, TermLds *terms, uint32_t *wb, Acc &acc) {
     const DPlan &pl = w.pl;
     const int s = pl.sib;
     const int lane = lane_id();
@@ -415,6 +428,8 @@ __device__ void count_last(const KParams &P, WaveLds &w, TermLds *terms, uint32_
         nsib = 1;
     } else {
         sibs = w.cand[s];
+        sibs.b = w.pos[s];
+        sibs.e = w.iend[s];
         nsib = sibs.e - sibs.b;
     }
     const uint32_t schk_mask = pl.check[s];
@@ -529,9 +544,15 @@ __device__ void count_last(const KParams &P, WaveLds &w, TermLds *terms, uint32_
             uint32_t i = r0 + lane;
             if (i < nb) {
                 if (kind == 2) {
-                    uint64_t n[MAX_TERMS];
-                    for (uint32_t j = 0; j < K; ++j) n[j] = w.cnt[i * K + j];
-                    local += ie_combine(n, pl.tail);
+                    const uint32_t *cn = &w.cnt[i * K];
+                    if (K == 3) {
+                        uint64_t a = cn[0], b = cn[1], ab = cn[2];
+                        local += a * b - ab;
+                    } else {
+                        // a=1 b=2 ab=3 c=4 ac=5 bc=6 abc=7 (index = mask - 1)
+                        uint64_t a = cn[0], b = cn[1], ab = cn[2], c = cn[3], ac = cn[4], bc = cn[5], abc = cn[6];
+                        local += a * b * c - ab * c - ac * b - bc * a + 2ull * abc;
+                    }
                 } else if (kind == 1) {
                     local += choosec(w.cnt[i], pl.tail);
                 } else {
@@ -589,6 +610,110 @@ __device__ void emit_last(const KParams &P, WaveLds &w, uint32_t *wb, Acc &acc) 
     }
 }
 
+__device__ void load_plan(const KParams &P, WaveLds &w, uint32_t plan) {
+    const uint32_t *srcw = reinterpret_cast<const uint32_t *>(P.plans + plan);
+    uint32_t *dstw = reinterpret_cast<uint32_t *>(&w.pl);
+    for (int q = lane_id(); q < (int)(sizeof(DPlan) / 4); q += WAVE) dstw[q] = srcw[q];
+    wave_sync();
+}
+
+// Write the untouched ranges of levels l0..d (below `stop`) as continuation
+// items; the deepest level gives away only the upper half of its range.
+// Returns false (and keeps all work) when the output buffer is full.
+__device__ __forceinline__ uint32_t give_from(const WaveLds &w, int j, int d) {
+    uint32_t b = w.pos[j], e = w.iend[j];
+    return (j == d) ? b + (e - b + 1) / 2 : b;
+}
+
+__device__ __forceinline__ void donate(const KParams &P, WaveLds &w, uint32_t plan, int l0, int d, int stop) {
+    const int lane = lane_id();
+    uint32_t n = 0;
+    for (int j = l0; j <= d && j < stop; ++j)
+        if (give_from(w, j, d) < w.iend[j]) ++n;
+    if (n == 0) return;
+    unsigned long long base = 0;
+    if (lane == 0) base = atomicAdd(P.out_count, (unsigned long long)n);
+    base = __shfl(base, 0);
+    if (base + n > P.item_cap) {
+        // no room: mark the reserved slots that exist as empty, keep the work
+        for (uint64_t slot = base; slot < P.item_cap && slot < base + n; ++slot)
+            if (lane == 0) P.out_items[slot * ITEM_WORDS] = NONE;
+        return;
+    }
+    uint64_t slot = base;
+    for (int j = l0; j <= d && j < stop; ++j) {
+        uint32_t b = give_from(w, j, d), e = w.iend[j];
+        if (b >= e) continue;
+        uint32_t *it = P.out_items + slot * ITEM_WORDS;
+        if (lane < ITEM_WORDS) {
+            uint32_t v;
+            if (lane == 0) v = plan;
+            else if (lane == 1) v = (uint32_t)j;
+            else if (lane == 2) v = b;
+            else if (lane == 3) v = e;
+            else v = (lane - 4 < j) ? w.phi[lane - 4] : 0u;
+            it[lane] = v;
+        }
+        w.iend[j] = b;  // this wave keeps [pos, b)
+        ++slot;
+    }
+}
+
+// Depth-first search from level l0 whose iteration range [pos, iend) is set,
+// images of levels < l0 fixed.  Levels below `stop` are enumerated; `stop`
+// is counted for all its siblings at once (or emitted when enumerating).
+template <bool ENUM>
+__device__ __forceinline__ void run_dfs(const KParams &P, WaveLds &w, TermLds *terms, uint32_t *wb, Acc &acc, uint32_t plan, int l0,
+                        int stop) {
+    const DPlan &pl = w.pl;
+    uint32_t spent = 0;
+    int d = l0;
+    while (d >= l0) {
+        if (d == stop) {
+            if (ENUM) {
+                const Src c = w.cand[d];
+                for (uint32_t i = w.pos[d]; i < w.iend[d]; ++i) {
+                    uint32_t x = c.ptr[i];
+                    if (is_checked(w, pl.check[d], x)) continue;
+                    w.phi[d] = x;
+                    emit_last(P, w, wb, acc);
+                }
+            } else {
+                count_last(P, w, terms, wb, acc);
+            }
+            --d;
+            continue;
+        }
+        const Src c = w.cand[d];
+        uint32_t p = w.pos[d];
+        const uint32_t pe = w.iend[d];
+        // next candidate, skipping images excluded by injectivity checks
+        uint32_t x = 0;
+        bool found = false;
+        while (p < pe) {
+            x = c.ptr[p];
+            ++p;
+            if (!is_checked(w, pl.check[d], x)) {
+                found = true;
+                break;
+            }
+        }
+        w.pos[d] = p;
+        if (found) {
+            w.phi[d] = x;
+            if (pl.mode == SGE_MODE_EXPLORE && !ENUM) explore_after(P, w, wb, d, acc);
+            materialize(P, w, wb, d + 1, acc);
+            ++d;
+            if (P.budget && P.out_items && ++spent >= P.budget) {
+                donate(P, w, plan, l0, d, stop);
+                spent = 0;
+            }
+        } else {
+            --d;
+        }
+    }
+}
+
 template <bool ENUM>
 __global__ void __launch_bounds__(256) wcoj_kernel(KParams P) {
     extern __shared__ unsigned char lds_raw[];
@@ -602,21 +727,49 @@ __global__ void __launch_bounds__(256) wcoj_kernel(KParams P) {
     Acc acc{0, 0, 0, 0, 0};
     uint32_t cur_plan = NONE;
     const uint64_t ntask = (P.task_end - P.task_begin + P.stride - 1) / P.stride;
+    const bool cont = P.in_items != nullptr;
+    const uint64_t nitems = cont ? min((uint64_t)*P.in_count, P.item_cap) : P.nitems;
     while (true) {
         unsigned long long item = 0;
         if (lane == 0) item = atomicAdd(P.queue, 1ull);
         item = __shfl(item, 0);
-        if (item >= P.nitems) break;
-        uint32_t plan = (uint32_t)(item % P.nplans);
-        uint64_t ch = item / P.nplans;
+        if (item >= nitems) break;
+        if (cont) {
+            // ---- continuation item: rebuild the path, then search its range
+            const uint32_t *it = P.in_items + item * ITEM_WORDS;
+            const uint32_t plan = it[0];
+            if (plan >= P.nplans) continue;  // empty slot
+            if (plan != cur_plan) {
+                if (cur_plan != NONE) flush(P, cur_plan, acc);
+                cur_plan = plan;
+                load_plan(P, w, plan);
+            }
+            const DPlan &pl = w.pl;
+            const int l = (int)it[1];
+            const int stop = ENUM ? (int)pl.k - 2 : (int)pl.sib;
+            if (l < 2 || l >= stop) continue;  // never written by donate()
+            for (int q = 0; q < l; ++q) w.phi[q] = it[4 + q];
+            if (pl.mode == SGE_MODE_EXPLORE && !ENUM) {
+                explore_after(P, w, wb, 0, acc);
+                explore_after(P, w, wb, 1, acc);
+            }
+            for (int j = 2; j <= l; ++j) {
+                materialize(P, w, wb, j, acc);
+                if (j < l && pl.mode == SGE_MODE_EXPLORE && !ENUM) explore_after(P, w, wb, j, acc);
+            }
+            const uint32_t rb = it[2], re = it[3];
+            w.pos[l] = max(rb, w.cand[l].b);
+            w.iend[l] = min(re, w.cand[l].e);
+            run_dfs<ENUM>(P, w, terms, wb, acc, plan, l, stop);
+            continue;
+        }
+        // ---- root pass: chunk of subtasks of one plan
+        const uint32_t plan = (uint32_t)(item % P.nplans);
+        const uint64_t ch = item / P.nplans;
         if (plan != cur_plan) {
             if (cur_plan != NONE) flush(P, cur_plan, acc);
             cur_plan = plan;
-            // load the plan into LDS, one dword per lane
-            const uint32_t *srcw = reinterpret_cast<const uint32_t *>(P.plans + plan);
-            uint32_t *dstw = reinterpret_cast<uint32_t *>(&w.pl);
-            for (int q = lane; q < (int)(sizeof(DPlan) / 4); q += WAVE) dstw[q] = srcw[q];
-            wave_sync();
+            load_plan(P, w, plan);
         }
         const DPlan &pl = w.pl;
         if (pl.k < 2) continue;  // never launched by the host; no edge tasks exist for it
@@ -663,54 +816,13 @@ __global__ void __launch_bounds__(256) wcoj_kernel(KParams P) {
             }
             const int stop = ENUM ? (int)pl.k - 2 : (int)pl.sib;  // level whose set feeds the leaf step
             materialize(P, w, wb, 2, acc);
-            if (nch > 1) {  // this subtask owns slice ci of the level-2 candidates
-                Src c2 = w.cand[2];
+            if (nch > 1) {  // this subtask iterates slice ci of the level-2 candidates
+                const Src c2 = w.cand[2];
                 uint64_t len = c2.e - c2.b;
-                uint32_t sb = c2.b + (uint32_t)(len * ci / nch), se = c2.b + (uint32_t)(len * (ci + 1) / nch);
-                c2.b = sb;
-                c2.e = se;
-                w.cand[2] = c2;
+                w.pos[2] = c2.b + (uint32_t)(len * ci / nch);
+                w.iend[2] = c2.b + (uint32_t)(len * (ci + 1) / nch);
             }
-            int d = 2;
-            while (d >= 2) {
-                if (d == stop) {
-                    if (ENUM) {
-                        const Src c = w.cand[d];
-                        for (uint32_t i = c.b; i < c.e; ++i) {
-                            uint32_t x = c.ptr[i];
-                            if (is_checked(w, pl.check[d], x)) continue;
-                            w.phi[d] = x;
-                            emit_last(P, w, wb, acc);
-                        }
-                    } else {
-                        count_last(P, w, terms, wb, acc);
-                    }
-                    --d;
-                    continue;
-                }
-                const Src c = w.cand[d];
-                uint32_t p = w.pos[d];
-                // next candidate, skipping images excluded by injectivity checks
-                uint32_t x = 0;
-                bool found = false;
-                while (c.b + p < c.e) {
-                    x = c.ptr[c.b + p];
-                    ++p;
-                    if (!is_checked(w, pl.check[d], x)) {
-                        found = true;
-                        break;
-                    }
-                }
-                w.pos[d] = p;
-                if (found) {
-                    w.phi[d] = x;
-                    if (pl.mode == SGE_MODE_EXPLORE && !ENUM) explore_after(P, w, wb, d, acc);
-                    materialize(P, w, wb, d + 1, acc);
-                    ++d;
-                } else {
-                    --d;
-                }
-            }
+            run_dfs<ENUM>(P, w, terms, wb, acc, plan, 2, stop);
         }
     }
     if (cur_plan != NONE) flush(P, cur_plan, acc);
@@ -735,8 +847,12 @@ struct DeviceCtx {
     size_t out_cap = 0;
     DPlan *plans = nullptr;
     size_t plans_cap = 0;
+    uint32_t *items[2] = {nullptr, nullptr};  // continuation items (ping-pong)
+    unsigned long long *icount = nullptr;     // [2] item counts
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
+
+constexpr uint64_t ITEM_CAP = 1ull << 21;  // continuation items per pass buffer (96 MB)
 
 DeviceCtx &dctx() {
     static DeviceCtx c;
@@ -764,6 +880,9 @@ int ensure_ctx() {
     HIP_TRY(hipMalloc(&c.queue, sizeof(unsigned long long) * 4));
     HIP_TRY(hipEventCreate(&c.ev0));
     HIP_TRY(hipEventCreate(&c.ev1));
+    HIP_TRY(hipMalloc(&c.items[0], sizeof(uint32_t) * ITEM_WORDS * ITEM_CAP));
+    HIP_TRY(hipMalloc(&c.items[1], sizeof(uint32_t) * ITEM_WORDS * ITEM_CAP));
+    HIP_TRY(hipMalloc(&c.icount, sizeof(unsigned long long) * 2));
     c.init = true;
     return SGE_OK;
 }
@@ -908,6 +1027,7 @@ struct LaunchCfg {
     int grid = 0;
     size_t lds = 0;
     uint32_t chunk = 64;
+    uint32_t budget = 4096;
 };
 
 int launch_cfg(const sge_tuning *t, bool enumerate, LaunchCfg &cfg) {
@@ -921,6 +1041,7 @@ int launch_cfg(const sge_tuning *t, bool enumerate, LaunchCfg &cfg) {
         return SGE_EINVAL;
     }
     cfg.chunk = t->chunk ? t->chunk : 64;
+    cfg.budget = t->donate_budget ? (t->donate_budget >= (1u << 30) ? 0u : t->donate_budget) : 4096u;
     cfg.lds = (size_t)(cfg.block / WAVE) * (sizeof(WaveLds) + MAX_TERMS * sizeof(TermLds));
     int per_cu = (int)t->blocks_per_cu;
     if (per_cu <= 0) {
@@ -946,6 +1067,13 @@ int ensure_scratch(const sge_graph *g, const LaunchCfg &cfg, KParams &kp) {
     if (rc) return rc;
     kp.scratch = c.scratch;
     return SGE_OK;
+}
+
+__global__ void reset_pass_kernel(unsigned long long *queue, unsigned long long *count) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        queue[0] = 0;
+        *count = 0;
+    }
 }
 
 __global__ void reset_kernel(PlanOut *out, uint32_t n, unsigned long long *queue, unsigned long long *emb_n) {
@@ -984,10 +1112,30 @@ int run_launch(sge_graph *g, const DPlan *dplans, uint32_t np, PlanOut *dout, ui
     kp.emb_n = emb_n;
     int rc = ensure_scratch(g, cfg, kp);
     if (rc) return rc;
+    kp.item_cap = ITEM_CAP;
+    kp.budget = cfg.budget;
     reset_kernel<<<(np + 255) / 256, 256, 0, c.stream>>>(dout, np, c.queue, emb_n);
+    reset_pass_kernel<<<1, 64, 0, c.stream>>>(c.queue, c.icount);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(c.ev0, c.stream));
-    if (kp.nitems > 0) {
+    int cur = 0;
+    for (int pass = 0;; ++pass) {
+        if (pass == 0) {
+            kp.in_items = nullptr;
+            kp.in_count = nullptr;
+        } else {
+            unsigned long long n = 0;
+            HIP_TRY(hipMemcpyAsync(&n, c.icount + cur, sizeof(n), hipMemcpyDeviceToHost, c.stream));
+            HIP_TRY(hipStreamSynchronize(c.stream));
+            if (n == 0) break;
+            kp.in_items = c.items[cur];
+            kp.in_count = c.icount + cur;
+            cur ^= 1;
+            reset_pass_kernel<<<1, 64, 0, c.stream>>>(c.queue, c.icount + cur);
+        }
+        kp.out_items = c.items[cur];
+        kp.out_count = c.icount + cur;
+        if (pass == 0 && kp.nitems == 0) break;
         if (enumerate)
             wcoj_kernel<true><<<cfg.grid, cfg.block, cfg.lds, c.stream>>>(kp);
         else
@@ -1324,11 +1472,29 @@ struct sge_batch {
     PlanOut *dout = nullptr;
     unsigned long long *queue = nullptr;
     uint32_t *scratch = nullptr;
+    uint32_t *items[2] = {nullptr, nullptr};
+    unsigned long long *icount = nullptr;
+    KParams kp;           // root-pass parameters (continuation passes derive from it)
+    LaunchCfg cfg;
+    int last_out = 0;     // item buffer written by the last captured pass
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
+
+constexpr int CAPTURED_PASSES = 8;  // continuation passes inside the graph (empty ones exit at once)
+
+// Enqueue one continuation pass reading items[in] and writing items[in ^ 1].
+static void enqueue_cont_pass(sge_batch *b, int in) {
+    KParams kp = b->kp;
+    kp.in_items = b->items[in];
+    kp.in_count = b->icount + in;
+    kp.out_items = b->items[in ^ 1];
+    kp.out_count = b->icount + (in ^ 1);
+    reset_pass_kernel<<<1, 64, 0, b->stream>>>(b->queue, b->icount + (in ^ 1));
+    wcoj_kernel<false><<<b->cfg.grid, b->cfg.block, b->cfg.lds, b->stream>>>(kp);
+}
 
 extern "C" int sge_batch_create_sampled(sge_graph *g, const sge_plan *plans, uint32_t nplans, uint64_t stride,
                                         uint64_t offset, const sge_tuning *tune, sge_batch **out);
@@ -1394,6 +1560,10 @@ extern "C" int sge_batch_create_sampled(sge_graph *g, const sge_plan *plans, uin
     if ((e = hipMalloc(&b->dout, sizeof(PlanOut) * nplans)) != hipSuccess) return fail(e, "out");
     if ((e = hipMalloc(&b->queue, sizeof(unsigned long long) * 2)) != hipSuccess) return fail(e, "queue");
     if ((e = hipMalloc(&b->scratch, waves * kp.wave_stride * sizeof(uint32_t))) != hipSuccess) return fail(e, "scratch");
+    for (int q = 0; q < 2; ++q)
+        if ((e = hipMalloc(&b->items[q], sizeof(uint32_t) * ITEM_WORDS * ITEM_CAP)) != hipSuccess)
+            return fail(e, "items");
+    if ((e = hipMalloc(&b->icount, sizeof(unsigned long long) * 2)) != hipSuccess) return fail(e, "item counts");
     if ((e = hipMemcpy(b->dplans, dp.data(), sizeof(DPlan) * nplans, hipMemcpyHostToDevice)) != hipSuccess)
         return fail(e, "plan upload");
     if ((e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking)) != hipSuccess) return fail(e, "stream");
@@ -1414,10 +1584,28 @@ extern "C" int sge_batch_create_sampled(sge_graph *g, const sge_plan *plans, uin
     kp.queue = b->queue;
     kp.scratch = b->scratch;
     kp.out = b->dout;
-    // capture: reset (a fill kernel, no memset node) + the persistent count kernel
+    kp.item_cap = ITEM_CAP;
+    kp.budget = cfg.budget;
+    kp.out_items = b->items[0];
+    kp.out_count = b->icount;
+    b->kp = kp;
+    b->cfg = cfg;
+    // capture: resets (fill kernels, no memset nodes), the root pass and a fixed
+    // number of continuation passes that read their item counts on the device
     if ((e = hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal)) != hipSuccess) return fail(e, "capture");
     reset_kernel<<<(nplans + 255) / 256, 256, 0, b->stream>>>(b->dout, nplans, b->queue, nullptr);
-    if (kp.nitems) wcoj_kernel<false><<<cfg.grid, cfg.block, cfg.lds, b->stream>>>(kp);
+    reset_pass_kernel<<<1, 64, 0, b->stream>>>(b->queue, b->icount);
+    if (kp.nitems) {
+        wcoj_kernel<false><<<cfg.grid, cfg.block, cfg.lds, b->stream>>>(kp);
+        int in = 0;
+        for (int pass = 0; pass < CAPTURED_PASSES; ++pass) {
+            enqueue_cont_pass(b, in);
+            in ^= 1;
+        }
+        b->last_out = in;
+    } else {
+        b->last_out = 0;
+    }
     if ((e = hipStreamEndCapture(b->stream, &b->graph)) != hipSuccess) return fail(e, "end capture");
     if ((e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0)) != hipSuccess) return fail(e, "instantiate");
     (void)c;
@@ -1434,6 +1622,17 @@ extern "C" int sge_batch_run(sge_batch *b, uint64_t *counts, sge_stats *stats) {
     uint32_t n = (uint32_t)b->plans.size();
     HIP_TRY(hipEventRecord(b->ev0, b->stream));
     HIP_TRY(hipGraphLaunch(b->exec, b->stream));
+    // items left after the captured passes: drain them eagerly
+    int in = b->last_out;
+    for (;;) {
+        unsigned long long left = 0;
+        HIP_TRY(hipMemcpyAsync(&left, b->icount + in, sizeof(left), hipMemcpyDeviceToHost, b->stream));
+        HIP_TRY(hipStreamSynchronize(b->stream));
+        if (left == 0) break;
+        enqueue_cont_pass(b, in);
+        HIP_TRY(hipGetLastError());
+        in ^= 1;
+    }
     HIP_TRY(hipEventRecord(b->ev1, b->stream));
     HIP_TRY(hipStreamSynchronize(b->stream));
     std::vector<PlanOut> host(n);
@@ -1461,6 +1660,9 @@ extern "C" void sge_batch_destroy(sge_batch *b) {
     if (b->dout) (void)hipFree(b->dout);
     if (b->queue) (void)hipFree(b->queue);
     if (b->scratch) (void)hipFree(b->scratch);
+    for (int q = 0; q < 2; ++q)
+        if (b->items[q]) (void)hipFree(b->items[q]);
+    if (b->icount) (void)hipFree(b->icount);
     delete b;
 }
 

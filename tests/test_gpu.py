@@ -149,14 +149,23 @@ def test_root_edge_split_same_counts():
     g = W.graph_c2()
     plans = [sge.build_plan(p, g, m) for p in W.six_vertex_set()[:3] for m in MODES]
     base = None
-    for span in (1 << 30, 64, 4, 1):
+    for span, budget in ((1 << 30, 1 << 30), (64, 1 << 30), (4, 1 << 30), (1, 1 << 30), (1 << 30, 16), (8, 2),
+                         (1, 1)):
         t = sge.default_tuning()
         t.split_span = span
-        say("split span", span)
+        t.donate_budget = budget
+        say("split span", span, "donate budget", budget)
         got = [sge.count_sampled(g, [pl], 97, 5, tuning=t)[0][0] for pl in plans]
         if base is None:
             base = got
-        assert got == base
+        assert got == base, (span, budget)
+    t = sge.default_tuning()
+    t.donate_budget = 2
+    say("batched with donation")
+    b = sge.Batch(g, plans, tuning=t, stride=97, offset=5)
+    for _ in range(2):
+        assert b.run()[0] == base
+    b.close()
     off, nbr = g.csr()
     for pl, c in zip(plans, base):
         assert c == O.count_strided(pl, off, nbr, 97, 5)
