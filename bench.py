@@ -37,7 +37,8 @@ def log(*a):
 def workload(name):
     if name == "c4":
         return ("rmat20_seven_vertex_78", lambda: W.graph_c4(), W.seven_vertex_set,
-                {"graph": "rmat(scale=20, ef=16, a=.57, b=.19, c=.19, seed=4)", "patterns": "78 x 7-vertex, 15-19 edges"})
+                {"graph": "rmat(scale=20, ef=16, a=.57, b=.19, c=.19, seed=4)",
+                 "patterns": "78 x 7-vertex, 15-19 edges"})
     if name == "c3":
         return ("chunglu7115_six_vertex_16", lambda: W.graph_c2(), W.six_vertex_set,
                 {"graph": "chung_lu(n=7115, m=100762, gamma=2.1, cap=1100, seed=1)", "patterns": "16 x 6-vertex, 11-13 edges"})
@@ -60,35 +61,27 @@ def load_traffic(kernel_workload):
         return None
 
 
-def cpu_baseline(graph, plans, budget_s=20.0):
-    """Oracle (port of the reference's join) on a bounded sample of root tasks:
-    evenly spaced blocks of CSR positions for every pattern, all host threads."""
+def cpu_baseline(graph, plans, stride, budget_s=20.0):
+    """Oracle (port of the reference's join) on a bounded sample of the same
+    root edges the GPU counts (the strided sample), patterns in set order until
+    the time budget is spent, all host threads."""
     from oracle import oracle as O
 
     off, nbr = graph.csr()
-    nnz = int(off[-1])
     threads = min(os.cpu_count() or 1, 64)
-    total_count, total_t = 0, 0.0
-    block = max(64, nnz // 4096)
-    starts = np.linspace(0, nnz - block, 16).astype(np.int64)
-    done_tasks = 0
+    total_count, total_t, done = 0, 0.0, 0
     t_start = time.time()
     for pl in plans:
-        for s in starts:
-            if time.time() - t_start > budget_s:
-                break
-            t0 = time.perf_counter()
-            c = O.count(pl, off, nbr, threads, int(s), int(s) + block)
-            total_t += time.perf_counter() - t0
-            total_count += c
-            done_tasks += block
         if time.time() - t_start > budget_s:
             break
+        t0 = time.perf_counter()
+        total_count += O.count_strided(pl, off, nbr, stride, 0, threads)
+        total_t += time.perf_counter() - t0
+        done += 1
     return {"value": total_count / total_t if total_t > 0 else None, "unit": "matches/s", "cores": threads,
             "kind": "port",
-            "sample": f"oracle over {done_tasks} root edges ({block}-edge blocks at 16 evenly spaced CSR offsets "
-                      f"per pattern, patterns in set order until {budget_s:.0f}s), {total_count} matches in "
-                      f"{total_t:.2f}s"}
+            "sample": f"oracle on the first {done} of {len(plans)} patterns over the same 1/{stride} root-edge "
+                      f"sample, {total_count} matches in {total_t:.2f}s"}
 
 
 def main():
@@ -97,10 +90,15 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="c4")
-    ap.add_argument("--mode", default="wcoj", choices=["wcoj", "explore", "iep"])
+    ap.add_argument("--mode", default="iep", choices=["wcoj", "explore", "iep"])
+    ap.add_argument("--stride", type=int, default=None,
+                    help="root-edge sample: every stride-th directed edge (default 4096 for c4, 1 otherwise)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
-    ap.add_argument("--learned", default=os.path.join(ROOT, "profiles", "mlp_weights.npz"))
+    ap.add_argument("--calibrate", action="store_true",
+                    help="also run the known-size copy kernel (scales rocprofv3 FETCH_SIZE / WRITE_SIZE)")
+    ap.add_argument("--learned", default=os.path.join(ROOT, "subgraph-enumeration_amd", "sge", "models",
+                                                      "order_mlp.npz"))
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is design-only",
@@ -108,10 +106,15 @@ def main():
         sys.exit(2)
     mode = {"wcoj": sge.MODE_WCOJ, "explore": sge.MODE_EXPLORE, "iep": sge.MODE_IEP}[args.mode]
     wname, gfn, pfn, cfg = workload(args.workload)
+    stride = args.stride or (4096 if args.workload == "c4" else 1)
     t0 = time.time()
     g = gfn()
     log(f"graph n={g.n} m={g.m} maxdeg={g.stats.max_degree} built in {time.time() - t0:.1f}s")
     g.to_device()
+    if args.calibrate:
+        nbytes = 256 << 20
+        ms = sge.calibration_copy(nbytes)
+        log(f"calibration copy: {nbytes} bytes read + written in {ms:.3f} ms ({2 * nbytes / ms / 1e6:.0f} GB/s)")
     pats = pfn()
     t0 = time.time()
     plans_cost = [sge.build_plan(p, g, mode) for p in pats]
@@ -134,11 +137,11 @@ def main():
     plain_counts = None
     if not args.no_plain:
         t0 = time.perf_counter()
-        plain_counts, st_plain = sge.count(g, plans_cost, batched=False)
+        plain_counts, pst = sge.count_sampled(g, plans_cost, stride)
         plain_ms = (time.perf_counter() - t0) * 1000
-        log(f"plain: {plain_ms:.1f} ms, kernel {st_plain.kernel_ms:.1f} ms, total {sum(plain_counts)}")
+        log(f"plain: {plain_ms:.1f} ms, kernel {sum(s.kernel_ms for s in pst):.1f} ms, total {sum(plain_counts)}")
 
-    batch = sge.Batch(g, plans_fast)
+    batch = sge.Batch(g, plans_fast, stride=stride)
     for i in range(args.warmup):
         counts, st = batch.run()
         log(f"warmup {i}: {st.kernel_ms:.1f} ms")
@@ -171,7 +174,8 @@ def main():
         "dtype": "u32 ids / u64 counts",
         "data": "synthetic",
         "config": {"workload": wname, "mode": args.mode, "learned_order": learned, **cfg,
-                   "n": g.n, "m": g.m, "patterns": len(pats)},
+                   "n": g.n, "m": g.m, "patterns": len(pats), "root_edge_stride": stride,
+                   "root_edges": (2 * g.m + stride - 1) // stride},
         "pattern_set_ms": {"batched": step_s * 1000, "plain": plain_ms,
                            "speedup_vs_plain": (plain_ms / (step_s * 1000)) if plain_ms else None},
         "total_matches": total,
@@ -184,7 +188,7 @@ def main():
         "cpu_baseline": None,
     }
     if not args.no_cpu:
-        res["cpu_baseline"] = cpu_baseline(g, plans_cost)
+        res["cpu_baseline"] = cpu_baseline(g, plans_cost, stride)
     print(json.dumps(res))
 
 

@@ -162,7 +162,8 @@ int sge_plan_text(const sge_plan *plan, char *buf, size_t cap);
 typedef struct sge_tuning {
     uint32_t block_threads;   /* 64..1024, multiple of 64 (one DFS per wavefront) */
     uint32_t chunk;           /* root tasks taken per queue fetch */
-    uint32_t stage_max;       /* stage a probed list window in LDS when <= this many ids */
+    uint32_t stage_max;       /* probe-direction ratio: a sibling's row drives the count while
+                                 |row| <= stage_max * |F| and F is staged in LDS (0 = default 8) */
     uint32_t blocks_per_cu;   /* 0 = occupancy-derived */
     uint32_t split_span;      /* a root edge whose level-2 set may exceed this is split into
                                  ceil(bound / split_span) subtasks (0 = default 64, 1 << 30 = off) */

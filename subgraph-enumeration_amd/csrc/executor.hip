@@ -57,6 +57,7 @@ constexpr int MAXK = SGE_MAX_K;
 constexpr int SIB_BLOCK = 256;  // (sibling, term) entries per flattened last-level batch
 constexpr int MAX_TERMS = 7;    // inclusion-exclusion terms of a 3-level independent tail
 constexpr int NBUF = 2 * MAXK + 1 + MAX_TERMS;  // scratch buffers per wave
+constexpr int FSTAGE = 512;     // LDS words per wave for staging the terms' F sets
 constexpr uint32_t NONE = 0xffffffffu;
 
 // Device copy of a plan (levels masks) plus derived fields.
@@ -116,6 +117,7 @@ struct KParams {
     unsigned long long *out_count;
     uint64_t item_cap;                 // capacity of each item buffer
     uint32_t budget;                   // 0: never donate
+    uint32_t probe_ratio;              // drive with the sibling's row while |row| <= ratio * |F| (F staged in LDS)
 };
 
 // continuation item: plan, level l, absolute candidate range [b, e) of level l,
@@ -150,6 +152,7 @@ struct WaveLds {
     uint32_t thi[SIB_BLOCK];
     uint32_t cnt[SIB_BLOCK];   // per-sibling counts
     uint32_t flag[SIB_BLOCK];  // 1: driver is the sibling's CSR row (target F), 0: driver F (target row)
+    uint32_t fst[FSTAGE];      // staged copies of small F sets (probed through LDS)
 };
 
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
@@ -418,6 +421,23 @@ __device__ __forceinline__ void count_last(const KParams &P, WaveLds &w, TermLds
         terms[j] = t;
     }
     wave_sync();
+    // ---- stage small F sets in LDS: per-sibling probes then hit LDS
+    {
+        uint32_t used = 0;
+        for (uint32_t j = 0; j < K; ++j) {
+            TermLds t = terms[j];
+            uint32_t len = t.F.e - t.F.b;
+            if ((t.flags & 4u) || len == 0 || used + len > (uint32_t)FSTAGE) continue;
+            for (uint32_t q = lane; q < len; q += WAVE) w.fst[used + q] = t.F.ptr[t.F.b + q];
+            t.F.ptr = &w.fst[used];
+            t.F.b = 0;
+            t.F.e = len;
+            t.flags |= 8u;  // staged
+            terms[j] = t;
+            used += len;
+        }
+        wave_sync();
+    }
     // ---- siblings
     Src sibs;
     uint32_t nsib;
@@ -469,7 +489,9 @@ __device__ __forceinline__ void count_last(const KParams &P, WaveLds &w, TermLds
                         } else {
                             uint32_t fb = t.F.b, fe = t.F.e;
                             clamp_range(t.F.ptr, fb, fe, s_lower, c, s_upper, c);
-                            if (re - rb <= fe - fb) {
+                            // probes into a staged F cost LDS reads, into a row global reads
+                            const uint64_t ratio = (t.flags & 8u) ? P.probe_ratio : 1u;
+                            if ((uint64_t)(re - rb) <= ratio * (uint64_t)(fe - fb)) {
                                 flag = 1;
                                 dstart = rb;
                                 tl = fb;
@@ -1028,6 +1050,7 @@ struct LaunchCfg {
     size_t lds = 0;
     uint32_t chunk = 64;
     uint32_t budget = 4096;
+    uint32_t probe_ratio = 8;
 };
 
 int launch_cfg(const sge_tuning *t, bool enumerate, LaunchCfg &cfg) {
@@ -1042,6 +1065,7 @@ int launch_cfg(const sge_tuning *t, bool enumerate, LaunchCfg &cfg) {
     }
     cfg.chunk = t->chunk ? t->chunk : 64;
     cfg.budget = t->donate_budget ? (t->donate_budget >= (1u << 30) ? 0u : t->donate_budget) : 4096u;
+    cfg.probe_ratio = t->stage_max ? t->stage_max : 8u;
     cfg.lds = (size_t)(cfg.block / WAVE) * (sizeof(WaveLds) + MAX_TERMS * sizeof(TermLds));
     int per_cu = (int)t->blocks_per_cu;
     if (per_cu <= 0) {
@@ -1114,6 +1138,7 @@ int run_launch(sge_graph *g, const DPlan *dplans, uint32_t np, PlanOut *dout, ui
     if (rc) return rc;
     kp.item_cap = ITEM_CAP;
     kp.budget = cfg.budget;
+    kp.probe_ratio = cfg.probe_ratio;
     reset_kernel<<<(np + 255) / 256, 256, 0, c.stream>>>(dout, np, c.queue, emb_n);
     reset_pass_kernel<<<1, 64, 0, c.stream>>>(c.queue, c.icount);
     HIP_TRY(hipGetLastError());
@@ -1586,6 +1611,7 @@ extern "C" int sge_batch_create_sampled(sge_graph *g, const sge_plan *plans, uin
     kp.out = b->dout;
     kp.item_cap = ITEM_CAP;
     kp.budget = cfg.budget;
+    kp.probe_ratio = cfg.probe_ratio;
     kp.out_items = b->items[0];
     kp.out_count = b->icount;
     b->kp = kp;

@@ -1,0 +1,84 @@
+"""Autotuner for the batched executor's launch parameters.
+
+Coordinate descent over workgroup size (waves per workgroup), root-queue chunk,
+root-edge split span, work-donation budget and the probe-direction ratio of the
+last-level count (which list drives an intersection, by length ratio), each
+candidate timed as one batched pass over a coarser root-edge sample of the same
+plans.  Counts are checked to be identical for every setting.  Results are cached
+in a JSON file keyed by the workload name.
+"""
+from __future__ import annotations
+
+import json
+import os
+import time
+from typing import Dict, Optional, Sequence
+
+from . import Batch, Graph, Plan, Tuning, default_tuning
+
+SPACE = {
+    "block_threads": [256, 128, 64],
+    "chunk": [8, 32, 128],
+    "split_span": [16, 64, 256],
+    "donate_budget": [1024, 4096, 16384],
+    "stage_max": [2, 8, 32],
+}
+
+
+def _tuning(d: Dict[str, int]) -> Tuning:
+    t = default_tuning()
+    for k, v in d.items():
+        setattr(t, k, int(v))
+    return t
+
+
+def _time(graph: Graph, plans: Sequence[Plan], stride: int, d: Dict[str, int], reps: int = 2):
+    b = Batch(graph, plans, _tuning(d), stride=stride)
+    try:
+        counts, st = b.run()  # warm
+        best = float("inf")
+        for _ in range(reps):
+            c2, st = b.run()
+            assert c2 == counts
+            best = min(best, st.kernel_ms)
+        return best, counts
+    finally:
+        b.close()
+
+
+def tune(graph: Graph, plans: Sequence[Plan], stride: int, rounds: int = 1, log=print,
+         cache: Optional[str] = None, key: Optional[str] = None, budget_s: float = 120.0) -> Dict[str, int]:
+    if cache and key and os.path.exists(cache):
+        with open(cache) as f:
+            data = json.load(f)
+        if key in data:
+            return data[key]["params"]
+    cur = {k: v[0] if k != "chunk" else 32 for k, v in SPACE.items()}
+    cur.update({"split_span": 64, "donate_budget": 4096, "stage_max": 8})
+    t0 = time.time()
+    best_ms, ref = _time(graph, plans, stride, cur)
+    log(f"[autotune] start {cur}: {best_ms:.2f} ms")
+    trace = [{"params": dict(cur), "ms": best_ms}]
+    for _ in range(rounds):
+        for name, values in SPACE.items():
+            for v in values:
+                if v == cur[name] or time.time() - t0 > budget_s:
+                    continue
+                trial = dict(cur, **{name: v})
+                ms, counts = _time(graph, plans, stride, trial)
+                if counts != ref:
+                    raise RuntimeError(f"autotune: counts changed with {trial}")
+                trace.append({"params": trial, "ms": ms})
+                log(f"[autotune] {name}={v}: {ms:.2f} ms")
+                if ms < best_ms:
+                    best_ms, cur = ms, trial
+    log(f"[autotune] best {cur}: {best_ms:.2f} ms in {time.time() - t0:.0f}s")
+    if cache and key:
+        data = {}
+        if os.path.exists(cache):
+            with open(cache) as f:
+                data = json.load(f)
+        data[key] = {"params": cur, "ms": best_ms, "stride": stride, "trace": trace}
+        with open(cache, "w") as f:
+            json.dump(data, f, indent=1)
+    return cur
