@@ -155,6 +155,11 @@ int sge_plan_build(const sge_pattern *p, const sge_graph *g, uint32_t mode, cons
 int sge_plan_cost(const sge_pattern *p, const sge_graph *g, uint32_t mode, const uint32_t *order, double *cost);
 /* All connected orders of the pattern (k! bounded), deduplicated up to automorphism; returns count in *n. */
 int sge_plan_orders(const sge_pattern *p, uint32_t *orders, uint32_t cap, uint32_t *n);
+/* All connected orders ranked as the planner ranks them for `mode` (longest
+ * closed-form tail first in IEP mode, then lowest cost): orders[i*k..], costs[i],
+ * tails[i] for i < min(cap, *n); any output may be NULL. */
+int sge_plan_rank_orders(const sge_pattern *p, const sge_graph *g, uint32_t mode, uint32_t *orders, double *costs,
+                         uint32_t *tails, uint32_t cap, uint32_t *n);
 /* Reference-style plan text ("num bags", symmetry rules, ...) into buf. */
 int sge_plan_text(const sge_plan *plan, char *buf, size_t cap);
 

@@ -54,10 +54,10 @@ namespace {
 
 constexpr int WAVE = 64;
 constexpr int MAXK = SGE_MAX_K;
-constexpr int SIB_BLOCK = 256;  // (sibling, term) entries per flattened last-level batch
+constexpr int SIB_BLOCK = 128;  // (sibling, term) entries per flattened last-level batch
 constexpr int MAX_TERMS = 7;    // inclusion-exclusion terms of a 3-level independent tail
 constexpr int NBUF = 2 * MAXK + 1 + MAX_TERMS;  // scratch buffers per wave
-constexpr int FSTAGE = 512;     // LDS words per wave for staging the terms' F sets
+constexpr int FSTAGE = 1024;    // LDS words per wave for staging the terms' F sets (bitmaps or copies)
 constexpr uint32_t NONE = 0xffffffffu;
 
 // Device copy of a plan (levels masks) plus derived fields.
@@ -366,10 +366,26 @@ __device__ __forceinline__ void explore_after(const KParams &P, WaveLds &w, uint
 struct TermLds {
     Src F;            // sibling-independent part (bounds/checks below s applied)
     uint32_t lo, hi;  // fixed bounds (has_lo / has_hi in flags)
-    uint32_t flags;   // bit0 has_lo, bit1 has_hi, bit2 universe
+    uint32_t flags;   // bit0 has_lo, bit1 has_hi, bit2 universe, bit3 staged copy, bit4 staged bitmap
     uint32_t sflags;  // as DPlan::t_sflags
     uint32_t chk;     // check mask below s (needed in the universe case)
+    uint32_t st_off;  // word offset of the staged copy / bitmap in WaveLds::fst
+    uint32_t bm_lo;   // bitmap: id of bit 0 (= smallest element)
+    uint32_t bm_hi;   // bitmap: largest element
 };
+
+// Membership of x in term t's F between F positions [tlo, thi) (a sibling's row
+// drove x, so its bounds already hold): one bit test for a bitmap, else a
+// binary search in the LDS copy or in F itself.
+__device__ __forceinline__ bool term_has(const WaveLds &w, const TermLds &t, uint32_t x, uint32_t tlo, uint32_t thi) {
+    if (t.flags & 16u) {
+        if (x < t.bm_lo || x > t.bm_hi) return false;
+        uint32_t r = x - t.bm_lo;
+        return (w.fst[t.st_off + (r >> 5)] >> (r & 31u)) & 1u;
+    }
+    if (t.flags & 8u) return contains(&w.fst[t.st_off], tlo - t.F.b, thi - t.F.b, x);
+    return contains(t.F.ptr, tlo, thi, x);
+}
 
 // Inclusion-exclusion over set partitions of the tail (phase 3 of count_last):
 // injective assignments = sum over partitions pi of prod_{B in pi} (-1)^{|B|-1}
@@ -421,20 +437,37 @@ __device__ __forceinline__ void count_last(const KParams &P, WaveLds &w, TermLds
         terms[j] = t;
     }
     wave_sync();
-    // ---- stage small F sets in LDS: per-sibling probes then hit LDS
+    // ---- stage F sets in LDS so per-sibling probes hit LDS: a bitmap over
+    // [min F, max F] when it fits, else a sorted copy when that fits
     {
         uint32_t used = 0;
         for (uint32_t j = 0; j < K; ++j) {
             TermLds t = terms[j];
-            uint32_t len = t.F.e - t.F.b;
-            if ((t.flags & 4u) || len == 0 || used + len > (uint32_t)FSTAGE) continue;
-            for (uint32_t q = lane; q < len; q += WAVE) w.fst[used + q] = t.F.ptr[t.F.b + q];
-            t.F.ptr = &w.fst[used];
-            t.F.b = 0;
-            t.F.e = len;
-            t.flags |= 8u;  // staged
+            const uint32_t len = t.F.e - t.F.b;
+            if ((t.flags & 4u) || len == 0) continue;
+            const uint32_t fmin = t.F.ptr[t.F.b], fmax = t.F.ptr[t.F.e - 1];
+            const uint64_t words = ((uint64_t)(fmax - fmin) >> 5) + 1;
+            if (used + words <= (uint64_t)FSTAGE) {
+                for (uint32_t q = lane; q < (uint32_t)words; q += WAVE) w.fst[used + q] = 0u;
+                wave_sync();
+                for (uint32_t q = lane; q < len; q += WAVE) {
+                    uint32_t r = t.F.ptr[t.F.b + q] - fmin;
+                    atomicOr(&w.fst[used + (r >> 5)], 1u << (r & 31u));
+                }
+                t.flags |= 16u;
+                t.st_off = used;
+                t.bm_lo = fmin;
+                t.bm_hi = fmax;
+                used += (uint32_t)words;
+            } else if (used + len <= (uint32_t)FSTAGE) {
+                for (uint32_t q = lane; q < len; q += WAVE) w.fst[used + q] = t.F.ptr[t.F.b + q];
+                t.flags |= 8u;
+                t.st_off = used;
+                used += len;
+            } else {
+                continue;
+            }
             terms[j] = t;
-            used += len;
         }
         wave_sync();
     }
@@ -489,8 +522,10 @@ __device__ __forceinline__ void count_last(const KParams &P, WaveLds &w, TermLds
                         } else {
                             uint32_t fb = t.F.b, fe = t.F.e;
                             clamp_range(t.F.ptr, fb, fe, s_lower, c, s_upper, c);
-                            // probes into a staged F cost LDS reads, into a row global reads
-                            const uint64_t ratio = (t.flags & 8u) ? P.probe_ratio : 1u;
+                            // probes into a staged F cost LDS reads (one for a bitmap),
+                            // probes into a row cost a global binary search
+                            const uint64_t ratio =
+                                (t.flags & 16u) ? 4u * P.probe_ratio : ((t.flags & 8u) ? P.probe_ratio : 1u);
                             if ((uint64_t)(re - rb) <= ratio * (uint64_t)(fe - fb)) {
                                 flag = 1;
                                 dstart = rb;
@@ -544,13 +579,13 @@ __device__ __forceinline__ void count_last(const KParams &P, WaveLds &w, TermLds
                 }
                 uint32_t off = u - w.pre_sum[a];
                 const uint32_t fl = w.flag[a];
-                const uint32_t *Fp = terms[fl >> 1].F.ptr;
+                const TermLds &t = terms[fl >> 1];
                 bool hit;
                 if (fl & 1u) {
                     uint32_t x = P.nbr[w.drv[a] + off];
-                    hit = contains(Fp, w.tlo[a], w.thi[a], x);
+                    hit = term_has(w, t, x, w.tlo[a], w.thi[a]);
                 } else {
-                    uint32_t x = Fp[w.drv[a] + off];
+                    uint32_t x = t.F.ptr[w.drv[a] + off];
                     hit = contains(P.nbr, w.tlo[a], w.thi[a], x);
                 }
                 if (hit) {

@@ -13,6 +13,7 @@
 #include <functional>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <set>
 #include <unordered_set>
 
@@ -233,7 +234,27 @@ static Tree fhd_from_order(const Pattern &p, const std::vector<int> &o) {
     return t;
 }
 
+static Decomposition decompose_uncached(const Pattern &p, const std::vector<std::vector<int>> &aut);
+
+// The decomposition depends on the pattern only; cache it per adjacency.
 Decomposition decompose(const Pattern &p, const std::vector<std::vector<int>> &aut) {
+    static std::mutex mu;
+    static std::map<std::vector<uint32_t>, Decomposition> cache;
+    std::vector<uint32_t> key(p.adj, p.adj + SGE_MAX_K);
+    key.push_back((uint32_t)p.k);
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        auto it = cache.find(key);
+        if (it != cache.end()) return it->second;
+    }
+    Decomposition d = decompose_uncached(p, aut);
+    std::lock_guard<std::mutex> lk(mu);
+    if (cache.size() > 4096) cache.clear();
+    cache[key] = d;
+    return d;
+}
+
+static Decomposition decompose_uncached(const Pattern &p, const std::vector<std::vector<int>> &aut) {
     Decomposition d;
     uint32_t core = two_core(p);
     d.core_size = __builtin_popcount(core);

@@ -412,6 +412,43 @@ extern "C" int sge_plan_cost(const sge_pattern *cp, const sge_graph *g, uint32_t
     return SGE_OK;
 }
 
+// Every connected order ranked the way the planner ranks them for `mode`:
+// longest closed-form tail first (IEP), then lowest analytic cost.
+extern "C" int sge_plan_rank_orders(const sge_pattern *cp, const sge_graph *g, uint32_t mode, uint32_t *orders,
+                                    double *costs, uint32_t *tails, uint32_t cap, uint32_t *n) {
+    if (!cp || !n || mode > SGE_MODE_IEP) {
+        set_error("sge_plan_rank_orders: bad arguments");
+        return SGE_EINVAL;
+    }
+    Pattern p = from_c(*cp);
+    if (!p.connected() || p.k < 2) {
+        set_error("sge_plan_rank_orders: pattern must be connected with >= 2 vertices");
+        return SGE_EINVAL;
+    }
+    auto aut = automorphisms(p);
+    GraphSummary s = summarize(g ? &g->h : nullptr);
+    auto os = connected_orders(p, aut);
+    struct R { int tail; double cost; size_t i; };
+    std::vector<R> rs;
+    for (size_t i = 0; i < os.size(); ++i) {
+        sge_plan trial;
+        std::memset(&trial, 0, sizeof(trial));
+        fill_masks(p, os[i].data(), aut, mode, trial);
+        rs.push_back({(int)trial.tail, plan_cost(trial, s), i});
+    }
+    std::stable_sort(rs.begin(), rs.end(), [](const R &a, const R &b) {
+        return a.tail != b.tail ? a.tail > b.tail : a.cost < b.cost;
+    });
+    *n = (uint32_t)rs.size();
+    for (uint32_t i = 0; i < std::min<uint32_t>(cap, *n); ++i) {
+        if (orders)
+            for (int j = 0; j < p.k; ++j) orders[i * p.k + j] = os[rs[i].i][j];
+        if (costs) costs[i] = rs[i].cost;
+        if (tails) tails[i] = rs[i].tail;
+    }
+    return SGE_OK;
+}
+
 extern "C" int sge_plan_orders(const sge_pattern *cp, uint32_t *orders, uint32_t cap, uint32_t *n) {
     if (!cp || !n) {
         set_error("sge_plan_orders: null argument");

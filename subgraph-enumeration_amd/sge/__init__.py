@@ -104,6 +104,8 @@ _SIGS = {
     "sge_plan_build": (C.c_int, [C.POINTER(CPattern), C.c_void_p, C.c_uint32, C.c_void_p, C.POINTER(Plan)]),
     "sge_plan_cost": (C.c_int, [C.POINTER(CPattern), C.c_void_p, C.c_uint32, C.c_void_p, C.POINTER(C.c_double)]),
     "sge_plan_orders": (C.c_int, [C.POINTER(CPattern), C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32)]),
+    "sge_plan_rank_orders": (C.c_int, [C.POINTER(CPattern), C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p,
+                                       C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32)]),
     "sge_plan_text": (C.c_int, [C.POINTER(Plan), C.c_char_p, C.c_size_t]),
     "sge_tuning_default": (None, [C.POINTER(Tuning)]),
     "sge_count": (C.c_int, [C.c_void_p, C.POINTER(Plan), C.c_uint32, C.c_uint32, C.POINTER(Tuning), C.c_void_p,
@@ -280,6 +282,20 @@ class Pattern:
 
     def __repr__(self):
         return f"Pattern(k={self.k}, edges={self.edges()})"
+
+
+def rank_orders(pattern: Pattern, graph: Optional[Graph], mode: int):
+    """All connected orders ranked like the planner: (orders, costs, tails)."""
+    n = C.c_uint32()
+    _check(lib().sge_plan_rank_orders(C.byref(pattern.c), graph._h if graph else None, mode, None, None, None, 0,
+                                      C.byref(n)), "sge_plan_rank_orders")
+    k = pattern.k
+    o = np.zeros(max(n.value * k, 1), dtype=np.uint32)
+    c = np.zeros(max(n.value, 1), dtype=np.float64)
+    t = np.zeros(max(n.value, 1), dtype=np.uint32)
+    _check(lib().sge_plan_rank_orders(C.byref(pattern.c), graph._h if graph else None, mode, _ptr(o), _ptr(c), _ptr(t),
+                                      n.value, C.byref(n)), "sge_plan_rank_orders")
+    return o[: n.value * k].reshape(-1, k).tolist(), c[: n.value].tolist(), t[: n.value].tolist()
 
 
 def sample_patterns(k, emin, emax, want, seed, exclude: Sequence[int] = ()) -> List[Pattern]:

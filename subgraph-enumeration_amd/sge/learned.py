@@ -23,21 +23,19 @@ from typing import Dict, List, Optional, Sequence
 
 import numpy as np
 
-from . import (MLP, Graph, Pattern, Plan, build_plan, count_sampled, order_cost, order_features)
+from . import (MLP, Graph, Pattern, Plan, build_plan, count_sampled, order_cost, order_features, rank_orders)
 
 FEATURE_SAMPLES = 512
 
 
-def candidate_orders(pattern: Pattern, graph: Optional[Graph], top: int = 16, rng=None) -> List[List[int]]:
-    """Connected orders (deduplicated up to automorphism), the `top` cheapest by
-    the analytic cost model first; with rng, the rest filled by random others."""
-    orders = pattern.orders()
-    costs = np.array([order_cost(pattern, graph, o) for o in orders])
-    idx = list(np.argsort(costs, kind="stable"))
-    pick = idx[:top]
-    if rng is not None and len(idx) > top:
-        extra = rng.choice(idx[top:], size=min(top, len(idx) - top), replace=False)
-        pick += list(extra)
+def candidate_orders(pattern: Pattern, graph: Optional[Graph], mode: int, top: int = 16, rng=None) -> List[List[int]]:
+    """Connected orders (deduplicated up to automorphism) ranked as the planner
+    ranks them for `mode`; the `top` best, and with rng as many random others."""
+    orders, costs, tails = rank_orders(pattern, graph, mode)
+    pick = list(range(min(top, len(orders))))
+    if rng is not None and len(orders) > top:
+        extra = rng.choice(np.arange(top, len(orders)), size=min(top, len(orders) - top), replace=False)
+        pick += [int(i) for i in extra]
     return [orders[i] for i in pick]
 
 
@@ -125,7 +123,7 @@ def choose_plan(model: LearnedOrder, pattern: Pattern, graph: Graph, mode: int, 
     the model's threshold (bits, calibrated scale)."""
     if fallback is None:
         fallback = build_plan(pattern, graph, mode)
-    cands = candidate_orders(pattern, graph, top)
+    cands = candidate_orders(pattern, graph, mode, top)
     X = np.stack([order_features(pattern, graph, o, mode, FEATURE_SAMPLES, 11) for o in cands])
     pred = model.predict_log2(X)
     best = int(np.argmin(pred))
@@ -147,12 +145,12 @@ def train(graph: Graph, patterns: Sequence[Pattern], mode: int, stride: int = 64
     X, y, c = [], [], []
     t0 = time.time()
     for i, p in enumerate(patterns):
-        orders = candidate_orders(p, graph, orders_per_pattern // 2, rng)[:orders_per_pattern]
+        orders = candidate_orders(p, graph, mode, orders_per_pattern // 2, rng)[:orders_per_pattern]
         ms, by, _ = log_work(graph, p, orders, mode, stride)
         for o, t in zip(orders, ms):
             X.append(order_features(p, graph, o, mode, FEATURE_SAMPLES, 11))
             y.append(np.log2(max(t, 1e-3)))
-            c.append(np.log2(max(order_cost(p, graph, o), 1.0)))
+            c.append(np.log2(max(order_cost(p, graph, o, mode), 1.0)))
         if i % 25 == 0:
             log(f"[learned] logged {i + 1}/{len(patterns)} patterns, {len(X)} orders, {time.time() - t0:.0f}s")
     X, y, c = np.array(X, np.float32), np.array(y, np.float64), np.array(c, np.float64)
