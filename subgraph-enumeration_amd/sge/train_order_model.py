@@ -1,14 +1,15 @@
-"""Train the learned attribute-order model on the GPU and evaluate it.
+"""Train the learned attribute-order model on the GPU and judge it.
 
-    python -m sge.train_order_model --out sge/models/order_mlp.npz [--patterns 400]
+    python -m sge.train_order_model --out sge/models/order_mlp.npz
 
 Logs (order, measured work) pairs with the plain executor on a strided
-root-edge sample of the scale-20 R-MAT graph (seed 4) for connected 7-vertex
-patterns disjoint from the 78-pattern benchmark set (sampler seed 5), fits the
-MLP (sge_mlp_* kernels), then judges it on the 78 benchmark patterns: predicted
-vs measured work, and the measured time of the learned choice against the
-analytic cost model's choice on the same sample.  Writes the weights (.npz) and
-a JSON report next to them.
+root-edge sample of the Chung-Lu graph (n=7,115, m=100,762, seed 1) for
+patterns disjoint from the benchmark set: 400 connected 7-vertex patterns with
+11-14 edges (sampler seed 5) and the connected 6-vertex patterns with 8-10
+edges.  Fits the MLP (sge_mlp_* kernels), then judges it on the 16 benchmark
+6-vertex patterns with full exact runs: predicted vs measured work, and the
+measured time of the learned choice against the analytic cost model's choice.
+Writes the weights (.npz) and a JSON report next to them.
 """
 import argparse
 import json
@@ -31,8 +32,8 @@ def main():
     ap.add_argument("--out", default=os.path.join(HERE, "models", "order_mlp.npz"))
     ap.add_argument("--patterns", type=int, default=400)
     ap.add_argument("--orders", type=int, default=6)
-    ap.add_argument("--stride", type=int, default=65536)
-    ap.add_argument("--eval-stride", type=int, default=65536)
+    ap.add_argument("--stride", type=int, default=128)
+    ap.add_argument("--eval-stride", type=int, default=1)
     ap.add_argument("--mode", default="iep", choices=["wcoj", "explore", "iep"])
     ap.add_argument("--epochs", type=int, default=300)
     ap.add_argument("--log", default=None)
@@ -48,10 +49,11 @@ def main():
             logf.flush()
 
     t0 = time.time()
-    g = W.graph_c4()
+    g = W.graph_c2()
     g.to_device()
     log(f"graph n={g.n} m={g.m} in {time.time() - t0:.1f}s")
-    train_p = W.training_set(args.patterns)
+    bench_codes = [p.info().canon for p in W.six_vertex_set()]
+    train_p = W.training_set(args.patterns) + sge.sample_patterns(6, 8, 10, 80, 5, bench_codes)
     model, (X, y, c) = L.train(g, train_p, mode, stride=args.stride, orders_per_pattern=args.orders,
                                epochs=args.epochs, log=log)
     pred = model.predict_log2(X)
@@ -62,7 +64,7 @@ def main():
     # judge on the benchmark set
     rows = []
     t_cost = t_learn = 0.0
-    for i, p in enumerate(W.seven_vertex_set()):
+    for i, p in enumerate(W.six_vertex_set()):
         cost_plan = sge.build_plan(p, g, mode)
         plan, info = L.choose_plan(model, p, g, mode, fallback=cost_plan)
         ms = []
@@ -78,7 +80,7 @@ def main():
                      "fallback": info["fallback"], "cost_order": info["cost_order"],
                      "learned_order": info["learned_order"]})
         log(f"eval {i}: cost-model {ms[0]:.2f} ms, learned {ms[1]:.2f} ms, fallback {info['fallback']}")
-    report = {"graph": "rmat(scale=20, ef=16, a=.57, b=.19, c=.19, seed=4)", "mode": args.mode,
+    report = {"graph": "chung_lu(n=7115, m=100762, gamma=2.1, cap=1100, seed=1)", "mode": args.mode,
               "train_patterns": len(train_p), "train_orders": int(len(X)), "train_stride": args.stride,
               "eval_stride": args.eval_stride, "fit": fit, "loss_first_last": [model.history["loss"][0],
                                                                               model.history["loss"][-1]],
