@@ -180,17 +180,22 @@ def main():
     except ImportError:
         pass
     log(f"{len(pats)} plans in {plan_s:.2f}s (learned order: {learned})")
+    from sge import autotune
+
+    tuning = autotune.tuned(wname)
+    tuning_desc = {k: getattr(tuning, k) for k, _ in sge.Tuning._fields_} if tuning else "library defaults"
+    log(f"tuning: {tuning_desc}")
 
     # plain executor: cost-model order, one launch per pattern
     plain_ms = None
     plain_counts = None
     if not args.no_plain:
         t0 = time.perf_counter()
-        plain_counts, pst = sge.count_sampled(g, plans_cost, stride)
+        plain_counts, pst = sge.count_sampled(g, plans_cost, stride, tuning=tuning)
         plain_ms = (time.perf_counter() - t0) * 1000
         log(f"plain: {plain_ms:.1f} ms, kernel {sum(s.kernel_ms for s in pst):.1f} ms, total {sum(plain_counts)}")
 
-    batch = sge.Batch(g, plans_fast, stride=stride)
+    batch = sge.Batch(g, plans_fast, tuning=tuning, stride=stride)
     for i in range(args.warmup):
         counts, st = batch.run()
         log(f"warmup {i}: {st.kernel_ms:.1f} ms")
@@ -222,7 +227,7 @@ def main():
         "vs_baseline": None,
         "dtype": "u32 ids / u64 counts",
         "data": "synthetic",
-        "config": {"workload": wname, "mode": args.mode, "learned_order": learned, **cfg,
+        "config": {"workload": wname, "mode": args.mode, "learned_order": learned, "tuning": tuning_desc, **cfg,
                    "n": g.n, "m": g.m, "patterns": len(pats), "root_edge_stride": stride,
                    "root_edges": (2 * g.m + stride - 1) // stride},
         "pattern_set_ms": {"batched": step_s * 1000, "plain": plain_ms,

@@ -36,6 +36,7 @@ extern "C" {
 #define SGE_EIO (-2)      /* file open / parse failure (reference: exit(1), graph.cpp:260,296,665,683) */
 #define SGE_EDEVICE (-3)  /* HIP runtime failure or no device */
 #define SGE_ENOMEM (-4)   /* allocation failure */
+#define SGE_ELIMIT (-6)   /* the tuning's node cap stopped the call; its counts are partial */
 #define SGE_ERANGE (-5)   /* capacity exceeded (output buffer, sizes) */
 
 /* executor modes (reference flags: default WCOJ, -i exploration, -u IEP) */
@@ -168,13 +169,20 @@ typedef struct sge_tuning {
     uint32_t block_threads;   /* 64..1024, multiple of 64 (one DFS per wavefront) */
     uint32_t chunk;           /* root tasks taken per queue fetch */
     uint32_t stage_max;       /* probe-direction ratio: a sibling's row drives the count while
-                                 |row| <= stage_max * |F| and F is staged in LDS (0 = default 8) */
+                                 |row| <= stage_max * |F| and F is staged in LDS as a sorted copy
+                                 (0 = default 2) */
     uint32_t blocks_per_cu;   /* 0 = occupancy-derived */
     uint32_t split_span;      /* a root edge whose level-2 set may exceed this is split into
                                  ceil(bound / split_span) subtasks (0 = default 64, 1 << 30 = off) */
-    uint32_t donate_budget;   /* search-tree nodes one work item may expand before its open
-                                 candidate ranges go to the next pass (0 = default 4096, 1 << 30 = off) */
-    uint32_t reserved[2];
+    uint32_t donate_budget;   /* wave steps (one per search-tree node plus one per 64 candidates it
+                                 forms, plus the 64-lane iterations of each last-level count) one
+                                 work item may spend before its open candidate ranges go to the
+                                 next pass (0 = default 4096, 1 << 30 = off) */
+    uint32_t bitmap_ratio;    /* the same ratio when F is staged as an LDS bitmap (0 = stage_max) */
+    uint32_t node_cap_k;      /* stop a counting call after about node_cap_k * 1024 wave steps (as
+                                 donate_budget counts them) and fail with SGE_ELIMIT (0 = no cap;
+                                 the batched executor ignores it).  For timing orders of unknown
+                                 cost. */
 } sge_tuning;
 
 typedef struct sge_stats {

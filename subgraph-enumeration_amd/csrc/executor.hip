@@ -54,9 +54,13 @@ namespace {
 
 constexpr int WAVE = 64;
 constexpr int MAXK = SGE_MAX_K;
-constexpr int SIB_BLOCK = 128;  // (sibling, term) entries per flattened last-level batch
+#ifndef SGE_SIB_BLOCK
+#define SGE_SIB_BLOCK 64
+#endif
+constexpr int SIB_BLOCK = SGE_SIB_BLOCK;  // (sibling, term) entries per flattened last-level batch
 constexpr int MAX_TERMS = 7;    // inclusion-exclusion terms of a 3-level independent tail
 constexpr int NBUF = 2 * MAXK + 1 + MAX_TERMS;  // scratch buffers per wave
+constexpr uint32_t NODE_STEP = 1024;  // work-cap accounting granularity per wavefront (wave steps)
 constexpr int FSTAGE = 1024;    // LDS words per wave for staging the terms' F sets (bitmaps or copies)
 constexpr uint32_t NONE = 0xffffffffu;
 
@@ -108,9 +112,11 @@ struct KParams {
     uint32_t *emb;
     uint64_t emb_cap;
     unsigned long long *emb_n;
-    // work donation between passes: a wave that expands more than `budget`
-    // search-tree nodes inside one item writes the untouched candidate ranges
-    // of its open levels as continuation items for the next pass
+    // work donation between passes: a wave that spends more than `budget`
+    // wave steps (one per node plus one per 64 candidates formed, and the
+    // 64-lane iterations of each last-level count) inside one item writes the
+    // untouched candidate ranges of its open levels as continuation items for
+    // the next pass
     const uint32_t *in_items;          // this pass's input items (null: root pass)
     const unsigned long long *in_count;
     uint32_t *out_items;
@@ -118,6 +124,9 @@ struct KParams {
     uint64_t item_cap;                 // capacity of each item buffer
     uint32_t budget;                   // 0: never donate
     uint32_t probe_ratio;              // drive with the sibling's row while |row| <= ratio * |F| (F staged in LDS)
+    uint32_t bitmap_ratio;             // the same when F is staged as a bitmap
+    unsigned long long node_cap;       // stop after about this many wave steps of search work (0 = no cap)
+    unsigned long long *node_ctr;      // wave steps spent so far, reported in blocks of >= NODE_STEP
 };
 
 // continuation item: plan, level l, absolute candidate range [b, e) of level l,
@@ -398,7 +407,9 @@ __device__ __forceinline__ bool term_has(const WaveLds &w, const TermLds &t, uin
 // depend on the sibling, with every bound/check from levels below the sibling
 // applied; per (sibling, term) the count is |F ∩ N(c) ∩ range(c)| (or a range
 // size), computed with the flattened probe loop.
-__device__ __forceinline__ void count_last(const KParams &P, WaveLds &w, TermLds *terms, uint32_t *wb, Acc &acc) {
+// Returns the wave steps spent (64-lane iterations of the entry setup and the
+// flattened probes), the unit of the donation budget and the node cap.
+__device__ __forceinline__ uint32_t count_last(const KParams &P, WaveLds &w, TermLds *terms, uint32_t *wb, Acc &acc) {
     const DPlan &pl = w.pl;
     const int s = pl.sib;
     const int lane = lane_id();
@@ -490,6 +501,7 @@ __device__ __forceinline__ void count_last(const KParams &P, WaveLds &w, TermLds
     const bool per_entry = (kind != 0) || K > 1;  // counts needed per (sibling, term)
     const uint32_t per_block = SIB_BLOCK / K;     // siblings per batch
     unsigned long long local = 0;                 // per-lane partial
+    uint32_t steps = 1;
     for (uint32_t sb = 0; sb < nsib; sb += per_block) {
         const uint32_t nb = min(per_block, nsib - sb);
         const uint32_t ne = nb * K;  // entries: sibling i, term j -> i*K + j
@@ -525,7 +537,7 @@ __device__ __forceinline__ void count_last(const KParams &P, WaveLds &w, TermLds
                             // probes into a staged F cost LDS reads (one for a bitmap),
                             // probes into a row cost a global binary search
                             const uint64_t ratio =
-                                (t.flags & 16u) ? 4u * P.probe_ratio : ((t.flags & 8u) ? P.probe_ratio : 1u);
+                                (t.flags & 16u) ? P.bitmap_ratio : ((t.flags & 8u) ? P.probe_ratio : 1u);
                             if ((uint64_t)(re - rb) <= ratio * (uint64_t)(fe - fb)) {
                                 flag = 1;
                                 dstart = rb;
@@ -565,6 +577,7 @@ __device__ __forceinline__ void count_last(const KParams &P, WaveLds &w, TermLds
         if (lane == 0) w.pre_sum[0] = 0;
         wave_sync();
         const uint32_t W = w.pre_sum[ne];
+        steps += (ne + W + WAVE - 1) / WAVE;
         acc.inter += ne;
         acc.bytes += 8ull * nb;
         // phase 2: flattened probes
@@ -621,6 +634,7 @@ __device__ __forceinline__ void count_last(const KParams &P, WaveLds &w, TermLds
     }
     for (int off = WAVE / 2; off > 0; off >>= 1) local += __shfl_xor(local, off);
     acc.count += local;
+    return steps;
 }
 
 __device__ void flush(const KParams &P, uint32_t plan, Acc &acc) {
@@ -723,7 +737,7 @@ template <bool ENUM>
 __device__ __forceinline__ void run_dfs(const KParams &P, WaveLds &w, TermLds *terms, uint32_t *wb, Acc &acc, uint32_t plan, int l0,
                         int stop) {
     const DPlan &pl = w.pl;
-    uint32_t spent = 0;
+    uint32_t spent = 0, since = 0;  // wave steps since the last donation / node-cap report
     int d = l0;
     while (d >= l0) {
         if (d == stop) {
@@ -736,7 +750,9 @@ __device__ __forceinline__ void run_dfs(const KParams &P, WaveLds &w, TermLds *t
                     emit_last(P, w, wb, acc);
                 }
             } else {
-                count_last(P, w, terms, wb, acc);
+                const uint32_t st = count_last(P, w, terms, wb, acc);
+                spent += st;
+                since += st;
             }
             --d;
             continue;
@@ -761,14 +777,25 @@ __device__ __forceinline__ void run_dfs(const KParams &P, WaveLds &w, TermLds *t
             if (pl.mode == SGE_MODE_EXPLORE && !ENUM) explore_after(P, w, wb, d, acc);
             materialize(P, w, wb, d + 1, acc);
             ++d;
-            if (P.budget && P.out_items && ++spent >= P.budget) {
+            const uint32_t st = 1 + ((w.cand[d].e - w.cand[d].b) >> 6);
+            spent += st;
+            since += st;
+            if (P.budget && P.out_items && spent >= P.budget) {
                 donate(P, w, plan, l0, d, stop);
                 spent = 0;
+            }
+            if (P.node_cap && since >= NODE_STEP) {
+                unsigned long long tot = 0;
+                if (lane_id() == 0) tot = atomicAdd(P.node_ctr, (unsigned long long)since);
+                tot = __shfl(tot, 0);
+                since = 0;
+                if (tot >= P.node_cap) return;  // the host reports the call as stopped
             }
         } else {
             --d;
         }
     }
+    if (P.node_cap && since && lane_id() == 0) atomicAdd(P.node_ctr, (unsigned long long)since);
 }
 
 template <bool ENUM>
@@ -791,6 +818,8 @@ __global__ void __launch_bounds__(256) wcoj_kernel(KParams P) {
         if (lane == 0) item = atomicAdd(P.queue, 1ull);
         item = __shfl(item, 0);
         if (item >= nitems) break;
+        if (P.node_cap && __hip_atomic_load(P.node_ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= P.node_cap)
+            break;
         if (cont) {
             // ---- continuation item: rebuild the path, then search its range
             const uint32_t *it = P.in_items + item * ITEM_WORDS;
@@ -1085,7 +1114,9 @@ struct LaunchCfg {
     size_t lds = 0;
     uint32_t chunk = 64;
     uint32_t budget = 4096;
-    uint32_t probe_ratio = 8;
+    uint32_t probe_ratio = 2;
+    uint32_t bitmap_ratio = 2;
+    unsigned long long node_cap = 0;
 };
 
 int launch_cfg(const sge_tuning *t, bool enumerate, LaunchCfg &cfg) {
@@ -1100,7 +1131,9 @@ int launch_cfg(const sge_tuning *t, bool enumerate, LaunchCfg &cfg) {
     }
     cfg.chunk = t->chunk ? t->chunk : 64;
     cfg.budget = t->donate_budget ? (t->donate_budget >= (1u << 30) ? 0u : t->donate_budget) : 4096u;
-    cfg.probe_ratio = t->stage_max ? t->stage_max : 8u;
+    cfg.probe_ratio = t->stage_max ? t->stage_max : 2u;
+    cfg.bitmap_ratio = t->bitmap_ratio ? t->bitmap_ratio : cfg.probe_ratio;
+    cfg.node_cap = (unsigned long long)t->node_cap_k * NODE_STEP;
     cfg.lds = (size_t)(cfg.block / WAVE) * (sizeof(WaveLds) + MAX_TERMS * sizeof(TermLds));
     int per_cu = (int)t->blocks_per_cu;
     if (per_cu <= 0) {
@@ -1140,6 +1173,7 @@ __global__ void reset_kernel(PlanOut *out, uint32_t n, unsigned long long *queue
     if (i < n) out[i] = PlanOut{0, 0, 0, 0, 0, {0, 0, 0}};
     if (i == 0) {
         queue[0] = 0;
+        queue[2] = 0;  // node-cap counter
         if (emb_n) *emb_n = 0;
     }
 }
@@ -1174,6 +1208,9 @@ int run_launch(sge_graph *g, const DPlan *dplans, uint32_t np, PlanOut *dout, ui
     kp.item_cap = ITEM_CAP;
     kp.budget = cfg.budget;
     kp.probe_ratio = cfg.probe_ratio;
+    kp.bitmap_ratio = cfg.bitmap_ratio;
+    kp.node_cap = cfg.node_cap;
+    kp.node_ctr = c.queue + 2;
     reset_kernel<<<(np + 255) / 256, 256, 0, c.stream>>>(dout, np, c.queue, emb_n);
     reset_pass_kernel<<<1, 64, 0, c.stream>>>(c.queue, c.icount);
     HIP_TRY(hipGetLastError());
@@ -1188,6 +1225,11 @@ int run_launch(sge_graph *g, const DPlan *dplans, uint32_t np, PlanOut *dout, ui
             HIP_TRY(hipMemcpyAsync(&n, c.icount + cur, sizeof(n), hipMemcpyDeviceToHost, c.stream));
             HIP_TRY(hipStreamSynchronize(c.stream));
             if (n == 0) break;
+            if (kp.node_cap) {
+                unsigned long long spent = 0;
+                HIP_TRY(hipMemcpy(&spent, kp.node_ctr, sizeof(spent), hipMemcpyDeviceToHost));
+                if (spent >= kp.node_cap) break;
+            }
             kp.in_items = c.items[cur];
             kp.in_count = c.icount + cur;
             cur ^= 1;
@@ -1207,6 +1249,14 @@ int run_launch(sge_graph *g, const DPlan *dplans, uint32_t np, PlanOut *dout, ui
     float t = 0;
     HIP_TRY(hipEventElapsedTime(&t, c.ev0, c.ev1));
     if (ms) *ms = t;
+    if (kp.node_cap) {
+        unsigned long long spent = 0;
+        HIP_TRY(hipMemcpy(&spent, kp.node_ctr, sizeof(spent), hipMemcpyDeviceToHost));
+        if (spent >= kp.node_cap) {
+            sge::set_error("node cap reached (" + std::to_string(spent) + " search-tree nodes): counts are partial");
+            return SGE_ELIMIT;
+        }
+    }
     return SGE_OK;
 }
 
@@ -1400,12 +1450,13 @@ static int count_impl(sge_graph *g, const sge_plan *plans, uint32_t nplans, uint
     if (per_plan) std::memset(per_plan, 0, sizeof(sge_stats) * nplans);
     std::vector<PlanOut> host(nplans);
     std::vector<float> ms(nplans, 0.f);
-    bool any_k1 = false;
+    bool any_k1 = false, limited = false;
     for (uint32_t i = 0; i < nplans; ++i) any_k1 = any_k1 || plans[i].k < 2;
     if ((flags & SGE_COUNT_BATCHED) && !any_k1) {
         float t = 0;
         rc = run_launch(g, c.plans, nplans, c.out, tb, te, stride, max_sub, cfg, false, nullptr, 0, nullptr, &t);
-        if (rc) return rc;
+        if (rc == SGE_ELIMIT) limited = true;
+        else if (rc) return rc;
         HIP_TRY(hipMemcpy(host.data(), c.out, sizeof(PlanOut) * nplans, hipMemcpyDeviceToHost));
         if (stats) stats->kernel_ms = t;
         for (uint32_t i = 0; i < nplans; ++i) {
@@ -1420,6 +1471,10 @@ static int count_impl(sge_graph *g, const sge_plan *plans, uint32_t nplans, uint
             }
             rc = run_launch(g, c.plans + i, 1, c.out + i, tb, te, stride, nsub[i], cfg, false, nullptr, 0, nullptr,
                             &ms[i]);
+            if (rc == SGE_ELIMIT) {  // keep going: the other plans get their own cap
+                limited = true;
+                continue;
+            }
             if (rc) return rc;
         }
         HIP_TRY(hipStreamSynchronize(c.stream));
@@ -1435,6 +1490,10 @@ static int count_impl(sge_graph *g, const sge_plan *plans, uint32_t nplans, uint
         counts[i] = cnt * (plans[i].multiplier ? plans[i].multiplier : 1);
     }
     if (stats) stats->wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (limited) {
+        sge::set_error("node cap reached: counts are partial (stats cover the work done)");
+        return SGE_ELIMIT;
+    }
     return SGE_OK;
 }
 
@@ -1647,6 +1706,7 @@ extern "C" int sge_batch_create_sampled(sge_graph *g, const sge_plan *plans, uin
     kp.item_cap = ITEM_CAP;
     kp.budget = cfg.budget;
     kp.probe_ratio = cfg.probe_ratio;
+    kp.bitmap_ratio = cfg.bitmap_ratio;
     kp.out_items = b->items[0];
     kp.out_count = b->icount;
     b->kp = kp;

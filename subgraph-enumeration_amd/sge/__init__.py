@@ -18,15 +18,18 @@ import numpy as np
 MAX_K = 8
 MODE_WCOJ, MODE_EXPLORE, MODE_IEP = 0, 1, 2
 COUNT_BATCHED = 1
+ELIMIT = -6
 GRAPH_DEGREE_ORDER = 1
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(os.path.dirname(_HERE), "libsge.so")
+LIB_PATH = os.environ.get("SGE_LIB") or os.path.join(os.path.dirname(_HERE), "libsge.so")
 CLI_PATH = os.path.join(os.path.dirname(_HERE), "graph_mining_system")
 
 
 class SgeError(RuntimeError):
-    pass
+    def __init__(self, msg: str, code: int = 0):
+        super().__init__(msg)
+        self.code = code
 
 
 class GraphStats(C.Structure):
@@ -65,7 +68,7 @@ class Plan(C.Structure):
 class Tuning(C.Structure):
     _fields_ = [("block_threads", C.c_uint32), ("chunk", C.c_uint32), ("stage_max", C.c_uint32),
                 ("blocks_per_cu", C.c_uint32), ("split_span", C.c_uint32), ("donate_budget", C.c_uint32),
-                ("reserved", C.c_uint32 * 2)]
+                ("bitmap_ratio", C.c_uint32), ("node_cap_k", C.c_uint32)]
 
 
 class Stats(C.Structure):
@@ -158,7 +161,7 @@ def lib():
 def _check(rc: int, what: str):
     if rc != 0:
         msg = lib().sge_last_error().decode(errors="replace")
-        raise SgeError(f"{what} failed ({rc}): {msg}")
+        raise SgeError(f"{what} failed ({rc}): {msg}", rc)
 
 
 def _ptr(a: np.ndarray):
@@ -374,6 +377,36 @@ def count_sampled(graph: Graph, plans, stride: int, offset: int = 0, tuning: Opt
     _check(lib().sge_count_sampled(graph._h, arr, n, stride, offset, C.byref(tuning) if tuning else None,
                                    _ptr(counts), pp), "sge_count_sampled")
     return [int(x) for x in counts], [pp[i] for i in range(n)]
+
+
+def work_steps_bound(st: "Stats") -> int:
+    """An upper bound on the wave steps (the unit of the donation budget and the
+    node cap) behind a call's Stats."""
+    return int(st.nodes + st.intersections + st.bytes // 256 + 1)
+
+
+def time_capped(graph: Graph, plans, stride: int, node_cap_k: int, tuning: Optional[Tuning] = None):
+    """Plain executor over the strided root sample with a work cap per plan (in
+    units of 1024 wave steps): per-plan Stats and whether each plan hit the cap
+    (its count is then partial and its time a lower bound)."""
+    if isinstance(plans, Plan):
+        plans = [plans]
+    out = []
+    for p in plans:
+        t = Tuning()
+        if tuning is not None:
+            C.memmove(C.byref(t), C.byref(tuning), C.sizeof(Tuning))
+        else:
+            lib().sge_tuning_default(C.byref(t))
+        t.node_cap_k = node_cap_k
+        arr = (Plan * 1)(p)
+        counts = np.zeros(1, dtype=np.uint64)
+        pp = (Stats * 1)()
+        rc = lib().sge_count_sampled(graph._h, arr, 1, stride, 0, C.byref(t), _ptr(counts), pp)
+        if rc not in (0, ELIMIT):
+            _check(rc, "sge_count_sampled")
+        out.append((pp[0], rc == ELIMIT))
+    return out
 
 
 def order_features(pattern: Pattern, graph: Optional[Graph], order, mode: int, nsamples: int = 1024,

@@ -11,18 +11,21 @@ from __future__ import annotations
 
 import json
 import os
+import sys
 import time
 from typing import Dict, Optional, Sequence
 
 from . import Batch, Graph, Plan, Tuning, default_tuning
 
-SPACE = {
+SPACE = {  # first value = the library default, the starting point
     "block_threads": [256, 128, 64],
-    "chunk": [8, 32, 128],
-    "split_span": [16, 64, 256],
-    "donate_budget": [1024, 4096, 16384],
-    "stage_max": [2, 8, 32],
+    "chunk": [32, 8, 128],
+    "split_span": [64, 16, 256],
+    "donate_budget": [4096, 1024, 16384],
+    "stage_max": [2, 1, 4],
+    "bitmap_ratio": [2, 1, 4],
 }
+CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "models", "tuning.json")
 
 
 def _tuning(d: Dict[str, int]) -> Tuning:
@@ -53,8 +56,7 @@ def tune(graph: Graph, plans: Sequence[Plan], stride: int, rounds: int = 1, log=
             data = json.load(f)
         if key in data:
             return data[key]["params"]
-    cur = {k: v[0] if k != "chunk" else 32 for k, v in SPACE.items()}
-    cur.update({"split_span": 64, "donate_budget": 4096, "stage_max": 8})
+    cur = {k: v[0] for k, v in SPACE.items()}
     t0 = time.time()
     best_ms, ref = _time(graph, plans, stride, cur)
     log(f"[autotune] start {cur}: {best_ms:.2f} ms")
@@ -82,3 +84,43 @@ def tune(graph: Graph, plans: Sequence[Plan], stride: int, rounds: int = 1, log=
         with open(cache, "w") as f:
             json.dump(data, f, indent=1)
     return cur
+
+
+def tuned(key: str, cache: str = CACHE) -> Optional[Tuning]:
+    """The cached tuning for `key`, or None."""
+    if not os.path.exists(cache):
+        return None
+    with open(cache) as f:
+        data = json.load(f)
+    return _tuning(data[key]["params"]) if key in data else None
+
+
+def main():
+    """Tune the batched executor for the 16 six-vertex patterns over the
+    Chung-Lu graph (n=7,115, m=100,762), IEP mode, whole job, and cache the
+    result under the workload's name."""
+    import argparse
+
+    from . import MODE_IEP, build_plan
+    from . import workloads as W
+
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--stride", type=int, default=1)
+    ap.add_argument("--rounds", type=int, default=1)
+    ap.add_argument("--budget-s", type=float, default=240.0)
+    args = ap.parse_args()
+    g = W.graph_c2()
+    g.to_device()
+    plans = [build_plan(p, g, MODE_IEP) for p in W.six_vertex_set()]
+    if os.path.exists(CACHE):
+        with open(CACHE) as f:
+            data = json.load(f)
+        data.pop("chunglu7115_sixvertex16", None)
+        with open(CACHE, "w") as f:
+            json.dump(data, f, indent=1)
+    tune(g, plans, args.stride, args.rounds, log=lambda *a: print(*a, flush=True), cache=CACHE,
+         key="chunglu7115_sixvertex16", budget_s=args.budget_s)
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -2,7 +2,7 @@
 
 A 64 -> 128 -> 128 -> 1 MLP (BF16 MFMA hidden layers, FP32 accumulation, Adam;
 all on the GPU through sge_mlp_*) predicts log2 of the plain GPU executor's
-measured device time for a (pattern, order) from the order's features
+measured device time (projected to the whole job) for a (pattern, order) from the order's features
 (sge_order_features: plan structure, degree moments, analytic cost, Knuth
 random-walk estimates over sampled root edges).  It only reorders: counts never
 change.  The analytic cost model (plan/optimizer.cpp:441-577 optCostOrder,
@@ -23,33 +23,51 @@ from typing import Dict, List, Optional, Sequence
 
 import numpy as np
 
-from . import (MLP, Graph, Pattern, Plan, build_plan, count_sampled, order_cost, order_features, rank_orders)
+from . import (MLP, Graph, Pattern, Plan, build_plan, count_sampled, order_cost, order_features, rank_orders,
+               time_capped, work_steps_bound)
 
 FEATURE_SAMPLES = 512
+CAPPED_PENALTY = 2.0  # bits added to the time of an order stopped by the work cap
 
 
 def candidate_orders(pattern: Pattern, graph: Optional[Graph], mode: int, top: int = 16, rng=None) -> List[List[int]]:
     """Connected orders (deduplicated up to automorphism) ranked as the planner
-    ranks them for `mode`; the `top` best, and with rng as many random others."""
+    ranks them for `mode`; the `top` best, and with rng up to `top` others drawn
+    from ranks [top, 4 * top) -- the neighbourhood the model has to choose in."""
     orders, costs, tails = rank_orders(pattern, graph, mode)
     pick = list(range(min(top, len(orders))))
-    if rng is not None and len(orders) > top:
-        extra = rng.choice(np.arange(top, len(orders)), size=min(top, len(orders) - top), replace=False)
+    hi = min(len(orders), 4 * top)
+    if rng is not None and hi > top:
+        extra = rng.choice(np.arange(top, hi), size=min(top, hi - top), replace=False)
         pick += [int(i) for i in extra]
     return [orders[i] for i in pick]
 
 
-def log_work(graph: Graph, pattern: Pattern, orders: Sequence[Sequence[int]], mode: int, stride: int,
-             reps: int = 2):
-    """Measured work of each order: plain executor on the strided root sample.
-    Returns (device ms [median of reps], algorithmic bytes, counts)."""
+def log_work(graph: Graph, pattern: Pattern, orders: Sequence[Sequence[int]], mode: int, stride: int = 1024,
+             fine_stride: int = 256, min_ms: float = 20.0, cap_factor: int = 8, pilot_cap_k: int = 1 << 20):
+    """Measured work of each order with the plain executor on a strided root
+    sample.  orders[0] (the cost model's choice) is the pilot: it runs under
+    the absolute cap `pilot_cap_k` (x1024 wave steps) -- None is returned when
+    even it is stopped -- and when it takes under `min_ms` the sample is refined
+    to `fine_stride`.  The other orders run under a cap of `cap_factor` times the
+    pilot's work.  Returns (projected full-job ms = sample ms x stride,
+    algorithmic bytes, capped flags, stride); a capped order's time is a lower
+    bound."""
     plans = [build_plan(pattern, graph, mode, o) for o in orders]
-    times = []
-    for _ in range(reps):
-        counts, st = count_sampled(graph, plans, stride)
-        times.append([s.kernel_ms for s in st])
-    ms = np.median(np.array(times), axis=0)
-    return ms, np.array([s.bytes for s in st], dtype=np.float64), counts
+    (pilot, lim), = time_capped(graph, plans[0], stride, pilot_cap_k)
+    if lim:
+        return None
+    if pilot.kernel_ms < min_ms and fine_stride < stride:
+        stride = fine_stride
+        (pilot, lim), = time_capped(graph, plans[0], stride, pilot_cap_k)
+        if lim:
+            return None
+    cap_k = max(64, cap_factor * work_steps_bound(pilot) // 1024 + 1)
+    res = [(pilot, False)] + time_capped(graph, plans[1:], stride, cap_k)
+    ms = np.array([st.kernel_ms * stride for st, _ in res])
+    by = np.array([st.bytes for st, _ in res], dtype=np.float64)
+    capped = np.array([lim for _, lim in res], dtype=bool)
+    return ms, by, capped, stride
 
 
 class LearnedOrder:
@@ -138,23 +156,74 @@ def choose_plan(model: LearnedOrder, pattern: Pattern, graph: Graph, mode: int, 
     return build_plan(pattern, graph, mode, cands[best]), info
 
 
-def train(graph: Graph, patterns: Sequence[Pattern], mode: int, stride: int = 64, orders_per_pattern: int = 8,
-          epochs: int = 300, seed: int = 0, log=print):
-    """Log (order, measured work) pairs with the plain executor and fit the MLP."""
+def judge(model: "LearnedOrder", graph: Graph, patterns: Sequence[Pattern], mode: int, stride: int = 1,
+          reps: int = 2, log=print):
+    """Learned choice against the cost model's choice, per pattern: the same
+    executor on the same root sample, best of `reps` device times.  The learned
+    plan runs under a work cap of 4x the cost-model plan's work (a runaway
+    choice is reported as capped, never run to the end); uncapped, both counts
+    must agree."""
+    rows = []
+    for i, p in enumerate(patterns):
+        cost_plan = build_plan(p, graph, mode)
+        plan, info = choose_plan(model, p, graph, mode, fallback=cost_plan)
+        t_cost, t_learn, capped = [], [], False
+        for _ in range(reps):
+            cnt, st = count_sampled(graph, [cost_plan], stride)
+            t_cost.append(st[0].kernel_ms)
+            cap_k = max(1024, 4 * work_steps_bound(st[0]) // 1024 + 1)
+            (sl, lim), = time_capped(graph, plan, stride, cap_k) if not info["fallback"] else [(st[0], False)]
+            capped = capped or lim
+            t_learn.append(sl.kernel_ms)
+        if not capped and not info["fallback"]:
+            cl, _ = count_sampled(graph, [plan], stride)
+            if cl[0] != cnt[0]:
+                raise AssertionError(f"pattern {i}: learned plan counted {cl[0]}, cost-model plan {cnt[0]}")
+        row = {"pattern": i, "cost_ms": float(min(t_cost)), "learned_ms": float(min(t_learn)), "capped": capped,
+               "pred_log2_ms": info["pred_log2"], "fallback": info["fallback"], "cost_order": info["cost_order"],
+               "learned_order": info["learned_order"]}
+        rows.append(row)
+        log(f"[judge] pattern {i}: cost-model {row['cost_ms']:.2f} ms, learned {row['learned_ms']:.2f} ms"
+            f"{' (capped)' if capped else ''}, fallback {info['fallback']}")
+    return rows
+
+
+def log_pairs(graph: Graph, patterns: Sequence[Pattern], mode: int, stride: int = 1024, orders_per_pattern: int = 8,
+              seed: int = 0, log=print):
+    """(features, log2 projected full-job ms, log2 analytic cost, capped) of
+    `orders_per_pattern` orders per pattern, measured with log_work.  An order
+    stopped by its work cap is logged CAPPED_PENALTY bits above its
+    (lower-bound) time; a pattern whose pilot order is stopped is skipped."""
     rng = np.random.default_rng(seed)
-    X, y, c = [], [], []
+    X, y, c, cap = [], [], [], []
     t0 = time.time()
     for i, p in enumerate(patterns):
         orders = candidate_orders(p, graph, mode, orders_per_pattern // 2, rng)[:orders_per_pattern]
-        ms, by, _ = log_work(graph, p, orders, mode, stride)
-        for o, t in zip(orders, ms):
+        r = log_work(graph, p, orders, mode, stride)
+        if r is None:
+            log(f"[learned] pattern {i}: pilot order stopped by the cap, skipped")
+            continue
+        ms, by, capped, used = r
+        for o, t, cp in zip(orders, ms, capped):
             X.append(order_features(p, graph, o, mode, FEATURE_SAMPLES, 11))
-            y.append(np.log2(max(t, 1e-3)))
+            y.append(np.log2(max(t, 1e-3)) + (CAPPED_PENALTY if cp else 0.0))
             c.append(np.log2(max(order_cost(p, graph, o, mode), 1.0)))
-        if i % 25 == 0:
-            log(f"[learned] logged {i + 1}/{len(patterns)} patterns, {len(X)} orders, {time.time() - t0:.0f}s")
-    X, y, c = np.array(X, np.float32), np.array(y, np.float64), np.array(c, np.float64)
+            cap.append(bool(cp))
+        log(f"[learned] logged {i + 1}/{len(patterns)} patterns, {len(X)} orders, stride {used}, "
+            f"projected ms {float(np.min(ms)):.0f}..{float(np.max(ms)):.0f}, capped {int(np.sum(capped))}, "
+            f"{time.time() - t0:.0f}s")
+    return (np.array(X, np.float32), np.array(y, np.float64), np.array(c, np.float64), np.array(cap, bool))
+
+
+def fit_model(X: np.ndarray, y: np.ndarray, c: np.ndarray, epochs: int = 300, seed: int = 0, log=print):
     model = LearnedOrder(seed=seed)
     losses = model.fit(X, y, c, epochs=epochs, seed=seed)
     log(f"[learned] trained on {len(X)} orders, loss {losses[0]:.3f} -> {losses[-1]:.3f}")
-    return model, (X, y, c)
+    return model
+
+
+def train(graph: Graph, patterns: Sequence[Pattern], mode: int, stride: int = 1024, orders_per_pattern: int = 8,
+          epochs: int = 300, seed: int = 0, log=print):
+    """Log (order, measured work) pairs with the plain executor and fit the MLP."""
+    X, y, c, cap = log_pairs(graph, patterns, mode, stride, orders_per_pattern, seed, log)
+    return fit_model(X, y, c, epochs, seed, log), (X, y, c)

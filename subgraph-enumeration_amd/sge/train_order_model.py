@@ -3,13 +3,20 @@
     python -m sge.train_order_model --out sge/models/order_mlp.npz
 
 Logs (order, measured work) pairs with the plain executor on a strided
-root-edge sample of the Chung-Lu graph (n=7,115, m=100,762, seed 1) for
+root-edge sample of the Chung-Lu graph (n=7,115, m=100,762, seed 1), each order
+under a work cap relative to the cost model's order (learned.log_work), for
 patterns disjoint from the benchmark set: 400 connected 7-vertex patterns with
 11-14 edges (sampler seed 5) and the connected 6-vertex patterns with 8-10
 edges.  Fits the MLP (sge_mlp_* kernels), then judges it on the 16 benchmark
 6-vertex patterns with full exact runs: predicted vs measured work, and the
 measured time of the learned choice against the analytic cost model's choice.
 Writes the weights (.npz) and a JSON report next to them.
+
+Logging can be split over several runs and fitted afterwards:
+
+    python -m sge.train_order_model --log-only pairs_0.npz --first 0 --count 200
+    python -m sge.train_order_model --log-only pairs_1.npz --first 200 --count 256
+    python -m sge.train_order_model --data pairs_0.npz,pairs_1.npz --out sge/models/order_mlp.npz
 """
 import argparse
 import json
@@ -32,11 +39,15 @@ def main():
     ap.add_argument("--out", default=os.path.join(HERE, "models", "order_mlp.npz"))
     ap.add_argument("--patterns", type=int, default=400)
     ap.add_argument("--orders", type=int, default=6)
-    ap.add_argument("--stride", type=int, default=128)
+    ap.add_argument("--stride", type=int, default=1024)
     ap.add_argument("--eval-stride", type=int, default=1)
     ap.add_argument("--mode", default="iep", choices=["wcoj", "explore", "iep"])
     ap.add_argument("--epochs", type=int, default=300)
     ap.add_argument("--log", default=None)
+    ap.add_argument("--log-only", default=None, help="log (order, work) pairs to this .npz and stop")
+    ap.add_argument("--first", type=int, default=0, help="first training pattern to log")
+    ap.add_argument("--count", type=int, default=None, help="number of training patterns to log")
+    ap.add_argument("--data", default=None, help="comma-separated .npz files of logged pairs to fit from")
     args = ap.parse_args()
     mode = {"wcoj": sge.MODE_WCOJ, "explore": sge.MODE_EXPLORE, "iep": sge.MODE_IEP}[args.mode]
     logf = open(args.log, "a") if args.log else None
@@ -54,39 +65,38 @@ def main():
     log(f"graph n={g.n} m={g.m} in {time.time() - t0:.1f}s")
     bench_codes = [p.info().canon for p in W.six_vertex_set()]
     train_p = W.training_set(args.patterns) + sge.sample_patterns(6, 8, 10, 80, 5, bench_codes)
-    model, (X, y, c) = L.train(g, train_p, mode, stride=args.stride, orders_per_pattern=args.orders,
-                               epochs=args.epochs, log=log)
+    if args.data:
+        parts = [np.load(f, allow_pickle=False) for f in args.data.split(",")]
+        X = np.concatenate([d["X"] for d in parts])
+        y = np.concatenate([d["y"] for d in parts])
+        c = np.concatenate([d["c"] for d in parts])
+        log(f"loaded {len(X)} logged orders from {args.data}")
+    else:
+        sel = train_p[args.first:args.first + args.count if args.count else None]
+        X, y, c, cap = L.log_pairs(g, sel, mode, stride=args.stride, orders_per_pattern=args.orders,
+                                   seed=args.first, log=log)
+        if args.log_only:
+            np.savez(args.log_only, X=X, y=y, c=c, capped=cap)
+            log(f"wrote {len(X)} orders to {args.log_only}")
+            return
+    model = L.fit_model(X, y, c, epochs=args.epochs, log=log)
     pred = model.predict_log2(X)
     fit = {"train_rmse_log2": float(np.sqrt(np.mean((pred - y) ** 2))),
            "train_corr": float(np.corrcoef(pred, y)[0, 1]),
            "cost_model_corr": float(np.corrcoef(c, y)[0, 1])}
     log("fit", fit)
     # judge on the benchmark set
-    rows = []
-    t_cost = t_learn = 0.0
-    for i, p in enumerate(W.six_vertex_set()):
-        cost_plan = sge.build_plan(p, g, mode)
-        plan, info = L.choose_plan(model, p, g, mode, fallback=cost_plan)
-        ms = []
-        for _ in range(2):
-            cnt, st = sge.count_sampled(g, [cost_plan, plan], args.eval_stride)
-            assert cnt[0] == cnt[1], (i, cnt)
-            ms.append([s.kernel_ms for s in st])
-        ms = np.min(np.array(ms), axis=0)
-        t_cost += ms[0]
-        t_learn += ms[1]
-        rows.append({"pattern": i, "cost_ms": float(ms[0]), "learned_ms": float(ms[1]),
-                     "pred_log2_ms": info["pred_log2"], "measured_log2_ms": float(np.log2(max(ms[1], 1e-3))),
-                     "fallback": info["fallback"], "cost_order": info["cost_order"],
-                     "learned_order": info["learned_order"]})
-        log(f"eval {i}: cost-model {ms[0]:.2f} ms, learned {ms[1]:.2f} ms, fallback {info['fallback']}")
+    rows = L.judge(model, g, W.six_vertex_set(), mode, args.eval_stride, log=log)
+    t_cost = sum(r["cost_ms"] for r in rows)
+    t_learn = sum(r["learned_ms"] for r in rows)
     report = {"graph": "chung_lu(n=7115, m=100762, gamma=2.1, cap=1100, seed=1)", "mode": args.mode,
               "train_patterns": len(train_p), "train_orders": int(len(X)), "train_stride": args.stride,
               "eval_stride": args.eval_stride, "fit": fit, "loss_first_last": [model.history["loss"][0],
                                                                               model.history["loss"][-1]],
               "eval_total_ms": {"cost_model": t_cost, "learned": t_learn,
                                 "speedup": t_cost / t_learn if t_learn else None},
-              "fallbacks": int(sum(r["fallback"] for r in rows)), "per_pattern": rows,
+              "fallbacks": int(sum(r["fallback"] for r in rows)), "capped": int(sum(r["capped"] for r in rows)),
+              "per_pattern": rows,
               "wall_s": time.time() - t0}
     model.save(args.out)
     with open(os.path.splitext(args.out)[0] + "_report.json", "w") as f:

@@ -144,6 +144,42 @@ def test_tuning_variants_same_counts():
         assert sge.count(g, plans, batched=True, tuning=t)[0] == want
 
 
+def test_probe_direction_ratios_same_counts():
+    """Which side of a last-level term drives the probes (LDS-staged F or the
+    sibling's CSR row) must not change a count."""
+    g = W.graph_c2()
+    plans = [sge.build_plan(p, g, sge.MODE_IEP) for p in W.six_vertex_set()[:6]]
+    base = None
+    for stage, bm in ((8, 32), (1, 1), (64, 256), (4, 8)):
+        t = sge.default_tuning()
+        t.stage_max, t.bitmap_ratio = stage, bm
+        say("probe ratio", stage, "bitmap ratio", bm)
+        got = sge.count_sampled(g, plans, 31, 3, tuning=t)[0]
+        if base is None:
+            base = got
+        assert got == base, (stage, bm)
+
+
+def test_node_cap_stops_and_reports():
+    """A work cap ends the call with SGE_ELIMIT and partial stats; a cap the
+    search never reaches changes nothing."""
+    g = W.graph_c2()
+    plan = sge.build_plan(W.six_vertex_set()[1], g, sge.MODE_WCOJ)
+    want, st = sge.count_sampled(g, [plan], 16)
+    say("node cap 1k")
+    (s1, lim1), = sge.time_capped(g, plan, 16, 1)
+    assert lim1 and 0 < s1.nodes < st[0].nodes
+    with pytest.raises(sge.SgeError) as e:
+        t = sge.default_tuning()
+        t.node_cap_k = 1
+        sge.count_sampled(g, [plan], 16, tuning=t)
+    assert e.value.code == sge.ELIMIT
+    say("node cap above the work")
+    (s2, lim2), = sge.time_capped(g, plan, 16, 4 * sge.work_steps_bound(st[0]) // 1024 + 1024)
+    assert not lim2 and s2.nodes == st[0].nodes
+    assert sge.count_sampled(g, [plan], 16)[0] == want
+
+
 def test_root_edge_split_same_counts():
     """Heavy root edges split into level-2 slices must not change any count."""
     g = W.graph_c2()
