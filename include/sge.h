@@ -167,13 +167,13 @@ int sge_plan_text(const sge_plan *plan, char *buf, size_t cap);
 /* ---------------------------------------------------------------- executor */
 typedef struct sge_tuning {
     uint32_t block_threads;   /* 64..1024, multiple of 64 (one DFS per wavefront) */
-    uint32_t chunk;           /* root tasks taken per queue fetch */
+    uint32_t chunk;           /* root subtasks per queue item after the head items (0 = default 4) */
     uint32_t stage_max;       /* probe-direction ratio: a sibling's row drives the count while
                                  |row| <= stage_max * |F| and F is staged in LDS as a sorted copy
                                  (0 = default 2) */
     uint32_t blocks_per_cu;   /* 0 = occupancy-derived */
     uint32_t split_span;      /* a root edge whose level-2 set may exceed this is split into
-                                 ceil(bound / split_span) subtasks (0 = default 64, 1 << 30 = off) */
+                                 ceil(bound / split_span) subtasks (0 = default 32, 1 << 30 = off) */
     uint32_t donate_budget;   /* wave steps (one per search-tree node plus one per 64 candidates it
                                  forms, plus the 64-lane iterations of each last-level count) one
                                  work item may spend before its open candidate ranges go to the
@@ -183,6 +183,9 @@ typedef struct sge_tuning {
                                  donate_budget counts them) and fail with SGE_ELIMIT (0 = no cap;
                                  the batched executor ignores it).  For timing orders of unknown
                                  cost. */
+    uint32_t head_items;      /* the heaviest head_items root subtasks of each plan are queue
+                                 items of their own, the rest come in chunks (0 = one per
+                                 resident wavefront, 1 << 30 = none) */
 } sge_tuning;
 
 typedef struct sge_stats {

@@ -103,6 +103,7 @@ struct KParams {
     uint32_t nplans;
     uint64_t task_begin, task_end, stride;  // tasks are CSR positions task_begin + i*stride < task_end
     uint32_t chunk;
+    uint32_t head;    // the heaviest `head` subtasks of each plan are one queue item each
     uint64_t nitems;
     unsigned long long *queue;
     uint32_t *scratch;
@@ -134,8 +135,10 @@ struct KParams {
 constexpr int ITEM_WORDS = 4 + MAXK;
 
 // list descriptor: [ptr + b, ptr + e)
+typedef const __attribute__((address_space(1))) uint32_t guint;
+__device__ __forceinline__ const guint *gl(const uint32_t *p) { return (const guint *)p; }
 struct Src {
-    const uint32_t *ptr;
+    const guint *ptr;
     uint32_t b, e;
 };
 
@@ -172,7 +175,8 @@ __device__ __forceinline__ void wave_sync() {
 }
 
 // first index in [lo, hi) with a[i] >= key
-__device__ __forceinline__ uint32_t lower_bound(const uint32_t *a, uint32_t lo, uint32_t hi, uint32_t key) {
+template <typename PT>
+__device__ __forceinline__ uint32_t lower_bound(PT a, uint32_t lo, uint32_t hi, uint32_t key) {
     while (lo < hi) {
         uint32_t mid = lo + ((hi - lo) >> 1);
         if (a[mid] < key) lo = mid + 1; else hi = mid;
@@ -180,7 +184,8 @@ __device__ __forceinline__ uint32_t lower_bound(const uint32_t *a, uint32_t lo, 
     return lo;
 }
 // first index in [lo, hi) with a[i] > key
-__device__ __forceinline__ uint32_t upper_bound(const uint32_t *a, uint32_t lo, uint32_t hi, uint32_t key) {
+template <typename PT>
+__device__ __forceinline__ uint32_t upper_bound(PT a, uint32_t lo, uint32_t hi, uint32_t key) {
     while (lo < hi) {
         uint32_t mid = lo + ((hi - lo) >> 1);
         if (a[mid] <= key) lo = mid + 1; else hi = mid;
@@ -188,14 +193,16 @@ __device__ __forceinline__ uint32_t upper_bound(const uint32_t *a, uint32_t lo, 
     return lo;
 }
 
-__device__ __forceinline__ bool contains(const uint32_t *a, uint32_t lo, uint32_t hi, uint32_t key) {
+template <typename PT>
+__device__ __forceinline__ bool contains(PT a, uint32_t lo, uint32_t hi, uint32_t key) {
     uint32_t p = lower_bound(a, lo, hi, key);
     return p < hi && a[p] == key;
 }
 
 // Clamp [b,e) of a sorted list to the open interval (lo, hi) (setBeginPos /
 // setEndPos).  Every lane computes the same result.
-__device__ __forceinline__ void clamp_range(const uint32_t *a, uint32_t &b, uint32_t &e, bool has_lo, uint32_t lo,
+template <typename PT>
+__device__ __forceinline__ void clamp_range(PT a, uint32_t &b, uint32_t &e, bool has_lo, uint32_t lo,
                                             bool has_hi, uint32_t hi) {
     if (has_lo) b = upper_bound(a, b, e, lo);
     if (has_hi) e = lower_bound(a, b, e, hi);
@@ -258,7 +265,7 @@ __device__ __forceinline__ uint32_t wave_intersect(WaveLds &w, int ns, int nchk,
     }
     acc.bytes += bytes;
     acc.inter += (ns > 1) ? (ns - 1) : 0;
-    const uint32_t *D = w.srcs[drv].ptr;
+    const guint *D = w.srcs[drv].ptr;
     const uint32_t db = w.srcs[drv].b, de = w.srcs[drv].e;
     const int lane = lane_id();
     uint32_t total = 0;
@@ -268,7 +275,7 @@ __device__ __forceinline__ uint32_t wave_intersect(WaveLds &w, int ns, int nchk,
         uint32_t x = ok ? D[idx] : 0xffffffffu;
         for (int i = 0; i < ns; ++i) {
             if (i == drv) continue;
-            const uint32_t *a = w.srcs[i].ptr;
+            const guint *a = w.srcs[i].ptr;
             uint32_t e = w.srcs[i].e;
             uint32_t p = lower_bound(a, w.cur[i], e, x);
             ok = ok && p < e && a[p] == x;
@@ -309,7 +316,7 @@ __device__ int gather_sources(const KParams &P, WaveLds &w, int d, uint32_t pmas
     for (uint32_t m = pmask; m; m &= m - 1) {
         uint32_t v = w.phi[__builtin_ctz(m)];
         Src s;
-        s.ptr = P.nbr;
+        s.ptr = gl(P.nbr);
         s.b = P.off[v];
         s.e = P.off[v + 1];
         clamp_range(s.ptr, s.b, s.e, has_lo, lo, has_hi, hi);
@@ -324,7 +331,7 @@ __device__ int gather_sources(const KParams &P, WaveLds &w, int d, uint32_t pmas
 __device__ Src form_set(WaveLds &w, int ns, int nchk, bool may_alias, uint32_t *dst, Acc &acc) {
     if (ns <= 0) {
         Src z;
-        z.ptr = dst;
+        z.ptr = gl(dst);
         z.b = z.e = 0;
         return z;
     }
@@ -333,7 +340,7 @@ __device__ Src form_set(WaveLds &w, int ns, int nchk, bool may_alias, uint32_t *
         return w.srcs[0];
     }
     Src r;
-    r.ptr = dst;
+    r.ptr = gl(dst);
     r.b = 0;
     r.e = wave_intersect(w, ns, nchk, dst, acc);
     return r;
@@ -860,10 +867,19 @@ __global__ void __launch_bounds__(256) wcoj_kernel(KParams P) {
         const DPlan &pl = w.pl;
         if (pl.k < 2) continue;  // never launched by the host; no edge tasks exist for it
         const uint64_t nsub = pl.split ? pl.nsub : ntask;
-        if (ch * P.chunk >= nsub) continue;
-        // heavy-first: chunk 0 covers the highest CSR positions
-        uint64_t hi_u = nsub - ch * P.chunk;
-        uint64_t lo_u = hi_u > P.chunk ? hi_u - P.chunk : 0;
+        // heavy-first: item 0 covers the highest CSR positions; the first
+        // `head` items are single subtasks, the rest chunks of P.chunk
+        const uint64_t head = min((uint64_t)P.head, nsub);
+        uint64_t hi_u, lo_u;
+        if (ch < head) {
+            hi_u = nsub - ch;
+            lo_u = hi_u - 1;
+        } else {
+            const uint64_t c2 = ch - head;
+            if (head + c2 * P.chunk >= nsub) continue;
+            hi_u = nsub - head - c2 * P.chunk;
+            lo_u = hi_u > P.chunk ? hi_u - P.chunk : 0;
+        }
         for (uint64_t u = lo_u; u < hi_u; ++u) {
             uint64_t ti = u, ci = 0, nch = 1;
             if (pl.split) {
@@ -1113,6 +1129,7 @@ struct LaunchCfg {
     int grid = 0;
     size_t lds = 0;
     uint32_t chunk = 64;
+    uint32_t head = 0;
     uint32_t budget = 4096;
     uint32_t probe_ratio = 2;
     uint32_t bitmap_ratio = 2;
@@ -1129,7 +1146,8 @@ int launch_cfg(const sge_tuning *t, bool enumerate, LaunchCfg &cfg) {
         sge::set_error("tuning: block_threads must be 64, 128, 192 or 256");
         return SGE_EINVAL;
     }
-    cfg.chunk = t->chunk ? t->chunk : 64;
+    cfg.chunk = t->chunk ? t->chunk : 4;
+    cfg.head = t->head_items;
     cfg.budget = t->donate_budget ? (t->donate_budget >= (1u << 30) ? 0u : t->donate_budget) : 4096u;
     cfg.probe_ratio = t->stage_max ? t->stage_max : 2u;
     cfg.bitmap_ratio = t->bitmap_ratio ? t->bitmap_ratio : cfg.probe_ratio;
@@ -1145,7 +1163,16 @@ int launch_cfg(const sge_tuning *t, bool enumerate, LaunchCfg &cfg) {
         per_cu = std::max(1, occ);
     }
     cfg.grid = c.num_cu * per_cu;
+    if (cfg.head == 0) cfg.head = (uint32_t)cfg.grid * (uint32_t)(cfg.block / WAVE);  // one per resident wave
+    else if (cfg.head >= (1u << 30)) cfg.head = 0;
     return SGE_OK;
+}
+
+// Root-queue items of one plan with `nsub` subtasks: single-subtask head items,
+// then chunks.
+uint64_t items_per_plan(uint64_t nsub, const LaunchCfg &cfg) {
+    const uint64_t head = std::min<uint64_t>(cfg.head, nsub);
+    return head + (nsub - head + cfg.chunk - 1) / cfg.chunk;
 }
 
 int ensure_scratch(const sge_graph *g, const LaunchCfg &cfg, KParams &kp) {
@@ -1196,8 +1223,8 @@ int run_launch(sge_graph *g, const DPlan *dplans, uint32_t np, PlanOut *dout, ui
     kp.task_end = te;
     kp.stride = stride ? stride : 1;
     kp.chunk = cfg.chunk;
-    uint64_t chunks = (max_sub + cfg.chunk - 1) / cfg.chunk;
-    kp.nitems = chunks * np;
+    kp.head = cfg.head;
+    kp.nitems = items_per_plan(max_sub, cfg) * np;
     kp.queue = c.queue;
     kp.out = dout;
     kp.emb = emb;
@@ -1367,7 +1394,7 @@ int upload_plans(sge_graph *g, const sge_plan *plans, uint32_t n, bool enumerate
     return SGE_OK;
 }
 
-uint32_t span_of(const sge_tuning *t) { return (t && t->split_span) ? t->split_span : 64u; }
+uint32_t span_of(const sge_tuning *t) { return (t && t->split_span) ? t->split_span : 32u; }
 
 void fill_stats(const PlanOut &o, double ms, sge_stats *s) {
     if (!s) return;
@@ -1384,7 +1411,7 @@ extern "C" void sge_tuning_default(sge_tuning *t) {
     if (!t) return;
     std::memset(t, 0, sizeof(*t));
     t->block_threads = 256;
-    t->chunk = 32;
+    t->chunk = 4;
     t->stage_max = 0;
     t->blocks_per_cu = 0;
 }
@@ -1699,7 +1726,8 @@ extern "C" int sge_batch_create_sampled(sge_graph *g, const sge_plan *plans, uin
     kp.task_end = g->h.nnz;
     kp.stride = stride;
     kp.chunk = cfg.chunk;
-    kp.nitems = ((max_sub + cfg.chunk - 1) / cfg.chunk) * nplans;
+    kp.head = cfg.head;
+    kp.nitems = items_per_plan(max_sub, cfg) * nplans;
     kp.queue = b->queue;
     kp.scratch = b->scratch;
     kp.out = b->dout;

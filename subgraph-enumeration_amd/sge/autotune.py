@@ -19,8 +19,8 @@ from . import Batch, Graph, Plan, Tuning, default_tuning
 
 SPACE = {  # first value = the library default, the starting point
     "block_threads": [256, 128, 64],
-    "chunk": [32, 8, 128],
-    "split_span": [64, 16, 256],
+    "chunk": [4, 2, 8],
+    "split_span": [32, 16, 64],
     "donate_budget": [4096, 1024, 16384],
     "stage_max": [2, 1, 4],
     "bitmap_ratio": [2, 1, 4],

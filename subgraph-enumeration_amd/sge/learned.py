@@ -2,13 +2,14 @@
 
 A 64 -> 128 -> 128 -> 1 MLP (BF16 MFMA hidden layers, FP32 accumulation, Adam;
 all on the GPU through sge_mlp_*) predicts log2 of the plain GPU executor's
-measured device time (projected to the whole job) for a (pattern, order) from the order's features
+measured device time of a (pattern, order), relative to the pattern's other
+orders, from the order's features
 (sge_order_features: plan structure, degree moments, analytic cost, Knuth
 random-walk estimates over sampled root edges).  It only reorders: counts never
 change.  The analytic cost model (plan/optimizer.cpp:441-577 optCostOrder,
-restated in the planner) stays the fallback: when the network's choice and the
-cost model's choice disagree by more than `threshold` bits on the calibrated
-scale, the cost-model order is kept.
+restated in the planner) stays the fallback: when the network's choice costs
+more than `threshold` bits above the cost model's own choice by the cost
+model's estimate, the cost-model order is kept.
 
 Training data are (order, measured work) pairs logged by the plain executor on
 a strided root-edge sample (sge_count_sampled), for patterns disjoint from the
@@ -71,29 +72,32 @@ def log_work(graph: Graph, pattern: Pattern, orders: Sequence[Sequence[int]], mo
 
 
 class LearnedOrder:
+    """The network predicts, for each candidate order of one pattern, log2 of
+    its measured time relative to the mean over that pattern's logged orders
+    (only the ranking within a pattern matters for the choice; the absolute
+    scale across patterns is left out of the target)."""
+
     def __init__(self, mlp: Optional[MLP] = None, seed: int = 0):
         self.mlp = mlp or MLP(seed)
         self.mu = np.zeros(64, np.float32)
         self.sd = np.ones(64, np.float32)
-        self.t_mu = 0.0
         self.t_sd = 1.0
-        self.calib = (1.0, 0.0)  # log2 analytic cost -> log2 ms
-        self.threshold = 3.0
+        self.threshold = 3.0  # bits of analytic cost the learned order may exceed the cost model's by
         self.history: Dict[str, list] = {}
 
     # ---- data
     def _norm(self, X):
         return (X - self.mu) / self.sd
 
-    def fit(self, X: np.ndarray, y_log2: np.ndarray, cost_log2: np.ndarray, epochs: int = 300, lr: float = 2e-3,
+    def fit(self, X: np.ndarray, y_log2: np.ndarray, groups: np.ndarray, epochs: int = 300, lr: float = 2e-3,
             batch: int = 256, seed: int = 0):
+        """X: features, y_log2: log2 measured time, groups: pattern id per row."""
         self.mu = X.mean(0).astype(np.float32)
         self.sd = (X.std(0) + 1e-3).astype(np.float32)
-        self.t_mu, self.t_sd = float(y_log2.mean()), float(y_log2.std() + 1e-6)
-        a, b = np.polyfit(cost_log2, y_log2, 1)
-        self.calib = (float(a), float(b))
+        rel = relative_targets(y_log2, groups)
+        self.t_sd = float(rel.std() + 1e-6)
         Xn = self._norm(X).astype(np.float32)
-        tn = ((y_log2 - self.t_mu) / self.t_sd).astype(np.float32)
+        tn = (rel / self.t_sd).astype(np.float32)
         rng = np.random.default_rng(seed)
         losses = []
         n = len(Xn)
@@ -107,15 +111,15 @@ class LearnedOrder:
         self.history["loss"] = losses
         return losses
 
-    def predict_log2(self, X: np.ndarray) -> np.ndarray:
+    def predict_rel_log2(self, X: np.ndarray) -> np.ndarray:
+        """Predicted log2 time of each order relative to its pattern's mean."""
         y = self.mlp.forward(self._norm(np.asarray(X, np.float32)).astype(np.float32))
-        return y * self.t_sd + self.t_mu
+        return y * self.t_sd
 
     # ---- persistence
     def save(self, path: str):
         np.savez(path, params=self.mlp.params(), mu=self.mu, sd=self.sd,
-                 meta=np.array(json.dumps({"t_mu": self.t_mu, "t_sd": self.t_sd, "calib": self.calib,
-                                           "threshold": self.threshold})))
+                 meta=np.array(json.dumps({"t_sd": self.t_sd, "threshold": self.threshold})))
 
     @classmethod
     def load(cls, path: str) -> "LearnedOrder":
@@ -124,10 +128,18 @@ class LearnedOrder:
         m.mlp.set_params(d["params"])
         m.mu, m.sd = d["mu"], d["sd"]
         meta = json.loads(str(d["meta"]))
-        m.t_mu, m.t_sd = meta["t_mu"], meta["t_sd"]
-        m.calib = tuple(meta["calib"])
+        m.t_sd = meta["t_sd"]
         m.threshold = meta["threshold"]
         return m
+
+
+def relative_targets(y_log2: np.ndarray, groups: np.ndarray) -> np.ndarray:
+    """y minus the mean of y over its group."""
+    y = np.asarray(y_log2, np.float64)
+    g = np.asarray(groups)
+    _, inv = np.unique(g, return_inverse=True)
+    mean = np.bincount(inv, weights=y) / np.bincount(inv)
+    return y - mean[inv]
 
 
 def load(path: str) -> LearnedOrder:
@@ -135,22 +147,24 @@ def load(path: str) -> LearnedOrder:
 
 
 def choose_plan(model: LearnedOrder, pattern: Pattern, graph: Graph, mode: int, fallback: Optional[Plan] = None,
-                top: int = 16):
-    """Order with the lowest predicted work among the cost model's `top`
-    candidates; the cost-model plan is kept when the two disagree by more than
-    the model's threshold (bits, calibrated scale)."""
+                top: int = 12):
+    """Order with the lowest predicted time among the cost model's `top`
+    candidates.  The cost model stays the fallback: its plan is kept when the
+    learned order's analytic cost exceeds the cost-model order's by more than
+    the model's threshold (bits), i.e. when the two disagree that much."""
     if fallback is None:
         fallback = build_plan(pattern, graph, mode)
     cands = candidate_orders(pattern, graph, mode, top)
     X = np.stack([order_features(pattern, graph, o, mode, FEATURE_SAMPLES, 11) for o in cands])
-    pred = model.predict_log2(X)
+    pred = model.predict_rel_log2(X)
     best = int(np.argmin(pred))
     cost_order = list(fallback.order)[: pattern.k]
-    a, b = model.calib
-    cost_pred = a * np.log2(max(fallback.cost, 1.0)) + b
-    info = {"pred_log2": float(pred[best]), "cost_model_log2": float(cost_pred),
-            "cost_order": cost_order, "learned_order": list(cands[best]), "fallback": False}
-    if abs(float(pred[best]) - float(cost_pred)) > model.threshold and cands[best] != cost_order:
+    ci = cands.index(cost_order) if cost_order in cands else None
+    learned_cost = order_cost(pattern, graph, cands[best], mode)
+    info = {"pred_rel_log2": float(pred[best] - (pred[ci] if ci is not None else 0.0)),
+            "cost_order": cost_order, "learned_order": list(cands[best]), "fallback": False,
+            "cost_bits_over": float(np.log2(max(learned_cost, 1.0)) - np.log2(max(fallback.cost, 1.0)))}
+    if info["cost_bits_over"] > model.threshold:
         info["fallback"] = True
         return fallback, info
     return build_plan(pattern, graph, mode, cands[best]), info
@@ -161,8 +175,8 @@ def judge(model: "LearnedOrder", graph: Graph, patterns: Sequence[Pattern], mode
     """Learned choice against the cost model's choice, per pattern: the same
     executor on the same root sample, best of `reps` device times.  The learned
     plan runs under a work cap of 4x the cost-model plan's work (a runaway
-    choice is reported as capped, never run to the end); uncapped, both counts
-    must agree."""
+    choice is reported as capped, never run to the end); uncapped whole-job
+    counts (stride 1) must agree."""
     rows = []
     for i, p in enumerate(patterns):
         cost_plan = build_plan(p, graph, mode)
@@ -175,12 +189,16 @@ def judge(model: "LearnedOrder", graph: Graph, patterns: Sequence[Pattern], mode
             (sl, lim), = time_capped(graph, plan, stride, cap_k) if not info["fallback"] else [(st[0], False)]
             capped = capped or lim
             t_learn.append(sl.kernel_ms)
-        if not capped and not info["fallback"]:
+        if stride == 1 and not capped and not info["fallback"]:
+            # whole-job counts do not depend on the order (a root-edge sample does:
+            # another order roots the search at another pattern edge)
             cl, _ = count_sampled(graph, [plan], stride)
             if cl[0] != cnt[0]:
                 raise AssertionError(f"pattern {i}: learned plan counted {cl[0]}, cost-model plan {cnt[0]}")
         row = {"pattern": i, "cost_ms": float(min(t_cost)), "learned_ms": float(min(t_learn)), "capped": capped,
-               "pred_log2_ms": info["pred_log2"], "fallback": info["fallback"], "cost_order": info["cost_order"],
+               "pred_rel_log2": info["pred_rel_log2"],
+               "measured_rel_log2": float(np.log2(max(min(t_learn), 1e-3) / max(min(t_cost), 1e-3))),
+               "fallback": info["fallback"], "cost_order": info["cost_order"],
                "learned_order": info["learned_order"]}
         rows.append(row)
         log(f"[judge] pattern {i}: cost-model {row['cost_ms']:.2f} ms, learned {row['learned_ms']:.2f} ms"
@@ -190,12 +208,12 @@ def judge(model: "LearnedOrder", graph: Graph, patterns: Sequence[Pattern], mode
 
 def log_pairs(graph: Graph, patterns: Sequence[Pattern], mode: int, stride: int = 1024, orders_per_pattern: int = 8,
               seed: int = 0, log=print):
-    """(features, log2 projected full-job ms, log2 analytic cost, capped) of
+    """(features, log2 projected full-job ms, log2 analytic cost, capped, pattern id) of
     `orders_per_pattern` orders per pattern, measured with log_work.  An order
     stopped by its work cap is logged CAPPED_PENALTY bits above its
     (lower-bound) time; a pattern whose pilot order is stopped is skipped."""
     rng = np.random.default_rng(seed)
-    X, y, c, cap = [], [], [], []
+    X, y, c, cap, grp = [], [], [], [], []
     t0 = time.time()
     for i, p in enumerate(patterns):
         orders = candidate_orders(p, graph, mode, orders_per_pattern // 2, rng)[:orders_per_pattern]
@@ -209,21 +227,24 @@ def log_pairs(graph: Graph, patterns: Sequence[Pattern], mode: int, stride: int 
             y.append(np.log2(max(t, 1e-3)) + (CAPPED_PENALTY if cp else 0.0))
             c.append(np.log2(max(order_cost(p, graph, o, mode), 1.0)))
             cap.append(bool(cp))
+            grp.append(seed + i)
         log(f"[learned] logged {i + 1}/{len(patterns)} patterns, {len(X)} orders, stride {used}, "
             f"projected ms {float(np.min(ms)):.0f}..{float(np.max(ms)):.0f}, capped {int(np.sum(capped))}, "
             f"{time.time() - t0:.0f}s")
-    return (np.array(X, np.float32), np.array(y, np.float64), np.array(c, np.float64), np.array(cap, bool))
+    return (np.array(X, np.float32), np.array(y, np.float64), np.array(c, np.float64), np.array(cap, bool),
+            np.array(grp, np.int64))
 
 
-def fit_model(X: np.ndarray, y: np.ndarray, c: np.ndarray, epochs: int = 300, seed: int = 0, log=print):
+def fit_model(X: np.ndarray, y: np.ndarray, groups: np.ndarray, epochs: int = 300, seed: int = 0, log=print):
     model = LearnedOrder(seed=seed)
-    losses = model.fit(X, y, c, epochs=epochs, seed=seed)
-    log(f"[learned] trained on {len(X)} orders, loss {losses[0]:.3f} -> {losses[-1]:.3f}")
+    losses = model.fit(X, y, groups, epochs=epochs, seed=seed)
+    log(f"[learned] trained on {len(X)} orders of {len(np.unique(groups))} patterns, "
+        f"loss {losses[0]:.3f} -> {losses[-1]:.3f}")
     return model
 
 
 def train(graph: Graph, patterns: Sequence[Pattern], mode: int, stride: int = 1024, orders_per_pattern: int = 8,
           epochs: int = 300, seed: int = 0, log=print):
     """Log (order, measured work) pairs with the plain executor and fit the MLP."""
-    X, y, c, cap = log_pairs(graph, patterns, mode, stride, orders_per_pattern, seed, log)
-    return fit_model(X, y, c, epochs, seed, log), (X, y, c)
+    X, y, c, cap, grp = log_pairs(graph, patterns, mode, stride, orders_per_pattern, seed, log)
+    return fit_model(X, y, grp, epochs, seed, log), (X, y, c, grp)

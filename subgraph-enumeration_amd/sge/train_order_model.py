@@ -17,6 +17,8 @@ Logging can be split over several runs and fitted afterwards:
     python -m sge.train_order_model --log-only pairs_0.npz --first 0 --count 200
     python -m sge.train_order_model --log-only pairs_1.npz --first 200 --count 256
     python -m sge.train_order_model --data pairs_0.npz,pairs_1.npz --out sge/models/order_mlp.npz
+
+The pairs logged for the committed model are in sge/models/order_pairs.npz.
 """
 import argparse
 import json
@@ -70,20 +72,22 @@ def main():
         X = np.concatenate([d["X"] for d in parts])
         y = np.concatenate([d["y"] for d in parts])
         c = np.concatenate([d["c"] for d in parts])
-        log(f"loaded {len(X)} logged orders from {args.data}")
+        grp = np.concatenate([d["group"] + 1_000_000 * k for k, d in enumerate(parts)])
+        log(f"loaded {len(X)} logged orders of {len(np.unique(grp))} patterns from {args.data}")
     else:
         sel = train_p[args.first:args.first + args.count if args.count else None]
-        X, y, c, cap = L.log_pairs(g, sel, mode, stride=args.stride, orders_per_pattern=args.orders,
-                                   seed=args.first, log=log)
+        X, y, c, cap, grp = L.log_pairs(g, sel, mode, stride=args.stride, orders_per_pattern=args.orders,
+                                        seed=args.first, log=log)
         if args.log_only:
-            np.savez(args.log_only, X=X, y=y, c=c, capped=cap)
+            np.savez(args.log_only, X=X, y=y, c=c, capped=cap, group=grp)
             log(f"wrote {len(X)} orders to {args.log_only}")
             return
-    model = L.fit_model(X, y, c, epochs=args.epochs, log=log)
-    pred = model.predict_log2(X)
-    fit = {"train_rmse_log2": float(np.sqrt(np.mean((pred - y) ** 2))),
-           "train_corr": float(np.corrcoef(pred, y)[0, 1]),
-           "cost_model_corr": float(np.corrcoef(c, y)[0, 1])}
+    model = L.fit_model(X, y, grp, epochs=args.epochs, log=log)
+    rel = L.relative_targets(y, grp)
+    pred = model.predict_rel_log2(X)
+    fit = {"train_rmse_rel_log2": float(np.sqrt(np.mean((pred - rel) ** 2))),
+           "train_corr_rel": float(np.corrcoef(pred, rel)[0, 1]),
+           "cost_model_corr_rel": float(np.corrcoef(L.relative_targets(c, grp), rel)[0, 1])}
     log("fit", fit)
     # judge on the benchmark set
     rows = L.judge(model, g, W.six_vertex_set(), mode, args.eval_stride, log=log)

@@ -137,11 +137,12 @@ def test_tuning_variants_same_counts():
     g = W.graph_c1()
     plans = [sge.build_plan(p, g) for p in W.six_vertex_set()[:4]]
     want, _ = sge.count(g, plans)
-    for bt, ch in [(64, 1), (128, 7), (192, 64), (256, 1000)]:
+    for bt, ch, head in [(64, 1, 0), (128, 7, 1 << 30), (192, 64, 3), (256, 1000, 0), (256, 5, 100000)]:
         t = sge.default_tuning()
-        t.block_threads, t.chunk = bt, ch
-        say("tuning", bt, ch)
+        t.block_threads, t.chunk, t.head_items = bt, ch, head
+        say("tuning", bt, ch, head)
         assert sge.count(g, plans, batched=True, tuning=t)[0] == want
+        assert sge.count(g, plans, batched=False, tuning=t)[0] == want
 
 
 def test_probe_direction_ratios_same_counts():
