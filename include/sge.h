@@ -195,6 +195,8 @@ typedef struct sge_stats {
     uint64_t nodes;           /* materialised search-tree nodes */
     double kernel_ms;         /* device time of the counting launch(es) */
     double wall_ms;           /* host wall time of the call */
+    uint64_t wave_cycles;     /* wavefront clock cycles spent on the plan's work (total work,
+                                 independent of how well the launch was balanced) */
 } sge_stats;
 
 void sge_tuning_default(sge_tuning *t);

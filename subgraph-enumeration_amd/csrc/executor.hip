@@ -92,7 +92,8 @@ struct DPlan {
 
 struct PlanOut {
     unsigned long long count, inter, bytes, nodes, tasks;
-    unsigned long long pad[3];
+    unsigned long long cycles;  // wavefront clock cycles spent on the plan's work items
+    unsigned long long pad[2];
 };
 
 struct KParams {
@@ -221,7 +222,7 @@ __device__ __forceinline__ uint64_t choosec(uint64_t n, uint32_t k) {
 }
 
 struct Acc {
-    unsigned long long count, inter, bytes, nodes, tasks;
+    unsigned long long count, inter, bytes, nodes, tasks, cycles;
 };
 
 // Bounds from the images of the levels in the masks.
@@ -652,8 +653,9 @@ __device__ void flush(const KParams &P, uint32_t plan, Acc &acc) {
         if (acc.bytes) atomicAdd(&o->bytes, acc.bytes);
         if (acc.nodes) atomicAdd(&o->nodes, acc.nodes);
         if (acc.tasks) atomicAdd(&o->tasks, acc.tasks);
+        if (acc.cycles) atomicAdd(&o->cycles, acc.cycles);
     }
-    acc = Acc{0, 0, 0, 0, 0};
+    acc = Acc{0, 0, 0, 0, 0, 0};
 }
 
 // Enumeration: form the last level and write one row per embedding, images
@@ -815,12 +817,19 @@ __global__ void __launch_bounds__(256) wcoj_kernel(KParams P) {
     TermLds *terms = reinterpret_cast<TermLds *>(lds_raw + sizeof(WaveLds) * nwaves) + wid * MAX_TERMS;
     const uint64_t gwave = (uint64_t)blockIdx.x * (blockDim.x / WAVE) + wid;
     uint32_t *wb = P.scratch + gwave * P.wave_stride;
-    Acc acc{0, 0, 0, 0, 0};
+    Acc acc{0, 0, 0, 0, 0, 0};
     uint32_t cur_plan = NONE;
     const uint64_t ntask = (P.task_end - P.task_begin + P.stride - 1) / P.stride;
     const bool cont = P.in_items != nullptr;
     const uint64_t nitems = cont ? min((uint64_t)*P.in_count, P.item_cap) : P.nitems;
+    unsigned long long t_item = __builtin_amdgcn_s_memtime();  // start of the current item
     while (true) {
+        {
+            // charge the time since the last item started to that item's plan
+            const unsigned long long now = __builtin_amdgcn_s_memtime();
+            if (cur_plan != NONE) acc.cycles += now - t_item;
+            t_item = now;
+        }
         unsigned long long item = 0;
         if (lane == 0) item = atomicAdd(P.queue, 1ull);
         item = __shfl(item, 0);
@@ -927,6 +936,7 @@ __global__ void __launch_bounds__(256) wcoj_kernel(KParams P) {
             run_dfs<ENUM>(P, w, terms, wb, acc, plan, 2, stop);
         }
     }
+    if (cur_plan != NONE) acc.cycles += __builtin_amdgcn_s_memtime() - t_item;
     if (cur_plan != NONE) flush(P, cur_plan, acc);
 }
 
@@ -1197,7 +1207,7 @@ __global__ void reset_pass_kernel(unsigned long long *queue, unsigned long long 
 
 __global__ void reset_kernel(PlanOut *out, uint32_t n, unsigned long long *queue, unsigned long long *emb_n) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) out[i] = PlanOut{0, 0, 0, 0, 0, {0, 0, 0}};
+    if (i < n) out[i] = PlanOut{0, 0, 0, 0, 0, 0, {0, 0}};
     if (i == 0) {
         queue[0] = 0;
         queue[2] = 0;  // node-cap counter
@@ -1402,6 +1412,7 @@ void fill_stats(const PlanOut &o, double ms, sge_stats *s) {
     s->bytes += o.bytes;
     s->tasks += o.tasks;
     s->nodes += o.nodes;
+    s->wave_cycles += o.cycles;
     s->kernel_ms += ms;
 }
 

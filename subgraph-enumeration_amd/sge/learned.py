@@ -1,9 +1,9 @@
 """Learned attribute-order cost model.
 
 A 64 -> 128 -> 128 -> 1 MLP (BF16 MFMA hidden layers, FP32 accumulation, Adam;
-all on the GPU through sge_mlp_*) predicts log2 of the plain GPU executor's
-measured device time of a (pattern, order), relative to the pattern's other
-orders, from the order's features
+all on the GPU through sge_mlp_*) predicts log2 of the work the plain GPU
+executor measures for a (pattern, order) -- wavefront clock cycles, summed over
+the wavefronts that ran it -- relative to the pattern's other orders, from the order's features
 (sge_order_features: plan structure, degree moments, analytic cost, Knuth
 random-walk estimates over sampled root edges).  It only reorders: counts never
 change.  The analytic cost model (plan/optimizer.cpp:441-577 optCostOrder,
@@ -51,9 +51,10 @@ def log_work(graph: Graph, pattern: Pattern, orders: Sequence[Sequence[int]], mo
     the absolute cap `pilot_cap_k` (x1024 wave steps) -- None is returned when
     even it is stopped -- and when it takes under `min_ms` the sample is refined
     to `fine_stride`.  The other orders run under a cap of `cap_factor` times the
-    pilot's work.  Returns (projected full-job ms = sample ms x stride,
-    algorithmic bytes, capped flags, stride); a capped order's time is a lower
-    bound."""
+    pilot's work.  Returns (projected full-job work = wavefront clock cycles
+    spent on the sample x stride, algorithmic bytes, capped flags, stride); a
+    capped order's work is a lower bound.  Work, unlike device time, does not
+    depend on how evenly a small sample spreads over the GPU."""
     plans = [build_plan(pattern, graph, mode, o) for o in orders]
     (pilot, lim), = time_capped(graph, plans[0], stride, pilot_cap_k)
     if lim:
@@ -65,7 +66,7 @@ def log_work(graph: Graph, pattern: Pattern, orders: Sequence[Sequence[int]], mo
             return None
     cap_k = max(64, cap_factor * work_steps_bound(pilot) // 1024 + 1)
     res = [(pilot, False)] + time_capped(graph, plans[1:], stride, cap_k)
-    ms = np.array([st.kernel_ms * stride for st, _ in res])
+    ms = np.array([float(st.wave_cycles) * stride for st, _ in res])
     by = np.array([st.bytes for st, _ in res], dtype=np.float64)
     capped = np.array([lim for _, lim in res], dtype=bool)
     return ms, by, capped, stride
@@ -73,7 +74,7 @@ def log_work(graph: Graph, pattern: Pattern, orders: Sequence[Sequence[int]], mo
 
 class LearnedOrder:
     """The network predicts, for each candidate order of one pattern, log2 of
-    its measured time relative to the mean over that pattern's logged orders
+    its measured work relative to the mean over that pattern's logged orders
     (only the ranking within a pattern matters for the choice; the absolute
     scale across patterns is left out of the target)."""
 
@@ -83,6 +84,7 @@ class LearnedOrder:
         self.sd = np.ones(64, np.float32)
         self.t_sd = 1.0
         self.threshold = 3.0  # bits of analytic cost the learned order may exceed the cost model's by
+        self.margin = 0.25    # predicted bits of improvement needed to leave the cost model's order
         self.history: Dict[str, list] = {}
 
     # ---- data
@@ -90,8 +92,10 @@ class LearnedOrder:
         return (X - self.mu) / self.sd
 
     def fit(self, X: np.ndarray, y_log2: np.ndarray, groups: np.ndarray, epochs: int = 300, lr: float = 2e-3,
-            batch: int = 256, seed: int = 0):
-        """X: features, y_log2: log2 measured time, groups: pattern id per row."""
+            batch: int = 256, seed: int = 0, val_frac: float = 0.15, check_every: int = 5):
+        """X: features, y_log2: log2 measured work, groups: pattern id per row.
+        A `val_frac` share of the patterns is held out; the weights with the
+        lowest held-out loss are kept (early stopping)."""
         self.mu = X.mean(0).astype(np.float32)
         self.sd = (X.std(0) + 1e-3).astype(np.float32)
         rel = relative_targets(y_log2, groups)
@@ -99,27 +103,39 @@ class LearnedOrder:
         Xn = self._norm(X).astype(np.float32)
         tn = (rel / self.t_sd).astype(np.float32)
         rng = np.random.default_rng(seed)
-        losses = []
-        n = len(Xn)
+        ug = np.unique(groups)
+        val_g = rng.choice(ug, size=max(1, int(len(ug) * val_frac)), replace=False) if val_frac > 0 else []
+        is_val = np.isin(groups, val_g)
+        tr = np.nonzero(~is_val)[0]
+        va = np.nonzero(is_val)[0]
+        losses, vlosses = [], []
+        best = (float("inf"), self.mlp.params(), 0)
         for ep in range(epochs):
-            perm = rng.permutation(n)
+            perm = rng.permutation(tr)
             el = []
-            for i in range(0, n, batch):
+            for i in range(0, len(perm), batch):
                 j = perm[i:i + batch]
                 el.append(self.mlp.train_step(Xn[j], tn[j], lr))
             losses.append(float(np.mean(el)))
-        self.history["loss"] = losses
+            if len(va) and (ep + 1) % check_every == 0:
+                v = float(np.mean((self.mlp.forward(Xn[va]) - tn[va]) ** 2))
+                vlosses.append(v)
+                if v < best[0]:
+                    best = (v, self.mlp.params(), ep + 1)
+        if len(va):
+            self.mlp.set_params(best[1])
+        self.history.update({"loss": losses, "val_loss": vlosses, "best_epoch": best[2]})
         return losses
 
     def predict_rel_log2(self, X: np.ndarray) -> np.ndarray:
-        """Predicted log2 time of each order relative to its pattern's mean."""
+        """Predicted log2 work of each order relative to its pattern's mean."""
         y = self.mlp.forward(self._norm(np.asarray(X, np.float32)).astype(np.float32))
         return y * self.t_sd
 
     # ---- persistence
     def save(self, path: str):
         np.savez(path, params=self.mlp.params(), mu=self.mu, sd=self.sd,
-                 meta=np.array(json.dumps({"t_sd": self.t_sd, "threshold": self.threshold})))
+                 meta=np.array(json.dumps({"t_sd": self.t_sd, "threshold": self.threshold, "margin": self.margin})))
 
     @classmethod
     def load(cls, path: str) -> "LearnedOrder":
@@ -130,6 +146,7 @@ class LearnedOrder:
         meta = json.loads(str(d["meta"]))
         m.t_sd = meta["t_sd"]
         m.threshold = meta["threshold"]
+        m.margin = meta.get("margin", m.margin)
         return m
 
 
@@ -148,10 +165,12 @@ def load(path: str) -> LearnedOrder:
 
 def choose_plan(model: LearnedOrder, pattern: Pattern, graph: Graph, mode: int, fallback: Optional[Plan] = None,
                 top: int = 12):
-    """Order with the lowest predicted time among the cost model's `top`
-    candidates.  The cost model stays the fallback: its plan is kept when the
-    learned order's analytic cost exceeds the cost-model order's by more than
-    the model's threshold (bits), i.e. when the two disagree that much."""
+    """Order with the lowest predicted work among the cost model's `top`
+    candidates, taken only when the network predicts it at least `margin` bits
+    below the cost model's own order.  The cost model stays the fallback: its
+    plan is kept when the learned order's analytic cost exceeds the cost-model
+    order's by more than the model's threshold (bits), i.e. when the two
+    disagree that much."""
     if fallback is None:
         fallback = build_plan(pattern, graph, mode)
     cands = candidate_orders(pattern, graph, mode, top)
@@ -164,6 +183,9 @@ def choose_plan(model: LearnedOrder, pattern: Pattern, graph: Graph, mode: int, 
     info = {"pred_rel_log2": float(pred[best] - (pred[ci] if ci is not None else 0.0)),
             "cost_order": cost_order, "learned_order": list(cands[best]), "fallback": False,
             "cost_bits_over": float(np.log2(max(learned_cost, 1.0)) - np.log2(max(fallback.cost, 1.0)))}
+    if ci is not None and pred[ci] - pred[best] < model.margin:
+        info["learned_order"] = cost_order  # not a clear enough improvement
+        return fallback, info
     if info["cost_bits_over"] > model.threshold:
         info["fallback"] = True
         return fallback, info
@@ -208,7 +230,7 @@ def judge(model: "LearnedOrder", graph: Graph, patterns: Sequence[Pattern], mode
 
 def log_pairs(graph: Graph, patterns: Sequence[Pattern], mode: int, stride: int = 1024, orders_per_pattern: int = 8,
               seed: int = 0, log=print):
-    """(features, log2 projected full-job ms, log2 analytic cost, capped, pattern id) of
+    """(features, log2 projected full-job work, log2 analytic cost, capped, pattern id) of
     `orders_per_pattern` orders per pattern, measured with log_work.  An order
     stopped by its work cap is logged CAPPED_PENALTY bits above its
     (lower-bound) time; a pattern whose pilot order is stopped is skipped."""
@@ -224,12 +246,12 @@ def log_pairs(graph: Graph, patterns: Sequence[Pattern], mode: int, stride: int 
         ms, by, capped, used = r
         for o, t, cp in zip(orders, ms, capped):
             X.append(order_features(p, graph, o, mode, FEATURE_SAMPLES, 11))
-            y.append(np.log2(max(t, 1e-3)) + (CAPPED_PENALTY if cp else 0.0))
+            y.append(np.log2(max(t, 1.0)) + (CAPPED_PENALTY if cp else 0.0))
             c.append(np.log2(max(order_cost(p, graph, o, mode), 1.0)))
             cap.append(bool(cp))
             grp.append(seed + i)
         log(f"[learned] logged {i + 1}/{len(patterns)} patterns, {len(X)} orders, stride {used}, "
-            f"projected ms {float(np.min(ms)):.0f}..{float(np.max(ms)):.0f}, capped {int(np.sum(capped))}, "
+            f"projected Gcycles {float(np.min(ms)) / 1e9:.1f}..{float(np.max(ms)) / 1e9:.1f}, capped {int(np.sum(capped))}, "
             f"{time.time() - t0:.0f}s")
     return (np.array(X, np.float32), np.array(y, np.float64), np.array(c, np.float64), np.array(cap, bool),
             np.array(grp, np.int64))
@@ -239,7 +261,8 @@ def fit_model(X: np.ndarray, y: np.ndarray, groups: np.ndarray, epochs: int = 30
     model = LearnedOrder(seed=seed)
     losses = model.fit(X, y, groups, epochs=epochs, seed=seed)
     log(f"[learned] trained on {len(X)} orders of {len(np.unique(groups))} patterns, "
-        f"loss {losses[0]:.3f} -> {losses[-1]:.3f}")
+        f"loss {losses[0]:.3f} -> {losses[-1]:.3f}, kept epoch {model.history['best_epoch']} "
+        f"(held-out loss {min(model.history['val_loss'] or [float('nan')]):.3f})")
     return model
 
 

@@ -167,6 +167,7 @@ def test_node_cap_stops_and_reports():
     g = W.graph_c2()
     plan = sge.build_plan(W.six_vertex_set()[1], g, sge.MODE_WCOJ)
     want, st = sge.count_sampled(g, [plan], 16)
+    assert st[0].wave_cycles > 0
     say("node cap 1k")
     (s1, lim1), = sge.time_capped(g, plan, 16, 1)
     assert lim1 and 0 < s1.nodes < st[0].nodes
