@@ -960,7 +960,7 @@ struct DeviceCtx {
     DPlan *plans = nullptr;
     size_t plans_cap = 0;
     uint32_t *items[2] = {nullptr, nullptr};  // continuation items (ping-pong)
-    unsigned long long *icount = nullptr;     // [2] item counts
+    unsigned long long *icount = nullptr;     // [2] item counts, [2] work-cap counter
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
 
@@ -994,7 +994,7 @@ int ensure_ctx() {
     HIP_TRY(hipEventCreate(&c.ev1));
     HIP_TRY(hipMalloc(&c.items[0], sizeof(uint32_t) * ITEM_WORDS * ITEM_CAP));
     HIP_TRY(hipMalloc(&c.items[1], sizeof(uint32_t) * ITEM_WORDS * ITEM_CAP));
-    HIP_TRY(hipMalloc(&c.icount, sizeof(unsigned long long) * 2));
+    HIP_TRY(hipMalloc(&c.icount, sizeof(unsigned long long) * 4));
     c.init = true;
     return SGE_OK;
 }
@@ -1205,13 +1205,14 @@ __global__ void reset_pass_kernel(unsigned long long *queue, unsigned long long 
     }
 }
 
-__global__ void reset_kernel(PlanOut *out, uint32_t n, unsigned long long *queue, unsigned long long *emb_n) {
+__global__ void reset_kernel(PlanOut *out, uint32_t n, unsigned long long *queue, unsigned long long *emb_n,
+                             unsigned long long *cap_ctr) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) out[i] = PlanOut{0, 0, 0, 0, 0, 0, {0, 0}};
     if (i == 0) {
         queue[0] = 0;
-        queue[2] = 0;  // node-cap counter
         if (emb_n) *emb_n = 0;
+        if (cap_ctr) *cap_ctr = 0;
     }
 }
 
@@ -1247,8 +1248,8 @@ int run_launch(sge_graph *g, const DPlan *dplans, uint32_t np, PlanOut *dout, ui
     kp.probe_ratio = cfg.probe_ratio;
     kp.bitmap_ratio = cfg.bitmap_ratio;
     kp.node_cap = cfg.node_cap;
-    kp.node_ctr = c.queue + 2;
-    reset_kernel<<<(np + 255) / 256, 256, 0, c.stream>>>(dout, np, c.queue, emb_n);
+    kp.node_ctr = c.icount + 2;  // read back with the item counts, in one copy per pass
+    reset_kernel<<<(np + 255) / 256, 256, 0, c.stream>>>(dout, np, c.queue, emb_n, kp.node_ctr);
     reset_pass_kernel<<<1, 64, 0, c.stream>>>(c.queue, c.icount);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(c.ev0, c.stream));
@@ -1258,15 +1259,11 @@ int run_launch(sge_graph *g, const DPlan *dplans, uint32_t np, PlanOut *dout, ui
             kp.in_items = nullptr;
             kp.in_count = nullptr;
         } else {
-            unsigned long long n = 0;
-            HIP_TRY(hipMemcpyAsync(&n, c.icount + cur, sizeof(n), hipMemcpyDeviceToHost, c.stream));
+            unsigned long long hc[3] = {0, 0, 0};  // item counts, work spent
+            HIP_TRY(hipMemcpyAsync(hc, c.icount, sizeof(hc), hipMemcpyDeviceToHost, c.stream));
             HIP_TRY(hipStreamSynchronize(c.stream));
-            if (n == 0) break;
-            if (kp.node_cap) {
-                unsigned long long spent = 0;
-                HIP_TRY(hipMemcpy(&spent, kp.node_ctr, sizeof(spent), hipMemcpyDeviceToHost));
-                if (spent >= kp.node_cap) break;
-            }
+            if (hc[cur] == 0) break;
+            if (kp.node_cap && hc[2] >= kp.node_cap) break;
             kp.in_items = c.items[cur];
             kp.in_count = c.icount + cur;
             cur ^= 1;
@@ -1288,9 +1285,10 @@ int run_launch(sge_graph *g, const DPlan *dplans, uint32_t np, PlanOut *dout, ui
     if (ms) *ms = t;
     if (kp.node_cap) {
         unsigned long long spent = 0;
-        HIP_TRY(hipMemcpy(&spent, kp.node_ctr, sizeof(spent), hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpyAsync(&spent, kp.node_ctr, sizeof(spent), hipMemcpyDeviceToHost, c.stream));
+        HIP_TRY(hipStreamSynchronize(c.stream));
         if (spent >= kp.node_cap) {
-            sge::set_error("node cap reached (" + std::to_string(spent) + " search-tree nodes): counts are partial");
+            sge::set_error("work cap reached (" + std::to_string(spent) + " wave steps): counts are partial");
             return SGE_ELIMIT;
         }
     }
@@ -1753,7 +1751,7 @@ extern "C" int sge_batch_create_sampled(sge_graph *g, const sge_plan *plans, uin
     // capture: resets (fill kernels, no memset nodes), the root pass and a fixed
     // number of continuation passes that read their item counts on the device
     if ((e = hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal)) != hipSuccess) return fail(e, "capture");
-    reset_kernel<<<(nplans + 255) / 256, 256, 0, b->stream>>>(b->dout, nplans, b->queue, nullptr);
+    reset_kernel<<<(nplans + 255) / 256, 256, 0, b->stream>>>(b->dout, nplans, b->queue, nullptr, nullptr);
     reset_pass_kernel<<<1, 64, 0, b->stream>>>(b->queue, b->icount);
     if (kp.nitems) {
         wcoj_kernel<false><<<cfg.grid, cfg.block, cfg.lds, b->stream>>>(kp);

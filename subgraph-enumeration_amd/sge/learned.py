@@ -204,14 +204,18 @@ def judge(model: "LearnedOrder", graph: Graph, patterns: Sequence[Pattern], mode
         cost_plan = build_plan(p, graph, mode)
         plan, info = choose_plan(model, p, graph, mode, fallback=cost_plan)
         t_cost, t_learn, capped = [], [], False
+        same = list(plan.order)[: p.k] == list(cost_plan.order)[: p.k]
         for _ in range(reps):
             cnt, st = count_sampled(graph, [cost_plan], stride)
             t_cost.append(st[0].kernel_ms)
+            if same:  # the model kept the cost model's order
+                t_learn.append(st[0].kernel_ms)
+                continue
             cap_k = max(1024, 4 * work_steps_bound(st[0]) // 1024 + 1)
             (sl, lim), = time_capped(graph, plan, stride, cap_k) if not info["fallback"] else [(st[0], False)]
             capped = capped or lim
             t_learn.append(sl.kernel_ms)
-        if stride == 1 and not capped and not info["fallback"]:
+        if stride == 1 and not capped and not same:
             # whole-job counts do not depend on the order (a root-edge sample does:
             # another order roots the search at another pattern edge)
             cl, _ = count_sampled(graph, [plan], stride)

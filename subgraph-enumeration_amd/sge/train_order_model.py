@@ -50,6 +50,8 @@ def main():
     ap.add_argument("--first", type=int, default=0, help="first training pattern to log")
     ap.add_argument("--count", type=int, default=None, help="number of training patterns to log")
     ap.add_argument("--data", default=None, help="comma-separated .npz files of logged pairs to fit from")
+    ap.add_argument("--judge-only", action="store_true",
+                    help="judge the model at --out on the benchmark set and rewrite its report; no training")
     args = ap.parse_args()
     mode = {"wcoj": sge.MODE_WCOJ, "explore": sge.MODE_EXPLORE, "iep": sge.MODE_IEP}[args.mode]
     logf = open(args.log, "a") if args.log else None
@@ -65,6 +67,22 @@ def main():
     g = W.graph_c2()
     g.to_device()
     log(f"graph n={g.n} m={g.m} in {time.time() - t0:.1f}s")
+    if args.judge_only:
+        model = L.load(args.out)
+        rows = L.judge(model, g, W.six_vertex_set(), mode, args.eval_stride, log=log)
+        rep_path = os.path.splitext(args.out)[0] + "_report.json"
+        report = json.load(open(rep_path)) if os.path.exists(rep_path) else {}
+        t_cost = sum(r["cost_ms"] for r in rows)
+        t_learn = sum(r["learned_ms"] for r in rows)
+        report.update({"eval_stride": args.eval_stride, "per_pattern": rows,
+                       "eval_total_ms": {"cost_model": t_cost, "learned": t_learn,
+                                         "speedup": t_cost / t_learn if t_learn else None},
+                       "fallbacks": int(sum(r["fallback"] for r in rows)),
+                       "capped": int(sum(r["capped"] for r in rows))})
+        with open(rep_path, "w") as f:
+            json.dump(report, f, indent=1)
+        log("total cost-model %.1f ms, learned %.1f ms" % (t_cost, t_learn))
+        return
     bench_codes = [p.info().canon for p in W.six_vertex_set()]
     train_p = W.training_set(args.patterns) + sge.sample_patterns(6, 8, 10, 80, 5, bench_codes)
     if args.data:
