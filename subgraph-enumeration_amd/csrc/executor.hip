@@ -58,6 +58,7 @@ constexpr int MAXK = SGE_MAX_K;
 #define SGE_SIB_BLOCK 64
 #endif
 constexpr int SIB_BLOCK = SGE_SIB_BLOCK;  // (sibling, term) entries per flattened last-level batch
+static_assert(SIB_BLOCK <= 64, "the probe-window entry mapping covers at most 64 entries per batch");
 constexpr int MAX_TERMS = 7;    // inclusion-exclusion terms of a 3-level independent tail
 constexpr int NBUF = 2 * MAXK + 1 + MAX_TERMS;  // scratch buffers per wave
 constexpr uint32_t NODE_STEP = 1024;  // work-cap accounting granularity per wavefront (wave steps)
@@ -165,6 +166,7 @@ struct WaveLds {
     uint32_t thi[SIB_BLOCK];
     uint32_t cnt[SIB_BLOCK];   // per-sibling counts
     uint32_t flag[SIB_BLOCK];  // 1: driver is the sibling's CSR row (target F), 0: driver F (target row)
+    uint32_t mark[WAVE];       // phase 2: entry starting at each lane's probe slot (index + 1)
     uint32_t fst[FSTAGE];      // staged copies of small F sets (probed through LDS)
 };
 
@@ -588,16 +590,30 @@ __device__ __forceinline__ uint32_t count_last(const KParams &P, WaveLds &w, Ter
         steps += (ne + W + WAVE - 1) / WAVE;
         acc.inter += ne;
         acc.bytes += 8ull * nb;
-        // phase 2: flattened probes
+        // phase 2: flattened probes, a window of 64 per round.  Lane x's entry is
+        // the last entry starting at or before u0 + x: entries e0+1.. (at most 64,
+        // as ne <= SIB_BLOCK <= 64) mark their start slot in the window, and each
+        // lane takes the highest marked slot at or below it.
+        uint32_t e0 = 0;  // an entry starting at or before u0 (uniform)
         for (uint32_t u0 = 0; u0 < W; u0 += WAVE) {
-            uint32_t u = u0 + lane;
-            if (u < W) {
-                // entry index: last e with pre_sum[e] <= u
-                uint32_t a = 0, b = ne;
-                while (b - a > 1) {
-                    uint32_t m = (a + b) >> 1;
-                    if (w.pre_sum[m] <= u) a = m; else b = m;
+            const uint32_t u = u0 + lane;
+            w.mark[lane] = 0;
+            wave_sync();
+            {
+                const uint32_t ej = e0 + 1 + lane;
+                if (ej < ne) {
+                    const uint32_t st = w.pre_sum[ej];
+                    if (st < u0 + WAVE) atomicMax(&w.mark[st - u0], lane + 1);
                 }
+            }
+            wave_sync();
+            const uint64_t marked = __ballot(w.mark[lane] != 0);
+            const uint64_t upto = marked & ((lane == WAVE - 1) ? ~0ull : ((2ull << lane) - 1ull));
+            uint32_t a = e0;
+            if (upto) a = e0 + w.mark[63 - __builtin_clzll(upto)];
+            e0 = __shfl(a, WAVE - 1);
+            wave_sync();  // marks are rewritten next round
+            if (u < W) {
                 uint32_t off = u - w.pre_sum[a];
                 const uint32_t fl = w.flag[a];
                 const TermLds &t = terms[fl >> 1];
