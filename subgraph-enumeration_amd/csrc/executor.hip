@@ -62,7 +62,10 @@ static_assert(SIB_BLOCK <= 64, "the probe-window entry mapping covers at most 64
 constexpr int MAX_TERMS = 7;    // inclusion-exclusion terms of a 3-level independent tail
 constexpr int NBUF = 2 * MAXK + 1 + MAX_TERMS;  // scratch buffers per wave
 constexpr uint32_t NODE_STEP = 1024;  // work-cap accounting granularity per wavefront (wave steps)
-constexpr int FSTAGE = 1024;    // LDS words per wave for staging the terms' F sets (bitmaps or copies)
+#ifndef SGE_FSTAGE
+#define SGE_FSTAGE 576
+#endif
+constexpr int FSTAGE = SGE_FSTAGE;  // LDS words per wave for staging the terms' F sets (bitmaps or copies)
 constexpr uint32_t NONE = 0xffffffffu;
 
 // Device copy of a plan (levels masks) plus derived fields.
@@ -161,9 +164,7 @@ struct WaveLds {
     uint32_t chk[MAXK];  // check images of the intersection being formed
     // flattened sibling batch
     uint32_t pre_sum[SIB_BLOCK + 1];
-    uint32_t drv[SIB_BLOCK];   // driver start
-    uint32_t tlo[SIB_BLOCK];   // target range
-    uint32_t thi[SIB_BLOCK];
+    uint4 ent[SIB_BLOCK];      // per entry: first probe slot, driver start, target range [z, w)
     uint32_t cnt[SIB_BLOCK];   // per-sibling counts
     uint32_t flag[SIB_BLOCK];  // 1: driver is the sibling's CSR row (target F), 0: driver F (target row)
     uint32_t mark[WAVE];       // phase 2: entry starting at each lane's probe slot (index + 1)
@@ -519,13 +520,13 @@ __device__ __forceinline__ uint32_t count_last(const KParams &P, WaveLds &w, Ter
         uint32_t carry = 0;
         for (uint32_t r0 = 0; r0 < ne; r0 += WAVE) {
             uint32_t ei = r0 + lane;
-            uint32_t work = 0;
+            uint32_t work = 0, dstart = 0, tl = 0, th = 0;
             if (ei < ne) {
                 const uint32_t i = ei / K, j = ei - i * K;
                 const TermLds &t = terms[j];
                 uint32_t c = sibs.ptr ? sibs.ptr[sibs.b + sb + i] : one;
                 uint32_t cntv = 0;
-                uint32_t flag = 0, dstart = 0, tl = 0, th = 0;
+                uint32_t flag = 0;
                 // a sibling aliasing a CSR row may still equal a checked image
                 bool valid = (s == 1) || !is_checked(w, schk_mask, c);
                 if (valid) {
@@ -571,9 +572,6 @@ __device__ __forceinline__ uint32_t count_last(const KParams &P, WaveLds &w, Ter
                 }
                 w.cnt[ei] = cntv;
                 w.flag[ei] = flag | (j << 1);
-                w.drv[ei] = dstart;
-                w.tlo[ei] = tl;
-                w.thi[ei] = th;
             }
             // inclusive scan of work within the wave
             uint32_t incl = work;
@@ -581,7 +579,10 @@ __device__ __forceinline__ uint32_t count_last(const KParams &P, WaveLds &w, Ter
                 uint32_t tt = __shfl_up(incl, off);
                 if (lane >= off) incl += tt;
             }
-            if (ei < ne) w.pre_sum[ei + 1] = carry + incl;
+            if (ei < ne) {
+                w.pre_sum[ei + 1] = carry + incl;
+                w.ent[ei] = make_uint4(carry + incl - work, dstart, tl, th);
+            }
             carry += __shfl(incl, WAVE - 1);
         }
         if (lane == 0) w.pre_sum[0] = 0;
@@ -614,16 +615,17 @@ __device__ __forceinline__ uint32_t count_last(const KParams &P, WaveLds &w, Ter
             e0 = __shfl(a, WAVE - 1);
             wave_sync();  // marks are rewritten next round
             if (u < W) {
-                uint32_t off = u - w.pre_sum[a];
+                const uint4 en = w.ent[a];
+                const uint32_t off = u - en.x;
                 const uint32_t fl = w.flag[a];
                 const TermLds &t = terms[fl >> 1];
                 bool hit;
                 if (fl & 1u) {
-                    uint32_t x = P.nbr[w.drv[a] + off];
-                    hit = term_has(w, t, x, w.tlo[a], w.thi[a]);
+                    uint32_t x = P.nbr[en.y + off];
+                    hit = term_has(w, t, x, en.z, en.w);
                 } else {
-                    uint32_t x = t.F.ptr[w.drv[a] + off];
-                    hit = contains(P.nbr, w.tlo[a], w.thi[a], x);
+                    uint32_t x = t.F.ptr[en.y + off];
+                    hit = contains(P.nbr, en.z, en.w, x);
                 }
                 if (hit) {
                     if (per_entry) atomicAdd(&w.cnt[a], 1u);
@@ -824,8 +826,16 @@ __device__ __forceinline__ void run_dfs(const KParams &P, WaveLds &w, TermLds *t
 }
 
 template <bool ENUM>
-__global__ void __launch_bounds__(256) wcoj_kernel(KParams P) {
-    extern __shared__ unsigned char lds_raw[];
+// Resident wavefronts per SIMD the register allocation is held to.  The search
+// is latency bound (dependent loads of binary searches and LDS round trips),
+// and 7 wavefronts hide more of it than the 5 that unconstrained allocation
+// gives, despite 128 B/lane of spills; FSTAGE is sized so that 28 wavefronts'
+// LDS fits a CU's 160 KB.
+#ifndef SGE_WAVES_PER_SIMD
+#define SGE_WAVES_PER_SIMD 7
+#endif
+__global__ void __launch_bounds__(256, SGE_WAVES_PER_SIMD) wcoj_kernel(KParams P) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
     const int lane = lane_id();
     const int wid = threadIdx.x / WAVE;
     const int nwaves = blockDim.x / WAVE;
