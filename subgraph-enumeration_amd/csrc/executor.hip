@@ -438,8 +438,19 @@ __device__ __forceinline__ uint32_t count_last(const KParams &P, WaveLds &w, Ter
         t.flags = (has_lo ? 1u : 0u) | (has_hi ? 2u : 0u);
         t.F.ptr = nullptr;
         t.F.b = t.F.e = 0;
+        // an inclusion-exclusion term over several tail members has exactly
+        // the union of their constraints, so its F is the intersection of
+        // two smaller terms' F sets (built earlier: their masks are smaller)
+        const uint32_t B = j + 1, b1 = B & (~B + 1u);
+        const bool from_terms = pl.tail_kind == 2 && B != b1 && pl.t_par[j] != 0 &&
+                                !((terms[b1 - 1].flags | terms[(B ^ b1) - 1].flags) & 4u);
         if (pl.t_par[j] == 0) {
             t.flags |= 4u;  // universe: only the sibling's row constrains it
+        } else if (from_terms) {
+            w.srcs[0] = terms[b1 - 1].F;
+            w.srcs[1] = terms[(B ^ b1) - 1].F;
+            t.F = form_set(w, 2, 0, false, buf_ptr(P, wb, 2 * MAXK + 1 + j), acc);
+            acc.nodes += 1;
         } else {
             int ns;
             if (pl.mode == SGE_MODE_EXPLORE && K == 1) {
