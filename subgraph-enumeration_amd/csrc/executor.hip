@@ -66,6 +66,8 @@ constexpr uint32_t NODE_STEP = 1024;  // work-cap accounting granularity per wav
 #define SGE_FSTAGE 576
 #endif
 constexpr int FSTAGE = SGE_FSTAGE;  // LDS words per wave for staging the terms' F sets (bitmaps or copies)
+constexpr uint32_t SEARCH_STAGE = 256;  // source elements staged in LDS per intersection chunk (in fst)
+static_assert(SEARCH_STAGE <= (uint32_t)FSTAGE, "the intersection stage reuses the F staging words");
 constexpr uint32_t NONE = 0xffffffffu;
 
 // Device copy of a plan (levels masks) plus derived fields.
@@ -255,8 +257,11 @@ __device__ __forceinline__ bool is_checked(const WaveLds &w, uint32_t mask, uint
 // Intersect the ns sources in w.srcs (already clamped), drop the nchk check
 // images in w.chk, and write survivors to out.  The shortest source drives; for
 // every other source each lane binary-searches its element within a window that
-// starts at the previous chunk's last position.  Survivors are compacted with a
-// wave ballot + popcount prefix.  Returns the survivor count (uniform).
+// starts at the previous chunk's last position.  When that window is expected to
+// be short (source and driver of similar density), its first SEARCH_STAGE
+// elements are loaded into LDS with coalesced reads and searched there; elements
+// beyond them are searched in global memory.  Survivors are compacted with a wave
+// ballot + popcount prefix.  Returns the survivor count (uniform).
 __device__ __forceinline__ uint32_t wave_intersect(WaveLds &w, int ns, int nchk, uint32_t *out, Acc &acc) {
     if (ns <= 0) return 0;
     int drv = 0;
@@ -277,14 +282,33 @@ __device__ __forceinline__ uint32_t wave_intersect(WaveLds &w, int ns, int nchk,
         uint32_t idx = base + lane;
         bool ok = idx < de;
         uint32_t x = ok ? D[idx] : 0xffffffffu;
+        const uint32_t clen = min((uint32_t)WAVE, de - base);
         for (int i = 0; i < ns; ++i) {
             if (i == drv) continue;
             const guint *a = w.srcs[i].ptr;
-            uint32_t e = w.srcs[i].e;
-            uint32_t p = lower_bound(a, w.cur[i], e, x);
-            ok = ok && p < e && a[p] == x;
-            uint32_t last = __shfl(p, WAVE - 1);
-            w.cur[i] = last;
+            const uint32_t e = w.srcs[i].e, c0 = w.cur[i];
+            const uint32_t rem = e - c0;
+            uint32_t p;
+            bool in;
+            if (rem > 0 && (uint64_t)rem * clen <= (uint64_t)SEARCH_STAGE * (de - base)) {
+                const uint32_t n = min((uint32_t)SEARCH_STAGE, rem);
+                for (uint32_t q = lane; q < n; q += WAVE) w.fst[q] = a[c0 + q];
+                wave_sync();
+                if (x <= w.fst[n - 1]) {
+                    const uint32_t pos = lower_bound(w.fst, 0u, n, x);
+                    p = c0 + pos;
+                    in = w.fst[pos] == x;  // pos < n: fst[n - 1] >= x
+                } else {
+                    p = lower_bound(a, c0 + n, e, x);
+                    in = p < e && a[p] == x;
+                }
+                wave_sync();  // the next source is staged over these words
+            } else {
+                p = lower_bound(a, c0, e, x);
+                in = p < e && a[p] == x;
+            }
+            ok = ok && in;
+            w.cur[i] = __shfl(p, WAVE - 1);
         }
         for (int c = 0; c < nchk; ++c) ok = ok && (x != w.chk[c]);
         uint64_t mask = __ballot(ok);

@@ -22,8 +22,8 @@ SPACE = {  # first value = the library default, the starting point
     "chunk": [4, 2, 8],
     "split_span": [32, 16, 64],
     "donate_budget": [4096, 1024, 16384],
-    "stage_max": [2, 1, 4],
-    "bitmap_ratio": [2, 1, 4],
+    "stage_max": [2, 1, 4, 8],
+    "bitmap_ratio": [2, 1, 4, 8],
 }
 CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "models", "tuning.json")
 
