@@ -50,6 +50,8 @@ extern "C" {
 
 /* graph construction flags */
 #define SGE_GRAPH_DEGREE_ORDER 1u /* relabel vertices by ascending degree (ties by id) */
+#define SGE_GRAPH_BUILD_DEVICE 2u /* deduplicate, order and build the CSR on the GPU (radix sorts);
+                                     identical arrays, and the graph is device-resident on return */
 
 /* ------------------------------------------------------------------ errors */
 const char *sge_last_error(void);
@@ -65,6 +67,7 @@ typedef struct sge_graph_stats {
     uint32_t degree_ordered;
     uint64_t csr_bytes;  /* offsets + neighbour bytes */
     uint64_t triangles_hint; /* 0 unless computed */
+    double build_ms;     /* wall time of the CSR build (normalise, order, CSR), host or device */
 } sge_graph_stats;
 
 /* Text "n m" + m lines "u v", or the reference's binary "BINGRPH" + u32 n + u32 m

@@ -1,4 +1,5 @@
 // C-ABI entry points for graphs and errors (include/sge.h).
+#include <chrono>
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
@@ -24,11 +25,13 @@ static int finish_graph(uint32_t n, EdgeVec &edges, uint32_t flags, sge_graph **
         return SGE_EINVAL;
     }
     sge_graph *g = new sge_graph();
-    int rc = build_graph(n, edges, flags, g->h);
+    const auto t0 = std::chrono::steady_clock::now();
+    int rc = (flags & SGE_GRAPH_BUILD_DEVICE) ? build_graph_device(n, edges, flags, g) : build_graph(n, edges, flags, g->h);
     if (rc) {
         delete g;
         return rc;
     }
+    g->build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     *out = g;
     return SGE_OK;
 }
@@ -123,6 +126,7 @@ extern "C" int sge_graph_stats_get(const sge_graph *g, sge_graph_stats *s) {
     s->max_degree = g->h.max_degree;
     s->degree_ordered = g->h.degree_ordered;
     s->csr_bytes = 4ull * ((uint64_t)g->h.n + 1) + 4ull * g->h.nnz;
+    s->build_ms = g->build_ms;
     return SGE_OK;
 }
 

@@ -7,6 +7,8 @@
 
 #include "sge.h"
 
+struct sge_graph;
+
 namespace sge {
 
 void set_error(const std::string &msg);
@@ -78,6 +80,9 @@ struct Decomposition {
 };
 Decomposition decompose(const Pattern &p, const std::vector<std::vector<int>> &aut);
 
+// graph_build.hip: the SGE_GRAPH_BUILD_DEVICE path (fills g->h and g's device mirror)
+int build_graph_device(uint32_t n, const EdgeVec &edges, uint32_t flags, sge_graph *g);
+
 // planner.cpp
 struct GraphSummary {
     double n = 1, m = 0, avg_deg = 0, d2 = 0;  // d2 = E[deg^2]/E[deg]
@@ -94,6 +99,7 @@ std::vector<std::vector<int>> connected_orders(const Pattern &p, const std::vect
 
 struct sge_graph {
     sge::HostGraph h;
+    double build_ms = 0;  // wall time of the CSR build
     // device mirror (executor.hip)
     void *d_off = nullptr, *d_nbr = nullptr, *d_src = nullptr;
     int device = -1;

@@ -20,6 +20,7 @@ MODE_WCOJ, MODE_EXPLORE, MODE_IEP = 0, 1, 2
 COUNT_BATCHED = 1
 ELIMIT = -6
 GRAPH_DEGREE_ORDER = 1
+GRAPH_BUILD_DEVICE = 2
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("SGE_LIB") or os.path.join(os.path.dirname(_HERE), "libsge.so")
@@ -34,7 +35,8 @@ class SgeError(RuntimeError):
 
 class GraphStats(C.Structure):
     _fields_ = [("n", C.c_uint32), ("m", C.c_uint64), ("max_degree", C.c_uint32),
-                ("degree_ordered", C.c_uint32), ("csr_bytes", C.c_uint64), ("triangles_hint", C.c_uint64)]
+                ("degree_ordered", C.c_uint32), ("csr_bytes", C.c_uint64), ("triangles_hint", C.c_uint64),
+                ("build_ms", C.c_double)]
 
 
 class CPattern(C.Structure):
@@ -170,6 +172,10 @@ def _ptr(a: np.ndarray):
     return a.ctypes.data_as(C.c_void_p)
 
 
+def _graph_flags(degree_order: bool, device_build: bool) -> int:
+    return (GRAPH_DEGREE_ORDER if degree_order else 0) | (GRAPH_BUILD_DEVICE if device_build else 0)
+
+
 class Graph:
     """Data graph: symmetric CSR (u32), optionally relabelled by ascending degree."""
 
@@ -177,31 +183,32 @@ class Graph:
         self._h = C.c_void_p(handle) if not isinstance(handle, C.c_void_p) else handle
 
     @classmethod
-    def load(cls, path: str, degree_order: bool = True) -> "Graph":
+    def load(cls, path: str, degree_order: bool = True, device_build: bool = False) -> "Graph":
         h = C.c_void_p()
-        _check(lib().sge_graph_load(path.encode(), GRAPH_DEGREE_ORDER if degree_order else 0, C.byref(h)),
+        _check(lib().sge_graph_load(path.encode(), _graph_flags(degree_order, device_build), C.byref(h)),
                "sge_graph_load")
         return cls(h)
 
     @classmethod
-    def from_edges(cls, n: int, edges, degree_order: bool = True) -> "Graph":
+    def from_edges(cls, n: int, edges, degree_order: bool = True, device_build: bool = False) -> "Graph":
         e = np.ascontiguousarray(np.asarray(edges, dtype=np.uint32).reshape(-1, 2))
         h = C.c_void_p()
-        _check(lib().sge_graph_from_edges(n, _ptr(e), e.shape[0], GRAPH_DEGREE_ORDER if degree_order else 0,
+        _check(lib().sge_graph_from_edges(n, _ptr(e), e.shape[0], _graph_flags(degree_order, device_build),
                                           C.byref(h)), "sge_graph_from_edges")
         return cls(h)
 
     @classmethod
-    def chung_lu(cls, n, m, gamma, cap=0.0, seed=0, degree_order=True) -> "Graph":
+    def chung_lu(cls, n, m, gamma, cap=0.0, seed=0, degree_order=True, device_build=False) -> "Graph":
         h = C.c_void_p()
-        _check(lib().sge_graph_chung_lu(n, m, gamma, cap, seed, GRAPH_DEGREE_ORDER if degree_order else 0,
+        _check(lib().sge_graph_chung_lu(n, m, gamma, cap, seed, _graph_flags(degree_order, device_build),
                                         C.byref(h)), "sge_graph_chung_lu")
         return cls(h)
 
     @classmethod
-    def rmat(cls, scale, edge_factor=16, a=0.57, b=0.19, c=0.19, seed=0, degree_order=True) -> "Graph":
+    def rmat(cls, scale, edge_factor=16, a=0.57, b=0.19, c=0.19, seed=0, degree_order=True,
+             device_build=False) -> "Graph":
         h = C.c_void_p()
-        _check(lib().sge_graph_rmat(scale, edge_factor, a, b, c, seed, GRAPH_DEGREE_ORDER if degree_order else 0,
+        _check(lib().sge_graph_rmat(scale, edge_factor, a, b, c, seed, _graph_flags(degree_order, device_build),
                                     C.byref(h)), "sge_graph_rmat")
         return cls(h)
 

@@ -176,3 +176,5 @@ def test_count_without_gpu_fails_loudly():
     g = W.graph_c1()
     with pytest.raises(sge.SgeError):
         sge.count(g, sge.build_plan(W.named("diamond"), g))
+    with pytest.raises(sge.SgeError):
+        sge.Graph.from_edges(4, [(0, 1), (1, 2)], device_build=True)
