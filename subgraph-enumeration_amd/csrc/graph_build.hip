@@ -6,7 +6,8 @@
 //
 //   1. normalise: (min, max) keys, self-loops to a sentinel that sorts last
 //   2. radix sort + unique  -> m' undirected edges
-//   3. degrees (atomics), rank = sort of (degree << 32 | id) -> perm / inverse
+//   3. degrees (atomics, one per run of equal smaller endpoints per wavefront),
+//      rank = sort of (degree << 32 | id) -> perm / inverse
 //   4. both directions in new ids, radix sort -> src (high word), nbr (low word)
 //   5. offsets = exclusive scan of the relabelled degrees
 // The device arrays become the graph's device mirror; off, nbr and perm are
@@ -41,11 +42,24 @@ __global__ void normalize_kernel(const uint64_t *__restrict__ in, uint64_t m, ui
     out[i] = (u == v) ? SENTINEL : (((uint64_t)min(u, v) << 32) | max(u, v));
 }
 
+// Degrees of the sorted unique edges.  The smaller endpoints arrive in runs (the
+// keys are sorted), so each wavefront adds a run's length once from its first
+// lane; hub vertices would otherwise serialise thousands of atomics.
 __global__ void degree_kernel(const uint64_t *__restrict__ e, uint64_t m, uint32_t *__restrict__ deg) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= m) return;
-    atomicAdd(&deg[(uint32_t)(e[i] >> 32)], 1u);
-    atomicAdd(&deg[(uint32_t)e[i]], 1u);
+    const int lane = __lane_id();
+    const bool valid = i < m;
+    const uint32_t u = valid ? (uint32_t)(e[i] >> 32) : 0xffffffffu;
+    const uint32_t prev = __shfl_up(u, 1);
+    const bool head = valid && (lane == 0 || prev != u);
+    const uint64_t heads = __ballot(head);
+    const uint64_t valid_lanes = __ballot(valid);  // outside the branch: every lane votes
+    if (head) {
+        const uint64_t after = heads & ~((lane == 63) ? ~0ull : ((2ull << lane) - 1ull));
+        const uint32_t end = after ? (uint32_t)__builtin_ctzll(after) : (uint32_t)(64 - __builtin_clzll(valid_lanes));
+        atomicAdd(&deg[u], end - (uint32_t)lane);
+    }
+    if (valid) atomicAdd(&deg[(uint32_t)e[i]], 1u);
 }
 
 __global__ void rank_key_kernel(const uint32_t *__restrict__ deg, uint32_t n, uint64_t *__restrict__ key) {
