@@ -51,7 +51,7 @@ def workload(name):
         return ("chunglu7115_house_k5e", lambda: W.graph_c2(), lambda: [W.named("house"), W.named("near5clique")],
                 {"graph": c2, "patterns": "house, K5-e"})
     if name in ("rmat20", "c4"):
-        return ("rmat20_sevenvertex78", lambda: W.graph_c4(), W.seven_vertex_set,
+        return ("rmat20_sevenvertex78", lambda: W.graph_c4(device_build=True), W.seven_vertex_set,
                 {"graph": "rmat(scale=20, ef=16, a=.57, b=.19, c=.19, seed=4)",
                  "patterns": "78 x 7-vertex, 15-19 edges"})
     raise SystemExit(f"unknown workload {name}")

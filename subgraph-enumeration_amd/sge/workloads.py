@@ -46,17 +46,17 @@ def graph_c2(degree_order=True) -> Graph:
     return Graph.chung_lu(7115, 100762, 2.1, 1100.0, 1, degree_order)
 
 
-def graph_rmat(scale: int, seed: int, degree_order=True) -> Graph:
+def graph_rmat(scale: int, seed: int, degree_order=True, device_build=False) -> Graph:
     """Undirected deduplicated R-MAT, edge factor 16, (a,b,c,d)=(.57,.19,.19,.05)."""
-    return Graph.rmat(scale, 16, 0.57, 0.19, 0.19, seed, degree_order)
+    return Graph.rmat(scale, 16, 0.57, 0.19, 0.19, seed, degree_order, device_build)
 
 
-def graph_c4(degree_order=True) -> Graph:
-    return graph_rmat(20, 4, degree_order)
+def graph_c4(degree_order=True, device_build=False) -> Graph:
+    return graph_rmat(20, 4, degree_order, device_build)
 
 
-def graph_c5(degree_order=True) -> Graph:
-    return graph_rmat(23, 6, degree_order)
+def graph_c5(degree_order=True, device_build=False) -> Graph:
+    return graph_rmat(23, 6, degree_order, device_build)
 
 
 # ---- pattern sets ---------------------------------------------------------------------
