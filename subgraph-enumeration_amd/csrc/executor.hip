@@ -78,8 +78,6 @@ struct DPlan {
     uint32_t ready[MAXK];
     // last-level decomposition (relative to the sibling level s)
     uint32_t f_parent;     // parents of `last` other than s
-    uint32_t s_is_parent;  // s adjacent to last
-    uint32_t s_lower, s_upper, s_check;  // last's clamp/check against s
     uint32_t f_ready;      // exploration: level after which F's intersection is formed
     uint32_t order[MAXK];  // level -> pattern vertex (enumeration output)
     // closed-form terms counted per sibling: kind 0/1 use one term (the first
@@ -1118,10 +1116,6 @@ DPlan to_dplan(const sge_plan &p) {
         term(0, 1u << L);
     }
     d.f_parent = d.parent[L] & ~(1u << s);
-    d.s_is_parent = (d.parent[L] >> s) & 1u;
-    d.s_lower = (d.lower[L] >> s) & 1u;
-    d.s_upper = (d.upper[L] >> s) & 1u;
-    d.s_check = (d.check[L] >> s) & 1u;
     d.f_ready = d.f_parent ? 31 - __builtin_clz(d.f_parent) : 0;
     return d;
 }

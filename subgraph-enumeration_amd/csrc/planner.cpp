@@ -146,18 +146,6 @@ std::vector<std::vector<int>> connected_orders(const Pattern &p, const std::vect
     return out;
 }
 
-// Largest set of pairwise non-adjacent vertices with identical neighbourhoods.
-static uint32_t largest_twin_group(const Pattern &p) {
-    uint32_t best = 0;
-    for (int a = 0; a < p.k; ++a) {
-        uint32_t grp = 0;
-        for (int b = 0; b < p.k; ++b)
-            if (p.adj[a] == p.adj[b] && !p.edge(a, b)) grp |= 1u << b;
-        if (__builtin_popcount(grp) > __builtin_popcount(best)) best = grp;
-    }
-    return __builtin_popcount(best) >= 2 ? best : 0;
-}
-
 // Trailing symmetric group (kind 1): the last r levels are pairwise
 // non-adjacent twins that share every parent, bound and check below the group
 // and are chained among themselves; choosec(|C|, r) counts them
