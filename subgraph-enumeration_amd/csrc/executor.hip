@@ -882,6 +882,7 @@ __global__ void __launch_bounds__(256, SGE_WAVES_PER_SIMD) wcoj_kernel(KParams P
     const bool cont = P.in_items != nullptr;
     const uint64_t nitems = cont ? min((uint64_t)*P.in_count, P.item_cap) : P.nitems;
     unsigned long long t_item = __builtin_amdgcn_s_memtime();  // start of the current item
+    uint32_t fetched = 0;
     while (true) {
         {
             // charge the time since the last item started to that item's plan
@@ -893,8 +894,11 @@ __global__ void __launch_bounds__(256, SGE_WAVES_PER_SIMD) wcoj_kernel(KParams P
         if (lane == 0) item = atomicAdd(P.queue, 1ull);
         item = __shfl(item, 0);
         if (item >= nitems) break;
-        if (P.node_cap && __hip_atomic_load(P.node_ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= P.node_cap)
-            break;
+        if (P.node_cap && (++fetched & 15u) == 0) {  // every 16th item: has the work cap been reached?
+            unsigned long long spent = 0;
+            if (lane == 0) spent = __hip_atomic_load(P.node_ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (__shfl(spent, 0) >= P.node_cap) break;
+        }
         if (cont) {
             // ---- continuation item: rebuild the path, then search its range
             const uint32_t *it = P.in_items + item * ITEM_WORDS;

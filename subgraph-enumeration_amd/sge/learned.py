@@ -214,6 +214,9 @@ def judge(model: "LearnedOrder", graph: Graph, patterns: Sequence[Pattern], mode
             cap_k = max(1024, 4 * work_steps_bound(st[0]) // 1024 + 1)
             (sl, lim), = time_capped(graph, plan, stride, cap_k) if not info["fallback"] else [(st[0], False)]
             capped = capped or lim
+            if not lim and not info["fallback"]:  # it finishes: time it without the cap's accounting
+                _, sl = count_sampled(graph, [plan], stride)
+                sl = sl[0]
             t_learn.append(sl.kernel_ms)
         if stride == 1 and not capped and not same:
             # whole-job counts do not depend on the order (a root-edge sample does:
