@@ -97,7 +97,8 @@ def tuned(key: str, cache: str = CACHE) -> Optional[Tuning]:
 
 def main():
     """Tune the batched executor for the 16 six-vertex patterns over the
-    Chung-Lu graph (n=7,115, m=100,762), IEP mode, whole job, and cache the
+    Chung-Lu graph (n=7,115, m=100,762), IEP mode, whole job, with the orders
+    the batch runs (the learned model's when it is present), and cache the
     result under the workload's name."""
     import argparse
 
@@ -112,6 +113,12 @@ def main():
     g = W.graph_c2()
     g.to_device()
     plans = [build_plan(p, g, MODE_IEP) for p in W.six_vertex_set()]
+    model_path = os.path.join(os.path.dirname(CACHE), "order_mlp.npz")
+    if os.path.exists(model_path):
+        from . import learned as L
+
+        model = L.load(model_path)
+        plans = [L.choose_plan(model, p, g, MODE_IEP, fallback=c)[0] for p, c in zip(W.six_vertex_set(), plans)]
     if os.path.exists(CACHE):
         with open(CACHE) as f:
             data = json.load(f)
