@@ -8,23 +8,29 @@
 //  * one depth-first search per wavefront (64 lanes); the wave's per-level
 //    candidate buffers live in a per-wave global scratch slab, its DFS cursor
 //    (images, lengths, positions, list descriptors) in LDS;
-//  * root tasks are directed data edges (CSR positions), handed out in chunks by
-//    a global atomic queue (the OpenMP schedule(dynamic) root split of
-//    parallel.cpp:1254-1299 / 1414-1458); chunks run from the high-degree end of
-//    the degree-ordered CSR down, so heavy subtrees start first;
+//  * root tasks are directed data edges (CSR positions), handed out by a global
+//    atomic queue (the OpenMP schedule(dynamic) root split of
+//    parallel.cpp:1254-1299 / 1414-1458) from the high-degree end of the
+//    degree-ordered CSR down: the heaviest subtasks one per item, the rest in
+//    chunks; a root edge whose level-2 set can be large is split into slices;
 //  * every level's candidate set is an intersection of sorted neighbour lists
 //    clamped by symmetry-breaking bounds (setBeginPos/setEndPos, utils.cpp:581-612)
-//    computed by lane-parallel binary search, the shortest list driving; survivors
-//    are compacted with a wave64 ballot + popcount prefix;
+//    computed by lane-parallel binary search (windows staged in LDS when short),
+//    the shortest list driving; survivors are compacted with a wave64 ballot +
+//    popcount prefix;
 //  * the last level is never enumerated: like TrieLevel::numResults
 //    (relation.cpp:301-338) it is counted in closed form, and all siblings of the
 //    second-to-last level are counted together with a flattened, load-balanced
-//    lane mapping (per-sibling work prefix-summed in LDS);
+//    lane mapping (per-sibling work prefix-summed in LDS, F sets staged in LDS as
+//    bitmaps or sorted copies);
 //  * IEP mode counts a trailing symmetric group with choosec (utils.cpp:80-94,
-//    join.cpp:1532-1537), wrapping uint64 arithmetic included;
+//    join.cpp:1532-1537), wrapping uint64 arithmetic included, and an
+//    independent 2- or 3-vertex tail by inclusion-exclusion;
 //  * exploration mode forms each level's parent intersection as soon as its last
 //    parent is matched (attributesAfter, join.cpp:423-563) and only filters it later;
-//  * batched mode runs many plans in the same launch: queue item i is chunk i/P of
+//  * a wavefront that has spent its work budget inside one item hands its open
+//    candidate ranges to the next pass as continuation items;
+//  * batched mode runs many plans in the same launch: queue item i is item i/P of
 //    plan i%P, so all patterns drain the shared queue together.
 // Counts are exact uint64.
 #include <hip/hip_runtime.h>
