@@ -72,7 +72,10 @@ constexpr uint32_t NODE_STEP = 1024;  // work-cap accounting granularity per wav
 #define SGE_FSTAGE 576
 #endif
 constexpr int FSTAGE = SGE_FSTAGE;  // LDS words per wave for staging the terms' F sets (bitmaps or copies)
-constexpr uint32_t SEARCH_STAGE = 256;  // source elements staged in LDS per intersection chunk (in fst)
+#ifndef SGE_SEARCH_STAGE
+#define SGE_SEARCH_STAGE 256
+#endif
+constexpr uint32_t SEARCH_STAGE = SGE_SEARCH_STAGE;  // source elements staged in LDS per intersection chunk (in fst)
 static_assert(SEARCH_STAGE <= (uint32_t)FSTAGE, "the intersection stage reuses the F staging words");
 constexpr uint32_t NONE = 0xffffffffu;
 

@@ -309,3 +309,21 @@ def test_device_graph_build_matches_host():
     # a device-built graph is device-resident and counts like the host-built one
     pl = sge.build_plan(W.named("house"), h, sge.MODE_IEP)
     assert sge.count_sampled(d, [pl], 7)[0] == sge.count_sampled(h, [pl], 7)[0]
+
+
+def test_vertex_relabeling_invariance():
+    """Counts do not depend on the data graph's vertex ids: the degree-ordered
+    CSR, the original ids and a random relabelling give the same counts."""
+    g0 = W.graph_c1()
+    off, nbr = g0.csr()
+    src = np.repeat(np.arange(len(off) - 1), np.diff(off))
+    edges = np.stack([src, nbr], 1)
+    edges = edges[edges[:, 0] < edges[:, 1]].astype(np.uint32)
+    perm = np.random.default_rng(3).permutation(g0.n).astype(np.uint32)
+    graphs = [g0, sge.Graph.from_edges(g0.n, edges, degree_order=False),
+              sge.Graph.from_edges(g0.n, perm[edges], degree_order=False)]
+    for p in [W.named("house"), W.named("diamond"), W.named("near5clique")] + W.six_vertex_set()[:4]:
+        for mode in MODES:
+            say("relabel", p.edges(), mode)
+            got = {sge.count(g, sge.build_plan(p, g, mode))[0][0] for g in graphs}
+            assert len(got) == 1, (p, mode, got)
