@@ -31,6 +31,7 @@ static void usage() {
               << "  -t <number>      Number of threads (accepted; the GPU executor ignores it)\n"
               << "  --embeddings <path>  Write canonical embeddings (extension)\n"
               << "  --order a,b,...  Fixed matching order (extension)\n"
+              << "  --device-build   Build the data graph's CSR on the GPU (extension)\n"
               << "  --help, -h       Show this help message\n"
               << std::endl;
 }
@@ -43,6 +44,7 @@ static int die(const char *what) {
 int main(int argc, char **argv) {
     std::string q, d, r, emb, order_s;
     bool explore = false, iep = false;
+    uint32_t gflags = SGE_GRAPH_DEGREE_ORDER;
     int threads = 1;
     for (int i = 1; i < argc; ++i) {
         std::string a = argv[i];
@@ -58,6 +60,7 @@ int main(int argc, char **argv) {
         else if (a == "-t") threads = std::atoi(next().c_str());
         else if (a == "--embeddings") emb = next();
         else if (a == "--order") order_s = next();
+        else if (a == "--device-build") gflags |= SGE_GRAPH_BUILD_DEVICE;
     }
     if (threads <= 0) threads = 1;  // atoi of a missing value -> 0 -> 1 (main.cpp:121)
     std::ofstream fout;
@@ -69,7 +72,7 @@ int main(int argc, char **argv) {
     sge_pattern p;
     if (sge_pattern_load(q.c_str(), &p)) return die("pattern");
     sge_graph *g = nullptr;
-    if (sge_graph_load(d.c_str(), SGE_GRAPH_DEGREE_ORDER, &g)) return die("data graph");
+    if (sge_graph_load(d.c_str(), gflags, &g)) return die("data graph");
     auto t0 = std::chrono::steady_clock::now();
     uint32_t mode = iep ? SGE_MODE_IEP : (explore ? SGE_MODE_EXPLORE : SGE_MODE_WCOJ);
     std::vector<uint32_t> order;
