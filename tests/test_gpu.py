@@ -327,3 +327,34 @@ def test_vertex_relabeling_invariance():
             say("relabel", p.edges(), mode)
             got = {sge.count(g, sge.build_plan(p, g, mode))[0][0] for g in graphs}
             assert len(got) == 1, (p, mode, got)
+
+
+def test_cli_drop_in(tmp_path):
+    """graph_mining_system with the reference's flags prints the reference's
+    count lines (C1 goldens) in every mode, writes -r files and canonical
+    embeddings, and loads through the GPU CSR builder on request."""
+    import subprocess
+
+    cli = os.path.join(os.path.dirname(sge.LIB_PATH), "graph_mining_system")
+    g = W.graph_c1()
+    dpath = str(tmp_path / "c1.txt")
+    g.write(dpath)
+    gold = golden("c1")["patterns"]
+    for name in ["diamond", "house"]:
+        qpath = tmp_path / f"{name}.txt"
+        e = gold[name]["edges"]
+        qpath.write_text(f"{gold[name]['k']} {len(e)}\n" + "".join(f"{a} {b}\n" for a, b in e))
+        for flags in ([], ["-i", "-t", "4"], ["-u"], ["--device-build"]):
+            say("cli", name, flags)
+            r = subprocess.run([cli, "-q", str(qpath), "-d", dpath] + flags, capture_output=True, text=True, timeout=120)
+            assert r.returncode == 0, r.stderr
+            lines = [l for l in r.stdout.splitlines() if l.startswith("Number of matches:")]
+            assert lines and int(lines[0].split(":")[1]) == gold[name]["wcoj"]["count"], (name, flags, r.stdout)
+    rpath, epath = str(tmp_path / "out.txt"), str(tmp_path / "emb.txt")
+    r = subprocess.run([cli, "-q", str(tmp_path / "diamond.txt"), "-d", dpath, "-r", rpath, "--embeddings", epath],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "Number of matches: %d" % gold["diamond"]["wcoj"]["count"] in open(rpath).read()
+    lines = [l for l in open(epath).read().splitlines() if l.strip()]
+    rows = [list(map(int, l.split())) for l in lines[1:]]
+    assert int(lines[0]) == len(rows) == gold["diamond"]["wcoj"]["count"] and rows == sorted(rows)
