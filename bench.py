@@ -20,7 +20,10 @@ the default: its patterns have ~1e17-1e19 matches each and single root edges
 carry ~1e13, beyond exact enumeration in a benchmark window; it runs only on a
 root-edge sample (`--stride`).
 
-    python bench.py [--steps K] [--warmup W] [--workload sixv|house|rmat20] [--no-cpu]
+The same 78 patterns over R-MAT scale 8 (seed 7, the graph of the reference's
+golden counts in tests/golden/rmat7.json) run exactly with `--workload rmat8`.
+
+    python bench.py [--steps K] [--warmup W] [--workload sixv|house|rmat8|rmat20] [--no-cpu]
 """
 import argparse
 import json
@@ -52,6 +55,10 @@ def workload(name):
     if name in ("house", "c2house"):
         return ("chunglu7115_house_k5e", lambda: W.graph_c2(), lambda: [W.named("house"), W.named("near5clique")],
                 {"graph": c2, "patterns": "house, K5-e"})
+    if name in ("rmat8", "rmat8_78"):
+        return ("rmat8_sevenvertex78", lambda: W.graph_rmat(8, 7, device_build=True), W.seven_vertex_set,
+                {"graph": "rmat(scale=8, ef=16, a=.57, b=.19, c=.19, seed=7)",
+                 "patterns": "78 x 7-vertex, 15-19 edges"})
     if name in ("rmat20", "c4"):
         return ("rmat20_sevenvertex78", lambda: W.graph_c4(device_build=True), W.seven_vertex_set,
                 {"graph": "rmat(scale=20, ef=16, a=.57, b=.19, c=.19, seed=4)",

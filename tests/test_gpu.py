@@ -358,3 +358,25 @@ def test_cli_drop_in(tmp_path):
     lines = [l for l in open(epath).read().splitlines() if l.strip()]
     rows = [list(map(int, l.split())) for l in lines[1:]]
     assert int(lines[0]) == len(rows) == gold["diamond"]["wcoj"]["count"] and rows == sorted(rows)
+
+
+def test_seven_vertex_set_rmat8_batched_goldens():
+    """All 78 seven-vertex patterns over R-MAT scale 8 (the reference's goldens),
+    IEP mode, batched and plain: every count equals the reference's WCOJ count
+    and its IEP count, except where the reference's IEP mode overcounts
+    (documented in test_oracle)."""
+    from test_oracle import REF_IEP_OVERCOUNTS
+
+    wcoj = golden("rmat7")["patterns"]
+    iep = golden("rmat7_iep")["patterns"]
+    g = W.graph_rmat(8, 7)
+    plans = [sge.build_plan(p, g, sge.MODE_IEP) for p in W.seven_vertex_set()]
+    say("rmat8 78 batched")
+    b = sge.Batch(g, plans)
+    got = b.run()[0]
+    b.close()
+    assert got == [r["wcoj"]["count"] for r in wcoj]
+    assert sge.count(g, plans)[0] == got
+    for i, (c, r) in enumerate(zip(got, iep)):
+        if i not in REF_IEP_OVERCOUNTS:
+            assert c == r["iep"]["count"], i
