@@ -18,7 +18,6 @@ ones the model is used on.
 from __future__ import annotations
 
 import json
-import os
 import time
 from typing import Dict, List, Optional, Sequence
 
