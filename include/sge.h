@@ -146,6 +146,12 @@ typedef struct sge_plan {
     uint32_t tail;                     /* IEP: trailing levels counted in closed form (0 = none) */
     uint32_t tail_kind;                /* 1: symmetric twin group, choosec (join.cpp:1532-1537);
                                           2: independent set, inclusion-exclusion over set partitions */
+    uint32_t bag;                      /* levels of one materialised bag (0 = none): its relation is
+                                          enumerated and built into a trie before the join, and each
+                                          bag level then reads its trie child list in place of the
+                                          rows of its bag parents (nodeJoin join.cpp:309-421, buildTrie
+                                          join.cpp:1713-1717, globalJoin join.cpp:1202-1420); set with
+                                          sge_plan_set_bag */
     uint32_t num_bags;                 /* decomposition bag count (reporting / features) */
     uint64_t multiplier;               /* |Aut| when the reference counts embeddings (single bag), else 1 */
     uint64_t aut_size;
@@ -166,6 +172,20 @@ int sge_plan_rank_orders(const sge_pattern *p, const sge_graph *g, uint32_t mode
                          uint32_t *tails, uint32_t cap, uint32_t *n);
 /* Reference-style plan text ("num bags", symmetry rules, ...) into buf. */
 int sge_plan_text(const sge_plan *plan, char *buf, size_t cap);
+/* Materialise the pattern vertices in vertex_mask as one bag of `plan` (0 clears
+ * it).  The bag relation is every assignment of the bag's levels that satisfies
+ * the plan's edges, bounds and injectivity checks among them (the reference's
+ * bag tuples, join.cpp:309-421); the join then intersects trie child lists with
+ * the rows of parents outside the bag (twoIntersection / multiIntersection,
+ * join.cpp:9-135).  Counts never change.  Valid bags: >= 3 vertices, WCOJ or
+ * IEP mode, every bag level below the closed-form level(s), the first two bag
+ * levels adjacent and every later bag level adjacent to an earlier bag level
+ * (SGE_EINVAL otherwise). */
+int sge_plan_set_bag(sge_plan *plan, uint32_t vertex_mask);
+/* The bags of the pattern's minimal fractional hypertree decomposition (vertex
+ * masks, as all_td.cpp:483-691 chooses them) that are valid bags of `plan`;
+ * count in *n (at most cap written). */
+int sge_plan_bags(const sge_pattern *p, const sge_plan *plan, uint32_t *masks, uint32_t cap, uint32_t *n);
 
 /* ---------------------------------------------------------------- executor */
 typedef struct sge_tuning {
@@ -200,6 +220,8 @@ typedef struct sge_stats {
     double wall_ms;           /* host wall time of the call */
     uint64_t wave_cycles;     /* wavefront clock cycles spent on the plan's work (total work,
                                  independent of how well the launch was balanced) */
+    double trie_ms;           /* host wall time spent materialising bag tries (sge_batch_run:
+                                 the one-off build at sge_batch_create) */
 } sge_stats;
 
 void sge_tuning_default(sge_tuning *t);
@@ -226,6 +248,23 @@ int sge_num_tasks(const sge_graph *g, const sge_plan *plan, uint64_t *n);
  * pattern-vertex order, original vertex ids) into out[cap*k]; *n_out gets the
  * total (may exceed cap -> SGE_ERANGE, nothing beyond cap written). */
 int sge_enumerate(sge_graph *g, const sge_plan *plan, uint32_t *out, uint64_t cap, uint64_t *n_out);
+
+/* Bag relation of plan->bag as a level-order trie on the device: the relation
+ * is enumerated by the executor (the bag's levels in plan order), radix sorted
+ * lexicographically, and grouped per depth (TrieLevel::buildTrieFromSortedMatchesBatch,
+ * relation.cpp:340-379; TrieLevel relation.h:49-69).  Depth j holds nodes[j]
+ * values (sorted within each parent's child list), stored after those of the
+ * shallower depths; the children of the node at absolute value index a of depth
+ * j < depth-1 are the values in [child[a + j], child[a + j + 1]) (each depth's
+ * offsets end with a closing sentinel).  Values are the graph's internal
+ * (degree-ordered) vertex ids. */
+typedef struct sge_trie sge_trie;
+int sge_trie_build(sge_graph *g, const sge_plan *plan, sge_trie **out);
+/* depth = bag levels, tuples = relation size, nodes[SGE_MAX_K] per depth, build_ms. */
+int sge_trie_shape(const sge_trie *t, uint32_t *depth, uint64_t *tuples, uint64_t *nodes, double *build_ms);
+/* values[sum nodes], child[sum over j < depth-1 of nodes[j] + 1] */
+int sge_trie_copy(const sge_trie *t, uint32_t *values, uint32_t *child);
+void sge_trie_destroy(sge_trie *t);
 
 /* Batched executor captured once in a hipGraph, replayed per run. */
 typedef struct sge_batch sge_batch;

@@ -39,6 +39,9 @@ def lib():
         L.oracle_count.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_int]
         L.oracle_enumerate.restype = C.c_uint64
         L.oracle_enumerate.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint64]
+        L.oracle_bag_trie.restype = C.c_int
+        L.oracle_bag_trie.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p,
+                                      C.c_uint64, C.c_void_p, C.c_uint64]
         L.oracle_count_brute.restype = C.c_uint64
         L.oracle_count_brute.argtypes = [C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32]
         _lib = L
@@ -93,6 +96,20 @@ def enumerate_embeddings(plan, off, nbr, cap):
     out = np.zeros(max(cap, 1) * plan.k, dtype=np.uint32)
     n = lib().oracle_enumerate(C.byref(plan), _p(off), _p(nbr), len(off) - 1, _p(out), cap)
     return out[: min(n, cap) * plan.k].reshape(-1, plan.k).copy(), int(n)
+
+
+def bag_trie(plan, off, nbr):
+    """Trie of plan.bag's relation: (vb, values, child), depth = len(vb) - 1."""
+    off, nbr = _u32(off), _u32(nbr)
+    vb = np.zeros(9, dtype=np.uint32)
+    p = lib().oracle_bag_trie(C.byref(plan), _p(off), _p(nbr), len(off) - 1, _p(vb), None, 0, None, 0)
+    if p < 0:
+        raise RuntimeError("oracle_bag_trie failed")
+    nval, nchild = int(vb[p]), int(vb[p - 1]) + p - 1
+    values = np.zeros(max(nval, 1), dtype=np.uint32)
+    child = np.zeros(max(nchild, 1), dtype=np.uint32)
+    lib().oracle_bag_trie(C.byref(plan), _p(off), _p(nbr), len(off) - 1, _p(vb), _p(values), nval, _p(child), nchild)
+    return [int(x) for x in vb[: p + 1]], values[:nval], child[:nchild]
 
 
 def count_brute(k, adj, off, nbr):

@@ -21,6 +21,12 @@
  *                         matching/parallel.cpp:1414-1458 (schedule(dynamic), + reduction)
  *   oracle_count_brute    no reference counterpart: all injective homomorphisms by plain
  *                         backtracking, the independent check for tiny graphs.
+ *   bag_trie_build        matching/join.cpp:309-421 (nodeJoin: the bag's tuples), join.cpp:1713-1717
+ *                         + relation/relation.cpp:340-379 (buildTrie: std::sort, then recursive
+ *                         grouping by depth), flattened depth by depth (oracle_bag_trie)
+ *   candidates (bag)      matching/join.cpp:1202-1300 (globalJoin: a level joined with a bag reads
+ *                         the bag's current trie level in place of the rows of its bag parents and
+ *                         descends into the matched value's children, join.cpp:244-264 setIters)
  *
  * Plans are the flat sge_plan of include/sge.h: level i matches pattern vertex
  * order[i]; parent/lower/upper/check masks are over levels.
@@ -139,8 +145,18 @@ uint64_t oracle_choosec(uint64_t n, int k) {
     return nom / denom;
 }
 
+/* Bag relation trie, depth-major: depth j's values are val[vb[j] .. vb[j+1]);
+ * the children of the depth-j value at index a are val[child[a + j] .. child[a + j + 1]). */
+typedef struct {
+    uint32_t p;
+    uint32_t vb[SGE_MAX_K + 1];
+    uint32_t *val, *child;
+} otrie_t;
+
 typedef struct {
     const sge_plan *pl;
+    const otrie_t *trie;       /* pl->bag's trie (NULL without a bag) */
+    uint32_t tpos[SGE_MAX_K];  /* bag levels: value index of phi[level] in the trie */
     const uint32_t *off, *nbr;
     uint32_t n, maxdeg;
     uint32_t phi[SGE_MAX_K];
@@ -175,6 +191,41 @@ static void bounds(const ctx_t *c, int d, uint32_t *lo_excl, int *has_lo, uint32
     *hi_excl = mn;
 }
 
+/* Trie list of bag level d: the root values for the first bag level, else the
+ * children of the previous bag level's matched value. */
+static const uint32_t *bag_list(const ctx_t *c, int d, uint32_t *len, uint32_t *base) {
+    const otrie_t *t = c->trie;
+    uint32_t below = c->pl->bag & ((1u << d) - 1u), b, e;
+    if (below == 0) {
+        b = t->vb[0];
+        e = t->vb[1];
+    } else {
+        uint32_t j = (uint32_t)__builtin_popcount(below), a = c->tpos[31 - __builtin_clz(below)];
+        b = t->child[a + j - 1];
+        e = t->child[a + j];
+    }
+    *len = e - b;
+    *base = b;
+    return t->val + b;
+}
+
+/* Record the trie position of phi[d] for bag level d; 0 when it is absent. */
+static int bag_enter(ctx_t *c, int d) {
+    uint32_t len, base;
+    const uint32_t *a = bag_list(c, d, &len, &base);
+    uint32_t p = lower_bound_u32(a, 0, len, c->phi[d]);
+    c->tpos[d] = base + p;
+    return p < len && a[p] == c->phi[d];
+}
+
+/* reuse is off for bag levels and for levels that would reuse a bag level's set */
+static int reuse_of(const ctx_t *c, int d) {
+    const sge_plan *pl = c->pl;
+    int r = pl->reuse[d];
+    if (pl->bag && r >= 0 && (((pl->bag >> d) & 1u) || ((pl->bag >> r) & 1u))) r = -1;
+    return r;
+}
+
 /* Candidate set of level d (ignoring the IEP group's internal chain when
  * `group_base` >= 0: members of the group beyond group_base are not matched yet).
  * Returns the size; elements in c->buf[d] when materialise != 0. */
@@ -185,7 +236,7 @@ static uint32_t candidates(ctx_t *c, int d, int materialise, uint32_t lower_mask
     const uint32_t *src = NULL;
     uint32_t srclen = 0;
     int j;
-    int r = c->no_reuse ? -1 : pl->reuse[d];
+    int r = c->no_reuse ? -1 : reuse_of(c, d);
     uint32_t mx = 0, mn = 0xffffffffu;
     int has_lo = 0, has_hi = 0;
     for (j = 0; j < d; ++j) {
@@ -196,6 +247,13 @@ static uint32_t candidates(ctx_t *c, int d, int materialise, uint32_t lower_mask
         src = c->buf[r];
         srclen = c->len[r];
         lists &= ~pl->parent[r];
+    }
+    if (c->trie && ((pl->bag >> d) & 1u)) {
+        /* the trie list satisfies every edge, bound and check among the bag's
+         * levels; rows of bag parents are not read (reuse is off here) */
+        uint32_t base;
+        src = bag_list(c, d, &srclen, &base);
+        if (pl->bag & ((1u << d) - 1u)) lists &= ~pl->bag;
     }
     /* generateCandidates: clamp every list, then chain pairwise intersections */
     uint32_t *cur = c->tmp, *nxt = c->tmp2;
@@ -306,11 +364,21 @@ static uint64_t dfs(ctx_t *c, int d) {
     candidates(c, d, 1, pl->lower[d], pl->upper[d], pl->check[d]);
     uint64_t sum = 0;
     uint32_t i;
+    int descend = c->trie && ((pl->bag >> d) & 1u) && (pl->bag >> (d + 1));
     for (i = 0; i < c->len[d]; ++i) {
         c->phi[d] = c->buf[d][i];
+        if (descend) bag_enter(c, d);
         sum += dfs(c, d + 1);
     }
     return sum;
+}
+
+/* Root edge (phi[0], phi[1]) of a plan with a bag: both ends must lie on a trie path. */
+static int bag_root_ok(ctx_t *c) {
+    if (!c->trie) return 1;
+    if ((c->pl->bag & 1u) && !bag_enter(c, 0)) return 0;
+    if ((c->pl->bag & 2u) && !bag_enter(c, 1)) return 0;
+    return 1;
 }
 
 static int ctx_init(ctx_t *c, const sge_plan *pl, const uint32_t *off, const uint32_t *nbr, uint32_t n) {
@@ -336,6 +404,16 @@ static int ctx_init(ctx_t *c, const sge_plan *pl, const uint32_t *off, const uin
     return (c->tmp && c->tmp2) ? 0 : -1;
 }
 
+/* Attach a bag trie: the working lists must also hold the trie's root values
+ * (up to n), which the first bag level copies before intersecting. */
+static void ctx_set_trie(ctx_t *c, const otrie_t *t) {
+    c->trie = t;
+    if (t->vb[1] > c->maxdeg) {
+        c->tmp = (uint32_t *)realloc(c->tmp, ((size_t)t->vb[1] + 1) * 4);
+        c->tmp2 = (uint32_t *)realloc(c->tmp2, ((size_t)t->vb[1] + 1) * 4);
+    }
+}
+
 static void ctx_free(ctx_t *c) {
     int i;
     for (i = 0; i < SGE_MAX_K; ++i) free(c->buf[i]);
@@ -354,13 +432,145 @@ static uint32_t src_of(const uint32_t *off, uint32_t n, uint64_t e) {
     return lo;
 }
 
+uint64_t oracle_enumerate(const sge_plan *pl, const uint32_t *off, const uint32_t *nbr, uint32_t n, uint32_t *emb,
+                          uint64_t cap);
+uint64_t oracle_count(const sge_plan *pl, const uint32_t *off, const uint32_t *nbr, uint32_t n, int threads);
+
+/* ---- materialised bag: relation and trie */
+
+/* The bag's own plan (nodeJoin over the bag's attributes): its levels in plan
+ * order, masks restricted to the bag, column i = bag level i. */
+static void bag_plan(const sge_plan *pl, sge_plan *sub) {
+    int lev[SGE_MAX_K], p = 0, i, j;
+    memset(sub, 0, sizeof(*sub));
+    for (i = 0; i < (int)pl->k; ++i)
+        if ((pl->bag >> i) & 1u) lev[p++] = i;
+    sub->k = (uint32_t)p;
+    sub->mode = SGE_MODE_WCOJ;
+    sub->multiplier = 1;
+    for (i = 0; i < SGE_MAX_K; ++i) sub->reuse[i] = -1;
+    for (i = 0; i < p; ++i) {
+        sub->order[i] = (uint32_t)i;
+        for (j = 0; j < i; ++j) {
+            uint32_t bit = 1u << lev[j];
+            if (pl->parent[lev[i]] & bit) sub->parent[i] |= 1u << j;
+            if (pl->lower[lev[i]] & bit) sub->lower[i] |= 1u << j;
+            if (pl->upper[lev[i]] & bit) sub->upper[i] |= 1u << j;
+            if (pl->check[lev[i]] & bit) sub->check[i] |= 1u << j;
+        }
+    }
+}
+
+static uint32_t g_sort_p; /* row width for cmp_rows (trie builds are not concurrent) */
+
+static int cmp_rows(const void *a, const void *b) {
+    const uint32_t *x = (const uint32_t *)a, *y = (const uint32_t *)b;
+    uint32_t j;
+    for (j = 0; j < g_sort_p; ++j)
+        if (x[j] != y[j]) return x[j] < y[j] ? -1 : 1;
+    return 0;
+}
+
+typedef struct {
+    uint32_t *v[SGE_MAX_K], *fc[SGE_MAX_K]; /* per depth: node values, first child (depth-local) */
+    uint32_t len[SGE_MAX_K];
+} groups_t;
+
+/* buildTrieFromSortedMatchesBatch: rows [start, end) share their first `depth`
+ * columns; one node per run of equal column-`depth` values, its children built
+ * from the run; the deepest column keeps every row. */
+static void group_rows(const uint32_t *rows, uint32_t p, uint64_t start, uint64_t end, uint32_t depth, groups_t *g) {
+    uint64_t i = start;
+    if (depth == p - 1) {
+        for (; i < end; ++i) g->v[depth][g->len[depth]++] = rows[i * p + depth];
+        return;
+    }
+    while (i < end) {
+        uint32_t key = rows[i * p + depth];
+        uint64_t g0 = i;
+        while (i < end && rows[i * p + depth] == key) ++i;
+        uint32_t a = g->len[depth]++;
+        g->v[depth][a] = key;
+        g->fc[depth][a] = g->len[depth + 1];
+        group_rows(rows, p, g0, i, depth + 1, g);
+    }
+}
+
+/* Build pl->bag's trie over the CSR; 0 on success. */
+static int bag_trie_build(const sge_plan *pl, const uint32_t *off, const uint32_t *nbr, uint32_t n, otrie_t *t) {
+    sge_plan sub;
+    groups_t g;
+    uint32_t j, p;
+    memset(t, 0, sizeof(*t));
+    memset(&g, 0, sizeof(g));
+    bag_plan(pl, &sub);
+    p = sub.k;
+    if (p < 3) return -1;
+    uint64_t N = oracle_count(&sub, off, nbr, n, 0);
+    uint32_t *rows = (uint32_t *)malloc((size_t)(N ? N : 1) * p * 4);
+    if (!rows) return -1;
+    if (oracle_enumerate(&sub, off, nbr, n, rows, N) != N) {
+        free(rows);
+        return -1;
+    }
+    g_sort_p = p;
+    qsort(rows, (size_t)N, (size_t)p * 4, cmp_rows);
+    for (j = 0; j < p; ++j) {
+        g.v[j] = (uint32_t *)malloc((size_t)(N ? N : 1) * 4);
+        g.fc[j] = (uint32_t *)malloc((size_t)(N ? N : 1) * 4);
+    }
+    group_rows(rows, p, 0, N, 0, &g);
+    free(rows);
+    t->p = p;
+    t->vb[0] = 0;
+    for (j = 0; j < p; ++j) t->vb[j + 1] = t->vb[j] + g.len[j];
+    for (j = p + 1; j <= SGE_MAX_K; ++j) t->vb[j] = t->vb[p];
+    t->val = (uint32_t *)malloc((size_t)(t->vb[p] ? t->vb[p] : 1) * 4);
+    t->child = (uint32_t *)malloc((size_t)(t->vb[p - 1] + p) * 4);
+    for (j = 0; j < p; ++j) {
+        uint32_t a;
+        for (a = 0; a < g.len[j]; ++a) {
+            t->val[t->vb[j] + a] = g.v[j][a];
+            if (j + 1 < p) t->child[t->vb[j] + j + a] = t->vb[j + 1] + g.fc[j][a];
+        }
+        if (j + 1 < p) t->child[t->vb[j] + j + g.len[j]] = t->vb[j + 1] + g.len[j + 1];
+        free(g.v[j]);
+        free(g.fc[j]);
+    }
+    return 0;
+}
+
+static void bag_trie_free(otrie_t *t) {
+    free(t->val);
+    free(t->child);
+    memset(t, 0, sizeof(*t));
+}
+
+/* The trie of pl->bag: depth returned (-1 on failure), vb[SGE_MAX_K + 1] filled;
+ * values / child copied when their capacities suffice (call with 0 to size). */
+int oracle_bag_trie(const sge_plan *pl, const uint32_t *off, const uint32_t *nbr, uint32_t n, uint32_t *vb,
+                    uint32_t *values, uint64_t values_cap, uint32_t *child, uint64_t child_cap) {
+    otrie_t t;
+    uint32_t j;
+    if (!pl->bag || bag_trie_build(pl, off, nbr, n, &t)) return -1;
+    for (j = 0; j <= SGE_MAX_K; ++j) vb[j] = t.vb[j];
+    if (values && values_cap >= t.vb[t.p]) memcpy(values, t.val, (size_t)t.vb[t.p] * 4);
+    if (child && child_cap >= (uint64_t)t.vb[t.p - 1] + t.p - 1)
+        memcpy(child, t.child, ((size_t)t.vb[t.p - 1] + t.p - 1) * 4);
+    j = t.p;
+    bag_trie_free(&t);
+    return (int)j;
+}
+
 /* Count over root tasks [tb, te) (tasks are CSR positions = directed edges).
  * threads <= 0: OpenMP default.  Returns the count (times plan multiplier). */
 uint64_t oracle_count_range(const sge_plan *pl, const uint32_t *off, const uint32_t *nbr, uint32_t n,
                             uint64_t tb, uint64_t te, int threads, uint64_t *intersections) {
     uint64_t total = 0, inter = 0;
     int k = (int)pl->k;
+    otrie_t trie;
     if (k == 1) return (uint64_t)n * pl->multiplier;
+    if (pl->bag && bag_trie_build(pl, off, nbr, n, &trie)) return ~0ull;
 #ifdef _OPENMP
     if (threads > 0) omp_set_num_threads(threads);
 #endif
@@ -368,6 +578,7 @@ uint64_t oracle_count_range(const sge_plan *pl, const uint32_t *off, const uint3
     {
         ctx_t c;
         ctx_init(&c, pl, off, nbr, n);
+        if (pl->bag) ctx_set_trie(&c, &trie);
         uint64_t t;
 #pragma omp for schedule(dynamic, 64)
         for (t = tb; t < te; ++t) {
@@ -375,6 +586,7 @@ uint64_t oracle_count_range(const sge_plan *pl, const uint32_t *off, const uint3
             if (((pl->lower[1] & 1u) && !(v1 > v0)) || ((pl->upper[1] & 1u) && !(v1 < v0))) continue;
             c.phi[0] = v0;
             c.phi[1] = v1;
+            if (!bag_root_ok(&c)) continue;
             if (k == 2) {
                 total += 1;
                 continue;
@@ -384,6 +596,7 @@ uint64_t oracle_count_range(const sge_plan *pl, const uint32_t *off, const uint3
         inter += c.intersections;
         ctx_free(&c);
     }
+    if (pl->bag) bag_trie_free(&trie);
     if (intersections) *intersections = inter;
     return total * pl->multiplier;
 }
@@ -393,7 +606,9 @@ uint64_t oracle_count_strided(const sge_plan *pl, const uint32_t *off, const uin
                               uint64_t stride, uint64_t offset, int threads) {
     uint64_t total = 0, nnz = off[n], ntask = offset < nnz ? (nnz - offset + stride - 1) / stride : 0, i;
     int k = (int)pl->k;
+    otrie_t trie;
     if (k < 2) return 0;
+    if (pl->bag && bag_trie_build(pl, off, nbr, n, &trie)) return ~0ull;
 #ifdef _OPENMP
     if (threads > 0) omp_set_num_threads(threads);
 #endif
@@ -401,6 +616,7 @@ uint64_t oracle_count_strided(const sge_plan *pl, const uint32_t *off, const uin
     {
         ctx_t c;
         ctx_init(&c, pl, off, nbr, n);
+        if (pl->bag) ctx_set_trie(&c, &trie);
 #pragma omp for schedule(dynamic, 16)
         for (i = 0; i < ntask; ++i) {
             uint64_t t = offset + i * stride;
@@ -408,10 +624,12 @@ uint64_t oracle_count_strided(const sge_plan *pl, const uint32_t *off, const uin
             if (((pl->lower[1] & 1u) && !(v1 > v0)) || ((pl->upper[1] & 1u) && !(v1 < v0))) continue;
             c.phi[0] = v0;
             c.phi[1] = v1;
+            if (!bag_root_ok(&c)) continue;
             total += (k == 2) ? 1 : dfs(&c, 2);
         }
         ctx_free(&c);
     }
+    if (pl->bag) bag_trie_free(&trie);
     return total * pl->multiplier;
 }
 
@@ -426,6 +644,7 @@ uint64_t oracle_enumerate(const sge_plan *pl, const uint32_t *off, const uint32_
     uint64_t found = 0;
     sge_plan p2 = *pl;
     p2.tail = 0; /* enumerate every member */
+    p2.bag = 0;  /* over CSR rows alone */
     if (p2.k < 3) return 0;
 #pragma omp parallel
     {

@@ -80,6 +80,20 @@ struct Decomposition {
 };
 Decomposition decompose(const Pattern &p, const std::vector<std::vector<int>> &aut);
 
+// trie.hip: level-order trie of a bag relation (device arrays, see sge_trie in sge.h)
+struct BagTrie {
+    uint32_t *val = nullptr, *child = nullptr;
+    uint32_t vb[SGE_MAX_K + 1] = {0};  // first value index of each depth
+    uint32_t depth = 0;
+    uint64_t tuples = 0;
+    uint64_t nodes[SGE_MAX_K] = {0};
+    double build_ms = 0;
+    void *desc = nullptr;  // executor's device descriptor
+};
+// rows: n x p images (row-major, device); ids < n_vertices; stream: hipStream_t
+int trie_from_rows(const uint32_t *d_rows, uint64_t n, uint32_t p, uint32_t n_vertices, void *stream, BagTrie &t);
+void trie_free(BagTrie &t);
+
 // graph_build.hip: the SGE_GRAPH_BUILD_DEVICE path (fills g->h and g's device mirror)
 int build_graph_device(uint32_t n, const EdgeVec &edges, uint32_t flags, sge_graph *g);
 
@@ -94,6 +108,8 @@ double order_cost(const Pattern &p, const GraphSummary &s, const int *order, con
                   uint32_t mode);
 int build_plan(const Pattern &p, const HostGraph *g, uint32_t mode, const uint32_t *order, sge_plan &out);
 std::vector<std::vector<int>> connected_orders(const Pattern &p, const std::vector<std::vector<int>> &aut);
+int bag_check(const sge_plan &pl, uint32_t level_mask);  // SGE_OK, or SGE_EINVAL with the reason
+void bag_subplan(const sge_plan &pl, sge_plan &sub);       // plan of pl.bag's relation (columns = bag levels)
 
 }  // namespace sge
 

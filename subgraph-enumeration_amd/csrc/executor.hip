@@ -79,6 +79,18 @@ constexpr uint32_t SEARCH_STAGE = SGE_SEARCH_STAGE;  // source elements staged i
 static_assert(SEARCH_STAGE <= (uint32_t)FSTAGE, "the intersection stage reuses the F staging words");
 constexpr uint32_t NONE = 0xffffffffu;
 
+// Bag relation as a level-order trie (built by trie.hip): depth j's values are
+// val[vb[j] .. vb[j+1]), sorted within each parent's child list; the children
+// of the depth-j value at absolute index a (j < depth-1) are
+// val[child[a + j] .. child[a + j + 1]).  The flat form of the reference's
+// nested TrieLevel (relation/relation.h:49-69).
+struct TrieDesc {
+    const uint32_t *val;
+    const uint32_t *child;
+    uint32_t vb[MAXK + 1];
+    uint32_t depth;
+};
+
 // Device copy of a plan (levels masks) plus derived fields.
 struct DPlan {
     uint32_t k, tail, mode, last, sib;  // last: level counted in closed form; sib = last - 1
@@ -96,6 +108,10 @@ struct DPlan {
     // bit2: upper bound from s, bit3: must differ from s's image.
     uint32_t tail_kind, nterms;
     uint32_t t_par[MAX_TERMS], t_lower[MAX_TERMS], t_upper[MAX_TERMS], t_check[MAX_TERMS], t_sflags[MAX_TERMS];
+    // materialised bag: levels whose candidates come from the bag trie (the
+    // trie list replaces the rows of bag parents; 0 = no bag)
+    uint32_t bag;
+    const TrieDesc *trie;
     // root-task split: split[i] = first subtask of sampled root edge i (prefix
     // over ntask + 1 entries); nsub = total subtasks.  Subtask (i, ci) of
     // split[i+1] - split[i] processes slice ci of the level-2 candidates.
@@ -166,6 +182,7 @@ struct WaveLds {
     uint32_t phi[MAXK];
     uint32_t pos[MAXK];  // next candidate position (absolute index into cand[d].ptr)
     uint32_t iend[MAXK]; // end of this wave's iteration range at each level
+    uint32_t tpos[MAXK]; // bag levels: absolute trie value index of phi[level]
     Src cand[MAXK];      // candidate set per level (whole set: reuse reads it)
     Src pre[MAXK + 1];   // exploration: parent intersections formed early ([MAXK] = F)
     Src srcs[MAXK + 2];  // sources of the intersection being formed
@@ -332,6 +349,42 @@ __device__ __forceinline__ uint32_t *buf_ptr(const KParams &P, uint32_t *wave_ba
     return wave_base + (uint64_t)i * P.buf_stride;
 }
 
+// Trie list of bag level d (unclamped): the first bag level reads the root
+// values, a later one the children of the previous bag level's trie node.
+__device__ __forceinline__ Src trie_list(const WaveLds &w, int d) {
+    const TrieDesc *T = w.pl.trie;
+    const uint32_t below = w.pl.bag & ((1u << d) - 1u);
+    Src s;
+    s.ptr = gl(T->val);
+    if (below == 0) {
+        s.b = T->vb[0];
+        s.e = T->vb[1];
+    } else {
+        const uint32_t j = __popc(below);              // depth of level d
+        const uint32_t a = w.tpos[31 - __clz(below)];  // parent node (depth j - 1)
+        s.b = T->child[a + j - 1];
+        s.e = T->child[a + j];
+    }
+    return s;
+}
+
+// Descend the trie at bag level d after phi[d] = x: record x's value index
+// (`idx` is its index in cand[d], NONE when unknown; when cand[d] is the trie
+// list itself that index is the value index).  Returns false when x is not in
+// the trie list, which only happens for the root levels (every other bag
+// level's candidates are a subset of its trie list).
+__device__ __forceinline__ bool trie_enter(WaveLds &w, int d, uint32_t x, uint32_t idx) {
+    const guint *val = gl(w.pl.trie->val);
+    if (idx != NONE && w.cand[d].ptr == val) {
+        w.tpos[d] = idx;
+        return true;
+    }
+    const Src s = trie_list(w, d);
+    const uint32_t p = lower_bound(s.ptr, s.b, s.e, x);
+    w.tpos[d] = p;
+    return p < s.e && val[p] == x;
+}
+
 // Sources of the parent intersection of level d into w.srcs, bounds applied.
 __device__ int gather_sources(const KParams &P, WaveLds &w, int d, uint32_t pmask, int reuse, bool use_pre,
                               bool has_lo, uint32_t lo, bool has_hi, uint32_t hi, Acc &acc) {
@@ -347,6 +400,15 @@ __device__ int gather_sources(const KParams &P, WaveLds &w, int d, uint32_t pmas
         pmask &= ~w.pl.parent[reuse];
         clamp_range(s.ptr, s.b, s.e, has_lo, lo, has_hi, hi);
         w.srcs[ns++] = s;
+    }
+    if ((w.pl.bag >> d) & 1u) {
+        // the trie list already satisfies every edge, bound and check among
+        // the bag's levels: the rows of bag parents are not read
+        Src s = trie_list(w, d);
+        if (w.pl.bag & ((1u << d) - 1u)) pmask &= ~w.pl.bag;
+        clamp_range(s.ptr, s.b, s.e, has_lo, lo, has_hi, hi);
+        w.srcs[ns++] = s;
+        acc.bytes += 8;
     }
     for (uint32_t m = pmask; m; m &= m - 1) {
         uint32_t v = w.phi[__builtin_ctz(m)];
@@ -812,10 +874,12 @@ __device__ __forceinline__ void run_dfs(const KParams &P, WaveLds &w, TermLds *t
         if (d == stop) {
             if (ENUM) {
                 const Src c = w.cand[d];
+                const bool descend = ((pl.bag >> d) & 1u) && (pl.bag >> (d + 1));
                 for (uint32_t i = w.pos[d]; i < w.iend[d]; ++i) {
                     uint32_t x = c.ptr[i];
                     if (is_checked(w, pl.check[d], x)) continue;
                     w.phi[d] = x;
+                    if (descend) trie_enter(w, d, x, i);
                     emit_last(P, w, wb, acc);
                 }
             } else {
@@ -843,6 +907,9 @@ __device__ __forceinline__ void run_dfs(const KParams &P, WaveLds &w, TermLds *t
         w.pos[d] = p;
         if (found) {
             w.phi[d] = x;
+            // x is in d's trie list (a source of cand[d]); a miss would leave no
+            // valid trie node to descend from, so it is skipped, never read
+            if (((pl.bag >> d) & 1u) && (pl.bag >> (d + 1)) && !trie_enter(w, d, x, p - 1)) continue;
             if (pl.mode == SGE_MODE_EXPLORE && !ENUM) explore_after(P, w, wb, d, acc);
             materialize(P, w, wb, d + 1, acc);
             ++d;
@@ -867,7 +934,6 @@ __device__ __forceinline__ void run_dfs(const KParams &P, WaveLds &w, TermLds *t
     if (P.node_cap && since && lane_id() == 0) atomicAdd(P.node_ctr, (unsigned long long)since);
 }
 
-template <bool ENUM>
 // Resident wavefronts per SIMD the register allocation is held to.  The search
 // is latency bound (dependent loads of binary searches and LDS round trips),
 // and 7 wavefronts hide more of it than the 5 that unconstrained allocation
@@ -876,6 +942,9 @@ template <bool ENUM>
 #ifndef SGE_WAVES_PER_SIMD
 #define SGE_WAVES_PER_SIMD 7
 #endif
+static_assert((sizeof(WaveLds) + MAX_TERMS * sizeof(TermLds)) * 4 * SGE_WAVES_PER_SIMD <= 160 * 1024,
+              "the resident wavefronts' LDS state must fit one CU");
+template <bool ENUM>
 __global__ void __launch_bounds__(256, SGE_WAVES_PER_SIMD) wcoj_kernel(KParams P) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     const int lane = lane_id();
@@ -927,10 +996,16 @@ __global__ void __launch_bounds__(256, SGE_WAVES_PER_SIMD) wcoj_kernel(KParams P
                 explore_after(P, w, wb, 0, acc);
                 explore_after(P, w, wb, 1, acc);
             }
-            for (int j = 2; j <= l; ++j) {
+            // trie nodes of the path's bag levels (all present: the path was live)
+            bool live = true;
+            if ((pl.bag & 1u) && !trie_enter(w, 0, w.phi[0], NONE)) live = false;
+            if (live && (pl.bag & 2u) && !trie_enter(w, 1, w.phi[1], NONE)) live = false;
+            for (int j = 2; j <= l && live; ++j) {
                 materialize(P, w, wb, j, acc);
+                if (j < l && ((pl.bag >> j) & 1u) && !trie_enter(w, j, w.phi[j], NONE)) live = false;
                 if (j < l && pl.mode == SGE_MODE_EXPLORE && !ENUM) explore_after(P, w, wb, j, acc);
             }
+            if (!live) continue;
             const uint32_t rb = it[2], re = it[3];
             w.pos[l] = max(rb, w.cand[l].b);
             w.iend[l] = min(re, w.cand[l].e);
@@ -978,9 +1053,12 @@ __global__ void __launch_bounds__(256, SGE_WAVES_PER_SIMD) wcoj_kernel(KParams P
             uint32_t v0 = P.src[t], v1 = P.nbr[t];
             if ((pl.lower[1] & 1u) && !(v1 > v0)) continue;
             if ((pl.upper[1] & 1u) && !(v1 < v0)) continue;
-            if (ci == 0) acc.tasks += 1;
             w.phi[0] = v0;
             w.phi[1] = v1;
+            // root levels of the bag: the edge must start a bag tuple
+            if ((pl.bag & 1u) && !trie_enter(w, 0, v0, NONE)) continue;
+            if ((pl.bag & 2u) && !trie_enter(w, 1, v1, NONE)) continue;
+            if (ci == 0) acc.tasks += 1;
             if (pl.k < 3) {  // an edge pattern counts its root tasks; k = 1 is counted on the host
                 if (pl.k == 2) acc.count += 1;
                 continue;
@@ -1130,6 +1208,15 @@ DPlan to_dplan(const sge_plan &p) {
     }
     d.f_parent = d.parent[L] & ~(1u << s);
     d.f_ready = d.f_parent ? 31 - __builtin_clz(d.f_parent) : 0;
+    d.bag = p.bag;
+    d.trie = nullptr;
+    if (d.bag) {
+        // a bag level's set is not a superset of another level's candidates
+        // (its trie list constrains that level alone), and a bag level's own
+        // sources are its trie list plus the rows of non-bag parents
+        for (int i = 0; i < MAXK; ++i)
+            if (((d.bag >> i) & 1u) || (d.reuse[i] >= 0 && ((d.bag >> d.reuse[i]) & 1u))) d.reuse[i] = -1;
+    }
     return d;
 }
 
@@ -1199,7 +1286,7 @@ int validate_plan(const sge_plan &p) {
             return SGE_EINVAL;
         }
     }
-    return SGE_OK;
+    return sge::bag_check(p, p.bag);
 }
 
 struct LaunchCfg {
@@ -1437,7 +1524,8 @@ int get_split(sge_graph *g, const sge_plan &p, uint64_t tb, uint64_t te, uint64_
 }
 
 int upload_plans(sge_graph *g, const sge_plan *plans, uint32_t n, bool enumerate, uint64_t tb, uint64_t te,
-                 uint64_t stride, uint32_t span, std::vector<uint64_t> &nsub) {
+                 uint64_t stride, uint32_t span, std::vector<uint64_t> &nsub,
+                 const std::vector<sge::BagTrie> *tries = nullptr) {
     DeviceCtx &c = dctx();
     std::vector<DPlan> dp(n);
     nsub.assign(n, 0);
@@ -1450,8 +1538,16 @@ int upload_plans(sge_graph *g, const sge_plan *plans, uint32_t n, bool enumerate
             p.mode = SGE_MODE_WCOJ;
             p.tail = 0;
             p.tail_kind = 0;
+            p.bag = 0;  // embeddings are enumerated over CSR rows alone
         }
         dp[i] = to_dplan(p);
+        if (p.bag) {
+            if (!tries || !(*tries)[i].desc) {
+                sge::set_error("plan has a bag but no trie was built for it");
+                return SGE_EINVAL;
+            }
+            dp[i].trie = static_cast<const TrieDesc *>((*tries)[i].desc);
+        }
         const unsigned long long *sp = nullptr;
         unsigned long long ns = 0;
         if (p.k >= 2 && ntask) {
@@ -1481,6 +1577,99 @@ void fill_stats(const PlanOut &o, double ms, sge_stats *s) {
     s->wave_cycles += o.cycles;
     s->kernel_ms += ms;
 }
+
+constexpr uint64_t BAG_TUPLE_CAP = 1ull << 28;  // tuples of one materialised bag relation
+
+// Materialise plan.bag's relation and build its trie (nodeJoin over the bag,
+// join.cpp:309-421, then buildTrie, join.cpp:1713-1717): count the relation
+// with the bag's own plan, enumerate it into exactly that many rows, sort and
+// group on the device.  Uses the context's plan/output buffers: call before the
+// caller's own plans are uploaded.
+int build_bag_trie(sge_graph *g, const sge_plan &plan, sge::BagTrie &t) {
+    auto t0 = std::chrono::steady_clock::now();
+    DeviceCtx &c = dctx();
+    sge_plan sub;
+    sge::bag_subplan(plan, sub);
+    const uint64_t nnz = g->h.nnz;
+    std::vector<uint64_t> nsub;
+    int rc = upload_plans(g, &sub, 1, false, 0, nnz, 1, span_of(nullptr), nsub);
+    if (rc) return rc;
+    LaunchCfg cfg;
+    rc = launch_cfg(nullptr, false, cfg);
+    if (rc) return rc;
+    float ms = 0;
+    rc = run_launch(g, c.plans, 1, c.out, 0, nnz, 1, nsub[0], cfg, false, nullptr, 0, nullptr, &ms);
+    if (rc) return rc;
+    PlanOut o;
+    HIP_TRY(hipMemcpy(&o, c.out, sizeof(o), hipMemcpyDeviceToHost));
+    const uint64_t N = o.count;
+    if (N > BAG_TUPLE_CAP) {
+        sge::set_error("bag relation has " + std::to_string(N) + " tuples, above the cap of " +
+                       std::to_string(BAG_TUPLE_CAP) + ": choose another bag");
+        return SGE_ERANGE;
+    }
+    uint32_t *rows = nullptr;
+    unsigned long long *d_n = nullptr;
+    HIP_TRY(hipMalloc(&rows, sizeof(uint32_t) * std::max<uint64_t>(N * sub.k, 1)));
+    struct Free {
+        void *a, *b;
+        ~Free() {
+            if (a) (void)hipFree(a);
+            if (b) (void)hipFree(b);
+        }
+    } fr{rows, nullptr};
+    HIP_TRY(hipMalloc(&d_n, sizeof(unsigned long long)));
+    fr.b = d_n;
+    if (N) {
+        rc = upload_plans(g, &sub, 1, true, 0, nnz, 1, span_of(nullptr), nsub);
+        if (rc) return rc;
+        LaunchCfg ecfg;
+        rc = launch_cfg(nullptr, true, ecfg);
+        if (rc) return rc;
+        rc = run_launch(g, c.plans, 1, c.out, 0, nnz, 1, nsub[0], ecfg, true, rows, N, d_n, &ms);
+        if (rc) return rc;
+        unsigned long long got = 0;
+        HIP_TRY(hipMemcpy(&got, d_n, sizeof(got), hipMemcpyDeviceToHost));
+        if (got != N) {
+            sge::set_error("bag relation: enumerated " + std::to_string(got) + " tuples, counted " + std::to_string(N));
+            return SGE_EDEVICE;
+        }
+    }
+    rc = sge::trie_from_rows(rows, N, sub.k, g->h.n, c.stream, t);
+    if (rc) return rc;
+    TrieDesc d;
+    std::memset(&d, 0, sizeof(d));
+    d.val = t.val;
+    d.child = t.child;
+    for (int j = 0; j <= MAXK; ++j) d.vb[j] = t.vb[j];
+    d.depth = t.depth;
+    HIP_TRY(hipMalloc(&t.desc, sizeof(TrieDesc)));
+    HIP_TRY(hipMemcpy(t.desc, &d, sizeof(d), hipMemcpyHostToDevice));
+    t.build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return SGE_OK;
+}
+
+// Tries of every plan that has a bag (entries of bagless plans stay empty).
+struct TrieSet {
+    std::vector<sge::BagTrie> t;
+    ~TrieSet() {
+        for (auto &x : t) sge::trie_free(x);
+    }
+    int build(sge_graph *g, const sge_plan *plans, uint32_t n, double *ms_total, std::vector<double> *ms_each) {
+        t.assign(n, sge::BagTrie());
+        if (ms_each) ms_each->assign(n, 0.0);
+        for (uint32_t i = 0; i < n; ++i) {
+            if (!plans[i].bag) continue;
+            int rc = validate_plan(plans[i]);
+            if (rc) return rc;
+            rc = build_bag_trie(g, plans[i], t[i]);
+            if (rc) return rc;
+            if (ms_total) *ms_total += t[i].build_ms;
+            if (ms_each) (*ms_each)[i] = t[i].build_ms;
+        }
+        return SGE_OK;
+    }
+};
 
 }  // namespace
 
@@ -1542,16 +1731,24 @@ static int count_impl(sge_graph *g, const sge_plan *plans, uint32_t nplans, uint
     if (te > g->h.nnz) te = g->h.nnz;
     if (tb > te) tb = te;
     if (stride == 0) stride = 1;
+    if (stats) std::memset(stats, 0, sizeof(*stats));
+    if (per_plan) std::memset(per_plan, 0, sizeof(sge_stats) * nplans);
+    TrieSet tries;
+    double trie_ms = 0;
+    std::vector<double> trie_each;
+    rc = tries.build(g, plans, nplans, &trie_ms, &trie_each);
+    if (rc) return rc;
+    if (stats) stats->trie_ms = trie_ms;
+    if (per_plan)
+        for (uint32_t i = 0; i < nplans; ++i) per_plan[i].trie_ms = trie_each[i];
     std::vector<uint64_t> nsub;
-    rc = upload_plans(g, plans, nplans, false, tb, te, stride, span_of(tune), nsub);
+    rc = upload_plans(g, plans, nplans, false, tb, te, stride, span_of(tune), nsub, &tries.t);
     if (rc) return rc;
     LaunchCfg cfg;
     rc = launch_cfg(tune, false, cfg);
     if (rc) return rc;
     uint64_t max_sub = 0;
     for (uint64_t x : nsub) max_sub = std::max(max_sub, x);
-    if (stats) std::memset(stats, 0, sizeof(*stats));
-    if (per_plan) std::memset(per_plan, 0, sizeof(sge_stats) * nplans);
     std::vector<PlanOut> host(nplans);
     std::vector<float> ms(nplans, 0.f);
     bool any_k1 = false, limited = false;
@@ -1687,10 +1884,69 @@ extern "C" int sge_enumerate(sge_graph *g, const sge_plan *plan, uint32_t *out, 
     return SGE_OK;
 }
 
+// ------------------------------------------------------------- bag tries
+struct sge_trie {
+    sge::BagTrie t;
+    ~sge_trie() { sge::trie_free(t); }
+};
+
+extern "C" int sge_trie_build(sge_graph *g, const sge_plan *plan, sge_trie **out) {
+    if (!g || !plan || !out) {
+        sge::set_error("sge_trie_build: null argument");
+        return SGE_EINVAL;
+    }
+    int rc = validate_plan(*plan);
+    if (rc) return rc;
+    if (!plan->bag) {
+        sge::set_error("sge_trie_build: the plan has no bag (sge_plan_set_bag)");
+        return SGE_EINVAL;
+    }
+    rc = sge_graph_to_device(g);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lk(dmutex());
+    sge_trie *t = new sge_trie();
+    rc = build_bag_trie(g, *plan, t->t);
+    if (rc) {
+        delete t;
+        return rc;
+    }
+    *out = t;
+    return SGE_OK;
+}
+
+extern "C" int sge_trie_shape(const sge_trie *t, uint32_t *depth, uint64_t *tuples, uint64_t *nodes, double *build_ms) {
+    if (!t) {
+        sge::set_error("sge_trie_shape: null trie");
+        return SGE_EINVAL;
+    }
+    if (depth) *depth = t->t.depth;
+    if (tuples) *tuples = t->t.tuples;
+    if (nodes)
+        for (int j = 0; j < MAXK; ++j) nodes[j] = j < (int)t->t.depth ? t->t.nodes[j] : 0;
+    if (build_ms) *build_ms = t->t.build_ms;
+    return SGE_OK;
+}
+
+extern "C" int sge_trie_copy(const sge_trie *t, uint32_t *values, uint32_t *child) {
+    if (!t || !t->t.depth) {
+        sge::set_error("sge_trie_copy: null trie");
+        return SGE_EINVAL;
+    }
+    const uint32_t p = t->t.depth;
+    const uint64_t nval = t->t.vb[p], nchild = (uint64_t)t->t.vb[p - 1] + (p - 1);
+    if (values && nval) HIP_TRY(hipMemcpy(values, t->t.val, sizeof(uint32_t) * nval, hipMemcpyDeviceToHost));
+    if (child && nchild) HIP_TRY(hipMemcpy(child, t->t.child, sizeof(uint32_t) * nchild, hipMemcpyDeviceToHost));
+    return SGE_OK;
+}
+
+extern "C" void sge_trie_destroy(sge_trie *t) { delete t; }
+
 // ---------------------------------------------------------------- batches
 struct sge_batch {
     sge_graph *g = nullptr;
     std::vector<sge_plan> plans;
+    TrieSet tries;        // bag tries the plans read (built at creation)
+    double trie_ms = 0;
     DPlan *dplans = nullptr;
     PlanOut *dout = nullptr;
     unsigned long long *queue = nullptr;
@@ -1748,6 +2004,12 @@ extern "C" int sge_batch_create_sampled(sge_graph *g, const sge_plan *plans, uin
         }
         dp[i] = to_dplan(plans[i]);
     }
+    TrieSet tries;
+    double trie_ms = 0;
+    rc = tries.build(g, plans, nplans, &trie_ms, nullptr);
+    if (rc) return rc;
+    for (uint32_t i = 0; i < nplans; ++i)
+        if (plans[i].bag) dp[i].trie = static_cast<const TrieDesc *>(tries.t[i].desc);
     LaunchCfg cfg;
     rc = launch_cfg(tune, false, cfg);
     if (rc) return rc;
@@ -1767,6 +2029,8 @@ extern "C" int sge_batch_create_sampled(sge_graph *g, const sge_plan *plans, uin
     sge_batch *b = new sge_batch();
     b->g = g;
     b->plans.assign(plans, plans + nplans);
+    b->tries.t.swap(tries.t);
+    b->trie_ms = trie_ms;
     KParams kp;
     std::memset(&kp, 0, sizeof(kp));
     uint32_t bstride = ((g->h.max_degree + 64 + 63) / 64) * 64;
@@ -1869,6 +2133,7 @@ extern "C" int sge_batch_run(sge_batch *b, uint64_t *counts, sge_stats *stats) {
         std::memset(stats, 0, sizeof(*stats));
         for (auto &o : host) fill_stats(o, 0.0, stats);
         stats->kernel_ms = ms;
+        stats->trie_ms = b->trie_ms;
     }
     for (uint32_t i = 0; i < n; ++i) counts[i] = host[i].count * (b->plans[i].multiplier ? b->plans[i].multiplier : 1);
     if (stats) stats->wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();

@@ -373,9 +373,116 @@ int build_plan(const Pattern &p, const HostGraph *g, uint32_t mode, const uint32
     return SGE_OK;
 }
 
+// ---- materialised bags
+// The level at which the closed-form count starts (every level below it is
+// enumerated).
+static uint32_t first_closed_level(const sge_plan &pl) {
+    return (pl.mode == SGE_MODE_IEP && pl.tail >= 2) ? pl.k - pl.tail : pl.k - 1;
+}
+
+int bag_check(const sge_plan &pl, uint32_t B) {
+    if (B == 0) return SGE_OK;
+    if (pl.mode == SGE_MODE_EXPLORE) {
+        set_error("bag: exploration mode forms parent intersections early and takes no bags");
+        return SGE_EINVAL;
+    }
+    if (__builtin_popcount(B) < 3) {
+        set_error("bag: a materialised bag needs at least 3 vertices");
+        return SGE_EINVAL;
+    }
+    if (pl.k < 4 || (B >> first_closed_level(pl))) {
+        set_error("bag: every bag level must lie below the closed-form level(s)");
+        return SGE_EINVAL;
+    }
+    const int b0 = __builtin_ctz(B);
+    const uint32_t rest = B & (B - 1);
+    const int b1 = __builtin_ctz(rest);
+    if (!((pl.parent[b1] >> b0) & 1u)) {
+        set_error("bag: the first two bag levels must be adjacent");
+        return SGE_EINVAL;
+    }
+    for (uint32_t m = rest & (rest - 1); m; m &= m - 1) {
+        const int l = __builtin_ctz(m);
+        if (!(pl.parent[l] & B)) {
+            set_error("bag: every bag level after the second needs an earlier neighbour in the bag");
+            return SGE_EINVAL;
+        }
+    }
+    return SGE_OK;
+}
+
+// The bag relation's own plan: bag levels in plan order, masks restricted to
+// the bag and renumbered, no closed-form tail, output column i = bag level i.
+void bag_subplan(const sge_plan &pl, sge_plan &sub) {
+    std::memset(&sub, 0, sizeof(sub));
+    int lev[SGE_MAX_K], p = 0;
+    for (uint32_t m = pl.bag; m; m &= m - 1) lev[p++] = __builtin_ctz(m);
+    auto remap = [&](uint32_t mask) {
+        uint32_t r = 0;
+        for (int i = 0; i < p; ++i)
+            if ((mask >> lev[i]) & 1u) r |= 1u << i;
+        return r;
+    };
+    sub.k = (uint32_t)p;
+    sub.mode = SGE_MODE_WCOJ;
+    for (int i = 0; i < SGE_MAX_K; ++i) sub.reuse[i] = -1;
+    for (int i = 0; i < p; ++i) {
+        sub.order[i] = (uint32_t)i;
+        sub.parent[i] = remap(pl.parent[lev[i]]);
+        sub.lower[i] = remap(pl.lower[lev[i]]);
+        sub.upper[i] = remap(pl.upper[lev[i]]);
+        sub.check[i] = remap(pl.check[lev[i]]);
+    }
+    sub.num_bags = 1;
+    sub.multiplier = 1;
+    sub.aut_size = 1;
+}
+
 }  // namespace sge
 
 using namespace sge;
+
+extern "C" int sge_plan_set_bag(sge_plan *plan, uint32_t vertex_mask) {
+    if (!plan || plan->k < 1 || plan->k > SGE_MAX_K) {
+        set_error("sge_plan_set_bag: bad plan");
+        return SGE_EINVAL;
+    }
+    if (vertex_mask >> plan->k) {
+        set_error("sge_plan_set_bag: vertex mask names vertices beyond k");
+        return SGE_EINVAL;
+    }
+    uint32_t B = 0;
+    for (uint32_t i = 0; i < plan->k; ++i)
+        if ((vertex_mask >> plan->order[i]) & 1u) B |= 1u << i;
+    int rc = bag_check(*plan, B);
+    if (rc) return rc;
+    plan->bag = B;
+    return SGE_OK;
+}
+
+extern "C" int sge_plan_bags(const sge_pattern *cp, const sge_plan *plan, uint32_t *masks, uint32_t cap,
+                             uint32_t *n) {
+    if (!cp || !plan || !n || (cap && !masks)) {
+        set_error("sge_plan_bags: null argument");
+        return SGE_EINVAL;
+    }
+    Pattern p = from_c(*cp);
+    if ((uint32_t)p.k != plan->k) {
+        set_error("sge_plan_bags: plan and pattern sizes differ");
+        return SGE_EINVAL;
+    }
+    Decomposition d = decompose(p, automorphisms(p));
+    uint32_t cnt = 0;
+    for (uint32_t vm : d.bags) {
+        sge_plan trial = *plan;
+        if (sge_plan_set_bag(&trial, vm) != SGE_OK) continue;
+        if (cnt < cap) masks[cnt] = vm;
+        ++cnt;
+    }
+    set_error("");
+    *n = cnt;
+    return SGE_OK;
+}
 
 extern "C" int sge_plan_build(const sge_pattern *p, const sge_graph *g, uint32_t mode, const uint32_t *order,
                               sge_plan *out) {

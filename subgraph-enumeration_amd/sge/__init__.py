@@ -53,7 +53,7 @@ class Plan(C.Structure):
     _fields_ = [("k", C.c_uint32), ("mode", C.c_uint32), ("order", C.c_uint32 * MAX_K),
                 ("parent", C.c_uint32 * MAX_K), ("lower", C.c_uint32 * MAX_K), ("upper", C.c_uint32 * MAX_K),
                 ("check", C.c_uint32 * MAX_K), ("reuse", C.c_int32 * MAX_K), ("ready", C.c_uint32 * MAX_K),
-                ("tail", C.c_uint32), ("tail_kind", C.c_uint32), ("num_bags", C.c_uint32),
+                ("tail", C.c_uint32), ("tail_kind", C.c_uint32), ("bag", C.c_uint32), ("num_bags", C.c_uint32),
                 ("multiplier", C.c_uint64),
                 ("aut_size", C.c_uint64), ("cost", C.c_double)]
 
@@ -62,7 +62,7 @@ class Plan(C.Structure):
         return {"k": k, "mode": self.mode, "order": list(self.order)[:k], "parent": list(self.parent)[:k],
                 "lower": list(self.lower)[:k], "upper": list(self.upper)[:k], "check": list(self.check)[:k],
                 "reuse": list(self.reuse)[:k], "ready": list(self.ready)[:k], "tail": self.tail,
-                "tail_kind": self.tail_kind,
+                "tail_kind": self.tail_kind, "bag": self.bag,
                 "num_bags": self.num_bags, "multiplier": self.multiplier, "aut_size": self.aut_size,
                 "cost": self.cost}
 
@@ -77,7 +77,7 @@ class Tuning(C.Structure):
 class Stats(C.Structure):
     _fields_ = [("intersections", C.c_uint64), ("bytes", C.c_uint64), ("tasks", C.c_uint64),
                 ("nodes", C.c_uint64), ("kernel_ms", C.c_double), ("wall_ms", C.c_double),
-                ("wave_cycles", C.c_uint64)]
+                ("wave_cycles", C.c_uint64), ("trie_ms", C.c_double)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -114,6 +114,14 @@ _SIGS = {
     "sge_plan_rank_orders": (C.c_int, [C.POINTER(CPattern), C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p,
                                        C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32)]),
     "sge_plan_text": (C.c_int, [C.POINTER(Plan), C.c_char_p, C.c_size_t]),
+    "sge_plan_set_bag": (C.c_int, [C.POINTER(Plan), C.c_uint32]),
+    "sge_plan_bags": (C.c_int, [C.POINTER(CPattern), C.POINTER(Plan), C.c_void_p, C.c_uint32,
+                                C.POINTER(C.c_uint32)]),
+    "sge_trie_build": (C.c_int, [C.c_void_p, C.POINTER(Plan), C.POINTER(C.c_void_p)]),
+    "sge_trie_shape": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint64), C.c_void_p,
+                                 C.POINTER(C.c_double)]),
+    "sge_trie_copy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "sge_trie_destroy": (None, [C.c_void_p]),
     "sge_tuning_default": (None, [C.POINTER(Tuning)]),
     "sge_count": (C.c_int, [C.c_void_p, C.POINTER(Plan), C.c_uint32, C.c_uint32, C.POINTER(Tuning), C.c_void_p,
                             C.POINTER(Stats), C.POINTER(Stats)]),
@@ -336,6 +344,76 @@ def order_cost(pattern: Pattern, graph: Optional[Graph], order: Sequence[int], m
     _check(lib().sge_plan_cost(C.byref(pattern.c), graph._h if graph else None, mode, _ptr(o), C.byref(c)),
            "sge_plan_cost")
     return c.value
+
+
+def set_bag(plan: Plan, vertices) -> Plan:
+    """Copy of `plan` with the pattern vertices `vertices` (iterable or bit
+    mask; 0 / empty clears) as its materialised bag.  Raises SgeError when the
+    bag is not valid for the plan's order."""
+    mask = vertices if isinstance(vertices, int) else sum(1 << int(v) for v in vertices)
+    q = Plan.from_buffer_copy(plan)
+    _check(lib().sge_plan_set_bag(C.byref(q), mask), "sge_plan_set_bag")
+    return q
+
+
+def plan_bags(pattern: Pattern, plan: Plan) -> List[int]:
+    """Vertex masks of the decomposition's bags that are valid bags of `plan`."""
+    out = np.zeros(16, dtype=np.uint32)
+    n = C.c_uint32()
+    _check(lib().sge_plan_bags(C.byref(pattern.c), C.byref(plan), _ptr(out), 16, C.byref(n)), "sge_plan_bags")
+    return [int(x) for x in out[: min(n.value, 16)]]
+
+
+def valid_bags(plan: Plan) -> List[int]:
+    """Every vertex mask that sge_plan_set_bag accepts for `plan`."""
+    out = []
+    for m in range(1, 1 << plan.k):
+        if bin(m).count("1") < 3:
+            continue
+        q = Plan.from_buffer_copy(plan)
+        if lib().sge_plan_set_bag(C.byref(q), m) == 0:
+            out.append(m)
+    return out
+
+
+class Trie:
+    """Level-order trie of a plan's bag relation, built on the device.
+    values[vb[j]:vb[j+1]] are depth j's node values; the children of the node
+    at value index a of depth j are values[child[a + j]:child[a + j + 1]]."""
+
+    def __init__(self, graph: Graph, plan: Plan):
+        self._h = C.c_void_p()
+        _check(lib().sge_trie_build(graph._h, C.byref(plan), C.byref(self._h)), "sge_trie_build")
+        depth, tuples, ms = C.c_uint32(), C.c_uint64(), C.c_double()
+        nodes = np.zeros(MAX_K, dtype=np.uint64)
+        _check(lib().sge_trie_shape(self._h, C.byref(depth), C.byref(tuples), _ptr(nodes), C.byref(ms)),
+               "sge_trie_shape")
+        self.depth = depth.value
+        self.tuples = tuples.value
+        self.nodes = [int(x) for x in nodes[: self.depth]]
+        self.build_ms = ms.value
+        self.vb = [0]
+        for x in self.nodes:
+            self.vb.append(self.vb[-1] + x)
+
+    def arrays(self):
+        """(values, child) as numpy u32 arrays."""
+        p = self.depth
+        values = np.zeros(max(self.vb[p], 1), dtype=np.uint32)
+        child = np.zeros(max(self.vb[p - 1] + p - 1, 1), dtype=np.uint32)
+        _check(lib().sge_trie_copy(self._h, _ptr(values), _ptr(child)), "sge_trie_copy")
+        return values[: self.vb[p]], child[: self.vb[p - 1] + p - 1]
+
+    def close(self):
+        if self._h:
+            lib().sge_trie_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def plan_text(plan: Plan) -> str:

@@ -380,3 +380,120 @@ def test_seven_vertex_set_rmat8_batched_goldens():
     for i, (c, r) in enumerate(zip(got, iep)):
         if i not in REF_IEP_OVERCOUNTS:
             assert c == r["iep"]["count"], i
+
+
+# ---------------------------------------------------------------- materialised bags
+def spread(bags, cap):
+    return bags[::max(1, len(bags) // cap)][:cap]
+
+
+def smallest_bags(plan, cap):
+    return sorted(sge.valid_bags(plan), key=lambda b: (bin(b).count("1"), b))[:cap]
+
+
+def test_bag_trie_matches_oracle():
+    """Device trie (enumerate, radix sort, per-depth scans) = the oracle's
+    recursive grouping of the sorted relation, array for array."""
+    for gname, g in (("c1", W.graph_c1()), ("random60", random_graph(60, 0.2, 7))):
+        off, nbr = g.csr()
+        checked = 0
+        for p in small_patterns() + W.six_vertex_set()[::3] + W.seven_vertex_set()[::13]:
+            if p.k < 4:
+                continue
+            for mode in (sge.MODE_WCOJ, sge.MODE_IEP):
+                plan = sge.build_plan(p, g, mode)
+                for b in spread(sge.valid_bags(plan), 3):
+                    q = sge.set_bag(plan, b)
+                    say("trie", gname, p.edges(), "mode", mode, "bag", b)
+                    t = sge.Trie(g, q)
+                    vb, val, child = O.bag_trie(q, off, nbr)
+                    values, ch = t.arrays()
+                    assert t.vb == vb and t.tuples == vb[-1] - vb[-2]
+                    assert np.array_equal(values, val) and np.array_equal(ch, child)
+                    t.close()
+                    checked += 1
+        assert checked >= 20
+
+
+def test_bag_counts_match_oracle_c1():
+    g = W.graph_c1()
+    off, nbr = g.csr()
+    for p in W.six_vertex_set() + W.seven_vertex_set()[::5] + small_patterns():
+        for mode in (sge.MODE_WCOJ, sge.MODE_IEP):
+            plan = sge.build_plan(p, g, mode)
+            want = O.count(plan, off, nbr)
+            for b in spread(sge.valid_bags(plan), 4):
+                say("bag count c1", p.edges(), "mode", mode, "bag", b)
+                got, st = sge.count(g, sge.set_bag(plan, b))
+                assert got[0] == want, (p.edges(), mode, b)
+                assert st.trie_ms > 0
+
+
+def test_bag_c2_goldens_plain_and_batched():
+    """The 16 six-vertex patterns over the Chung-Lu graph n=7,115 with a
+    three-vertex bag each (IEP mode), plain and batched, equal the reference's
+    counts; so do house and K5-e in WCOJ and IEP mode with every 3-vertex bag."""
+    g = W.graph_c2()
+    gold3 = golden("c3")["patterns"]
+    pats = W.six_vertex_set()
+    plans = []
+    for rec in gold3:
+        plan = sge.build_plan(pats[rec["index"]], g, sge.MODE_IEP)
+        bags = smallest_bags(plan, 1)
+        plans.append(sge.set_bag(plan, bags[0]) if bags else plan)
+    assert sum(1 for pl in plans if pl.bag) >= 8
+    say("c2 six-vertex with bags, plain")
+    plain, st = sge.count(g, plans)
+    assert plain == [r["iep"]["count"] for r in gold3]
+    say("c2 six-vertex with bags, batched")
+    b = sge.Batch(g, plans)
+    for rep in range(2):
+        got, st = b.run()
+        assert got == plain
+        assert st.trie_ms > 0
+    b.close()
+    gold2 = golden("c2")["patterns"]
+    for name in ["near5clique", "house"]:
+        for mode, key in ((sge.MODE_WCOJ, "wcoj"), (sge.MODE_IEP, "iep")):
+            plan = sge.build_plan(W.named(name), g, mode)
+            for bag in [b for b in sge.valid_bags(plan) if bin(b).count("1") == 3]:
+                say("c2", name, key, "bag", bag)
+                assert sge.count(g, sge.set_bag(plan, bag))[0][0] == gold2[name][key]["count"]
+
+
+def test_bag_with_donation_and_tuning_variants():
+    """Continuation items rebuild their path's trie positions: tiny donation
+    budgets and unusual launch shapes give the same counts."""
+    g = W.graph_c1()
+    plans = []
+    for p in W.six_vertex_set()[::2] + W.seven_vertex_set()[::11]:
+        plan = sge.build_plan(p, g, sge.MODE_WCOJ)
+        bags = spread(sge.valid_bags(plan), 2)
+        plans += [sge.set_bag(plan, b) for b in bags]
+    want, _ = sge.count(g, [sge.set_bag(pl, 0) for pl in plans])
+    for bt, ch, budget, span in [(64, 1, 16, 8), (256, 3, 64, 1 << 30), (128, 8, 1, 4)]:
+        t = sge.default_tuning()
+        t.block_threads, t.chunk, t.donate_budget, t.split_span = bt, ch, budget, span
+        say("bag tuning", bt, ch, budget, span)
+        assert sge.count(g, plans, batched=True, tuning=t)[0] == want
+        assert sge.count(g, plans, batched=False, tuning=t)[0] == want
+
+
+def test_bag_rmat8_seven_vertex_goldens():
+    gold = golden("rmat7")["patterns"]
+    g = W.graph_rmat(8, 7)
+    pats = W.seven_vertex_set()
+    for rec in gold[::4]:
+        plan = sge.build_plan(pats[rec["index"]], g, sge.MODE_IEP)
+        for b in spread(smallest_bags(plan, 6), 2):
+            say("rmat8 bag", rec["index"], b)
+            assert sge.count(g, sge.set_bag(plan, b))[0][0] == rec["wcoj"]["count"], (rec["index"], b)
+
+
+def test_bag_enumeration_unchanged():
+    g = W.graph_c1()
+    plan = sge.build_plan(W.named("house"), g, sge.MODE_WCOJ, [2, 3, 4, 0, 1])
+    house = W.named("house")
+    a = sge.enumerate_embeddings(g, plan, house, 100000)
+    b = sge.enumerate_embeddings(g, sge.set_bag(plan, [2, 3, 4]), house, 100000)
+    assert len(a) == sge.count(g, plan)[0][0] and np.array_equal(a, b)

@@ -178,3 +178,28 @@ def test_count_without_gpu_fails_loudly():
         sge.count(g, sge.build_plan(W.named("diamond"), g))
     with pytest.raises(sge.SgeError):
         sge.Graph.from_edges(4, [(0, 1), (1, 2)], device_build=True)
+
+
+def test_bag_validation():
+    g = W.graph_c1()
+    house = W.named("house")  # square 0-1-3-2, roof 4 on 2 and 3
+    plan = sge.build_plan(house, g, sge.MODE_WCOJ, [2, 3, 4, 0, 1])  # vertex 1 is the last level
+    q = sge.set_bag(plan, [2, 3, 4])              # the roof triangle: levels 0, 1, 2
+    assert q.bag == 0b111 and plan.bag == 0
+    assert sge.set_bag(q, 0).bag == 0            # cleared
+    assert sge.set_bag(plan, [2, 4, 0]).bag == 0b1101  # levels 0, 2, 3: 2-4 adjacent, 0 adjacent to 2
+    for bad in ([2, 3],          # fewer than 3 vertices
+                [2, 3, 1],       # vertex 1 is the closed-form level
+                [3, 4, 0]):      # vertex 0 has no neighbour among 3 and 4
+        with pytest.raises(sge.SgeError):
+            sge.set_bag(plan, bad)
+    with pytest.raises(sge.SgeError):  # first two bag levels (vertices 1 and 2) not adjacent
+        sge.set_bag(sge.build_plan(house, g, sge.MODE_WCOJ, [0, 1, 2, 3, 4]), [1, 2, 3])
+    with pytest.raises(sge.SgeError):  # exploration mode takes no bags
+        sge.set_bag(sge.build_plan(house, g, sge.MODE_EXPLORE, [2, 3, 4, 0, 1]), [2, 3, 4])
+    # every bag the decomposition proposes is accepted
+    for p in W.six_vertex_set():
+        pl = sge.build_plan(p, g, sge.MODE_IEP)
+        for b in sge.plan_bags(p, pl):
+            assert b in sge.valid_bags(pl)
+            assert sge.set_bag(pl, b).bag

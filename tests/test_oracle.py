@@ -217,3 +217,110 @@ def test_iep_mode_counts_subgraphs_on_the_reference_divergence():
     assert (4 in [v for e in p.edges() for v in e]) and not any(set(e) == {4, 6} for e in p.edges())
     for mode in (sge.MODE_IEP, sge.MODE_WCOJ):
         assert O.count(sge.build_plan(p, g, mode), off, nbr) == gold[19]["wcoj"]["count"]
+
+
+# ---------------------------------------------------------------- materialised bags
+def bag_levels(plan):
+    return [i for i in range(plan.k) if (plan.bag >> i) & 1]
+
+
+def brute_relation(plan, adj, n):
+    """Every assignment of the bag's levels (in level order) with the plan's
+    edges and bounds among them and pairwise distinct images, by plain
+    backtracking over all vertices."""
+    lev = bag_levels(plan)
+    out = []
+
+    def rec(i, img):
+        if i == len(lev):
+            out.append(list(img))
+            return
+        L = lev[i]
+        for v in range(n):
+            ok = True
+            for j in range(i):
+                a, x = lev[j], img[j]
+                if v == x or ((plan.parent[L] >> a) & 1 and v not in adj[x]) or \
+                        ((plan.lower[L] >> a) & 1 and not v > x) or ((plan.upper[L] >> a) & 1 and not v < x):
+                    ok = False
+                    break
+            if ok:
+                img.append(v)
+                rec(i + 1, img)
+                img.pop()
+
+    rec(0, [])
+    return out
+
+
+def walk_trie(vb, val, child):
+    """Root-to-leaf paths of a level-order trie, checking that every child
+    list is strictly increasing and that child ranges tile each depth."""
+    P = len(vb) - 1
+    assert np.all(np.diff(val[vb[0]:vb[1]].astype(np.int64)) > 0)
+    for j in range(P - 1):
+        c = child[vb[j] + j: vb[j + 1] + j + 1].astype(np.int64)
+        assert c[0] == vb[j + 1] and c[-1] == vb[j + 2] and np.all(np.diff(c) > 0)
+        for a in range(vb[j], vb[j + 1]):
+            seg = val[child[a + j]:child[a + j + 1]].astype(np.int64)
+            assert np.all(np.diff(seg) > 0)
+    out = []
+
+    def rec(j, a, pre):
+        pre = pre + [int(val[a])]
+        if j == P - 1:
+            out.append(pre)
+            return
+        for b in range(child[a + j], child[a + j + 1]):
+            rec(j + 1, b, pre)
+
+    for a in range(vb[0], vb[1]):
+        rec(0, a, [])
+    return out
+
+
+def some_bags(plan, cap=4):
+    bags = sge.valid_bags(plan)
+    step = max(1, len(bags) // cap)
+    return bags[::step][:cap]
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_bag_trie_is_the_brute_force_relation(seed):
+    g = random_graph(18, 0.4, 40 + seed)
+    off, nbr = g.csr()
+    n = len(off) - 1
+    adj = [set(nbr[off[v]:off[v + 1]].tolist()) for v in range(n)]
+    checked = 0
+    for p in small_patterns() + W.six_vertex_set()[::5]:
+        for mode in (sge.MODE_WCOJ, sge.MODE_IEP):
+            plan = sge.build_plan(p, g, mode)
+            for b in some_bags(plan, 3):
+                q = sge.set_bag(plan, b)
+                vb, val, child = O.bag_trie(q, off, nbr)
+                assert len(vb) - 1 == len(bag_levels(q))
+                assert walk_trie(vb, val, child) == sorted(brute_relation(q, adj, n)), (p.edges(), mode, b)
+                checked += 1
+    assert checked >= 10
+
+
+def test_bag_join_counts_equal_plain_counts():
+    """A bag changes how candidates are formed, never the count: the oracle's
+    trie join equals its plain join (and the reference goldens) for every
+    sampled valid bag."""
+    g = W.graph_c1()
+    off, nbr = g.csr()
+    gold = load_golden("c1")
+    checked = 0
+    for p in W.six_vertex_set() + W.seven_vertex_set()[::6] + [W.named(k) for k in gold["patterns"]]:
+        for mode in (sge.MODE_WCOJ, sge.MODE_IEP):
+            plan = sge.build_plan(p, g, mode)
+            base = O.count(plan, off, nbr)
+            for b in some_bags(plan, 4):
+                assert O.count(sge.set_bag(plan, b), off, nbr) == base, (p.edges(), mode, b)
+                checked += 1
+    for name, rec in gold["patterns"].items():
+        plan = sge.build_plan(W.named(name), g)
+        for b in sge.valid_bags(plan):
+            assert O.count(sge.set_bag(plan, b), off, nbr) == rec["wcoj"]["count"], (name, b)
+    assert checked >= 50
