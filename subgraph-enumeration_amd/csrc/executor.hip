@@ -937,7 +937,7 @@ __device__ __forceinline__ void run_dfs(const KParams &P, WaveLds &w, TermLds *t
 // Resident wavefronts per SIMD the register allocation is held to.  The search
 // is latency bound (dependent loads of binary searches and LDS round trips),
 // and 7 wavefronts hide more of it than the 5 that unconstrained allocation
-// gives, despite 128 B/lane of spills; FSTAGE is sized so that 28 wavefronts'
+// gives, despite 144 B/lane of spills; FSTAGE is sized so that 28 wavefronts'
 // LDS fits a CU's 160 KB.
 #ifndef SGE_WAVES_PER_SIMD
 #define SGE_WAVES_PER_SIMD 7

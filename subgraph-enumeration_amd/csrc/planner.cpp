@@ -576,6 +576,12 @@ extern "C" int sge_plan_text(const sge_plan *pl, char *buf, size_t cap) {
             if ((pl->lower[j] >> i) & 1u) larger += std::to_string(pl->order[j]) + ",";
         if (!larger.empty()) s += std::to_string(pl->order[i]) + " < " + larger + "\n";
     }
+    if (pl->bag) {
+        s += "materialised bag:";
+        for (uint32_t i = 0; i < pl->k; ++i)
+            if ((pl->bag >> i) & 1u) s += " " + std::to_string(pl->order[i]);
+        s += "\n";
+    }
     if (s.size() + 1 > cap) {
         set_error("sge_plan_text: buffer too small");
         return SGE_ERANGE;

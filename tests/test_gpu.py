@@ -350,6 +350,22 @@ def test_cli_drop_in(tmp_path):
             assert r.returncode == 0, r.stderr
             lines = [l for l in r.stdout.splitlines() if l.startswith("Number of matches:")]
             assert lines and int(lines[0].split(":")[1]) == gold[name]["wcoj"]["count"], (name, flags, r.stdout)
+    # --bag: the same count through a materialised bag, reported in the plan text
+    ph = sge.Pattern.from_edges(gold["house"]["k"], [tuple(e) for e in gold["house"]["edges"]])
+    for mode_flags, mode in (([], sge.MODE_WCOJ), (["-u"], sge.MODE_IEP)):
+        plan = sge.build_plan(ph, g, mode)
+        bags = sge.valid_bags(plan)
+        assert bags
+        for bag in bags[:2]:
+            verts = [v for v in range(ph.k) if (bag >> v) & 1]
+            args = mode_flags + ["--order", ",".join(str(v) for v in list(plan.order)[:plan.k]),
+                                 "--bag", ",".join(map(str, verts))]
+            say("cli bag", args)
+            r = subprocess.run([cli, "-q", str(tmp_path / "house.txt"), "-d", dpath] + args, capture_output=True,
+                               text=True, timeout=120)
+            assert r.returncode == 0, r.stderr
+            assert "materialised bag:" in r.stdout
+            assert "Number of matches: %d" % gold["house"]["wcoj"]["count"] in r.stdout, (args, r.stdout)
     rpath, epath = str(tmp_path / "out.txt"), str(tmp_path / "emb.txt")
     r = subprocess.run([cli, "-q", str(tmp_path / "diamond.txt"), "-d", dpath, "-r", rpath, "--embeddings", epath],
                        capture_output=True, text=True, timeout=120)
