@@ -183,6 +183,7 @@ struct WaveLds {
     uint32_t pos[MAXK];  // next candidate position (absolute index into cand[d].ptr)
     uint32_t iend[MAXK]; // end of this wave's iteration range at each level
     uint32_t tpos[MAXK]; // bag levels: absolute trie value index of phi[level]
+    unsigned long long accv[6];  // statistics counters (Acc), lane 0 only
     Src cand[MAXK];      // candidate set per level (whole set: reuse reads it)
     Src pre[MAXK + 1];   // exploration: parent intersections formed early ([MAXK] = F)
     Src srcs[MAXK + 2];  // sources of the intersection being formed
@@ -250,8 +251,15 @@ __device__ __forceinline__ uint64_t choosec(uint64_t n, uint32_t k) {
     return nom / den;
 }
 
+// Per-wave statistics counters (matches, pairwise intersections, algorithmic
+// bytes, search-tree nodes, root tasks, clock cycles), kept in the wave's LDS and
+// updated by lane 0 only, so they occupy no registers.
+enum { A_COUNT, A_INTER, A_BYTES, A_NODES, A_TASKS, A_CYCLES, A_NUM };
 struct Acc {
-    unsigned long long count, inter, bytes, nodes, tasks, cycles;
+    unsigned long long *c;  // WaveLds::accv
+    __device__ __forceinline__ void add(int k, unsigned long long v) const {
+        if (__lane_id() == 0) c[k] += v;
+    }
 };
 
 // Bounds from the images of the levels in the masks.
@@ -296,8 +304,8 @@ __device__ __forceinline__ uint32_t wave_intersect(WaveLds &w, int ns, int nchk,
         if (len < w.srcs[drv].e - w.srcs[drv].b) drv = i;
         w.cur[i] = w.srcs[i].b;
     }
-    acc.bytes += bytes;
-    acc.inter += (ns > 1) ? (ns - 1) : 0;
+    acc.add(A_BYTES, bytes);
+    acc.add(A_INTER, (ns > 1) ? (ns - 1) : 0);
     const guint *D = w.srcs[drv].ptr;
     const uint32_t db = w.srcs[drv].b, de = w.srcs[drv].e;
     const int lane = lane_id();
@@ -340,7 +348,7 @@ __device__ __forceinline__ uint32_t wave_intersect(WaveLds &w, int ns, int nchk,
         if (ok) out[total + pre] = x;
         total += __popcll(mask);
     }
-    acc.bytes += 4ull * total;
+    acc.add(A_BYTES, 4ull * total);
     wave_sync();  // other lanes read the survivors next
     return total;
 }
@@ -408,7 +416,7 @@ __device__ int gather_sources(const KParams &P, WaveLds &w, int d, uint32_t pmas
         if (w.pl.bag & ((1u << d) - 1u)) pmask &= ~w.pl.bag;
         clamp_range(s.ptr, s.b, s.e, has_lo, lo, has_hi, hi);
         w.srcs[ns++] = s;
-        acc.bytes += 8;
+        acc.add(A_BYTES, 8);
     }
     for (uint32_t m = pmask; m; m &= m - 1) {
         uint32_t v = w.phi[__builtin_ctz(m)];
@@ -418,7 +426,7 @@ __device__ int gather_sources(const KParams &P, WaveLds &w, int d, uint32_t pmas
         s.e = P.off[v + 1];
         clamp_range(s.ptr, s.b, s.e, has_lo, lo, has_hi, hi);
         w.srcs[ns++] = s;
-        acc.bytes += 8;
+        acc.add(A_BYTES, 8);
     }
     return ns;
 }
@@ -433,7 +441,7 @@ __device__ Src form_set(WaveLds &w, int ns, int nchk, bool may_alias, uint32_t *
         return z;
     }
     if (ns == 1 && may_alias) {
-        acc.bytes += 4ull * (w.srcs[0].e - w.srcs[0].b);
+        acc.add(A_BYTES, 4ull * (w.srcs[0].e - w.srcs[0].b));
         return w.srcs[0];
     }
     Src r;
@@ -457,7 +465,7 @@ __device__ __forceinline__ void materialize(const KParams &P, WaveLds &w, uint32
     w.cand[d] = c;
     w.pos[d] = c.b;
     w.iend[d] = c.e;
-    acc.nodes += 1;
+    acc.add(A_NODES, 1);
 }
 
 // Exploration: form the parent intersection of every level whose last parent
@@ -543,7 +551,7 @@ __device__ __forceinline__ uint32_t count_last(const KParams &P, WaveLds &w, Ter
             w.srcs[0] = terms[b1 - 1].F;
             w.srcs[1] = terms[(B ^ b1) - 1].F;
             t.F = form_set(w, 2, 0, false, buf_ptr(P, wb, 2 * MAXK + 1 + j), acc);
-            acc.nodes += 1;
+            acc.add(A_NODES, 1);
         } else {
             int ns;
             if (pl.mode == SGE_MODE_EXPLORE && K == 1) {
@@ -558,7 +566,7 @@ __device__ __forceinline__ uint32_t count_last(const KParams &P, WaveLds &w, Ter
             }
             int nchk = load_checks(w, pl.t_check[j]);
             t.F = form_set(w, ns, nchk, nchk == 0, buf_ptr(P, wb, 2 * MAXK + 1 + j), acc);
-            acc.nodes += 1;
+            acc.add(A_NODES, 1);
         }
         terms[j] = t;
     }
@@ -693,8 +701,8 @@ __device__ __forceinline__ uint32_t count_last(const KParams &P, WaveLds &w, Ter
         wave_sync();
         const uint32_t W = w.pre_sum[ne];
         steps += (ne + W + WAVE - 1) / WAVE;
-        acc.inter += ne;
-        acc.bytes += 8ull * nb;
+        acc.add(A_INTER, ne);
+        acc.add(A_BYTES, 8ull * nb);
         // phase 2: flattened probes, a window of 64 per round.  Lane x's entry is
         // the last entry starting at or before u0 + x: entries e0+1.. (at most 64,
         // as ne <= SIB_BLOCK <= 64) mark their start slot in the window, and each
@@ -737,7 +745,7 @@ __device__ __forceinline__ uint32_t count_last(const KParams &P, WaveLds &w, Ter
                 }
             }
         }
-        acc.bytes += 4ull * W * 2;
+        acc.add(A_BYTES, 4ull * W * 2);
         wave_sync();
         // phase 3: closed-form contributions (lane per sibling)
         for (uint32_t r0 = 0; r0 < nb; r0 += WAVE) {
@@ -763,21 +771,18 @@ __device__ __forceinline__ uint32_t count_last(const KParams &P, WaveLds &w, Ter
         wave_sync();
     }
     for (int off = WAVE / 2; off > 0; off >>= 1) local += __shfl_xor(local, off);
-    acc.count += local;
+    acc.add(A_COUNT, local);
     return steps;
 }
 
 __device__ void flush(const KParams &P, uint32_t plan, Acc &acc) {
     if (lane_id() == 0) {
-        PlanOut *o = P.out + plan;
-        if (acc.count) atomicAdd(&o->count, acc.count);
-        if (acc.inter) atomicAdd(&o->inter, acc.inter);
-        if (acc.bytes) atomicAdd(&o->bytes, acc.bytes);
-        if (acc.nodes) atomicAdd(&o->nodes, acc.nodes);
-        if (acc.tasks) atomicAdd(&o->tasks, acc.tasks);
-        if (acc.cycles) atomicAdd(&o->cycles, acc.cycles);
+        unsigned long long *o = &P.out[plan].count;  // PlanOut: the counters in Acc order
+        for (int k = 0; k < A_NUM; ++k) {
+            if (acc.c[k]) atomicAdd(o + k, acc.c[k]);
+            acc.c[k] = 0;
+        }
     }
-    acc = Acc{0, 0, 0, 0, 0, 0};
 }
 
 // Enumeration: form the last level and write one row per embedding, images
@@ -808,7 +813,7 @@ __device__ void emit_last(const KParams &P, WaveLds &w, uint32_t *wb, Acc &acc) 
                 row[pl.order[L]] = x;
             }
         }
-        acc.count += n;
+        acc.add(A_COUNT, n);
     }
 }
 
@@ -954,7 +959,9 @@ __global__ void __launch_bounds__(256, SGE_WAVES_PER_SIMD) wcoj_kernel(KParams P
     TermLds *terms = reinterpret_cast<TermLds *>(lds_raw + sizeof(WaveLds) * nwaves) + wid * MAX_TERMS;
     const uint64_t gwave = (uint64_t)blockIdx.x * (blockDim.x / WAVE) + wid;
     uint32_t *wb = P.scratch + gwave * P.wave_stride;
-    Acc acc{0, 0, 0, 0, 0, 0};
+    Acc acc{w.accv};
+    if (lane == 0)
+        for (int k = 0; k < A_NUM; ++k) w.accv[k] = 0;
     uint32_t cur_plan = NONE;
     const uint64_t ntask = (P.task_end - P.task_begin + P.stride - 1) / P.stride;
     const bool cont = P.in_items != nullptr;
@@ -965,7 +972,7 @@ __global__ void __launch_bounds__(256, SGE_WAVES_PER_SIMD) wcoj_kernel(KParams P
         {
             // charge the time since the last item started to that item's plan
             const unsigned long long now = __builtin_amdgcn_s_memtime();
-            if (cur_plan != NONE) acc.cycles += now - t_item;
+            if (cur_plan != NONE) acc.add(A_CYCLES, now - t_item);
             t_item = now;
         }
         unsigned long long item = 0;
@@ -1058,9 +1065,9 @@ __global__ void __launch_bounds__(256, SGE_WAVES_PER_SIMD) wcoj_kernel(KParams P
             // root levels of the bag: the edge must start a bag tuple
             if ((pl.bag & 1u) && !trie_enter(w, 0, v0, NONE)) continue;
             if ((pl.bag & 2u) && !trie_enter(w, 1, v1, NONE)) continue;
-            if (ci == 0) acc.tasks += 1;
+            if (ci == 0) acc.add(A_TASKS, 1);
             if (pl.k < 3) {  // an edge pattern counts its root tasks; k = 1 is counted on the host
-                if (pl.k == 2) acc.count += 1;
+                if (pl.k == 2) acc.add(A_COUNT, 1);
                 continue;
             }
             if (pl.mode == SGE_MODE_EXPLORE && !ENUM) {
@@ -1086,7 +1093,7 @@ __global__ void __launch_bounds__(256, SGE_WAVES_PER_SIMD) wcoj_kernel(KParams P
             run_dfs<ENUM>(P, w, terms, wb, acc, plan, 2, stop);
         }
     }
-    if (cur_plan != NONE) acc.cycles += __builtin_amdgcn_s_memtime() - t_item;
+    if (cur_plan != NONE) acc.add(A_CYCLES, __builtin_amdgcn_s_memtime() - t_item);
     if (cur_plan != NONE) flush(P, cur_plan, acc);
 }
 
