@@ -953,7 +953,7 @@ template <bool ENUM>
 __global__ void __launch_bounds__(256, SGE_WAVES_PER_SIMD) wcoj_kernel(KParams P) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     const int lane = lane_id();
-    const int wid = threadIdx.x / WAVE;
+    const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);  // uniform: wave bases in SGPRs
     const int nwaves = blockDim.x / WAVE;
     WaveLds &w = reinterpret_cast<WaveLds *>(lds_raw)[wid];
     TermLds *terms = reinterpret_cast<TermLds *>(lds_raw + sizeof(WaveLds) * nwaves) + wid * MAX_TERMS;
