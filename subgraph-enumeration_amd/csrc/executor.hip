@@ -69,7 +69,7 @@ constexpr int MAX_TERMS = 7;    // inclusion-exclusion terms of a 3-level indepe
 constexpr int NBUF = 2 * MAXK + 1 + MAX_TERMS;  // scratch buffers per wave
 constexpr uint32_t NODE_STEP = 1024;  // work-cap accounting granularity per wavefront (wave steps)
 #ifndef SGE_FSTAGE
-#define SGE_FSTAGE 576
+#define SGE_FSTAGE 400
 #endif
 constexpr int FSTAGE = SGE_FSTAGE;  // LDS words per wave for staging the terms' F sets (bitmaps or copies)
 #ifndef SGE_SEARCH_STAGE
@@ -939,13 +939,13 @@ __device__ __forceinline__ void run_dfs(const KParams &P, WaveLds &w, TermLds *t
     if (P.node_cap && since && lane_id() == 0) atomicAdd(P.node_ctr, (unsigned long long)since);
 }
 
-// Resident wavefronts per SIMD the register allocation is held to.  The search
-// is latency bound (dependent loads of binary searches and LDS round trips),
-// and 7 wavefronts hide more of it than the 5 that unconstrained allocation
-// gives, despite 144 B/lane of spills; FSTAGE is sized so that 28 wavefronts'
-// LDS fits a CU's 160 KB.
+// Resident wavefronts per SIMD the register allocation is held to: the
+// hardware maximum, 8 (64 VGPRs).  The search is latency bound (dependent loads
+// of binary searches and LDS round trips) and every extra resident wavefront
+// hides more of it; FSTAGE is sized so that 32 wavefronts' LDS fits a CU's
+// 160 KB.
 #ifndef SGE_WAVES_PER_SIMD
-#define SGE_WAVES_PER_SIMD 7
+#define SGE_WAVES_PER_SIMD 8
 #endif
 static_assert((sizeof(WaveLds) + MAX_TERMS * sizeof(TermLds)) * 4 * SGE_WAVES_PER_SIMD <= 160 * 1024,
               "the resident wavefronts' LDS state must fit one CU");
