@@ -45,6 +45,7 @@ def run(label, g, pats, mode, max_bag, reps):
             if r[1] != base[1]:
                 raise SystemExit(f"{label} pattern {i} bag {b}: count {r[1]} != {base[1]}")
             rows.append({"bag": b, "ms": r[0], "trie_ms": r[2]})
+            print(f"[{label}] pattern {i} bag {b}: {r[0]:.3f} ms (trie {r[2]:.3f} ms)", flush=True)
         timed = [r for r in rows if "ms" in r]
         best = min(timed, key=lambda r: r["ms"]) if timed else None
         print(f"[{label}] pattern {i}: no bag {base[0]:.3f} ms | best bag {best} of {len(rows)} | "
