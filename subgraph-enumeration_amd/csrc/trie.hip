@@ -142,7 +142,7 @@ int trie_from_rows(const uint32_t *d_rows, uint64_t n, uint32_t p, uint32_t n_ve
     TRIE_TRY(s.alloc(&S, n * p));
     // 1. lexicographic order: stable radix passes from the last column to the first
     if (n) iota_kernel<<<blocks(n), 256, 0, st>>>(perm, n);
-    for (int c = (int)p - 1; c >= 0; --c) {
+    for (int c = (int)p - 1; c >= 0 && n; --c) {
         if (n) column_kernel<<<blocks(n), 256, 0, st>>>(d_rows, perm, n, p, (uint32_t)c, keys);
         TRIE_TRY(hipGetLastError());
         TRIE_TRY(with_temp(s, [&](void *tmp, size_t &b) {

@@ -513,3 +513,40 @@ def test_bag_enumeration_unchanged():
     a = sge.enumerate_embeddings(g, plan, house, 100000)
     b = sge.enumerate_embeddings(g, sge.set_bag(plan, [2, 3, 4]), house, 100000)
     assert len(a) == sge.count(g, plan)[0][0] and np.array_equal(a, b)
+
+
+def test_bag_empty_relation():
+    """A triangle bag over a triangle-free graph (a 4 x 4 grid): the relation and
+    the trie are empty, every count is 0 and equals the oracle's."""
+    edges = [(r * 4 + c, r * 4 + c + 1) for r in range(4) for c in range(3)] + \
+            [(r * 4 + c, (r + 1) * 4 + c) for r in range(3) for c in range(4)]
+    g = sge.Graph.from_edges(16, edges)
+    off, nbr = g.csr()
+    house = W.named("house")
+    for mode in (sge.MODE_WCOJ, sge.MODE_IEP):
+        plan = sge.set_bag(sge.build_plan(house, g, mode, [2, 3, 4, 0, 1]), [2, 3, 4])
+        say("empty bag relation, mode", mode)
+        t = sge.Trie(g, plan)
+        assert t.tuples == 0 and t.nodes == [0, 0, 0]
+        values, child = t.arrays()
+        assert len(values) == 0 and list(child) == [0, 0]
+        t.close()
+        assert sge.count(g, plan)[0][0] == 0 == O.count(plan, off, nbr)
+
+
+def test_bag_rmat8_all_78_batched():
+    """One bag per seven-vertex pattern (the smallest valid one), all 78 in one
+    batched launch over R-MAT scale 8: the reference's WCOJ counts."""
+    gold = golden("rmat7")["patterns"]
+    g = W.graph_rmat(8, 7)
+    plans = []
+    for p in W.seven_vertex_set():
+        plan = sge.build_plan(p, g, sge.MODE_IEP)
+        bags = smallest_bags(plan, 1)
+        plans.append(sge.set_bag(plan, bags[0]) if bags else plan)
+    assert sum(1 for pl in plans if pl.bag) >= 60
+    say("rmat8 78 with bags, batched")
+    b = sge.Batch(g, plans)
+    got = b.run()[0]
+    b.close()
+    assert got == [r["wcoj"]["count"] for r in gold]
