@@ -7,7 +7,8 @@
 // reference never writes; --order a,b,c,... fixes the matching order;
 // --bag a,b,c,... materialises those pattern vertices as one bag (its relation
 // built into a trie on the GPU before the join; Execution Time includes the
-// build, as the reference's includes its bag joins and tries);
+// build, as the reference's includes its bag joins and tries); --bag auto takes
+// the first bag of the pattern's decomposition that is valid for the order;
 // --device-build builds the data graph's CSR on the GPU.
 #include <chrono>
 #include <cstdio>
@@ -34,7 +35,8 @@ static void usage() {
               << "  -t <number>      Number of threads (accepted; the GPU executor ignores it)\n"
               << "  --embeddings <path>  Write canonical embeddings (extension)\n"
               << "  --order a,b,...  Fixed matching order (extension)\n"
-              << "  --bag a,b,...    Materialise these pattern vertices as one bag (extension)\n"
+              << "  --bag a,b,...|auto  Materialise these pattern vertices (or the first valid\n"
+              << "                   decomposition bag) as one bag (extension)\n"
               << "  --device-build   Build the data graph's CSR on the GPU (extension)\n"
               << "  --help, -h       Show this help message\n"
               << std::endl;
@@ -92,7 +94,11 @@ int main(int argc, char **argv) {
     }
     sge_plan plan;
     if (sge_plan_build(&p, g, mode, order.empty() ? nullptr : order.data(), &plan)) return die("plan");
-    if (!bag_s.empty()) {
+    if (bag_s == "auto") {
+        uint32_t bags[16], nb = 0;
+        if (sge_plan_bags(&p, &plan, bags, 16, &nb)) return die("bag");
+        if (nb && sge_plan_set_bag(&plan, bags[0])) return die("bag");
+    } else if (!bag_s.empty()) {
         std::stringstream ss(bag_s);
         std::string tok;
         uint32_t mask = 0;

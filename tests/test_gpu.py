@@ -366,6 +366,14 @@ def test_cli_drop_in(tmp_path):
             assert r.returncode == 0, r.stderr
             assert "materialised bag:" in r.stdout
             assert "Number of matches: %d" % gold["house"]["wcoj"]["count"] in r.stdout, (args, r.stdout)
+    # --bag auto: the first decomposition bag valid for the cost model's order
+    for name in ["diamond", "house"]:
+        say("cli bag auto", name)
+        r = subprocess.run([cli, "-q", str(tmp_path / f"{name}.txt"), "-d", dpath, "--bag", "auto"],
+                           capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        assert "materialised bag:" in r.stdout
+        assert "Number of matches: %d" % gold[name]["wcoj"]["count"] in r.stdout, (name, r.stdout)
     rpath, epath = str(tmp_path / "out.txt"), str(tmp_path / "emb.txt")
     r = subprocess.run([cli, "-q", str(tmp_path / "diamond.txt"), "-d", dpath, "-r", rpath, "--embeddings", epath],
                        capture_output=True, text=True, timeout=120)
