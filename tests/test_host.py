@@ -178,6 +178,11 @@ def test_count_without_gpu_fails_loudly():
         sge.count(g, sge.build_plan(W.named("diamond"), g))
     with pytest.raises(sge.SgeError):
         sge.Graph.from_edges(4, [(0, 1), (1, 2)], device_build=True)
+    bag_plan = sge.set_bag(sge.build_plan(W.named("house"), g, sge.MODE_WCOJ, [2, 3, 4, 0, 1]), [2, 3, 4])
+    with pytest.raises(sge.SgeError):
+        sge.Trie(g, bag_plan)
+    with pytest.raises(sge.SgeError):
+        sge.count(g, bag_plan)
 
 
 def test_bag_validation():
