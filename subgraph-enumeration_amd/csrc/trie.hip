@@ -107,17 +107,6 @@ struct Scratch {
     void release(void *q) { p.erase(std::remove(p.begin(), p.end(), q), p.end()); }
 };
 
-template <typename F>
-hipError_t with_temp(Scratch &s, F f) {
-    size_t bytes = 0;
-    hipError_t e = f(nullptr, bytes);
-    if (e != hipSuccess) return e;
-    unsigned char *t = nullptr;
-    e = s.alloc(&t, bytes);
-    if (e != hipSuccess) return e;
-    return f(t, bytes);
-}
-
 }  // namespace
 
 namespace sge {
@@ -140,14 +129,20 @@ int trie_from_rows(const uint32_t *d_rows, uint64_t n, uint32_t p, uint32_t n_ve
     TRIE_TRY(s.alloc(&keys, n));
     TRIE_TRY(s.alloc(&keys2, n));
     TRIE_TRY(s.alloc(&S, n * p));
+    // one temporary buffer serves every radix pass and every scan below
+    size_t sort_bytes = 0, scan_bytes = 0;
+    TRIE_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, keys, keys2, perm, perm2, (int)n, 0, key_bits, st));
+    TRIE_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, keys2, perm, (int)n, st));
+    size_t tmp_bytes = std::max(sort_bytes, scan_bytes);
+    unsigned char *tmp;
+    TRIE_TRY(s.alloc(&tmp, tmp_bytes));
     // 1. lexicographic order: stable radix passes from the last column to the first
     if (n) iota_kernel<<<blocks(n), 256, 0, st>>>(perm, n);
     for (int c = (int)p - 1; c >= 0 && n; --c) {
-        if (n) column_kernel<<<blocks(n), 256, 0, st>>>(d_rows, perm, n, p, (uint32_t)c, keys);
+        column_kernel<<<blocks(n), 256, 0, st>>>(d_rows, perm, n, p, (uint32_t)c, keys);
         TRIE_TRY(hipGetLastError());
-        TRIE_TRY(with_temp(s, [&](void *tmp, size_t &b) {
-            return hipcub::DeviceRadixSort::SortPairs(tmp, b, keys, keys2, perm, perm2, (int)n, 0, key_bits, st);
-        }));
+        size_t b = tmp_bytes;
+        TRIE_TRY(hipcub::DeviceRadixSort::SortPairs(tmp, b, keys, keys2, perm, perm2, (int)n, 0, key_bits, st));
         std::swap(perm, perm2);
     }
     for (uint32_t c = 0; c < p && n; ++c) column_kernel<<<blocks(n), 256, 0, st>>>(d_rows, perm, n, p, c, S + (uint64_t)c * n);
@@ -188,9 +183,8 @@ int trie_from_rows(const uint32_t *d_rows, uint64_t n, uint32_t p, uint32_t n_ve
         if (n) {
             flag_kernel<<<blocks(n), 256, 0, st>>>(dcol, n, (uint32_t)j, f);
             TRIE_TRY(hipGetLastError());
-            TRIE_TRY(with_temp(s, [&](void *tmp, size_t &b) {
-                return hipcub::DeviceScan::ExclusiveSum(tmp, b, f, pos, (int)n, st);
-            }));
+            size_t b = tmp_bytes;
+            TRIE_TRY(hipcub::DeviceScan::ExclusiveSum(tmp, b, f, pos, (int)n, st));
             const bool inner = j < (int)p - 1;
             scatter_kernel<<<blocks(n), 256, 0, st>>>(S + (uint64_t)j * n, f, pos, inner ? pos_next : nullptr, n,
                                                       t.vb[j], inner ? t.vb[j + 1] : 0u, (uint32_t)j, val, child);
