@@ -84,6 +84,7 @@ class LearnedOrder:
         self.t_sd = 1.0
         self.threshold = 3.0  # bits of analytic cost the learned order may exceed the cost model's by
         self.margin = 0.25    # predicted bits of improvement needed to leave the cost model's order
+        self.sizes: Optional[List[int]] = None  # pattern sizes trained on (None: no restriction)
         self.history: Dict[str, list] = {}
 
     # ---- data
@@ -134,7 +135,8 @@ class LearnedOrder:
     # ---- persistence
     def save(self, path: str):
         np.savez(path, params=self.mlp.params(), mu=self.mu, sd=self.sd,
-                 meta=np.array(json.dumps({"t_sd": self.t_sd, "threshold": self.threshold, "margin": self.margin})))
+                 meta=np.array(json.dumps({"t_sd": self.t_sd, "threshold": self.threshold, "margin": self.margin,
+                                           "sizes": self.sizes})))
 
     @classmethod
     def load(cls, path: str) -> "LearnedOrder":
@@ -146,6 +148,7 @@ class LearnedOrder:
         m.t_sd = meta["t_sd"]
         m.threshold = meta["threshold"]
         m.margin = meta.get("margin", m.margin)
+        m.sizes = meta.get("sizes")
         return m
 
 
@@ -172,6 +175,12 @@ def choose_plan(model: LearnedOrder, pattern: Pattern, graph: Graph, mode: int, 
     disagree that much."""
     if fallback is None:
         fallback = build_plan(pattern, graph, mode)
+    if model.sizes and pattern.k not in model.sizes:
+        # outside the pattern sizes the network was trained on its ranking is
+        # unfounded (5-vertex house in IEP mode: a 3.1x slower order)
+        order = list(fallback.order)[: pattern.k]
+        return fallback, {"pred_rel_log2": 0.0, "cost_order": order, "learned_order": order, "fallback": True,
+                          "cost_bits_over": 0.0, "reason": "pattern size outside the training sizes"}
     cands = candidate_orders(pattern, graph, mode, top)
     X = np.stack([order_features(pattern, graph, o, mode, FEATURE_SAMPLES, 11) for o in cands])
     pred = model.predict_rel_log2(X)
@@ -276,4 +285,6 @@ def train(graph: Graph, patterns: Sequence[Pattern], mode: int, stride: int = 10
           epochs: int = 300, seed: int = 0, log=print):
     """Log (order, measured work) pairs with the plain executor and fit the MLP."""
     X, y, c, cap, grp = log_pairs(graph, patterns, mode, stride, orders_per_pattern, seed, log)
-    return fit_model(X, y, grp, epochs, seed, log), (X, y, c, grp)
+    model = fit_model(X, y, grp, epochs, seed, log)
+    model.sizes = sorted({p.k for p in patterns})
+    return model, (X, y, c, grp)

@@ -101,6 +101,7 @@ def main():
             log(f"wrote {len(X)} orders to {args.log_only}")
             return
     model = L.fit_model(X, y, grp, epochs=args.epochs, log=log)
+    model.sizes = sorted({p.k for p in train_p})  # the model only ranks orders of these sizes
     rel = L.relative_targets(y, grp)
     pred = model.predict_rel_log2(X)
     fit = {"train_rmse_rel_log2": float(np.sqrt(np.mean((pred - rel) ** 2))),

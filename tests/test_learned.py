@@ -100,3 +100,18 @@ def test_learned_order_gain_over_cost_model():
     print(f"cost-model orders {t_cost:.2f} ms, learned orders {t_learn:.2f} ms")
     assert not any(r["capped"] for r in rows)
     assert t_learn <= 1.05 * t_cost
+
+
+@pytest.mark.gpu
+def test_model_keeps_to_its_training_sizes():
+    """Outside the pattern sizes it was trained on, the model returns the cost
+    model's plan."""
+    from sge import learned as L
+
+    m = L.load(os.path.join(os.path.dirname(sge.__file__), "models", "order_mlp.npz"))
+    assert m.sizes == [6, 7]
+    g = W.graph_c1()
+    house = W.named("house")
+    fb = sge.build_plan(house, g, sge.MODE_IEP)
+    plan, info = L.choose_plan(m, house, g, sge.MODE_IEP, fallback=fb)
+    assert info["fallback"] and bytes(plan) == bytes(fb)
