@@ -66,7 +66,7 @@ constexpr int MAXK = SGE_MAX_K;
 constexpr int SIB_BLOCK = SGE_SIB_BLOCK;  // (sibling, term) entries per flattened last-level batch
 static_assert(SIB_BLOCK <= 64, "the probe-window entry mapping covers at most 64 entries per batch");
 constexpr int MAX_TERMS = 7;    // inclusion-exclusion terms of a 3-level independent tail
-constexpr int NBUF = 2 * MAXK + 1 + MAX_TERMS;  // scratch buffers per wave
+constexpr int NBUF_MAX = 2 * MAXK + 1 + MAX_TERMS;  // scratch buffers per wave at most (see BufLayout)
 constexpr uint32_t NODE_STEP = 1024;  // work-cap accounting granularity per wavefront (wave steps)
 #ifndef SGE_FSTAGE
 #define SGE_FSTAGE 400
@@ -139,6 +139,10 @@ struct KParams {
     uint32_t *scratch;
     uint64_t wave_stride;  // u32 per wave
     uint32_t buf_stride;   // u32 per buffer
+    // buffer indices in a wave's slab: level d's set at d; exploration's early
+    // parent intersections at ex_base + level and F's at pre_f; the last-level
+    // terms' F sets at term_base + term
+    uint32_t ex_base, pre_f, term_base;
     PlanOut *out;
     uint32_t *emb;
     uint64_t emb_cap;
@@ -475,11 +479,11 @@ __device__ __forceinline__ void explore_after(const KParams &P, WaveLds &w, uint
     for (uint32_t j = d + 1; j <= pl.sib; ++j) {
         if (pl.ready[j] != (uint32_t)d || j < 2) continue;
         int ns = gather_sources(P, w, j, pl.parent[j], -1, false, false, 0, false, 0, acc);
-        w.pre[j] = form_set(w, ns, 0, true, buf_ptr(P, wb, MAXK + j), acc);
+        w.pre[j] = form_set(w, ns, 0, true, buf_ptr(P, wb, P.ex_base + j), acc);
     }
     if (pl.f_parent && pl.f_ready == (uint32_t)d) {
         int ns = gather_sources(P, w, pl.last, pl.f_parent, -1, false, false, 0, false, 0, acc);
-        w.pre[MAXK] = form_set(w, ns, 0, true, buf_ptr(P, wb, 2 * MAXK), acc);
+        w.pre[MAXK] = form_set(w, ns, 0, true, buf_ptr(P, wb, P.pre_f), acc);
     }
 }
 
@@ -550,7 +554,7 @@ __device__ __forceinline__ uint32_t count_last(const KParams &P, WaveLds &w, Ter
         } else if (from_terms) {
             w.srcs[0] = terms[b1 - 1].F;
             w.srcs[1] = terms[(B ^ b1) - 1].F;
-            t.F = form_set(w, 2, 0, false, buf_ptr(P, wb, 2 * MAXK + 1 + j), acc);
+            t.F = form_set(w, 2, 0, false, buf_ptr(P, wb, P.term_base + j), acc);
             acc.add(A_NODES, 1);
         } else {
             int ns;
@@ -565,7 +569,7 @@ __device__ __forceinline__ uint32_t count_last(const KParams &P, WaveLds &w, Ter
                 ns = gather_sources(P, w, pl.last, pl.t_par[j], reuse, false, has_lo, lo, has_hi, hi, acc);
             }
             int nchk = load_checks(w, pl.t_check[j]);
-            t.F = form_set(w, ns, nchk, nchk == 0, buf_ptr(P, wb, 2 * MAXK + 1 + j), acc);
+            t.F = form_set(w, ns, nchk, nchk == 0, buf_ptr(P, wb, P.term_base + j), acc);
             acc.add(A_NODES, 1);
         }
         terms[j] = t;
@@ -1119,6 +1123,7 @@ struct DeviceCtx {
     uint32_t *items[2] = {nullptr, nullptr};  // continuation items (ping-pong)
     unsigned long long *icount = nullptr;     // [2] item counts, [2] work-cap counter
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    uint32_t ex_base = 0, pre_f = 0, term_base = 0, nbuf = NBUF_MAX;  // slab layout of the uploaded plans
 };
 
 constexpr uint64_t ITEM_CAP = 1ull << 21;  // continuation items per pass buffer (96 MB)
@@ -1347,11 +1352,46 @@ uint64_t items_per_plan(uint64_t nsub, const LaunchCfg &cfg) {
     return head + (nsub - head + cfg.chunk - 1) / cfg.chunk;
 }
 
+// Slab layout for a set of plans: only the buffers they use (level sets up to
+// the largest k, exploration's early intersections only when a plan explores,
+// as many F sets as the most terms).  At max degree D a wavefront's slab is
+// nbuf * (D + 64) words, so this sets the scratch footprint.
+struct BufLayout {
+    uint32_t ex_base, pre_f, term_base, nbuf;
+};
+
+BufLayout buf_layout(const DPlan *dp, uint32_t n) {
+    uint32_t kmax = 2, terms = 1;
+    bool explore = false;
+    for (uint32_t i = 0; i < n; ++i) {
+        kmax = std::max(kmax, dp[i].k);
+        terms = std::max(terms, dp[i].nterms);
+        explore = explore || dp[i].mode == SGE_MODE_EXPLORE;
+    }
+    BufLayout L;
+    if (explore) {
+        L.ex_base = kmax;         // levels 2..sib at ex_base + level < 2 kmax
+        L.pre_f = 2 * kmax;
+        L.term_base = 2 * kmax + 1;
+    } else {
+        L.ex_base = L.pre_f = L.term_base = kmax;  // no early intersections
+    }
+    L.nbuf = L.term_base + terms;
+    return L;
+}
+
+void set_layout(KParams &kp, uint32_t ex_base, uint32_t pre_f, uint32_t term_base) {
+    kp.ex_base = ex_base;
+    kp.pre_f = pre_f;
+    kp.term_base = term_base;
+}
+
 int ensure_scratch(const sge_graph *g, const LaunchCfg &cfg, KParams &kp) {
     DeviceCtx &c = dctx();
     uint32_t stride = ((g->h.max_degree + 64 + 63) / 64) * 64;
     kp.buf_stride = stride;
-    kp.wave_stride = (uint64_t)stride * NBUF;
+    kp.wave_stride = (uint64_t)stride * c.nbuf;
+    set_layout(kp, c.ex_base, c.pre_f, c.term_base);
     size_t waves = (size_t)cfg.grid * (cfg.block / WAVE);
     size_t need = waves * kp.wave_stride * sizeof(uint32_t);
     int rc = ensure_buffer((void **)&c.scratch, c.scratch_bytes, need);
@@ -1565,6 +1605,11 @@ int upload_plans(sge_graph *g, const sge_plan *plans, uint32_t n, bool enumerate
         dp[i].nsub = ns;
         nsub[i] = sp ? ns : ntask;
     }
+    const BufLayout L = buf_layout(dp.data(), n);
+    c.ex_base = L.ex_base;
+    c.pre_f = L.pre_f;
+    c.term_base = L.term_base;
+    c.nbuf = L.nbuf;
     int rc = ensure_buffer((void **)&c.plans, c.plans_cap, sizeof(DPlan) * std::max<uint32_t>(n, 1));
     if (rc) return rc;
     rc = ensure_buffer((void **)&c.out, c.out_cap, sizeof(PlanOut) * std::max<uint32_t>(n, 1));
@@ -2041,8 +2086,10 @@ extern "C" int sge_batch_create_sampled(sge_graph *g, const sge_plan *plans, uin
     KParams kp;
     std::memset(&kp, 0, sizeof(kp));
     uint32_t bstride = ((g->h.max_degree + 64 + 63) / 64) * 64;
+    const BufLayout L = buf_layout(dp.data(), nplans);
     kp.buf_stride = bstride;
-    kp.wave_stride = (uint64_t)bstride * NBUF;
+    kp.wave_stride = (uint64_t)bstride * L.nbuf;
+    set_layout(kp, L.ex_base, L.pre_f, L.term_base);
     size_t waves = (size_t)cfg.grid * (cfg.block / WAVE);
     auto fail = [&](hipError_t e, const char *what) {
         sge::set_error(std::string("HIP error: ") + hipGetErrorString(e) + " at " + what);
