@@ -26,15 +26,23 @@ def main():
     ap.add_argument("--scale", type=int, default=23)
     ap.add_argument("--seed", type=int, default=6)
     ap.add_argument("--target-ms", type=float, default=4000.0)
+    ap.add_argument("--patterns", default="near5clique,sixvertex0")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="run the 256 MiB copy kernel first (scales rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     ap.add_argument("--out", default=None)
     args = ap.parse_args()
+    if args.calibrate:
+        ms = sge.calibration_copy(256 << 20)
+        print(f"calibration copy: 256 MiB read + written in {ms:.3f} ms", flush=True)
     print(f"building R-MAT scale {args.scale} (host edges, CSR on the GPU)", flush=True)
     t0 = time.time()
     g = W.graph_rmat(args.scale, args.seed, device_build=True)
     print(f"graph n={g.n} m={g.m} max degree {g.stats.max_degree} in {time.time() - t0:.1f} s", flush=True)
     res = {"graph": f"rmat(scale={args.scale}, ef=16, a=.57, b=.19, c=.19, seed={args.seed})", "n": g.n, "m": g.m,
            "max_degree": g.stats.max_degree, "runs": []}
-    for name, p in (("near5clique", W.named("near5clique")), ("sixvertex0", W.six_vertex_set()[0])):
+    pats = {"near5clique": W.named("near5clique"), "sixvertex0": W.six_vertex_set()[0]}
+    for name in args.patterns.split(","):
+        p = pats[name]
         plan = sge.build_plan(p, g, sge.MODE_IEP)
         cap = 1 << 12
         for _ in range(3):
