@@ -200,7 +200,7 @@ def choose_plan(model: LearnedOrder, pattern: Pattern, graph: Graph, mode: int, 
     return build_plan(pattern, graph, mode, cands[best]), info
 
 
-def judge(model: "LearnedOrder", graph: Graph, patterns: Sequence[Pattern], mode: int, stride: int = 1,
+def evaluate(model: "LearnedOrder", graph: Graph, patterns: Sequence[Pattern], mode: int, stride: int = 1,
           reps: int = 2, log=print):
     """Learned choice against the cost model's choice, per pattern: the same
     executor on the same root sample, best of `reps` device times.  The learned
@@ -238,7 +238,7 @@ def judge(model: "LearnedOrder", graph: Graph, patterns: Sequence[Pattern], mode
                "fallback": info["fallback"], "cost_order": info["cost_order"],
                "learned_order": info["learned_order"]}
         rows.append(row)
-        log(f"[judge] pattern {i}: cost-model {row['cost_ms']:.2f} ms, learned {row['learned_ms']:.2f} ms"
+        log(f"[evaluate] pattern {i}: cost-model {row['cost_ms']:.2f} ms, learned {row['learned_ms']:.2f} ms"
             f"{' (capped)' if capped else ''}, fallback {info['fallback']}")
     return rows
 

@@ -1,4 +1,4 @@
-"""Train the learned attribute-order model on the GPU and judge it.
+"""Train the learned attribute-order model on the GPU and evaluate it.
 
     python -m sge.train_order_model --out sge/models/order_mlp.npz
 
@@ -7,7 +7,7 @@ root-edge sample of the Chung-Lu graph (n=7,115, m=100,762, seed 1), each order
 under a work cap relative to the cost model's order (learned.log_work), for
 patterns disjoint from the benchmark set: 400 connected 7-vertex patterns with
 11-14 edges (sampler seed 5) and the connected 6-vertex patterns with 8-10
-edges.  Fits the MLP (sge_mlp_* kernels), then judges it on the 16 benchmark
+edges.  Fits the MLP (sge_mlp_* kernels), then evaluates it on the 16 benchmark
 6-vertex patterns with full exact runs: predicted vs measured work, and the
 measured time of the learned choice against the analytic cost model's choice.
 Writes the weights (.npz) and a JSON report next to them.
@@ -50,8 +50,8 @@ def main():
     ap.add_argument("--first", type=int, default=0, help="first training pattern to log")
     ap.add_argument("--count", type=int, default=None, help="number of training patterns to log")
     ap.add_argument("--data", default=None, help="comma-separated .npz files of logged pairs to fit from")
-    ap.add_argument("--judge-only", action="store_true",
-                    help="judge the model at --out on the benchmark set and rewrite its report; no training")
+    ap.add_argument("--eval-only", action="store_true",
+                    help="evaluate the model at --out on the benchmark set and rewrite its report; no training")
     args = ap.parse_args()
     mode = {"wcoj": sge.MODE_WCOJ, "explore": sge.MODE_EXPLORE, "iep": sge.MODE_IEP}[args.mode]
     logf = open(args.log, "a") if args.log else None
@@ -67,9 +67,9 @@ def main():
     g = W.graph_c2()
     g.to_device()
     log(f"graph n={g.n} m={g.m} in {time.time() - t0:.1f}s")
-    if args.judge_only:
+    if args.eval_only:
         model = L.load(args.out)
-        rows = L.judge(model, g, W.six_vertex_set(), mode, args.eval_stride, log=log)
+        rows = L.evaluate(model, g, W.six_vertex_set(), mode, args.eval_stride, log=log)
         rep_path = os.path.splitext(args.out)[0] + "_report.json"
         report = json.load(open(rep_path)) if os.path.exists(rep_path) else {}
         t_cost = sum(r["cost_ms"] for r in rows)
@@ -108,8 +108,8 @@ def main():
            "train_corr_rel": float(np.corrcoef(pred, rel)[0, 1]),
            "cost_model_corr_rel": float(np.corrcoef(L.relative_targets(c, grp), rel)[0, 1])}
     log("fit", fit)
-    # judge on the benchmark set
-    rows = L.judge(model, g, W.six_vertex_set(), mode, args.eval_stride, log=log)
+    # evaluate on the benchmark set
+    rows = L.evaluate(model, g, W.six_vertex_set(), mode, args.eval_stride, log=log)
     t_cost = sum(r["cost_ms"] for r in rows)
     t_learn = sum(r["learned_ms"] for r in rows)
     report = {"graph": "chung_lu(n=7115, m=100762, gamma=2.1, cap=1100, seed=1)", "mode": args.mode,

@@ -35,7 +35,7 @@ def named(name: str) -> Pattern:
     return Pattern.from_edges(k, e)
 
 
-# ---- data graphs -------------------------------------------------
+# ---- data graphs (benchmark configurations) -------------------------------------------------
 def graph_c1(degree_order=True) -> Graph:
     """Chung-Lu n=379, m=914, gamma=2.5, no cap, seed 0."""
     return Graph.chung_lu(379, 914, 2.5, 0.0, 0, degree_order)

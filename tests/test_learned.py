@@ -94,7 +94,7 @@ def test_learned_order_gain_over_cost_model():
     assert os.path.exists(path), "trained order model missing: python -m sge.train_order_model"
     model = L.load(path)
     g = W.graph_c2()
-    rows = L.judge(model, g, W.six_vertex_set(), sge.MODE_IEP, stride=1, reps=2)
+    rows = L.evaluate(model, g, W.six_vertex_set(), sge.MODE_IEP, stride=1, reps=2)
     t_cost = sum(r["cost_ms"] for r in rows)
     t_learn = sum(r["learned_ms"] for r in rows)
     print(f"cost-model orders {t_cost:.2f} ms, learned orders {t_learn:.2f} ms")
