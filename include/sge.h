@@ -209,6 +209,10 @@ typedef struct sge_tuning {
     uint32_t head_items;      /* the heaviest head_items root subtasks of each plan are queue
                                  items of their own, the rest come in chunks (0 = one per
                                  resident wavefront, 1 << 30 = none) */
+    uint32_t global_bitmap_min; /* a last-level F set too large for the LDS stage, with at least
+                                 this many elements, is staged as a bitmap in a per-wave global
+                                 region when its siblings' rows will probe it (0 = default 2048,
+                                 1 << 30 = off) */
 } sge_tuning;
 
 typedef struct sge_stats {

@@ -143,6 +143,11 @@ struct KParams {
     // parent intersections at ex_base + level and F's at pre_f; the last-level
     // terms' F sets at term_base + term
     uint32_t ex_base, pre_f, term_base;
+    // per-wave bitmap region in global memory for last-level F sets too large
+    // for the LDS stage (all zero between count_last calls; null: not used)
+    uint32_t *gbm;
+    uint32_t gbm_words;
+    uint32_t gbm_min;  // F sets staged as global bitmaps have at least this many elements
     PlanOut *out;
     uint32_t *emb;
     uint64_t emb_cap;
@@ -502,11 +507,16 @@ struct TermLds {
 // Membership of x in term t's F between F positions [tlo, thi) (a sibling's row
 // drove x, so its bounds already hold): one bit test for a bitmap, else a
 // binary search in the LDS copy or in F itself.
-__device__ __forceinline__ bool term_has(const WaveLds &w, const TermLds &t, uint32_t x, uint32_t tlo, uint32_t thi) {
-    if (t.flags & 16u) {
+__device__ __forceinline__ bool term_has(const WaveLds &w, const TermLds &t, const uint32_t *gb, uint32_t x,
+                                         uint32_t tlo, uint32_t thi) {
+    if (t.flags & 48u) {
         if (x < t.bm_lo || x > t.bm_hi) return false;
         uint32_t r = x - t.bm_lo;
-        return (w.fst[t.st_off + (r >> 5)] >> (r & 31u)) & 1u;
+        // a global bitmap is read at L2 (agent scope), where its atomics landed
+        const uint32_t word = (t.flags & 16u) ? w.fst[t.st_off + (r >> 5)]
+                                              : __hip_atomic_load(gb + t.st_off + (r >> 5), __ATOMIC_RELAXED,
+                                                                  __HIP_MEMORY_SCOPE_AGENT);
+        return (word >> (r & 31u)) & 1u;
     }
     if (t.flags & 8u) return contains(&w.fst[t.st_off], tlo - t.F.b, thi - t.F.b, x);
     return contains(t.F.ptr, tlo, thi, x);
@@ -518,6 +528,26 @@ __device__ __forceinline__ bool term_has(const WaveLds &w, const TermLds &t, uin
 // |bc|a + 2|abc|.  u64 arithmetic is exact modulo 2^64 and the true count is
 // below 2^64.
 
+// Whether a global bitmap of a large F (len elements) pays for its build and
+// clear (about 2 len / 64 wave steps): the siblings' rows short enough to
+// probe F (|row| <= 2 len) save about log2(len) loads per element.  Estimated
+// from the first 64 siblings' degrees, scaled to all nsib.
+__device__ __forceinline__ bool gbm_pays(const KParams &P, const WaveLds &w, int s, uint32_t len) {
+    const int lane = lane_id();
+    const uint32_t b = (s == 1) ? 0u : w.pos[s];
+    const uint32_t nsib = (s == 1) ? 1u : w.iend[s] - b;
+    const uint32_t nsamp = min(nsib, (uint32_t)WAVE);
+    uint64_t probes = 0;
+    if ((uint32_t)lane < nsamp) {
+        const uint32_t c = (s == 1) ? w.phi[1] : w.cand[s].ptr[b + lane];
+        const uint32_t d = P.off[c + 1] - P.off[c];
+        if ((uint64_t)d <= 2ull * len) probes = d;
+    }
+    for (int off = WAVE / 2; off > 0; off >>= 1) probes += __shfl_xor(probes, off);
+    const uint64_t est = probes * nsib / max(nsamp, 1u);
+    return est * (uint64_t)(32 - __clz(len)) >= 4ull * len;
+}
+
 // Count the closed-form level(s) for every sibling in cand[sib] (or the single
 // sibling phi[1] when sib == 1).  Each term's F = the candidates that do not
 // depend on the sibling, with every bound/check from levels below the sibling
@@ -525,7 +555,8 @@ __device__ __forceinline__ bool term_has(const WaveLds &w, const TermLds &t, uin
 // size), computed with the flattened probe loop.
 // Returns the wave steps spent (64-lane iterations of the entry setup and the
 // flattened probes), the unit of the donation budget and the node cap.
-__device__ __forceinline__ uint32_t count_last(const KParams &P, WaveLds &w, TermLds *terms, uint32_t *wb, Acc &acc) {
+__device__ __forceinline__ uint32_t count_last(const KParams &P, WaveLds &w, TermLds *terms, uint32_t *wb, uint32_t *gb,
+                                               Acc &acc) {
     const DPlan &pl = w.pl;
     const int s = pl.sib;
     const int lane = lane_id();
@@ -576,7 +607,10 @@ __device__ __forceinline__ uint32_t count_last(const KParams &P, WaveLds &w, Ter
     }
     wave_sync();
     // ---- stage F sets in LDS so per-sibling probes hit LDS: a bitmap over
-    // [min F, max F] when it fits, else a sorted copy when that fits
+    // [min F, max F] when it fits, else a sorted copy when that fits, else (a
+    // large F that the siblings' rows will probe) a bitmap in the wave's
+    // global region
+    uint32_t used_g = 0;
     {
         uint32_t used = 0;
         for (uint32_t j = 0; j < K; ++j) {
@@ -602,11 +636,25 @@ __device__ __forceinline__ uint32_t count_last(const KParams &P, WaveLds &w, Ter
                 t.flags |= 8u;
                 t.st_off = used;
                 used += len;
+            } else if (gb && (t.sflags & 1u) && len >= P.gbm_min && used_g + words <= (uint64_t)P.gbm_words &&
+                       gbm_pays(P, w, s, len)) {
+                // too large for LDS: a bitmap in the wave's global region, so a
+                // probe is one L2 read instead of a binary search of F
+                for (uint32_t q = lane; q < len; q += WAVE) {
+                    const uint32_t r = t.F.ptr[t.F.b + q] - fmin;
+                    atomicOr(gb + used_g + (r >> 5), 1u << (r & 31u));
+                }
+                t.flags |= 32u;
+                t.st_off = used_g;
+                t.bm_lo = fmin;
+                t.bm_hi = fmax;
+                used_g += (uint32_t)words;
             } else {
                 continue;
             }
             terms[j] = t;
         }
+        if (used_g) __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");  // the bits are set before any probe
         wave_sync();
     }
     // ---- siblings
@@ -664,7 +712,7 @@ __device__ __forceinline__ uint32_t count_last(const KParams &P, WaveLds &w, Ter
                             // probes into a staged F cost LDS reads (one for a bitmap),
                             // probes into a row cost a global binary search
                             const uint64_t ratio =
-                                (t.flags & 16u) ? P.bitmap_ratio : ((t.flags & 8u) ? P.probe_ratio : 1u);
+                                (t.flags & 48u) ? P.bitmap_ratio : ((t.flags & 8u) ? P.probe_ratio : 1u);
                             if ((uint64_t)(re - rb) <= ratio * (uint64_t)(fe - fb)) {
                                 flag = 1;
                                 dstart = rb;
@@ -738,7 +786,7 @@ __device__ __forceinline__ uint32_t count_last(const KParams &P, WaveLds &w, Ter
                 bool hit;
                 if (fl & 1u) {
                     uint32_t x = P.nbr[en.y + off];
-                    hit = term_has(w, t, x, en.z, en.w);
+                    hit = term_has(w, t, gb, x, en.z, en.w);
                 } else {
                     uint32_t x = t.F.ptr[en.y + off];
                     hit = contains(P.nbr, en.z, en.w, x);
@@ -776,6 +824,16 @@ __device__ __forceinline__ uint32_t count_last(const KParams &P, WaveLds &w, Ter
     }
     for (int off = WAVE / 2; off > 0; off >>= 1) local += __shfl_xor(local, off);
     acc.add(A_COUNT, local);
+    if (used_g) {
+        // leave the global region all zero for the next call
+        for (uint32_t k = 0; k < K; ++k) {
+            const TermLds t = terms[k];
+            if (!(t.flags & 32u)) continue;
+            for (uint32_t q = lane; q < t.F.e - t.F.b; q += WAVE)
+                gb[t.st_off + ((t.F.ptr[t.F.b + q] - t.bm_lo) >> 5)] = 0u;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");
+    }
     return steps;
 }
 
@@ -874,7 +932,8 @@ __device__ __forceinline__ void donate(const KParams &P, WaveLds &w, uint32_t pl
 // images of levels < l0 fixed.  Levels below `stop` are enumerated; `stop`
 // is counted for all its siblings at once (or emitted when enumerating).
 template <bool ENUM>
-__device__ __forceinline__ void run_dfs(const KParams &P, WaveLds &w, TermLds *terms, uint32_t *wb, Acc &acc, uint32_t plan, int l0,
+__device__ __forceinline__ void run_dfs(const KParams &P, WaveLds &w, TermLds *terms, uint32_t *wb, uint32_t *gb, Acc &acc,
+                                        uint32_t plan, int l0,
                         int stop) {
     const DPlan &pl = w.pl;
     uint32_t spent = 0, since = 0;  // wave steps since the last donation / node-cap report
@@ -892,7 +951,7 @@ __device__ __forceinline__ void run_dfs(const KParams &P, WaveLds &w, TermLds *t
                     emit_last(P, w, wb, acc);
                 }
             } else {
-                const uint32_t st = count_last(P, w, terms, wb, acc);
+                const uint32_t st = count_last(P, w, terms, wb, gb, acc);
                 spent += st;
                 since += st;
             }
@@ -963,6 +1022,7 @@ __global__ void __launch_bounds__(256, SGE_WAVES_PER_SIMD) wcoj_kernel(KParams P
     TermLds *terms = reinterpret_cast<TermLds *>(lds_raw + sizeof(WaveLds) * nwaves) + wid * MAX_TERMS;
     const uint64_t gwave = (uint64_t)blockIdx.x * (blockDim.x / WAVE) + wid;
     uint32_t *wb = P.scratch + gwave * P.wave_stride;
+    uint32_t *gb = P.gbm ? P.gbm + gwave * P.gbm_words : nullptr;
     Acc acc{w.accv};
     if (lane == 0)
         for (int k = 0; k < A_NUM; ++k) w.accv[k] = 0;
@@ -1020,7 +1080,7 @@ __global__ void __launch_bounds__(256, SGE_WAVES_PER_SIMD) wcoj_kernel(KParams P
             const uint32_t rb = it[2], re = it[3];
             w.pos[l] = max(rb, w.cand[l].b);
             w.iend[l] = min(re, w.cand[l].e);
-            run_dfs<ENUM>(P, w, terms, wb, acc, plan, l, stop);
+            run_dfs<ENUM>(P, w, terms, wb, gb, acc, plan, l, stop);
             continue;
         }
         // ---- root pass: chunk of subtasks of one plan
@@ -1079,7 +1139,7 @@ __global__ void __launch_bounds__(256, SGE_WAVES_PER_SIMD) wcoj_kernel(KParams P
                 explore_after(P, w, wb, 1, acc);
             }
             if (!ENUM && pl.sib == 1) {
-                count_last(P, w, terms, wb, acc);
+                count_last(P, w, terms, wb, gb, acc);
                 continue;
             }
             if (ENUM && pl.k == 3) {
@@ -1094,7 +1154,7 @@ __global__ void __launch_bounds__(256, SGE_WAVES_PER_SIMD) wcoj_kernel(KParams P
                 w.pos[2] = c2.b + (uint32_t)(len * ci / nch);
                 w.iend[2] = c2.b + (uint32_t)(len * (ci + 1) / nch);
             }
-            run_dfs<ENUM>(P, w, terms, wb, acc, plan, 2, stop);
+            run_dfs<ENUM>(P, w, terms, wb, gb, acc, plan, 2, stop);
         }
     }
     if (cur_plan != NONE) acc.add(A_CYCLES, __builtin_amdgcn_s_memtime() - t_item);
@@ -1124,6 +1184,8 @@ struct DeviceCtx {
     unsigned long long *icount = nullptr;     // [2] item counts, [2] work-cap counter
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     uint32_t ex_base = 0, pre_f = 0, term_base = 0, nbuf = NBUF_MAX;  // slab layout of the uploaded plans
+    uint32_t *gbm = nullptr;  // per-wave global bitmap regions (kept all zero between uses)
+    size_t gbm_bytes = 0;
 };
 
 constexpr uint64_t ITEM_CAP = 1ull << 21;  // continuation items per pass buffer (96 MB)
@@ -1311,6 +1373,7 @@ struct LaunchCfg {
     uint32_t probe_ratio = 2;
     uint32_t bitmap_ratio = 2;
     unsigned long long node_cap = 0;
+    uint32_t gbm_min = 2048;
 };
 
 int launch_cfg(const sge_tuning *t, bool enumerate, LaunchCfg &cfg) {
@@ -1329,6 +1392,7 @@ int launch_cfg(const sge_tuning *t, bool enumerate, LaunchCfg &cfg) {
     cfg.probe_ratio = t->stage_max ? t->stage_max : 2u;
     cfg.bitmap_ratio = t->bitmap_ratio ? t->bitmap_ratio : cfg.probe_ratio;
     cfg.node_cap = (unsigned long long)t->node_cap_k * NODE_STEP;
+    cfg.gbm_min = t->global_bitmap_min ? t->global_bitmap_min : 2048u;
     cfg.lds = (size_t)(cfg.block / WAVE) * (sizeof(WaveLds) + MAX_TERMS * sizeof(TermLds));
     int per_cu = (int)t->blocks_per_cu;
     if (per_cu <= 0) {
@@ -1386,6 +1450,14 @@ void set_layout(KParams &kp, uint32_t ex_base, uint32_t pre_f, uint32_t term_bas
     kp.term_base = term_base;
 }
 
+// Words of a wave's global bitmap region: three terms' bitmaps over the whole
+// id range, at most GBM_MAX_WORDS (3 MB per wave; 26 GB for 8,192 wavefronts).
+constexpr uint64_t GBM_MAX_WORDS = 3ull << 18;
+uint32_t gbm_words_for(const sge_graph *g) {
+    const uint64_t w = 3ull * (((uint64_t)g->h.n >> 5) + 1) + 64;
+    return (uint32_t)((std::min(w, GBM_MAX_WORDS) + 63) / 64 * 64);
+}
+
 int ensure_scratch(const sge_graph *g, const LaunchCfg &cfg, KParams &kp) {
     DeviceCtx &c = dctx();
     uint32_t stride = ((g->h.max_degree + 64 + 63) / 64) * 64;
@@ -1397,6 +1469,18 @@ int ensure_scratch(const sge_graph *g, const LaunchCfg &cfg, KParams &kp) {
     int rc = ensure_buffer((void **)&c.scratch, c.scratch_bytes, need);
     if (rc) return rc;
     kp.scratch = c.scratch;
+    // the bitmap region starts all zero and every use clears what it set
+    kp.gbm_words = gbm_words_for(g);
+    const size_t gneed = waves * (size_t)kp.gbm_words * sizeof(uint32_t);
+    if (gneed > c.gbm_bytes) {
+        if (c.gbm) HIP_TRY(hipFree(c.gbm));
+        c.gbm = nullptr;
+        c.gbm_bytes = 0;
+        HIP_TRY(hipMalloc(&c.gbm, gneed));
+        HIP_TRY(hipMemset(c.gbm, 0, gneed));
+        c.gbm_bytes = gneed;
+    }
+    kp.gbm = c.gbm;
     return SGE_OK;
 }
 
@@ -1450,6 +1534,7 @@ int run_launch(sge_graph *g, const DPlan *dplans, uint32_t np, PlanOut *dout, ui
     kp.probe_ratio = cfg.probe_ratio;
     kp.bitmap_ratio = cfg.bitmap_ratio;
     kp.node_cap = cfg.node_cap;
+    kp.gbm_min = cfg.gbm_min;
     kp.node_ctr = c.icount + 2;  // read back with the item counts, in one copy per pass
     reset_kernel<<<(np + 255) / 256, 256, 0, c.stream>>>(dout, np, c.queue, emb_n, kp.node_ctr);
     reset_pass_kernel<<<1, 64, 0, c.stream>>>(c.queue, c.icount);
@@ -2003,6 +2088,7 @@ struct sge_batch {
     PlanOut *dout = nullptr;
     unsigned long long *queue = nullptr;
     uint32_t *scratch = nullptr;
+    uint32_t *gbm = nullptr;  // global bitmap regions, all zero between uses
     uint32_t *items[2] = {nullptr, nullptr};
     unsigned long long *icount = nullptr;
     KParams kp;           // root-pass parameters (continuation passes derive from it)
@@ -2101,6 +2187,10 @@ extern "C" int sge_batch_create_sampled(sge_graph *g, const sge_plan *plans, uin
     if ((e = hipMalloc(&b->dout, sizeof(PlanOut) * nplans)) != hipSuccess) return fail(e, "out");
     if ((e = hipMalloc(&b->queue, sizeof(unsigned long long) * 2)) != hipSuccess) return fail(e, "queue");
     if ((e = hipMalloc(&b->scratch, waves * kp.wave_stride * sizeof(uint32_t))) != hipSuccess) return fail(e, "scratch");
+    kp.gbm_words = gbm_words_for(g);
+    if ((e = hipMalloc(&b->gbm, waves * (size_t)kp.gbm_words * sizeof(uint32_t))) != hipSuccess) return fail(e, "bitmaps");
+    if ((e = hipMemset(b->gbm, 0, waves * (size_t)kp.gbm_words * sizeof(uint32_t))) != hipSuccess)
+        return fail(e, "bitmaps");
     for (int q = 0; q < 2; ++q)
         if ((e = hipMalloc(&b->items[q], sizeof(uint32_t) * ITEM_WORDS * ITEM_CAP)) != hipSuccess)
             return fail(e, "items");
@@ -2125,11 +2215,13 @@ extern "C" int sge_batch_create_sampled(sge_graph *g, const sge_plan *plans, uin
     kp.nitems = items_per_plan(max_sub, cfg) * nplans;
     kp.queue = b->queue;
     kp.scratch = b->scratch;
+    kp.gbm = b->gbm;
     kp.out = b->dout;
     kp.item_cap = ITEM_CAP;
     kp.budget = cfg.budget;
     kp.probe_ratio = cfg.probe_ratio;
     kp.bitmap_ratio = cfg.bitmap_ratio;
+    kp.gbm_min = cfg.gbm_min;
     kp.out_items = b->items[0];
     kp.out_count = b->icount;
     b->kp = kp;
@@ -2205,6 +2297,7 @@ extern "C" void sge_batch_destroy(sge_batch *b) {
     if (b->dout) (void)hipFree(b->dout);
     if (b->queue) (void)hipFree(b->queue);
     if (b->scratch) (void)hipFree(b->scratch);
+    if (b->gbm) (void)hipFree(b->gbm);
     for (int q = 0; q < 2; ++q)
         if (b->items[q]) (void)hipFree(b->items[q]);
     if (b->icount) (void)hipFree(b->icount);

@@ -558,3 +558,30 @@ def test_bag_rmat8_all_78_batched():
     got = b.run()[0]
     b.close()
     assert got == [r["wcoj"]["count"] for r in gold]
+
+
+def test_global_bitmaps_forced_c2_goldens():
+    """F sets too large for the LDS stage go to per-wave global bitmaps when
+    their siblings' rows will probe them.  With the size threshold forced low
+    the path runs on C2 (by default only hub-sized sets use it): counts equal
+    the reference's, plain and batched, and repeated calls find the region
+    cleared."""
+    g = W.graph_c2()
+    gold3 = golden("c3")["patterns"]
+    pats = W.six_vertex_set()
+    plans = [sge.build_plan(pats[r["index"]], g, sge.MODE_IEP) for r in gold3]
+    want = [r["iep"]["count"] for r in gold3]
+    t = sge.default_tuning()
+    t.global_bitmap_min = 16
+    for rep in range(2):
+        say("forced global bitmaps, plain, rep", rep)
+        assert sge.count(g, plans, tuning=t)[0] == want
+    say("forced global bitmaps, batched")
+    b = sge.Batch(g, plans, tuning=t)
+    for _ in range(2):
+        assert b.run()[0] == want
+    b.close()
+    gold2 = golden("c2")["patterns"]
+    for name in ["near5clique", "house"]:
+        for mode, key in ((sge.MODE_WCOJ, "wcoj"), (sge.MODE_IEP, "iep")):
+            assert sge.count(g, sge.build_plan(W.named(name), g, mode), tuning=t)[0][0] == gold2[name][key]["count"]
