@@ -208,3 +208,34 @@ def test_bag_validation():
         for b in sge.plan_bags(p, pl):
             assert b in sge.valid_bags(pl)
             assert sge.set_bag(pl, b).bag
+
+
+def test_ctypes_mirrors_match_the_c_abi(tmp_path):
+    """Every ctypes mirror in the Python binding has the size and field offsets
+    of its struct in include/sge.h (compiled with the host C compiler)."""
+    import ctypes
+    import shutil
+    import subprocess
+
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    mirrors = {"sge_graph_stats": sge.GraphStats, "sge_pattern": sge.CPattern, "sge_pattern_info": sge.PatternInfo,
+               "sge_plan": sge.Plan, "sge_tuning": sge.Tuning, "sge_stats": sge.Stats}
+    inc = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include")
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "sge.h"', "int main(void) {"]
+    for cname, py in mirrors.items():
+        lines.append(f'    printf("{cname} size %zu\\n", sizeof({cname}));')
+        for fname, _ in py._fields_:
+            lines.append(f'    printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines += ["    return 0;", "}"]
+    src = tmp_path / "abi.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "abi"
+    subprocess.run([cc, "-I", inc, str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {tuple(l.split()[:2]): int(l.split()[2]) for l in out if l.strip()}
+    for cname, py in mirrors.items():
+        assert got[(cname, "size")] == ctypes.sizeof(py), cname
+        for fname, _ in py._fields_:
+            assert got[(cname, fname)] == getattr(py, fname).offset, (cname, fname)
