@@ -178,6 +178,20 @@ def test_count_without_gpu_fails_loudly():
         sge.count(g, sge.build_plan(W.named("diamond"), g))
     with pytest.raises(sge.SgeError):
         sge.Graph.from_edges(4, [(0, 1), (1, 2)], device_build=True)
+    # the CLI as well: an error and a non-zero exit, never a count
+    import subprocess
+
+    cli = os.path.join(os.path.dirname(sge.LIB_PATH), "graph_mining_system")
+    if os.path.exists(cli):
+        import tempfile
+
+        with tempfile.TemporaryDirectory() as d:
+            g.write(os.path.join(d, "g.txt"))
+            with open(os.path.join(d, "q.txt"), "w") as f:
+                f.write("4 5\n0 1\n0 2\n1 2\n1 3\n2 3\n")
+            r = subprocess.run([cli, "-q", os.path.join(d, "q.txt"), "-d", os.path.join(d, "g.txt")],
+                               capture_output=True, text=True, timeout=60)
+        assert r.returncode != 0 and "error" in r.stderr and "Number of matches" not in r.stdout
     bag_plan = sge.set_bag(sge.build_plan(W.named("house"), g, sge.MODE_WCOJ, [2, 3, 4, 0, 1]), [2, 3, 4])
     with pytest.raises(sge.SgeError):
         sge.Trie(g, bag_plan)
