@@ -3,7 +3,7 @@ from its own sources by oracle/ref.mk into oracle/_ref/, on this repo's
 synthetic graphs.  Run in a container that has /root/reference and the built
 binary:  python tests/golden/make_golden.py [section ...]
 
-Sections: c1, c2, c3, rmat7, rmat7_iep, dataset.  Output: tests/golden/<section>.json
+Sections: c1, c2, c3, rmat7, rmat7_iep, queries, dataset.  Output: tests/golden/<section>.json
 (inputs are named by generator + parameters; counts are the reference's
 "Number of matches" lines; "bags" its "num bags" line).
 """
@@ -121,6 +121,24 @@ def section_rmat7_iep():
         r = run_ref(pp, gp, ["-u"], 8)
         res["patterns"].append({"index": i, "edges": p.edges(), "canon": p.info().canon, "iep": r})
         print(i, r, flush=True)
+    return res
+
+
+def section_queries():
+    """The reference's query files (dataset/light_query, dataset/huge_query) on
+    C1, WCOJ and IEP modes with one thread (its exploration mode needs -t > 1,
+    where single-bag plans crash it)."""
+    g = W.graph_c1(False)
+    gp = graph_file("c1", g)
+    res = {"graph": "chung_lu(n=379, m=914, gamma=2.5, cap=none, seed=0)", "n": g.n, "m": g.m, "patterns": []}
+    for d in ["light_query", "huge_query"]:
+        for f in sorted(os.listdir(os.path.join(REF_DATA, d))):
+            path = os.path.join(REF_DATA, d, f)
+            p = sge.Pattern.load(path)
+            rec = {"file": f"{d}/{f}", "k": p.k, "edges": p.edges(),
+                   "wcoj": run_ref(path, gp, [], 1), "iep": run_ref(path, gp, ["-u"], 1)}
+            res["patterns"].append(rec)
+            print(rec["file"], rec["wcoj"].get("count"), rec["iep"].get("count"), flush=True)
     return res
 
 

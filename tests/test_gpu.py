@@ -585,3 +585,16 @@ def test_global_bitmaps_forced_c2_goldens():
     for name in ["near5clique", "house"]:
         for mode, key in ((sge.MODE_WCOJ, "wcoj"), (sge.MODE_IEP, "iep")):
             assert sge.count(g, sge.build_plan(W.named(name), g, mode), tuning=t)[0][0] == gold2[name][key]["count"]
+
+
+def test_query_files_c1_goldens():
+    """The reference's 13 query files (dataset/light_query, huge_query) on C1 in
+    every mode: the reference's WCOJ counts (its IEP mode overcounts one file,
+    documented in test_oracle)."""
+    gold = golden("queries")
+    g = W.graph_c1()
+    for rec in gold["patterns"]:
+        p = sge.Pattern.from_edges(rec["k"], [tuple(e) for e in rec["edges"]])
+        for mode in MODES:
+            say("query file", rec["file"], "mode", mode)
+            assert sge.count(g, sge.build_plan(p, g, mode))[0][0] == rec["wcoj"]["count"], (rec["file"], mode)

@@ -324,3 +324,30 @@ def test_bag_join_counts_equal_plain_counts():
         for b in sge.valid_bags(plan):
             assert O.count(sge.set_bag(plan, b), off, nbr) == rec["wcoj"]["count"], (name, b)
     assert checked >= 50
+
+
+# The reference's IEP mode also overcounts huge_query/p8 on C1 (two triangles
+# joined by two edges): 64,398 against its own WCOJ count and brute force, 19,305.
+REF_IEP_OVERCOUNT_QUERIES = {"huge_query/p8.txt"}
+
+
+def test_golden_query_files_c1():
+    """The reference's 13 query files on C1: the oracle in every mode equals the
+    reference's WCOJ count and brute force; the reference's IEP count agrees
+    except for the documented overcount."""
+    gold = load_golden("queries")
+    g = W.graph_c1()
+    assert (g.n, g.m) == (gold["n"], gold["m"])
+    off, nbr = g.csr()
+    for rec in gold["patterns"]:
+        p = sge.Pattern.from_edges(rec["k"], [tuple(e) for e in rec["edges"]])
+        info = p.info()
+        brute = O.count_brute(p.k, p.c.adj, off, nbr)
+        want = brute if info.single_bag else brute // info.aut_size
+        assert rec["wcoj"]["count"] == want, rec["file"]
+        for mode in (sge.MODE_WCOJ, sge.MODE_EXPLORE, sge.MODE_IEP):
+            assert O.count(sge.build_plan(p, g, mode), off, nbr) == want, (rec["file"], mode)
+        if rec["file"] in REF_IEP_OVERCOUNT_QUERIES:
+            assert rec["iep"]["count"] > want
+        else:
+            assert rec["iep"]["count"] == want, rec["file"]
