@@ -3,7 +3,7 @@ from its own sources by oracle/ref.mk into oracle/_ref/, on this repo's
 synthetic graphs.  Run in a container that has /root/reference and the built
 binary:  python tests/golden/make_golden.py [section ...]
 
-Sections: c1, c2, c3, rmat7, rmat7_iep, queries, dataset.  Output: tests/golden/<section>.json
+Sections: c1, c2, c3, rmat7, rmat7_iep, queries, queries_c2, dataset.  Output: tests/golden/<section>.json
 (inputs are named by generator + parameters; counts are the reference's
 "Number of matches" lines; "bags" its "num bags" line).
 """
@@ -137,6 +137,24 @@ def section_queries():
             p = sge.Pattern.load(path)
             rec = {"file": f"{d}/{f}", "k": p.k, "edges": p.edges(),
                    "wcoj": run_ref(path, gp, [], 1), "iep": run_ref(path, gp, ["-u"], 1)}
+            res["patterns"].append(rec)
+            print(rec["file"], rec["wcoj"].get("count"), rec["iep"].get("count"), flush=True)
+    return res
+
+
+def section_queries_c2():
+    """The reference's query files on C2, WCOJ and IEP modes, 8 threads (one
+    thread for single-bag plans, which crash it with more)."""
+    g = W.graph_c2(False)
+    gp = graph_file("c2", g)
+    res = {"graph": "chung_lu(n=7115, m=100762, gamma=2.1, cap=1100, seed=1)", "n": g.n, "m": g.m, "patterns": []}
+    for d in ["light_query", "huge_query"]:
+        for f in sorted(os.listdir(os.path.join(REF_DATA, d))):
+            path = os.path.join(REF_DATA, d, f)
+            p = sge.Pattern.load(path)
+            t = 1 if p.info().single_bag else 8
+            rec = {"file": f"{d}/{f}", "k": p.k, "edges": p.edges(),
+                   "wcoj": run_ref(path, gp, [], t), "iep": run_ref(path, gp, ["-u"], t)}
             res["patterns"].append(rec)
             print(rec["file"], rec["wcoj"].get("count"), rec["iep"].get("count"), flush=True)
     return res

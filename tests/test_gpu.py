@@ -598,3 +598,17 @@ def test_query_files_c1_goldens():
         for mode in MODES:
             say("query file", rec["file"], "mode", mode)
             assert sge.count(g, sge.build_plan(p, g, mode))[0][0] == rec["wcoj"]["count"], (rec["file"], mode)
+
+
+def test_query_files_c2_goldens():
+    """The reference's 13 query files on C2 in every mode, plain and batched:
+    the reference's WCOJ counts."""
+    gold = golden("queries_c2")
+    g = W.graph_c2()
+    pats = [sge.Pattern.from_edges(r["k"], [tuple(e) for e in r["edges"]]) for r in gold["patterns"]]
+    want = [r["wcoj"]["count"] for r in gold["patterns"]]
+    for mode in MODES:
+        plans = [sge.build_plan(p, g, mode) for p in pats]
+        say("query files c2, mode", mode)
+        assert sge.count(g, plans)[0] == want
+        assert sge.count(g, plans, batched=True)[0] == want

@@ -351,3 +351,26 @@ def test_golden_query_files_c1():
             assert rec["iep"]["count"] > want
         else:
             assert rec["iep"]["count"] == want, rec["file"]
+
+
+# On C2 the reference's IEP mode overcounts these two query files (which ones
+# depends on its randomised plan choice: p5 agreed on C1).
+REF_IEP_OVERCOUNT_QUERIES_C2 = {"light_query/p5.txt", "huge_query/p8.txt"}
+
+
+def test_golden_query_files_c2():
+    """The reference's query files on C2: its IEP counts equal its WCOJ counts
+    except for the documented overcounts, and the oracle (IEP mode) equals the
+    WCOJ counts on the files it finishes within seconds (the GPU test covers
+    all 13 in every mode)."""
+    gold = load_golden("queries_c2")
+    g = W.graph_c2()
+    assert (g.n, g.m) == (gold["n"], gold["m"])
+    off, nbr = g.csr()
+    for rec in gold["patterns"]:
+        w, i = rec["wcoj"]["count"], rec["iep"]["count"]
+        assert (i > w) if rec["file"] in REF_IEP_OVERCOUNT_QUERIES_C2 else (i == w), rec["file"]
+        if rec["file"] in ("light_query/p5.txt", "huge_query/p8.txt", "light_query/p4.txt", "huge_query/p4.txt"):
+            continue  # 3-85 s each in the oracle
+        p = sge.Pattern.from_edges(rec["k"], [tuple(e) for e in rec["edges"]])
+        assert O.count(sge.build_plan(p, g, sge.MODE_IEP), off, nbr) == w, rec["file"]
