@@ -612,3 +612,17 @@ def test_query_files_c2_goldens():
         say("query files c2, mode", mode)
         assert sge.count(g, plans)[0] == want
         assert sge.count(g, plans, batched=True)[0] == want
+
+
+def test_bag_joins_on_query_files_c1():
+    """Every valid bag of every query file's plan (WCOJ and IEP) on C1: the
+    reference's WCOJ counts through the device tries."""
+    gold = golden("queries")
+    g = W.graph_c1()
+    for rec in gold["patterns"]:
+        p = sge.Pattern.from_edges(rec["k"], [tuple(e) for e in rec["edges"]])
+        for mode in (sge.MODE_WCOJ, sge.MODE_IEP):
+            plan = sge.build_plan(p, g, mode)
+            for b in sge.valid_bags(plan):
+                say("query file bag", rec["file"], mode, b)
+                assert sge.count(g, sge.set_bag(plan, b))[0][0] == rec["wcoj"]["count"], (rec["file"], mode, b)

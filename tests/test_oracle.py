@@ -374,3 +374,20 @@ def test_golden_query_files_c2():
             continue  # 3-85 s each in the oracle
         p = sge.Pattern.from_edges(rec["k"], [tuple(e) for e in rec["edges"]])
         assert O.count(sge.build_plan(p, g, sge.MODE_IEP), off, nbr) == w, rec["file"]
+
+
+def test_bag_joins_on_query_files_c1():
+    """Every valid bag of every query file's plan (WCOJ and IEP) gives the
+    reference's WCOJ count in the oracle's trie join."""
+    gold = load_golden("queries")
+    g = W.graph_c1()
+    off, nbr = g.csr()
+    checked = 0
+    for rec in gold["patterns"]:
+        p = sge.Pattern.from_edges(rec["k"], [tuple(e) for e in rec["edges"]])
+        for mode in (sge.MODE_WCOJ, sge.MODE_IEP):
+            plan = sge.build_plan(p, g, mode)
+            for b in sge.valid_bags(plan):
+                assert O.count(sge.set_bag(plan, b), off, nbr) == rec["wcoj"]["count"], (rec["file"], mode, b)
+                checked += 1
+    assert checked >= 20
