@@ -37,6 +37,7 @@ static int finish_graph(uint32_t n, EdgeVec &edges, uint32_t flags, sge_graph **
 }
 
 extern "C" int sge_graph_load(const char *path, uint32_t flags, sge_graph **out) {
+    SGE_API_BEGIN
     if (!path) {
         set_error("null path");
         return SGE_EINVAL;
@@ -46,9 +47,11 @@ extern "C" int sge_graph_load(const char *path, uint32_t flags, sge_graph **out)
     int rc = read_edge_file(path, n, edges);
     if (rc) return rc;
     return finish_graph(n, edges, flags, out);
+    SGE_API_END
 }
 
 extern "C" int sge_graph_from_edges(uint32_t n, const uint32_t *uv, uint64_t m, uint32_t flags, sge_graph **out) {
+    SGE_API_BEGIN
     if (m && !uv) {
         set_error("null edge list");
         return SGE_EINVAL;
@@ -62,10 +65,12 @@ extern "C" int sge_graph_from_edges(uint32_t n, const uint32_t *uv, uint64_t m, 
         edges[i] = ((uint64_t)uv[2 * i] << 32) | uv[2 * i + 1];
     }
     return finish_graph(n, edges, flags, out);
+    SGE_API_END
 }
 
 extern "C" int sge_graph_from_csr(const uint32_t *off, const uint32_t *nbr, uint32_t n, uint64_t nnz, uint32_t flags,
                                   sge_graph **out) {
+    SGE_API_BEGIN
     if (!off || (nnz && !nbr)) {
         set_error("null CSR array");
         return SGE_EINVAL;
@@ -91,10 +96,12 @@ extern "C" int sge_graph_from_csr(const uint32_t *off, const uint32_t *nbr, uint
         }
     }
     return finish_graph(n, edges, flags, out);
+    SGE_API_END
 }
 
 extern "C" int sge_graph_chung_lu(uint32_t n, uint64_t m, double gamma, double cap, uint64_t seed, uint32_t flags,
                                   sge_graph **out) {
+    SGE_API_BEGIN
     if (n < 2 || gamma <= 1.0 || m > (uint64_t)n * (n - 1) / 2) {
         set_error("chung_lu: need n >= 2, gamma > 1, m <= n(n-1)/2");
         return SGE_EINVAL;
@@ -102,10 +109,12 @@ extern "C" int sge_graph_chung_lu(uint32_t n, uint64_t m, double gamma, double c
     EdgeVec edges;
     chung_lu_edges(n, m, gamma, cap > 0 ? cap : 1e300, seed, edges);
     return finish_graph(n, edges, flags, out);
+    SGE_API_END
 }
 
 extern "C" int sge_graph_rmat(uint32_t scale, uint32_t ef, double a, double b, double c, uint64_t seed, uint32_t flags,
                               sge_graph **out) {
+    SGE_API_BEGIN
     if (scale < 1 || scale > 30 || ef == 0 || a < 0 || b < 0 || c < 0 || a + b + c > 1.0) {
         set_error("rmat: bad parameters");
         return SGE_EINVAL;
@@ -113,9 +122,11 @@ extern "C" int sge_graph_rmat(uint32_t scale, uint32_t ef, double a, double b, d
     EdgeVec edges;
     rmat_edges(scale, ef, a, b, c, seed, edges);
     return finish_graph(1u << scale, edges, flags, out);
+    SGE_API_END
 }
 
 extern "C" int sge_graph_stats_get(const sge_graph *g, sge_graph_stats *s) {
+    SGE_API_BEGIN
     if (!g || !s) {
         set_error("null argument");
         return SGE_EINVAL;
@@ -128,9 +139,11 @@ extern "C" int sge_graph_stats_get(const sge_graph *g, sge_graph_stats *s) {
     s->csr_bytes = 4ull * ((uint64_t)g->h.n + 1) + 4ull * g->h.nnz;
     s->build_ms = g->build_ms;
     return SGE_OK;
+    SGE_API_END
 }
 
 extern "C" int sge_graph_csr(const sge_graph *g, uint32_t *off, uint32_t *nbr) {
+    SGE_API_BEGIN
     if (!g || !off || (g->h.nnz && !nbr)) {
         set_error("null argument");
         return SGE_EINVAL;
@@ -138,18 +151,22 @@ extern "C" int sge_graph_csr(const sge_graph *g, uint32_t *off, uint32_t *nbr) {
     std::memcpy(off, g->h.off.data(), 4ull * ((uint64_t)g->h.n + 1));
     if (g->h.nnz) std::memcpy(nbr, g->h.nbr.data(), 4ull * g->h.nnz);
     return SGE_OK;
+    SGE_API_END
 }
 
 extern "C" int sge_graph_perm(const sge_graph *g, uint32_t *perm) {
+    SGE_API_BEGIN
     if (!g || !perm) {
         set_error("null argument");
         return SGE_EINVAL;
     }
     std::memcpy(perm, g->h.perm.data(), 4ull * g->h.n);
     return SGE_OK;
+    SGE_API_END
 }
 
 extern "C" int sge_graph_write(const sge_graph *g, const char *path, int binary) {
+    SGE_API_BEGIN
     if (!g || !path) {
         set_error("null argument");
         return SGE_EINVAL;
@@ -163,25 +180,35 @@ extern "C" int sge_graph_write(const sge_graph *g, const char *path, int binary)
             if (a < b) es.push_back(((uint64_t)a << 32) | b);
         }
     std::sort(es.begin(), es.end());
+    if (binary && es.size() > 0xffffffffull) {  // BINGRPH stores m as u32
+        set_error("BINGRPH holds at most 2^32 - 1 edges");
+        return SGE_ERANGE;
+    }
     FILE *f = std::fopen(path, binary ? "wb" : "w");
     if (!f) {
         set_error(std::string("Can not create file ") + path + "!");
         return SGE_EIO;
     }
+    bool ok = true;
     if (binary) {
         uint32_t hdr[2] = {h.n, (uint32_t)es.size()};
-        std::fwrite("BINGRPH", 1, 7, f);
-        std::fwrite(hdr, 4, 2, f);
-        for (uint64_t e : es) {
-            uint32_t uv[2] = {(uint32_t)(e >> 32), (uint32_t)e};
-            std::fwrite(uv, 4, 2, f);
+        ok = std::fwrite("BINGRPH", 1, 7, f) == 7 && std::fwrite(hdr, 4, 2, f) == 2;
+        for (size_t i = 0; ok && i < es.size(); ++i) {
+            uint32_t uv[2] = {(uint32_t)(es[i] >> 32), (uint32_t)es[i]};
+            ok = std::fwrite(uv, 4, 2, f) == 2;
         }
     } else {
-        std::fprintf(f, "%u %llu\n", h.n, (unsigned long long)es.size());
-        for (uint64_t e : es) std::fprintf(f, "%u %u\n", (uint32_t)(e >> 32), (uint32_t)e);
+        ok = std::fprintf(f, "%u %llu\n", h.n, (unsigned long long)es.size()) > 0;
+        for (size_t i = 0; ok && i < es.size(); ++i)
+            ok = std::fprintf(f, "%u %u\n", (uint32_t)(es[i] >> 32), (uint32_t)es[i]) > 0;
     }
-    std::fclose(f);
+    ok = (std::fclose(f) == 0) && ok;
+    if (!ok) {
+        set_error(std::string("write failed: ") + path);
+        return SGE_EIO;
+    }
     return SGE_OK;
+    SGE_API_END
 }
 
 extern "C" void sge_graph_destroy(sge_graph *g) {

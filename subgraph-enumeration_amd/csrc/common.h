@@ -1,6 +1,8 @@
 // Internal host-side structures shared by the C++ translation units of libsge.
 #pragma once
 #include <cstdint>
+#include <new>
+#include <stdexcept>
 #include <string>
 #include <vector>
 #include <utility>
@@ -12,6 +14,26 @@ struct sge_graph;
 namespace sge {
 
 void set_error(const std::string &msg);
+
+// Every int-returning C entry point runs its body inside this pair, so that no
+// C++ exception unwinds into a C, ctypes or cgo caller (which would abort the
+// process): host allocation failures become SGE_ENOMEM, anything else
+// SGE_EINVAL, each with its message in sge_last_error().
+#define SGE_API_BEGIN try {
+#define SGE_API_END                                                               \
+    }                                                                             \
+    catch (const std::bad_alloc &) {                                              \
+        sge::set_error("host memory allocation failed");                          \
+        return SGE_ENOMEM;                                                        \
+    }                                                                             \
+    catch (const std::length_error &e) {                                          \
+        sge::set_error(std::string("host allocation too large: ") + e.what());    \
+        return SGE_ENOMEM;                                                        \
+    }                                                                             \
+    catch (const std::exception &e) {                                             \
+        sge::set_error(std::string("internal error: ") + e.what());               \
+        return SGE_EINVAL;                                                        \
+    }
 
 // Symmetric CSR, ascending neighbour lists, u32 offsets (the reference's EdgeID
 // is u32 too, config/config.h:43-45).  perm[new] = original id.

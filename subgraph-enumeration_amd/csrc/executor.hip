@@ -1820,6 +1820,7 @@ extern "C" void sge_tuning_default(sge_tuning *t) {
 }
 
 extern "C" int sge_graph_to_device(sge_graph *g) {
+    SGE_API_BEGIN
     if (!g) {
         sge::set_error("sge_graph_to_device: null graph");
         return SGE_EINVAL;
@@ -1842,6 +1843,7 @@ extern "C" int sge_graph_to_device(sge_graph *g) {
     }
     g->device = dctx().dev;
     return SGE_OK;
+    SGE_API_END
 }
 
 extern "C" void sge_graph_device_free(sge_graph *g) {
@@ -1937,24 +1939,29 @@ static int count_impl(sge_graph *g, const sge_plan *plans, uint32_t nplans, uint
 
 extern "C" int sge_count(sge_graph *g, const sge_plan *plans, uint32_t nplans, uint32_t flags, const sge_tuning *tune,
                          uint64_t *counts, sge_stats *stats, sge_stats *per_plan) {
+    SGE_API_BEGIN
     if (!g) {
         sge::set_error("sge_count: null graph");
         return SGE_EINVAL;
     }
     return count_impl(g, plans, nplans, flags, tune, 0, g->h.nnz, 1, counts, stats, per_plan);
+    SGE_API_END
 }
 
 extern "C" int sge_count_range(sge_graph *g, const sge_plan *plan, uint64_t task_begin, uint64_t task_end,
                                const sge_tuning *tune, uint64_t *count, sge_stats *stats) {
+    SGE_API_BEGIN
     if (plan && plan->k == 1) {
         sge::set_error("sge_count_range: single-vertex patterns have no edge tasks");
         return SGE_EINVAL;
     }
     return count_impl(g, plan, 1, 0, tune, task_begin, task_end, 1, count, stats, nullptr);
+    SGE_API_END
 }
 
 extern "C" int sge_count_sampled(sge_graph *g, const sge_plan *plans, uint32_t nplans, uint64_t stride,
                                  uint64_t offset, const sge_tuning *tune, uint64_t *counts, sge_stats *per_plan) {
+    SGE_API_BEGIN
     if (!g || stride == 0) {
         sge::set_error("sge_count_sampled: null graph or zero stride");
         return SGE_EINVAL;
@@ -1965,9 +1972,11 @@ extern "C" int sge_count_sampled(sge_graph *g, const sge_plan *plans, uint32_t n
             return SGE_EINVAL;
         }
     return count_impl(g, plans, nplans, 0, tune, offset, g->h.nnz, stride, counts, nullptr, per_plan);
+    SGE_API_END
 }
 
 extern "C" int sge_num_tasks(const sge_graph *g, const sge_plan *plan, uint64_t *n) {
+    SGE_API_BEGIN
     if (!g || !n) {
         sge::set_error("sge_num_tasks: null argument");
         return SGE_EINVAL;
@@ -1975,9 +1984,11 @@ extern "C" int sge_num_tasks(const sge_graph *g, const sge_plan *plan, uint64_t 
     (void)plan;
     *n = g->h.nnz;
     return SGE_OK;
+    SGE_API_END
 }
 
 extern "C" int sge_enumerate(sge_graph *g, const sge_plan *plan, uint32_t *out, uint64_t cap, uint64_t *n_out) {
+    SGE_API_BEGIN
     if (!g || !plan || !n_out || (cap && !out)) {
         sge::set_error("sge_enumerate: bad arguments");
         return SGE_EINVAL;
@@ -2019,6 +2030,7 @@ extern "C" int sge_enumerate(sge_graph *g, const sge_plan *plan, uint32_t *out, 
         return SGE_ERANGE;
     }
     return SGE_OK;
+    SGE_API_END
 }
 
 // ------------------------------------------------------------- bag tries
@@ -2028,6 +2040,7 @@ struct sge_trie {
 };
 
 extern "C" int sge_trie_build(sge_graph *g, const sge_plan *plan, sge_trie **out) {
+    SGE_API_BEGIN
     if (!g || !plan || !out) {
         sge::set_error("sge_trie_build: null argument");
         return SGE_EINVAL;
@@ -2049,9 +2062,11 @@ extern "C" int sge_trie_build(sge_graph *g, const sge_plan *plan, sge_trie **out
     }
     *out = t;
     return SGE_OK;
+    SGE_API_END
 }
 
 extern "C" int sge_trie_shape(const sge_trie *t, uint32_t *depth, uint64_t *tuples, uint64_t *nodes, double *build_ms) {
+    SGE_API_BEGIN
     if (!t) {
         sge::set_error("sge_trie_shape: null trie");
         return SGE_EINVAL;
@@ -2062,9 +2077,11 @@ extern "C" int sge_trie_shape(const sge_trie *t, uint32_t *depth, uint64_t *tupl
         for (int j = 0; j < MAXK; ++j) nodes[j] = j < (int)t->t.depth ? t->t.nodes[j] : 0;
     if (build_ms) *build_ms = t->t.build_ms;
     return SGE_OK;
+    SGE_API_END
 }
 
 extern "C" int sge_trie_copy(const sge_trie *t, uint32_t *values, uint32_t *child) {
+    SGE_API_BEGIN
     if (!t || !t->t.depth) {
         sge::set_error("sge_trie_copy: null trie");
         return SGE_EINVAL;
@@ -2074,6 +2091,7 @@ extern "C" int sge_trie_copy(const sge_trie *t, uint32_t *values, uint32_t *chil
     if (values && nval) HIP_TRY(hipMemcpy(values, t->t.val, sizeof(uint32_t) * nval, hipMemcpyDeviceToHost));
     if (child && nchild) HIP_TRY(hipMemcpy(child, t->t.child, sizeof(uint32_t) * nchild, hipMemcpyDeviceToHost));
     return SGE_OK;
+    SGE_API_END
 }
 
 extern "C" void sge_trie_destroy(sge_trie *t) { delete t; }
@@ -2118,11 +2136,14 @@ extern "C" int sge_batch_create_sampled(sge_graph *g, const sge_plan *plans, uin
 
 extern "C" int sge_batch_create(sge_graph *g, const sge_plan *plans, uint32_t nplans, const sge_tuning *tune,
                                 sge_batch **out) {
+    SGE_API_BEGIN
     return sge_batch_create_sampled(g, plans, nplans, 1, 0, tune, out);
+    SGE_API_END
 }
 
 extern "C" int sge_batch_create_sampled(sge_graph *g, const sge_plan *plans, uint32_t nplans, uint64_t stride,
                                         uint64_t offset, const sge_tuning *tune, sge_batch **out) {
+    SGE_API_BEGIN
     if (stride == 0) stride = 1;
     if (!g || !plans || !nplans || !out) {
         sge::set_error("sge_batch_create: bad arguments");
@@ -2247,9 +2268,11 @@ extern "C" int sge_batch_create_sampled(sge_graph *g, const sge_plan *plans, uin
     (void)c;
     *out = b;
     return SGE_OK;
+    SGE_API_END
 }
 
 extern "C" int sge_batch_run(sge_batch *b, uint64_t *counts, sge_stats *stats) {
+    SGE_API_BEGIN
     if (!b || !counts) {
         sge::set_error("sge_batch_run: bad arguments");
         return SGE_EINVAL;
@@ -2284,6 +2307,7 @@ extern "C" int sge_batch_run(sge_batch *b, uint64_t *counts, sge_stats *stats) {
     for (uint32_t i = 0; i < n; ++i) counts[i] = host[i].count * (b->plans[i].multiplier ? b->plans[i].multiplier : 1);
     if (stats) stats->wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return SGE_OK;
+    SGE_API_END
 }
 
 extern "C" void sge_batch_destroy(sge_batch *b) {
@@ -2305,6 +2329,7 @@ extern "C" void sge_batch_destroy(sge_batch *b) {
 }
 
 extern "C" int sge_device_info(int *num_cu, int *num_xcd_hint, char *name, size_t cap) {
+    SGE_API_BEGIN
     std::lock_guard<std::mutex> lk(dmutex());
     int rc = ensure_ctx();
     if (rc) return rc;
@@ -2316,6 +2341,7 @@ extern "C" int sge_device_info(int *num_cu, int *num_xcd_hint, char *name, size_
         std::snprintf(name, cap, "%s", prop.gcnArchName);
     }
     return SGE_OK;
+    SGE_API_END
 }
 
 // ------------------------------------------------------------ calibration
@@ -2331,6 +2357,7 @@ __global__ void __launch_bounds__(256) calib_copy_kernel(const uint4 *__restrict
 }  // namespace
 
 extern "C" int sge_calibration_copy(uint64_t bytes, double *ms) {
+    SGE_API_BEGIN
     if (bytes < 16 || bytes % 16) {
         sge::set_error("sge_calibration_copy: bytes must be a positive multiple of 16");
         return SGE_EINVAL;
@@ -2355,4 +2382,5 @@ extern "C" int sge_calibration_copy(uint64_t bytes, double *ms) {
     (void)hipFree(src);
     (void)hipFree(dst);
     return SGE_OK;
+    SGE_API_END
 }

@@ -308,6 +308,7 @@ extern "C" void sge_mlp_destroy(sge_mlp *M) {
 }
 
 extern "C" int sge_mlp_create(uint64_t seed, sge_mlp **out) {
+    SGE_API_BEGIN
     if (!out) {
         sge::set_error("sge_mlp_create: null output");
         return SGE_EINVAL;
@@ -346,9 +347,11 @@ extern "C" int sge_mlp_create(uint64_t seed, sge_mlp **out) {
     if (hipStreamSynchronize(M->stream)) return fail("sync");
     *out = M;
     return SGE_OK;
+    SGE_API_END
 }
 
 extern "C" int sge_mlp_set_params(sge_mlp *M, const float *params, size_t n) {
+    SGE_API_BEGIN
     if (!M || !params || n != (size_t)NPARAM) {
         sge::set_error("sge_mlp_set_params: bad arguments");
         return SGE_EINVAL;
@@ -358,15 +361,18 @@ extern "C" int sge_mlp_set_params(sge_mlp *M, const float *params, size_t n) {
     if (rc) return rc;
     HIP_TRY(hipStreamSynchronize(M->stream));
     return SGE_OK;
+    SGE_API_END
 }
 
 extern "C" int sge_mlp_get_params(sge_mlp *M, float *params, size_t n) {
+    SGE_API_BEGIN
     if (!M || !params || n != (size_t)NPARAM) {
         sge::set_error("sge_mlp_get_params: bad arguments");
         return SGE_EINVAL;
     }
     HIP_TRY(hipMemcpy(params, M->P, sizeof(float) * NPARAM, hipMemcpyDeviceToHost));
     return SGE_OK;
+    SGE_API_END
 }
 
 static int upload_x(sge_mlp *M, const float *X, uint32_t N) {
@@ -386,6 +392,7 @@ static int run_forward(sge_mlp *M, uint32_t N) {
 }
 
 extern "C" int sge_mlp_forward(sge_mlp *M, const float *X, uint32_t N, float *y) {
+    SGE_API_BEGIN
     if (!M || !X || !y || N == 0) {
         sge::set_error("sge_mlp_forward: bad arguments");
         return SGE_EINVAL;
@@ -397,6 +404,7 @@ extern "C" int sge_mlp_forward(sge_mlp *M, const float *X, uint32_t N, float *y)
     HIP_TRY(hipMemcpyAsync(y, M->y, sizeof(float) * N, hipMemcpyDeviceToHost, M->stream));
     HIP_TRY(hipStreamSynchronize(M->stream));
     return SGE_OK;
+    SGE_API_END
 }
 
 // forward + backward into M->G (no update)
@@ -418,6 +426,7 @@ static int run_grads(sge_mlp *M, const float *X, const float *t, uint32_t N) {
 }
 
 extern "C" int sge_mlp_gradients(sge_mlp *M, const float *X, const float *t, uint32_t N, float *grads, float *loss) {
+    SGE_API_BEGIN
     if (!M || !X || !t || !grads || N == 0) {
         sge::set_error("sge_mlp_gradients: bad arguments");
         return SGE_EINVAL;
@@ -428,9 +437,11 @@ extern "C" int sge_mlp_gradients(sge_mlp *M, const float *X, const float *t, uin
     if (loss) HIP_TRY(hipMemcpyAsync(loss, M->loss, sizeof(float), hipMemcpyDeviceToHost, M->stream));
     HIP_TRY(hipStreamSynchronize(M->stream));
     return SGE_OK;
+    SGE_API_END
 }
 
 extern "C" int sge_mlp_train_step(sge_mlp *M, const float *X, const float *t, uint32_t N, float lr, float *loss) {
+    SGE_API_BEGIN
     if (!M || !X || !t || N == 0) {
         sge::set_error("sge_mlp_train_step: bad arguments");
         return SGE_EINVAL;
@@ -446,4 +457,5 @@ extern "C" int sge_mlp_train_step(sge_mlp *M, const float *X, const float *t, ui
     if (loss) HIP_TRY(hipMemcpyAsync(loss, M->loss, sizeof(float), hipMemcpyDeviceToHost, M->stream));
     HIP_TRY(hipStreamSynchronize(M->stream));
     return SGE_OK;
+    SGE_API_END
 }

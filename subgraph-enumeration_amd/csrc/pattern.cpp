@@ -361,6 +361,7 @@ static Decomposition decompose_uncached(const Pattern &p, const std::vector<std:
 using namespace sge;
 
 extern "C" int sge_pattern_from_edges(uint32_t k, const uint32_t *uv, uint32_t m, sge_pattern *out) {
+    SGE_API_BEGIN
     if (!out || k == 0 || k > SGE_MAX_K || (m && !uv)) {
         set_error("sge_pattern_from_edges: bad arguments (k must be 1..8)");
         return SGE_EINVAL;
@@ -377,9 +378,11 @@ extern "C" int sge_pattern_from_edges(uint32_t k, const uint32_t *uv, uint32_t m
         out->adj[b] |= 1u << a;
     }
     return SGE_OK;
+    SGE_API_END
 }
 
 extern "C" int sge_pattern_load(const char *path, sge_pattern *out) {
+    SGE_API_BEGIN
     uint32_t n = 0;
     EdgeVec edges;
     // pattern files share the text format of data graphs; the optional orbit
@@ -397,9 +400,11 @@ extern "C" int sge_pattern_load(const char *path, sge_pattern *out) {
         uv.push_back((uint32_t)e);
     }
     return sge_pattern_from_edges(n, uv.data(), (uint32_t)edges.size(), out);
+    SGE_API_END
 }
 
 extern "C" int sge_pattern_analyze(const sge_pattern *cp, sge_pattern_info *out) {
+    SGE_API_BEGIN
     if (!cp || !out || cp->k == 0 || cp->k > SGE_MAX_K) {
         set_error("sge_pattern_analyze: bad arguments");
         return SGE_EINVAL;
@@ -423,10 +428,12 @@ extern "C" int sge_pattern_analyze(const sge_pattern *cp, sge_pattern_info *out)
     out->single_bag = d.single_bag;
     out->fhw = d.fhw;
     return SGE_OK;
+    SGE_API_END
 }
 
 extern "C" int sge_pattern_sample(uint32_t k, uint32_t emin, uint32_t emax, uint32_t want, uint64_t seed,
                                   const uint64_t *exclude, uint32_t n_exclude, sge_pattern *out, uint32_t *got) {
+    SGE_API_BEGIN
     if (!out || !got || k < 2 || k > SGE_MAX_K || emin > emax || emax > k * (k - 1) / 2) {
         set_error("sge_pattern_sample: bad arguments");
         return SGE_EINVAL;
@@ -457,11 +464,13 @@ extern "C" int sge_pattern_sample(uint32_t k, uint32_t emin, uint32_t emax, uint
     }
     *got = n;
     return SGE_OK;
+    SGE_API_END
 }
 
 // Canonical embedding rows: each row (images per pattern vertex) is replaced by
 // its lexicographically smallest relabelling over Aut(P); rows are then sorted.
 extern "C" int sge_canonicalize_embeddings(const sge_pattern *cp, uint32_t *rows, uint64_t n) {
+    SGE_API_BEGIN
     if (!cp || (n && !rows)) {
         set_error("sge_canonicalize_embeddings: bad arguments");
         return SGE_EINVAL;
@@ -488,4 +497,5 @@ extern "C" int sge_canonicalize_embeddings(const sge_pattern *cp, uint32_t *rows
     for (uint64_t i = 0; i < n; ++i) std::memcpy(tmp.data() + i * k, rows + idx[i] * k, 4ull * k);
     if (n) std::memcpy(rows, tmp.data(), 4ull * n * k);
     return SGE_OK;
+    SGE_API_END
 }

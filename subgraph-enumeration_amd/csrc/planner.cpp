@@ -443,6 +443,7 @@ void bag_subplan(const sge_plan &pl, sge_plan &sub) {
 using namespace sge;
 
 extern "C" int sge_plan_set_bag(sge_plan *plan, uint32_t vertex_mask) {
+    SGE_API_BEGIN
     if (!plan || plan->k < 1 || plan->k > SGE_MAX_K) {
         set_error("sge_plan_set_bag: bad plan");
         return SGE_EINVAL;
@@ -458,10 +459,12 @@ extern "C" int sge_plan_set_bag(sge_plan *plan, uint32_t vertex_mask) {
     if (rc) return rc;
     plan->bag = B;
     return SGE_OK;
+    SGE_API_END
 }
 
 extern "C" int sge_plan_bags(const sge_pattern *cp, const sge_plan *plan, uint32_t *masks, uint32_t cap,
                              uint32_t *n) {
+    SGE_API_BEGIN
     if (!cp || !plan || !n || (cap && !masks)) {
         set_error("sge_plan_bags: null argument");
         return SGE_EINVAL;
@@ -482,19 +485,23 @@ extern "C" int sge_plan_bags(const sge_pattern *cp, const sge_plan *plan, uint32
     set_error("");
     *n = cnt;
     return SGE_OK;
+    SGE_API_END
 }
 
 extern "C" int sge_plan_build(const sge_pattern *p, const sge_graph *g, uint32_t mode, const uint32_t *order,
                               sge_plan *out) {
+    SGE_API_BEGIN
     if (!p || !out) {
         set_error("sge_plan_build: null argument");
         return SGE_EINVAL;
     }
     return build_plan(from_c(*p), g ? &g->h : nullptr, mode, order, *out);
+    SGE_API_END
 }
 
 extern "C" int sge_plan_cost(const sge_pattern *cp, const sge_graph *g, uint32_t mode, const uint32_t *order,
                              double *cost) {
+    SGE_API_BEGIN
     if (!cp || !order || !cost) {
         set_error("sge_plan_cost: null argument");
         return SGE_EINVAL;
@@ -505,12 +512,14 @@ extern "C" int sge_plan_cost(const sge_pattern *cp, const sge_graph *g, uint32_t
     for (int i = 0; i < p.k; ++i) o[i] = (int)order[i];
     *cost = order_cost(p, summarize(g ? &g->h : nullptr), o, aut, mode);
     return SGE_OK;
+    SGE_API_END
 }
 
 // Every connected order ranked the way the planner ranks them for `mode`:
 // longest closed-form tail first (IEP), then lowest analytic cost.
 extern "C" int sge_plan_rank_orders(const sge_pattern *cp, const sge_graph *g, uint32_t mode, uint32_t *orders,
                                     double *costs, uint32_t *tails, uint32_t cap, uint32_t *n) {
+    SGE_API_BEGIN
     if (!cp || !n || mode > SGE_MODE_IEP) {
         set_error("sge_plan_rank_orders: bad arguments");
         return SGE_EINVAL;
@@ -542,9 +551,11 @@ extern "C" int sge_plan_rank_orders(const sge_pattern *cp, const sge_graph *g, u
         if (tails) tails[i] = rs[i].tail;
     }
     return SGE_OK;
+    SGE_API_END
 }
 
 extern "C" int sge_plan_orders(const sge_pattern *cp, uint32_t *orders, uint32_t cap, uint32_t *n) {
+    SGE_API_BEGIN
     if (!cp || !n) {
         set_error("sge_plan_orders: null argument");
         return SGE_EINVAL;
@@ -557,11 +568,13 @@ extern "C" int sge_plan_orders(const sge_pattern *cp, uint32_t *orders, uint32_t
         for (uint32_t i = 0; i < std::min<uint32_t>(cap, *n); ++i)
             for (int j = 0; j < p.k; ++j) orders[i * p.k + j] = os[i][j];
     return SGE_OK;
+    SGE_API_END
 }
 
 // Plan text in the reference's -r / stdout style (decomposition.cpp:537-553
 // prints "num bags" and bags; symmetry rules are printed as "u < a,b,").
 extern "C" int sge_plan_text(const sge_plan *pl, char *buf, size_t cap) {
+    SGE_API_BEGIN
     if (!pl || !buf || cap == 0) {
         set_error("sge_plan_text: bad arguments");
         return SGE_EINVAL;
@@ -588,6 +601,7 @@ extern "C" int sge_plan_text(const sge_plan *pl, char *buf, size_t cap) {
     }
     std::memcpy(buf, s.c_str(), s.size() + 1);
     return SGE_OK;
+    SGE_API_END
 }
 
 // ---------------------------------------------------------------------------
@@ -716,9 +730,11 @@ int order_features(const Pattern &p, const HostGraph *g, const uint32_t *order, 
 
 extern "C" int sge_order_features(const sge_pattern *cp, const sge_graph *g, const uint32_t *order, uint32_t mode,
                                   uint32_t nsamples, uint64_t seed, float *feat) {
+    SGE_API_BEGIN
     if (!cp || !order || !feat) {
         set_error("sge_order_features: null argument");
         return SGE_EINVAL;
     }
     return order_features(from_c(*cp), g ? &g->h : nullptr, order, mode, nsamples, seed, feat);
+    SGE_API_END
 }

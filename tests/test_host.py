@@ -164,6 +164,25 @@ def test_bad_inputs_raise(tmp_path):
         sge.build_plan(W.named("house"), None, 0, [0, 4, 1, 2, 3])  # level 1 not adjacent to level 0
 
 
+def test_host_failures_return_codes():
+    """A host allocation failure inside an entry point comes back as SGE_ENOMEM
+    (no C++ exception reaches the caller), and a failed write as SGE_EIO."""
+    import ctypes
+
+    h = ctypes.c_void_p()
+    uv = np.zeros(2, dtype=np.uint32)
+    # 2^61 edges: the edge vector cannot be sized, before any edge is read
+    rc = sge.lib().sge_graph_from_edges(4, uv.ctypes.data_as(ctypes.c_void_p), 1 << 61, 0, ctypes.byref(h))
+    assert rc == -4 and not h.value
+    assert b"allocation" in sge.lib().sge_last_error()
+    if os.path.exists("/dev/full"):  # every write fails with ENOSPC
+        g = W.graph_c1(False)
+        for binary in (False, True):
+            with pytest.raises(sge.SgeError) as e:
+                g.write("/dev/full", binary=binary)
+            assert e.value.code == -2
+
+
 def test_count_without_gpu_fails_loudly():
     import ctypes
     try:
