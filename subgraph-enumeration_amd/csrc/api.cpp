@@ -4,6 +4,7 @@
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <memory>
 
 #include "common.h"
 
@@ -24,15 +25,13 @@ static int finish_graph(uint32_t n, EdgeVec &edges, uint32_t flags, sge_graph **
         set_error("null output handle");
         return SGE_EINVAL;
     }
-    sge_graph *g = new sge_graph();
+    std::unique_ptr<sge_graph> g(new sge_graph());  // freed if the build fails or throws
     const auto t0 = std::chrono::steady_clock::now();
-    int rc = (flags & SGE_GRAPH_BUILD_DEVICE) ? build_graph_device(n, edges, flags, g) : build_graph(n, edges, flags, g->h);
-    if (rc) {
-        delete g;
-        return rc;
-    }
+    int rc = (flags & SGE_GRAPH_BUILD_DEVICE) ? build_graph_device(n, edges, flags, g.get())
+                                              : build_graph(n, edges, flags, g->h);
+    if (rc) return rc;
     g->build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    *out = g;
+    *out = g.release();
     return SGE_OK;
 }
 
@@ -108,6 +107,11 @@ extern "C" int sge_graph_chung_lu(uint32_t n, uint64_t m, double gamma, double c
     }
     EdgeVec edges;
     chung_lu_edges(n, m, gamma, cap > 0 ? cap : 1e300, seed, edges);
+    if (edges.size() != m) {  // the draw limit ran out (m near n(n-1)/2 under a steep degree law)
+        set_error("chung_lu: drew only " + std::to_string(edges.size()) + " of " + std::to_string(m) +
+                  " distinct edges; lower m or gamma");
+        return SGE_ERANGE;
+    }
     return finish_graph(n, edges, flags, out);
     SGE_API_END
 }

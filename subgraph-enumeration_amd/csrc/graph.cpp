@@ -60,6 +60,19 @@ int read_edge_file(const char *path, uint32_t &n, EdgeVec &edges) {
         }
         n = hdr[0];
         uint64_t m = hdr[1];
+        // the header's m is checked against the file length before anything is sized by it
+        long here = std::ftell(f);
+        if (here < 0 || std::fseek(f, 0, SEEK_END) != 0) {
+            std::fclose(f);
+            set_error(std::string("can not seek in ") + path);
+            return SGE_EIO;
+        }
+        long end = std::ftell(f);
+        if (end < here || (uint64_t)(end - here) < 8 * m || std::fseek(f, here, SEEK_SET) != 0) {
+            std::fclose(f);
+            set_error("truncated BINGRPH edge list");
+            return SGE_EIO;
+        }
         std::vector<uint32_t> raw(2 * m);
         if (m && std::fread(raw.data(), 8, m, f) != m) {
             std::fclose(f);
@@ -86,8 +99,14 @@ int read_edge_file(const char *path, uint32_t &n, EdgeVec &edges) {
         set_error(std::string("bad header in ") + path);
         return SGE_EIO;
     }
+    if (nn > 0xffffffffull) {
+        set_error("vertex count above 2^32 - 1 in " + std::string(path));
+        return SGE_EIO;
+    }
     n = (uint32_t)nn;
-    edges.resize(m);
+    // grown as lines arrive: a header's m alone sizes nothing
+    edges.clear();
+    edges.reserve(std::min<uint64_t>(m, 1ull << 24));
     for (uint64_t i = 0; i < m; ++i) {
         uint64_t u, v;
         if (!(in >> u >> v)) {
@@ -98,7 +117,7 @@ int read_edge_file(const char *path, uint32_t &n, EdgeVec &edges) {
             set_error("edge endpoint out of range");
             return SGE_EIO;
         }
-        edges[i] = (u << 32) | v;
+        edges.push_back((u << 32) | v);
     }
     return SGE_OK;
 }

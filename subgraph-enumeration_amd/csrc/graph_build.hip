@@ -246,14 +246,16 @@ int build_graph_device(uint32_t n, const EdgeVec &edges, uint32_t flags, sge_gra
     BUILD_TRY(hipStreamSynchronize(st));
     h.max_degree = 0;
     for (uint32_t v = 0; v < n; ++v) h.max_degree = std::max(h.max_degree, h.off[v + 1] - h.off[v]);
-    // the device arrays become the graph's device mirror
+    // the device arrays become the graph's device mirror (nothing can fail after this)
+    int dev = -1;
+    BUILD_TRY(hipGetDevice(&dev));
     s.release(d_off);
     s.release(d_nbr);
     s.release(d_src);
     g->d_off = d_off;
     g->d_nbr = d_nbr;
     g->d_src = d_src;
-    BUILD_TRY(hipGetDevice(&g->device));
+    g->device = dev;
     return SGE_OK;
 }
 

@@ -175,6 +175,28 @@ def test_host_failures_return_codes():
     rc = sge.lib().sge_graph_from_edges(4, uv.ctypes.data_as(ctypes.c_void_p), 1 << 61, 0, ctypes.byref(h))
     assert rc == -4 and not h.value
     assert b"allocation" in sge.lib().sge_last_error()
+    # headers are checked against the file before they size anything
+    import struct
+    import tempfile
+
+    with tempfile.TemporaryDirectory() as d:
+        b = os.path.join(d, "short.bin")
+        with open(b, "wb") as f:  # claims 2^32 - 1 edges, holds one
+            f.write(b"BINGRPH" + struct.pack("<II", 4, 0xFFFFFFFF) + struct.pack("<II", 0, 1))
+        with pytest.raises(sge.SgeError) as e:
+            sge.Graph.load(b)
+        assert e.value.code == -2 and "truncated" in str(e.value)
+        t = os.path.join(d, "big_n.txt")
+        with open(t, "w") as f:
+            f.write("4294967296 1\n0 1\n")
+        with pytest.raises(sge.SgeError) as e:
+            sge.Graph.load(t)
+        assert e.value.code == -2
+    # a Chung-Lu request the draw limit cannot meet is an error, not a smaller
+    # graph: the complete graph on 100 vertices under weights ~ (i+1)^-5 needs
+    # the pair (98, 99), drawn with probability ~1e-20 per draw
+    rc = sge.lib().sge_graph_chung_lu(100, 4950, 1.2, 0.0, 1, 0, ctypes.byref(h))
+    assert rc == -5 and not h.value and b"distinct edges" in sge.lib().sge_last_error()
     if os.path.exists("/dev/full"):  # every write fails with ENOSPC
         g = W.graph_c1(False)
         for binary in (False, True):
