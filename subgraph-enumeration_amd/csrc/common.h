@@ -90,6 +90,10 @@ struct Pattern {
     bool connected() const;
 };
 Pattern from_c(const sge_pattern &p);
+// A caller's pattern for entry point `fn`: non-null with 1..8 vertices (SGE_EINVAL otherwise).
+int pattern_arg(const sge_pattern *cp, const char *fn, Pattern &out);
+// A caller's vertex order for a k-vertex pattern: non-null and a permutation of 0..k-1.
+int order_arg(const uint32_t *order, int k, const char *fn);
 sge_pattern to_c(const Pattern &p);
 std::vector<std::vector<int>> automorphisms(const Pattern &p);
 uint64_t canonical_code(const Pattern &p);

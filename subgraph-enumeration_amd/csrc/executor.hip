@@ -1717,6 +1717,15 @@ void fill_stats(const PlanOut &o, double ms, sge_stats *s) {
 
 constexpr uint64_t BAG_TUPLE_CAP = 1ull << 28;  // tuples of one materialised bag relation
 
+// Frees up to two device buffers when the scope ends, on every return path.
+struct DevFree {
+    void *a = nullptr, *b = nullptr;
+    ~DevFree() {
+        if (a) (void)hipFree(a);
+        if (b) (void)hipFree(b);
+    }
+};
+
 // Materialise plan.bag's relation and build its trie (nodeJoin over the bag,
 // join.cpp:309-421, then buildTrie, join.cpp:1713-1717): count the relation
 // with the bag's own plan, enumerate it into exactly that many rows, sort and
@@ -1747,14 +1756,9 @@ int build_bag_trie(sge_graph *g, const sge_plan &plan, sge::BagTrie &t) {
     }
     uint32_t *rows = nullptr;
     unsigned long long *d_n = nullptr;
+    DevFree fr;
     HIP_TRY(hipMalloc(&rows, sizeof(uint32_t) * std::max<uint64_t>(N * sub.k, 1)));
-    struct Free {
-        void *a, *b;
-        ~Free() {
-            if (a) (void)hipFree(a);
-            if (b) (void)hipFree(b);
-        }
-    } fr{rows, nullptr};
+    fr.a = rows;
     HIP_TRY(hipMalloc(&d_n, sizeof(unsigned long long)));
     fr.b = d_n;
     if (N) {
@@ -1997,6 +2001,10 @@ extern "C" int sge_enumerate(sge_graph *g, const sge_plan *plan, uint32_t *out, 
         sge::set_error("sge_enumerate: patterns need at least 3 vertices");
         return SGE_EINVAL;
     }
+    if (cap > (SIZE_MAX / sizeof(uint32_t)) / SGE_MAX_K) {  // cap * k * 4 bytes must not wrap
+        sge::set_error("sge_enumerate: capacity too large");
+        return SGE_ERANGE;
+    }
     int rc = sge_graph_to_device(g);
     if (rc) return rc;
     std::lock_guard<std::mutex> lk(dmutex());
@@ -2009,21 +2017,19 @@ extern "C" int sge_enumerate(sge_graph *g, const sge_plan *plan, uint32_t *out, 
     if (rc) return rc;
     uint32_t *d_emb = nullptr;
     unsigned long long *d_n = nullptr;
-    size_t bytes = (size_t)std::max<uint64_t>(cap, 1) * plan->k * sizeof(uint32_t);
-    HIP_TRY(hipMalloc(&d_emb, bytes));
+    DevFree fr;
+    HIP_TRY(hipMalloc(&d_emb, (size_t)std::max<uint64_t>(cap, 1) * plan->k * sizeof(uint32_t)));
+    fr.a = d_emb;
     HIP_TRY(hipMalloc(&d_n, sizeof(unsigned long long)));
+    fr.b = d_n;
     float t = 0;
     rc = run_launch(g, c.plans, 1, c.out, 0, g->h.nnz, 1, nsub[0], cfg, true, d_emb, cap, d_n, &t);
-    unsigned long long n = 0;
-    if (!rc) {
-        HIP_TRY(hipMemcpy(&n, d_n, sizeof(n), hipMemcpyDeviceToHost));
-        uint64_t w = std::min<uint64_t>(n, cap);
-        if (w) HIP_TRY(hipMemcpy(out, d_emb, w * plan->k * sizeof(uint32_t), hipMemcpyDeviceToHost));
-        for (uint64_t i = 0; i < w * plan->k; ++i) out[i] = g->h.perm[out[i]];
-    }
-    (void)hipFree(d_emb);
-    (void)hipFree(d_n);
     if (rc) return rc;
+    unsigned long long n = 0;
+    HIP_TRY(hipMemcpy(&n, d_n, sizeof(n), hipMemcpyDeviceToHost));
+    const uint64_t w = std::min<uint64_t>(n, cap);
+    if (w) HIP_TRY(hipMemcpy(out, d_emb, w * plan->k * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    for (uint64_t i = 0; i < w * plan->k; ++i) out[i] = g->h.perm[out[i]];
     *n_out = n;
     if (n > cap) {
         sge::set_error("sge_enumerate: output capacity exceeded");

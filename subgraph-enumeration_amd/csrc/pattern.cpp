@@ -50,6 +50,28 @@ Pattern from_c(const sge_pattern &c) {
     return p;
 }
 
+int pattern_arg(const sge_pattern *cp, const char *fn, Pattern &out) {
+    if (!cp || cp->k == 0 || cp->k > SGE_MAX_K) {
+        set_error(std::string(fn) + (cp ? ": pattern must have 1..8 vertices" : ": null pattern"));
+        return SGE_EINVAL;
+    }
+    out = from_c(*cp);
+    return SGE_OK;
+}
+
+int order_arg(const uint32_t *order, int k, const char *fn) {
+    uint32_t seen = 0;
+    for (int i = 0; order && i < k; ++i) {
+        if (order[i] >= (uint32_t)k || ((seen >> order[i]) & 1u)) break;
+        seen |= 1u << order[i];
+    }
+    if (!order || seen != (1u << k) - 1) {
+        set_error(std::string(fn) + ": order is not a permutation of the pattern vertices");
+        return SGE_EINVAL;
+    }
+    return SGE_OK;
+}
+
 sge_pattern to_c(const Pattern &p) {
     sge_pattern c;
     std::memset(&c, 0, sizeof(c));
@@ -383,6 +405,10 @@ extern "C" int sge_pattern_from_edges(uint32_t k, const uint32_t *uv, uint32_t m
 
 extern "C" int sge_pattern_load(const char *path, sge_pattern *out) {
     SGE_API_BEGIN
+    if (!path || !out) {
+        set_error("sge_pattern_load: null argument");
+        return SGE_EINVAL;
+    }
     uint32_t n = 0;
     EdgeVec edges;
     // pattern files share the text format of data graphs; the optional orbit
@@ -471,11 +497,12 @@ extern "C" int sge_pattern_sample(uint32_t k, uint32_t emin, uint32_t emax, uint
 // its lexicographically smallest relabelling over Aut(P); rows are then sorted.
 extern "C" int sge_canonicalize_embeddings(const sge_pattern *cp, uint32_t *rows, uint64_t n) {
     SGE_API_BEGIN
-    if (!cp || (n && !rows)) {
-        set_error("sge_canonicalize_embeddings: bad arguments");
+    if (n && !rows) {
+        set_error("sge_canonicalize_embeddings: null rows");
         return SGE_EINVAL;
     }
-    Pattern p = from_c(*cp);
+    Pattern p;
+    if (int rc = pattern_arg(cp, "sge_canonicalize_embeddings", p)) return rc;
     auto aut = automorphisms(p);
     int k = p.k;
     std::vector<uint32_t> best(k), cand(k);

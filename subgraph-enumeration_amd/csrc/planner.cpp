@@ -465,11 +465,12 @@ extern "C" int sge_plan_set_bag(sge_plan *plan, uint32_t vertex_mask) {
 extern "C" int sge_plan_bags(const sge_pattern *cp, const sge_plan *plan, uint32_t *masks, uint32_t cap,
                              uint32_t *n) {
     SGE_API_BEGIN
-    if (!cp || !plan || !n || (cap && !masks)) {
+    if (!plan || !n || (cap && !masks)) {
         set_error("sge_plan_bags: null argument");
         return SGE_EINVAL;
     }
-    Pattern p = from_c(*cp);
+    Pattern p;
+    if (int rc = pattern_arg(cp, "sge_plan_bags", p)) return rc;
     if ((uint32_t)p.k != plan->k) {
         set_error("sge_plan_bags: plan and pattern sizes differ");
         return SGE_EINVAL;
@@ -491,22 +492,30 @@ extern "C" int sge_plan_bags(const sge_pattern *cp, const sge_plan *plan, uint32
 extern "C" int sge_plan_build(const sge_pattern *p, const sge_graph *g, uint32_t mode, const uint32_t *order,
                               sge_plan *out) {
     SGE_API_BEGIN
-    if (!p || !out) {
+    if (!out) {
         set_error("sge_plan_build: null argument");
         return SGE_EINVAL;
     }
-    return build_plan(from_c(*p), g ? &g->h : nullptr, mode, order, *out);
+    Pattern pat;
+    if (int rc = pattern_arg(p, "sge_plan_build", pat)) return rc;
+    return build_plan(pat, g ? &g->h : nullptr, mode, order, *out);
     SGE_API_END
 }
 
 extern "C" int sge_plan_cost(const sge_pattern *cp, const sge_graph *g, uint32_t mode, const uint32_t *order,
                              double *cost) {
     SGE_API_BEGIN
-    if (!cp || !order || !cost) {
+    if (!cost) {
         set_error("sge_plan_cost: null argument");
         return SGE_EINVAL;
     }
-    Pattern p = from_c(*cp);
+    if (mode > SGE_MODE_IEP) {
+        set_error("sge_plan_cost: unknown mode");
+        return SGE_EINVAL;
+    }
+    Pattern p;
+    if (int rc = pattern_arg(cp, "sge_plan_cost", p)) return rc;
+    if (int rc = order_arg(order, p.k, "sge_plan_cost")) return rc;
     auto aut = automorphisms(p);
     int o[SGE_MAX_K];
     for (int i = 0; i < p.k; ++i) o[i] = (int)order[i];
@@ -520,11 +529,12 @@ extern "C" int sge_plan_cost(const sge_pattern *cp, const sge_graph *g, uint32_t
 extern "C" int sge_plan_rank_orders(const sge_pattern *cp, const sge_graph *g, uint32_t mode, uint32_t *orders,
                                     double *costs, uint32_t *tails, uint32_t cap, uint32_t *n) {
     SGE_API_BEGIN
-    if (!cp || !n || mode > SGE_MODE_IEP) {
+    if (!n || mode > SGE_MODE_IEP) {
         set_error("sge_plan_rank_orders: bad arguments");
         return SGE_EINVAL;
     }
-    Pattern p = from_c(*cp);
+    Pattern p;
+    if (int rc = pattern_arg(cp, "sge_plan_rank_orders", p)) return rc;
     if (!p.connected() || p.k < 2) {
         set_error("sge_plan_rank_orders: pattern must be connected with >= 2 vertices");
         return SGE_EINVAL;
@@ -556,11 +566,12 @@ extern "C" int sge_plan_rank_orders(const sge_pattern *cp, const sge_graph *g, u
 
 extern "C" int sge_plan_orders(const sge_pattern *cp, uint32_t *orders, uint32_t cap, uint32_t *n) {
     SGE_API_BEGIN
-    if (!cp || !n) {
+    if (!n) {
         set_error("sge_plan_orders: null argument");
         return SGE_EINVAL;
     }
-    Pattern p = from_c(*cp);
+    Pattern p;
+    if (int rc = pattern_arg(cp, "sge_plan_orders", p)) return rc;
     auto aut = automorphisms(p);
     auto os = connected_orders(p, aut);
     *n = (uint32_t)os.size();
@@ -731,10 +742,12 @@ int order_features(const Pattern &p, const HostGraph *g, const uint32_t *order, 
 extern "C" int sge_order_features(const sge_pattern *cp, const sge_graph *g, const uint32_t *order, uint32_t mode,
                                   uint32_t nsamples, uint64_t seed, float *feat) {
     SGE_API_BEGIN
-    if (!cp || !order || !feat) {
+    if (!order || !feat) {
         set_error("sge_order_features: null argument");
         return SGE_EINVAL;
     }
-    return order_features(from_c(*cp), g ? &g->h : nullptr, order, mode, nsamples, seed, feat);
+    Pattern p;
+    if (int rc = pattern_arg(cp, "sge_order_features", p)) return rc;
+    return order_features(p, g ? &g->h : nullptr, order, mode, nsamples, seed, feat);
     SGE_API_END
 }

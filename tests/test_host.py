@@ -164,6 +164,43 @@ def test_bad_inputs_raise(tmp_path):
         sge.build_plan(W.named("house"), None, 0, [0, 4, 1, 2, 3])  # level 1 not adjacent to level 0
 
 
+def test_pattern_arguments_are_checked():
+    """Every entry point that takes a caller's pattern rejects k outside 1..8
+    (and a null pattern) with SGE_EINVAL before reading it; plan cost rejects an
+    order that is not a permutation."""
+    import ctypes
+
+    L = sge.lib()
+    good = W.named("house").c
+    for k in (0, 9, 40):
+        bad = sge.CPattern()
+        bad.k = k
+        n = ctypes.c_uint32()
+        cost = ctypes.c_double()
+        order = np.arange(8, dtype=np.uint32)
+        feat = np.zeros(64, dtype=np.float32)
+        plan = sge.Plan()
+        rows = np.zeros(8, dtype=np.uint32)
+        for pat in (ctypes.byref(bad), None):
+            assert L.sge_plan_build(pat, None, 0, None, ctypes.byref(plan)) == -1
+            assert L.sge_plan_cost(pat, None, 0, order.ctypes.data_as(ctypes.c_void_p), ctypes.byref(cost)) == -1
+            assert L.sge_plan_orders(pat, None, 0, ctypes.byref(n)) == -1
+            assert L.sge_plan_rank_orders(pat, None, 0, None, None, None, 0, ctypes.byref(n)) == -1
+            assert L.sge_plan_bags(pat, ctypes.byref(plan), None, 0, ctypes.byref(n)) == -1
+            assert L.sge_canonicalize_embeddings(pat, rows.ctypes.data_as(ctypes.c_void_p), 1) == -1
+            assert L.sge_order_features(pat, None, order.ctypes.data_as(ctypes.c_void_p), 0, 0, 0,
+                                        feat.ctypes.data_as(ctypes.c_void_p)) == -1
+    cost = ctypes.c_double()
+    for o in ([0, 1, 2, 3, 3], [0, 1, 2, 3, 5]):
+        oa = np.array(o, dtype=np.uint32)
+        assert L.sge_plan_cost(ctypes.byref(good), None, 0, oa.ctypes.data_as(ctypes.c_void_p), ctypes.byref(cost)) == -1
+    assert L.sge_plan_cost(ctypes.byref(good), None, 0, None, ctypes.byref(cost)) == -1
+    oa = np.array([2, 3, 4, 0, 1], dtype=np.uint32)
+    assert L.sge_plan_cost(ctypes.byref(good), None, 0, oa.ctypes.data_as(ctypes.c_void_p), ctypes.byref(cost)) == 0
+    assert cost.value > 0
+    assert L.sge_pattern_load(None, ctypes.byref(sge.CPattern())) == -1
+
+
 def test_host_failures_return_codes():
     """A host allocation failure inside an entry point comes back as SGE_ENOMEM
     (no C++ exception reaches the caller), and a failed write as SGE_EIO."""
