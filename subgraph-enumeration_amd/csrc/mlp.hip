@@ -270,12 +270,25 @@ struct sge_mlp {
     uint32_t step = 0;
 };
 
+// Rows per call: keeps N * IN and every grid size below 2^31.
+constexpr uint32_t MLP_MAX_ROWS = 1u << 24;
+
 static int mlp_reserve(sge_mlp *M, uint32_t N) {
+    if (N > MLP_MAX_ROWS) {
+        sge::set_error("MLP: at most 2^24 rows per call");
+        return SGE_ERANGE;
+    }
     if (N <= M->cap) return SGE_OK;
-    uint32_t c = std::max<uint32_t>(N, 2 * M->cap);
+    uint32_t c = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(N, 2ull * M->cap), MLP_MAX_ROWS);
     c = (c + 63) / 64 * 64;
-    (void)hipFree(M->Xf); (void)hipFree(M->y); (void)hipFree(M->t); (void)hipFree(M->X); (void)hipFree(M->H1); (void)hipFree(M->H2);
-    (void)hipFree(M->DZ2); (void)hipFree(M->DZ1);
+    // released first, so that a failed allocation below leaves no freed pointer behind
+    // (cap 0: the next call allocates again; destroy frees what was allocated)
+    auto drop = [](auto *&q) {
+        (void)hipFree(q);
+        q = nullptr;
+    };
+    drop(M->Xf), drop(M->y), drop(M->t), drop(M->X), drop(M->H1), drop(M->H2), drop(M->DZ2), drop(M->DZ1);
+    M->cap = 0;
     HIP_TRY(hipMalloc(&M->Xf, sizeof(float) * c * IN));
     HIP_TRY(hipMalloc(&M->y, sizeof(float) * c));
     HIP_TRY(hipMalloc(&M->t, sizeof(float) * c));
