@@ -11,8 +11,9 @@
  *   relation/graph.cpp:661      PatternGraph::loadPatternGraph  -> sge_pattern_load
  *
  * Conventions: every function returns 0 on success and a negative SGE_E* code
- * on failure; sge_last_error() gives the message (thread-local).  Only POD types
- * cross the boundary.  The caller owns every host buffer it passes; the library
+ * on failure; sge_last_error() gives the message (thread-local).  No C++
+ * exception crosses the boundary: a host allocation failure inside a call is
+ * SGE_ENOMEM.  Only POD types cross the boundary.  The caller owns every host buffer it passes; the library
  * owns graph/batch/model handles and their device memory (freed by *_destroy).
  * Counts are assigned, never accumulated, and are exact uint64 (the IEP
  * closed form reproduces the reference's wrapping choosec, utility/utils.cpp:80-94).
