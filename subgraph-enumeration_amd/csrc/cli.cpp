@@ -74,6 +74,10 @@ int main(int argc, char **argv) {
     std::ostream *out = &std::cout;
     if (!r.empty()) {
         fout.open(r);
+        if (!fout) {
+            std::fprintf(stderr, "error: can not create result file %s\n", r.c_str());
+            return 1;
+        }
         out = &fout;
     }
     sge_pattern p;
@@ -102,7 +106,15 @@ int main(int argc, char **argv) {
         std::stringstream ss(bag_s);
         std::string tok;
         uint32_t mask = 0;
-        while (std::getline(ss, tok, ',')) mask |= 1u << (std::atoi(tok.c_str()) & 31);
+        while (std::getline(ss, tok, ',')) {
+            char *end = nullptr;
+            const unsigned long v = std::strtoul(tok.c_str(), &end, 10);
+            if (tok.empty() || *end || v >= p.k) {
+                std::fprintf(stderr, "error: --bag: '%s' is not a pattern vertex (0..%u)\n", tok.c_str(), p.k - 1);
+                return 1;
+            }
+            mask |= 1u << v;
+        }
         if (sge_plan_set_bag(&plan, mask)) return die("bag");
     }
     double plan_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -129,11 +141,20 @@ int main(int argc, char **argv) {
     if (!emb.empty()) {
         uint64_t n = 0;
         uint64_t cap = count / (plan.multiplier ? plan.multiplier : 1) + 16;
+        if (cap > (1ull << 32)) {  // the host copy alone would exceed 128 GiB
+            std::fprintf(stderr, "error: --embeddings: %llu embeddings are too many to write\n",
+                         (unsigned long long)cap);
+            return 1;
+        }
         std::vector<uint32_t> rows((size_t)cap * p.k);
         int rc = sge_enumerate(g, &plan, rows.data(), cap, &n);
         if (rc) return die("enumerate");
-        sge_canonicalize_embeddings(&p, rows.data(), n);
+        if (sge_canonicalize_embeddings(&p, rows.data(), n)) return die("embeddings");
         std::ofstream ef(emb);
+        if (!ef) {
+            std::fprintf(stderr, "error: can not create embeddings file %s\n", emb.c_str());
+            return 1;
+        }
         ef << n << "\n";
         for (uint64_t i = 0; i < n; ++i) {
             for (uint32_t j = 0; j < p.k; ++j) ef << rows[i * p.k + j] << (j + 1 < p.k ? " " : "");

@@ -201,6 +201,25 @@ def test_pattern_arguments_are_checked():
     assert L.sge_pattern_load(None, ctypes.byref(sge.CPattern())) == -1
 
 
+def test_cli_rejects_bad_arguments(tmp_path):
+    """Bag vertex ids outside the pattern and an unwritable result file end the
+    CLI with an error before anything is counted."""
+    import subprocess
+
+    cli = os.path.join(os.path.dirname(sge.LIB_PATH), "graph_mining_system")
+    if not os.path.exists(cli):
+        pytest.skip("CLI not built")
+    W.graph_c1(False).write(str(tmp_path / "g.txt"))
+    (tmp_path / "q.txt").write_text("5 6\n0 1\n0 2\n1 3\n2 3\n2 4\n3 4\n")  # house
+    base = [cli, "-q", str(tmp_path / "q.txt"), "-d", str(tmp_path / "g.txt"), "--order", "2,3,4,0,1"]
+    for bag in ("2,3,33", "2,3,x", "2,,3", "5"):
+        r = subprocess.run(base + ["--bag", bag], capture_output=True, text=True, timeout=60)
+        assert r.returncode == 1 and "--bag" in r.stderr and "Number of matches" not in r.stdout, bag
+    r = subprocess.run(base + ["-r", str(tmp_path / "missing" / "out.txt")], capture_output=True, text=True,
+                       timeout=60)
+    assert r.returncode == 1 and "result file" in r.stderr
+
+
 def test_host_failures_return_codes():
     """A host allocation failure inside an entry point comes back as SGE_ENOMEM
     (no C++ exception reaches the caller), and a failed write as SGE_EIO."""
