@@ -3,7 +3,7 @@ from its own sources by oracle/ref.mk into oracle/_ref/, on this repo's
 synthetic graphs.  Run in a container that has /root/reference and the built
 binary:  python tests/golden/make_golden.py [section ...]
 
-Sections: c1, c2, c3, rmat7, rmat7_iep, queries, queries_c2, dataset.  Output: tests/golden/<section>.json
+Sections: c1, c2, c3, rmat7, rmat7_iep, queries, queries_c2, classes, dataset.  Output: tests/golden/<section>.json
 (inputs are named by generator + parameters; counts are the reference's
 "Number of matches" lines; "bags" its "num bags" line).
 """
@@ -157,6 +157,33 @@ def section_queries_c2():
                    "wcoj": run_ref(path, gp, [], t), "iep": run_ref(path, gp, ["-u"], t)}
             res["patterns"].append(rec)
             print(rec["file"], rec["wcoj"].get("count"), rec["iep"].get("count"), flush=True)
+    return res
+
+
+def section_classes():
+    """Every connected pattern class on 4, 5 and 6 vertices (6 + 21 + 112),
+    reference WCOJ and -u counts and its bag count, over two small graphs: the
+    C1 Chung-Lu graph and a dense Chung-Lu graph (n=60, m=600, gamma 3, seed 2)."""
+    graphs = {"c1": ("chung_lu(n=379, m=914, gamma=2.5, cap=none, seed=0)", W.graph_c1(False)),
+              "cl60": ("chung_lu(n=60, m=600, gamma=3.0, cap=none, seed=2)",
+                       sge.Graph.chung_lu(60, 600, 3.0, 0.0, 2, False))}
+    pats = []
+    for k in (4, 5, 6):
+        pats += sge.sample_patterns(k, k - 1, k * (k - 1) // 2, 200, 11)
+    res = {"graphs": {}, "patterns": [{"k": p.k, "edges": p.edges()} for p in pats]}
+    for name, (desc, g) in graphs.items():
+        gp = graph_file(name, g)
+        rows = []
+        for i, p in enumerate(pats):
+            pp = os.path.join(TMP, f"class{i}.txt")
+            write_pattern(p.k, p.edges(), pp)
+            w, u = run_ref(pp, gp, [], 1), run_ref(pp, gp, ["-u"], 1)
+            # the reference binary ends with a signal on a few classes: its exit status is recorded
+            rows.append({"wcoj": w.get("count"), "iep": u.get("count"), "bags": w.get("bags"),
+                         "ref_exit": w.get("error", 0) or u.get("error", 0)})
+        res["graphs"][name] = {"graph": desc, "n": g.n, "m": g.m, "counts": rows}
+        print(name, sum(r["wcoj"] != r["iep"] for r in rows if r["wcoj"] is not None), "patterns where -u differs;",
+              [i for i, r in enumerate(rows) if r["ref_exit"]], "reference failures", flush=True)
     return res
 
 

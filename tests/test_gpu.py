@@ -626,3 +626,21 @@ def test_bag_joins_on_query_files_c1():
             for b in sge.valid_bags(plan):
                 say("query file bag", rec["file"], mode, b)
                 assert sge.count(g, sge.set_bag(plan, b))[0][0] == rec["wcoj"]["count"], (rec["file"], mode, b)
+
+
+def test_every_small_class_matches_reference():
+    """All 139 connected classes on 4-6 vertices over two graphs in every mode,
+    plain and batched: the reference's WCOJ counts, and the oracle's (pinned by
+    brute force) on the four classes where the reference binary fails."""
+    gold = golden("classes")
+    pats = [sge.Pattern.from_edges(r["k"], [tuple(e) for e in r["edges"]]) for r in gold["patterns"]]
+    graphs = {"c1": W.graph_c1(), "cl60": sge.Graph.chung_lu(60, 600, 3.0, 0.0, 2, True)}
+    for name, g in graphs.items():
+        off, nbr = g.csr()
+        rows = gold["graphs"][name]["counts"]
+        for mode in MODES:
+            plans = [sge.build_plan(p, g, mode) for p in pats]
+            want = [r["wcoj"] if not r["ref_exit"] else O.count(pl, off, nbr) for r, pl in zip(rows, plans)]
+            say("every small class", name, "mode", mode)
+            assert sge.count(g, plans)[0] == want
+            assert sge.count(g, plans, batched=True)[0] == want

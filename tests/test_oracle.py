@@ -219,6 +219,85 @@ def test_iep_mode_counts_subgraphs_on_the_reference_divergence():
         assert O.count(sge.build_plan(p, g, mode), off, nbr) == gold[19]["wcoj"]["count"]
 
 
+# ---------------------------------------------------------------- every class with 4-6 vertices
+# tests/golden/classes.json holds all 6 + 21 + 112 connected pattern classes on
+# 4, 5 and 6 vertices and the reference binary's WCOJ and -u counts over two
+# graphs.  The binary ends with a signal on four of them, in both modes and with
+# one or two threads (deterministic): on the 6-cycle (134) and on class 42
+# every minimal-width elimination tree has a bag that is disconnected in the
+# pattern, so its optimal-tree search (plan/all_td.cpp:527-549 clears the list,
+# then skips such trees) returns none; 45 ends with SIGSEGV and 74 with
+# std::bad_array_new_length.  Those counts are pinned by brute force below.
+REF_CLASS_FAILURES = {42, 45, 74, 134}
+# Classes where the reference's -u count differs from its WCOJ count; brute
+# force sides with WCOJ on each one checked below (over- and under-counts).
+REF_CLASS_IEP_DIFFERS = {
+    "c1": {5, 15, 24, 53, 59, 62, 65, 68, 73, 77, 78, 86, 87, 98, 99, 110, 121, 125, 131, 132},
+    "cl60": {5, 15, 24, 59, 62, 65, 68, 73, 77, 79, 87, 90, 98, 99, 110, 112, 121, 125, 127, 131, 132},
+}
+
+
+def class_graphs():
+    return {"c1": W.graph_c1(), "cl60": sge.Graph.chung_lu(60, 600, 3.0, 0.0, 2, True)}
+
+
+def class_patterns(gold):
+    return [sge.Pattern.from_edges(r["k"], [tuple(e) for e in r["edges"]]) for r in gold["patterns"]]
+
+
+def bfs_relabelled(p):
+    """The same pattern with every vertex after the first adjacent to an
+    earlier one (brute force then prunes at every level)."""
+    nb = {v: {b for a, b in p.edges() if a == v} | {a for a, b in p.edges() if b == v} for v in range(p.k)}
+    order = [max(range(p.k), key=lambda v: len(nb[v]))]
+    while len(order) < p.k:
+        rest = [u for u in range(p.k) if u not in order]
+        order.append(max(rest, key=lambda u: (len(nb[u] & set(order)), len(nb[u]))))
+    pos = {v: i for i, v in enumerate(order)}
+    return sge.Pattern.from_edges(p.k, [(pos[a], pos[b]) for a, b in p.edges()])
+
+
+def test_golden_every_small_class():
+    gold = load_golden("classes")
+    pats = class_patterns(gold)
+    assert [sum(p.k == k for p in pats) for k in (4, 5, 6)] == [6, 21, 112]
+    assert len({p.info().canon for p in pats}) == len(pats)  # pairwise non-isomorphic: every class once
+    for name, g in class_graphs().items():
+        rec = gold["graphs"][name]
+        assert (g.n, g.m) == (rec["n"], rec["m"])
+        off, nbr = g.csr()
+        rows = rec["counts"]
+        assert {i for i, r in enumerate(rows) if r["ref_exit"]} == REF_CLASS_FAILURES
+        assert {i for i, r in enumerate(rows) if not r["ref_exit"] and r["iep"] != r["wcoj"]} == \
+            REF_CLASS_IEP_DIFFERS[name]
+        for i, (p, r) in enumerate(zip(pats, rows)):
+            if r["ref_exit"]:
+                continue
+            # embeddings for a single-bag pattern, subgraphs otherwise: the same rule as the reference
+            assert bool(p.info().single_bag) == (r["bags"] == 1), (name, i)
+            for mode in (sge.MODE_WCOJ, sge.MODE_EXPLORE, sge.MODE_IEP):
+                assert O.count(sge.build_plan(p, g, mode), off, nbr) == r["wcoj"], (name, i, mode)
+
+
+def test_classes_the_reference_fails_or_miscounts_by_brute_force():
+    gold = load_golden("classes")
+    pats = class_patterns(gold)
+    g = class_graphs()["cl60"]
+    off, nbr = g.csr()
+    rows = gold["graphs"]["cl60"]["counts"]
+    for i in sorted(REF_CLASS_FAILURES) + [5, 15, 24, 77, 127]:
+        p = pats[i]
+        info = p.info()
+        q = bfs_relabelled(p)
+        assert q.info().canon == info.canon
+        b = O.count_brute(q.k, q.c.adj, off, nbr)
+        want = b if info.single_bag else b // info.aut_size
+        for mode in (sge.MODE_WCOJ, sge.MODE_IEP):
+            assert O.count(sge.build_plan(p, g, mode), off, nbr) == want, (i, mode)
+        if not rows[i]["ref_exit"]:
+            assert rows[i]["wcoj"] == want != rows[i]["iep"], i
+
+
 # ---------------------------------------------------------------- materialised bags
 def bag_levels(plan):
     return [i for i in range(plan.k) if (plan.bag >> i) & 1]
