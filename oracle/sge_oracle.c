@@ -32,7 +32,9 @@
  * order[i]; parent/lower/upper/check masks are over levels.
  *
  * Parity pins (tests/test_oracle.py): golden counts written by the reference
- * binary built from its own sources (tests/golden/*.json), K_n closed forms
+ * binary built from its own sources (tests/golden/*.json, including every
+ * connected class on 4-6 vertices over two graphs; brute force where the binary
+ * crashes or its -u mode disagrees with its WCOJ mode), K_n closed forms
  * n!/(n-k)!/|Aut|, brute force on random small graphs, the diamond formula
  * sum over edges of C(common neighbours, 2), mode/order/relabelling invariance,
  * std-library set operations for the primitives, and a hand-computed wrap case
