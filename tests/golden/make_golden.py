@@ -3,7 +3,7 @@ from its own sources by oracle/ref.mk into oracle/_ref/, on this repo's
 synthetic graphs.  Run in a container that has /root/reference and the built
 binary:  python tests/golden/make_golden.py [section ...]
 
-Sections: c1, c2, c3, rmat7, rmat7_iep, queries, queries_c2, classes, dataset.  Output: tests/golden/<section>.json
+Sections: c1, c2, c3, c3_modes, rmat7, rmat7_iep, queries, queries_c2, classes, dataset.  Output: tests/golden/<section>.json
 (inputs are named by generator + parameters; counts are the reference's
 "Number of matches" lines; "bags" its "num bags" line).
 """
@@ -94,6 +94,24 @@ def section_c3():
         r = run_ref(pp, gp, ["-u"], 8)
         res["patterns"].append({"index": i, "edges": p.edges(), "canon": p.info().canon, "iep": r})
         print(i, r, flush=True)
+    return res
+
+
+def section_c3_modes():
+    """The 16 six-vertex patterns on C2 in the reference's other two modes:
+    WCOJ (no flag) and exploration (-i), both with 8 threads."""
+    g = W.graph_c2(False)
+    gp = graph_file("c2", g)
+    res = {"graph": "chung_lu(n=7115, m=100762, gamma=2.1, cap=1100, seed=1)", "modes": ["", "-i"],
+           "threads": 8, "cores": os.cpu_count(), "patterns": []}
+    for i, p in enumerate(W.six_vertex_set()):
+        pp = os.path.join(TMP, f"six{i}.txt")
+        write_pattern(p.k, p.edges(), pp)
+        rec = {"index": i, "edges": p.edges(), "canon": p.info().canon,
+               "wcoj": run_ref(pp, gp, [], 8), "explore": run_ref(pp, gp, ["-i"], 8)}
+        res["patterns"].append(rec)
+        print(i, rec["wcoj"].get("count"), rec["wcoj"].get("exec_s"), rec["explore"].get("count"),
+              rec["explore"].get("exec_s"), flush=True)
     return res
 
 

@@ -644,3 +644,27 @@ def test_every_small_class_matches_reference():
             say("every small class", name, "mode", mode)
             assert sge.count(g, plans)[0] == want
             assert sge.count(g, plans, batched=True)[0] == want
+
+
+def test_every_reference_pattern_file_on_cliques():
+    """The reference's 107 pattern files (94 benchmark patterns, 13 query files)
+    on K6, K7 and K8 in every mode, plain and batched: n!/(n-k)! embeddings,
+    divided by |Aut| unless the pattern is single-bag."""
+    import math
+
+    gold = golden("dataset")
+    pats = [sge.Pattern.from_edges(r["k"], [tuple(e) for e in r["edges"]]) for d in gold.values() for r in d]
+    assert len(pats) == 107
+    for n in (6, 7, 8):
+        kn = sge.Graph.from_edges(n, [(i, j) for i in range(n) for j in range(i + 1, n)])
+        fit = [p for p in pats if p.k <= n]
+        want = []
+        for p in fit:
+            info = p.info()
+            emb = math.perm(n, p.k)
+            want.append(emb if info.single_bag else emb // info.aut_size)
+        for mode in MODES:
+            plans = [sge.build_plan(p, kn, mode) for p in fit]
+            say("pattern files on K", n, "mode", mode)
+            assert sge.count(kn, plans)[0] == want
+            assert sge.count(kn, plans, batched=True)[0] == want

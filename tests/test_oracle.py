@@ -109,18 +109,28 @@ def test_oracle_every_order_agrees():
         assert len(counts) == 1
 
 
+def dataset_patterns():
+    """The reference's pattern files (dataset/6vertex, 7vertex, light_query,
+    huge_query) from tests/golden/dataset.json."""
+    gold = load_golden("dataset")
+    return [sge.Pattern.from_edges(r["k"], [tuple(e) for e in r["edges"]]) for d in gold.values() for r in d]
+
+
 @pytest.mark.parametrize("n", [6, 7, 8])
 def test_closed_form_on_cliques(n):
+    """Every pattern file of the reference (the 94 benchmark patterns and the 13
+    query files) and the small patterns on K_n in every mode: n!/(n-k)!
+    embeddings, divided by |Aut| unless the pattern is single-bag."""
     kn = sge.Graph.from_edges(n, [(i, j) for i in range(n) for j in range(i + 1, n)])
     off, nbr = kn.csr()
-    for p in W.six_vertex_set()[:8] + W.seven_vertex_set()[:6] + small_patterns():
+    for p in dataset_patterns() + small_patterns():
         if p.k > n:
             continue
         info = p.info()
         emb = math.perm(n, p.k)
         want = emb if info.single_bag else emb // info.aut_size
-        plan = sge.build_plan(p, kn, sge.MODE_IEP)
-        assert O.count(plan, off, nbr) == want
+        for mode in (sge.MODE_WCOJ, sge.MODE_EXPLORE, sge.MODE_IEP):
+            assert O.count(sge.build_plan(p, kn, mode), off, nbr) == want, (p.edges(), mode)
 
 
 def test_diamond_formula():
