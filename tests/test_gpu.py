@@ -668,3 +668,17 @@ def test_every_reference_pattern_file_on_cliques():
             say("pattern files on K", n, "mode", mode)
             assert sge.count(kn, plans)[0] == want
             assert sge.count(kn, plans, batched=True)[0] == want
+
+
+def test_c3_wcoj_and_exploration_match_reference_wcoj():
+    """The 16 six-vertex patterns on C2 in WCOJ and exploration mode, plain and
+    batched: the reference's WCOJ counts (tests/golden/c3_modes.json)."""
+    g = W.graph_c2()
+    gold = golden("c3_modes")["patterns"]
+    pats = W.six_vertex_set()
+    want = [r["wcoj"]["count"] for r in gold]
+    for mode in (sge.MODE_WCOJ, sge.MODE_EXPLORE):
+        plans = [sge.build_plan(pats[r["index"]], g, mode) for r in gold]
+        say("c3 six-vertex reference WCOJ counts, mode", mode)
+        assert sge.count(g, plans)[0] == want
+        assert sge.count(g, plans, batched=True)[0] == want
