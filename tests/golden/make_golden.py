@@ -3,7 +3,7 @@ from its own sources by oracle/ref.mk into oracle/_ref/, on this repo's
 synthetic graphs.  Run in a container that has /root/reference and the built
 binary:  python tests/golden/make_golden.py [section ...]
 
-Sections: c1, c2, c3, c3_modes, rmat7, rmat7_iep, queries, queries_c2, classes, dataset.  Output: tests/golden/<section>.json
+Sections: c1, c2, c3, c3_modes, rmat7, rmat7_iep, rmat7_explore, queries, queries_c2, classes, dataset.  Output: tests/golden/<section>.json
 (inputs are named by generator + parameters; counts are the reference's
 "Number of matches" lines; "bags" its "num bags" line).
 """
@@ -139,6 +139,20 @@ def section_rmat7_iep():
         r = run_ref(pp, gp, ["-u"], 8)
         res["patterns"].append({"index": i, "edges": p.edges(), "canon": p.info().canon, "iep": r})
         print(i, r, flush=True)
+    return res
+
+
+def section_rmat7_explore():
+    """The same 78 patterns and graph as rmat7, reference exploration mode (-i, 8 threads)."""
+    g = W.graph_rmat(8, 7, False)
+    gp = graph_file("rmat8", g)
+    res = {"graph": "rmat(scale=8, ef=16, a=.57, b=.19, c=.19, seed=7)", "n": g.n, "m": g.m, "patterns": []}
+    for i, p in enumerate(W.seven_vertex_set()):
+        pp = os.path.join(TMP, f"seven{i}.txt")
+        write_pattern(p.k, p.edges(), pp)
+        r = run_ref(pp, gp, ["-i"], 8)
+        res["patterns"].append({"index": i, "edges": p.edges(), "canon": p.info().canon, "explore": r})
+        print(i, r.get("count"), flush=True)
     return res
 
 

@@ -682,3 +682,16 @@ def test_c3_wcoj_and_exploration_match_reference_wcoj():
         say("c3 six-vertex reference WCOJ counts, mode", mode)
         assert sge.count(g, plans)[0] == want
         assert sge.count(g, plans, batched=True)[0] == want
+
+
+def test_rmat8_seven_vertex_wcoj_and_exploration_batched():
+    """All 78 seven-vertex patterns over R-MAT scale 8 in WCOJ and exploration
+    mode, one batched launch each: the reference's WCOJ counts (its own -i mode
+    differs on ten of them, documented in test_oracle)."""
+    want = [r["wcoj"]["count"] for r in golden("rmat7")["patterns"]]
+    g = W.graph_rmat(8, 7)
+    pats = W.seven_vertex_set()
+    for mode in (sge.MODE_WCOJ, sge.MODE_EXPLORE):
+        plans = [sge.build_plan(p, g, mode) for p in pats]
+        say("rmat8 78 batched, mode", mode)
+        assert sge.count(g, plans, batched=True)[0] == want
