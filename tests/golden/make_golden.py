@@ -3,7 +3,7 @@ from its own sources by oracle/ref.mk into oracle/_ref/, on this repo's
 synthetic graphs.  Run in a container that has /root/reference and the built
 binary:  python tests/golden/make_golden.py [section ...]
 
-Sections: c1, c2, c3, c3_modes, rmat7, rmat7_iep, rmat7_explore, queries, queries_c2, classes, dataset.  Output: tests/golden/<section>.json
+Sections: c1, c2, c3, c3_modes, rmat7, rmat7_iep, rmat7_explore, queries, queries_c2, classes, classes7, dataset.  Output: tests/golden/<section>.json
 (inputs are named by generator + parameters; counts are the reference's
 "Number of matches" lines; "bags" its "num bags" line).
 """
@@ -217,6 +217,28 @@ def section_classes():
         print(name, sum(r["wcoj"] != r["iep"] for r in rows if r["wcoj"] is not None), "patterns where -u differs;",
               [i for i, r in enumerate(rows) if r["ref_exit"]], "reference failures", flush=True)
     return res
+
+
+def section_classes7():
+    """Every connected pattern class on 7 vertices (853) over C1: the reference's
+    WCOJ and -u counts, its bag count and its exit status."""
+    g = W.graph_c1(False)
+    gp = graph_file("c1", g)
+    pats = sge.sample_patterns(7, 6, 21, 1000, 11)
+    rows = []
+    for i, p in enumerate(pats):
+        pp = os.path.join(TMP, f"class7_{i}.txt")
+        write_pattern(p.k, p.edges(), pp)
+        w, u = run_ref(pp, gp, [], 1), run_ref(pp, gp, ["-u"], 1)
+        rows.append({"wcoj": w.get("count"), "iep": u.get("count"), "bags": w.get("bags"),
+                     "ref_exit": w.get("error", 0) or u.get("error", 0)})
+        if i % 50 == 0:
+            print(i, flush=True)
+    print(len(pats), "classes;", [i for i, r in enumerate(rows) if r["ref_exit"]], "reference failures;",
+          sum(r["wcoj"] != r["iep"] for r in rows if not r["ref_exit"]), "where -u differs", flush=True)
+    return {"patterns": [{"k": p.k, "edges": p.edges()} for p in pats],
+            "graphs": {"c1": {"graph": "chung_lu(n=379, m=914, gamma=2.5, cap=none, seed=0)", "n": g.n, "m": g.m,
+                              "counts": rows}}}
 
 
 def section_dataset():

@@ -349,6 +349,55 @@ def test_classes_the_reference_fails_or_miscounts_by_brute_force():
             assert rows[i]["wcoj"] == want != rows[i]["iep"], i
 
 
+# tests/golden/classes7.json: every connected class on 7 vertices (853) over
+# C1.  The reference binary ends with a signal (SIGSEGV or SIGABRT) on 83 of
+# them, and its -u count differs from its WCOJ count on 153 of the 770 it
+# completes.  Brute-force counts for those 236 classes were added to the fixture
+# by tests/golden/brute_classes7.py; each equals the reference's WCOJ count where
+# it has one.
+REF_CLASS7_FAILURES = 83
+REF_CLASS7_IEP_DIFFERS = 153
+
+
+def test_golden_every_seven_vertex_class():
+    gold = load_golden("classes7")
+    pats = class_patterns(gold)
+    assert len(pats) == 853 and all(p.k == 7 for p in pats)
+    assert len({p.info().canon for p in pats}) == 853
+    rec = gold["graphs"]["c1"]
+    g = W.graph_c1()
+    assert (g.n, g.m) == (rec["n"], rec["m"])
+    off, nbr = g.csr()
+    rows = rec["counts"]
+    assert sum(1 for r in rows if r["ref_exit"]) == REF_CLASS7_FAILURES
+    assert sum(1 for r in rows if not r["ref_exit"] and r["iep"] != r["wcoj"]) == REF_CLASS7_IEP_DIFFERS
+    for i, (p, r) in enumerate(zip(pats, rows)):
+        if r["ref_exit"] or r["iep"] != r["wcoj"]:
+            assert "brute" in r, i
+        if not r["ref_exit"]:
+            assert bool(p.info().single_bag) == (r["bags"] == 1), i
+            assert r.get("brute", r["wcoj"]) == r["wcoj"], i
+        want = r["brute"] if r["ref_exit"] else r["wcoj"]
+        assert O.count(sge.build_plan(p, g, sge.MODE_IEP), off, nbr) == want, i
+
+
+def test_seven_vertex_brute_force_counts_reproduce():
+    """The fixture's brute-force counts, recomputed for the 16 cheapest classes
+    (crashes and -u divergences both)."""
+    gold = load_golden("classes7")
+    pats = class_patterns(gold)
+    rows = gold["graphs"]["c1"]["counts"]
+    g = W.graph_c1()
+    off, nbr = g.csr()
+    picked = sorted((i for i, r in enumerate(rows) if "brute" in r), key=lambda i: rows[i]["brute"])[:16]
+    assert any(rows[i]["ref_exit"] for i in picked) and any(not rows[i]["ref_exit"] for i in picked)
+    for i in picked:
+        p = pats[i]
+        q = bfs_relabelled(p)
+        b = O.count_brute(q.k, q.c.adj, off, nbr)
+        assert (b if p.info().single_bag else b // p.info().aut_size) == rows[i]["brute"], i
+
+
 # ---------------------------------------------------------------- materialised bags
 def bag_levels(plan):
     return [i for i in range(plan.k) if (plan.bag >> i) & 1]
