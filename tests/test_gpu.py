@@ -695,3 +695,22 @@ def test_rmat8_seven_vertex_wcoj_and_exploration_batched():
         plans = [sge.build_plan(p, g, mode) for p in pats]
         say("rmat8 78 batched, mode", mode)
         assert sge.count(g, plans, batched=True)[0] == want
+
+
+def test_every_seven_vertex_class_matches_reference():
+    """All 853 connected 7-vertex classes over C1 (IEP plain and batched, WCOJ
+    batched): the reference's WCOJ counts, and the fixture's brute-force counts
+    on the 83 classes where the reference binary fails."""
+    gold = golden("classes7")
+    pats = [sge.Pattern.from_edges(r["k"], [tuple(e) for e in r["edges"]]) for r in gold["patterns"]]
+    rows = gold["graphs"]["c1"]["counts"]
+    want = [r["brute"] if r["ref_exit"] else r["wcoj"] for r in rows]
+    g = W.graph_c1()
+    plans = [sge.build_plan(p, g, sge.MODE_IEP) for p in pats]
+    say("853 seven-vertex classes, IEP batched")
+    assert sge.count(g, plans, batched=True)[0] == want
+    say("853 seven-vertex classes, IEP plain")
+    assert sge.count(g, plans)[0] == want
+    plans = [sge.build_plan(p, g, sge.MODE_WCOJ) for p in pats]
+    say("853 seven-vertex classes, WCOJ batched")
+    assert sge.count(g, plans, batched=True)[0] == want
