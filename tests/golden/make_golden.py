@@ -3,7 +3,7 @@ from its own sources by oracle/ref.mk into oracle/_ref/, on this repo's
 synthetic graphs.  Run in a container that has /root/reference and the built
 binary:  python tests/golden/make_golden.py [section ...]
 
-Sections: c1, c2, c3, c3_modes, rmat7, rmat7_iep, rmat7_explore, queries, queries_c2, classes, classes7, dataset.  Output: tests/golden/<section>.json
+Sections: c1, c2, c3, c3_modes, rmat7, rmat7_iep, rmat7_explore, queries, queries_c2, classes, classes_explore, classes7, dataset.  Output: tests/golden/<section>.json
 (inputs are named by generator + parameters; counts are the reference's
 "Number of matches" lines; "bags" its "num bags" line).
 """
@@ -216,6 +216,27 @@ def section_classes():
         res["graphs"][name] = {"graph": desc, "n": g.n, "m": g.m, "counts": rows}
         print(name, sum(r["wcoj"] != r["iep"] for r in rows if r["wcoj"] is not None), "patterns where -u differs;",
               [i for i, r in enumerate(rows) if r["ref_exit"]], "reference failures", flush=True)
+    return res
+
+
+def section_classes_explore():
+    """The reference's exploration mode (-i, 8 threads) on the same 139 classes
+    and graphs as section classes (pattern order from classes.json)."""
+    with open(os.path.join(OUT, "classes.json")) as f:
+        base = json.load(f)
+    pats = [sge.Pattern.from_edges(r["k"], [tuple(e) for e in r["edges"]]) for r in base["patterns"]]
+    graphs = {"c1": W.graph_c1(False), "cl60": sge.Graph.chung_lu(60, 600, 3.0, 0.0, 2, False)}
+    res = {"graphs": {}}
+    for name, g in graphs.items():
+        gp = graph_file(name, g)
+        rows = []
+        for i, p in enumerate(pats):
+            pp = os.path.join(TMP, f"classx{i}.txt")
+            write_pattern(p.k, p.edges(), pp)
+            x = run_ref(pp, gp, ["-i"], 8)
+            rows.append({"explore": x.get("count"), "ref_exit": x.get("error", 0)})
+        res["graphs"][name] = {"n": g.n, "m": g.m, "counts": rows}
+        print(name, [i for i, r in enumerate(rows) if r["ref_exit"]], "failures", flush=True)
     return res
 
 

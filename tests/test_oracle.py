@@ -330,6 +330,29 @@ def test_golden_every_small_class():
                 assert O.count(sge.build_plan(p, g, mode), off, nbr) == r["wcoj"], (name, i, mode)
 
 
+# The reference's exploration mode (-i -t 8) on the same classes: it also
+# crashes on K4, K5 and K6 (classes 0, 7, 29: single-bag plans with more than
+# one thread, parallel.cpp:1205), and its count differs from its WCOJ count on
+# these classes (mostly exactly 2x; class 56 on cl60 prints 0).
+REF_CLASS_EXPLORE_FAILURES = REF_CLASS_FAILURES | {0, 7, 29}
+REF_CLASS_EXPLORE_DIFFERS = {"c1": {8, 57, 96, 101, 102, 126, 128},
+                             "cl60": {8, 30, 56, 57, 96, 101, 102, 126, 128}}
+
+
+def test_reference_exploration_mode_on_small_classes():
+    base = load_golden("classes")
+    expl = load_golden("classes_explore")
+    pats = class_patterns(base)
+    for name in ("c1", "cl60"):
+        b = base["graphs"][name]["counts"]
+        e = expl["graphs"][name]["counts"]
+        assert len(b) == len(e) == len(pats)
+        assert {i for i, r in enumerate(e) if r["ref_exit"]} == REF_CLASS_EXPLORE_FAILURES
+        assert all(pats[i].info().single_bag for i in REF_CLASS_EXPLORE_FAILURES - REF_CLASS_FAILURES)
+        assert {i for i, r in enumerate(e) if not r["ref_exit"] and r["explore"] != b[i]["wcoj"]} == \
+            REF_CLASS_EXPLORE_DIFFERS[name]
+
+
 def test_classes_the_reference_fails_or_miscounts_by_brute_force():
     gold = load_golden("classes")
     pats = class_patterns(gold)
