@@ -3,7 +3,7 @@ from its own sources by oracle/ref.mk into oracle/_ref/, on this repo's
 synthetic graphs.  Run in a container that has /root/reference and the built
 binary:  python tests/golden/make_golden.py [section ...]
 
-Sections: c1, c2, c3, c3_modes, rmat7, rmat7_iep, rmat7_explore, queries, queries_c2, classes, classes_explore, classes7, dataset.  Output: tests/golden/<section>.json
+Sections: c1, c2, c3, c3_modes, rmat7, rmat7_iep, rmat7_explore, queries, queries_c2, classes, classes_explore, classes7, classes7_explore, dataset.  Output: tests/golden/<section>.json
 (inputs are named by generator + parameters; counts are the reference's
 "Number of matches" lines; "bags" its "num bags" line).
 """
@@ -260,6 +260,25 @@ def section_classes7():
     return {"patterns": [{"k": p.k, "edges": p.edges()} for p in pats],
             "graphs": {"c1": {"graph": "chung_lu(n=379, m=914, gamma=2.5, cap=none, seed=0)", "n": g.n, "m": g.m,
                               "counts": rows}}}
+
+
+def section_classes7_explore():
+    """The reference's exploration mode (-i, 8 threads) on the 853 classes of
+    section classes7 over C1 (pattern order from classes7.json)."""
+    with open(os.path.join(OUT, "classes7.json")) as f:
+        base = json.load(f)
+    pats = [sge.Pattern.from_edges(r["k"], [tuple(e) for e in r["edges"]]) for r in base["patterns"]]
+    g = W.graph_c1(False)
+    gp = graph_file("c1", g)
+    rows = []
+    for i, p in enumerate(pats):
+        pp = os.path.join(TMP, f"class7x_{i}.txt")
+        write_pattern(p.k, p.edges(), pp)
+        x = run_ref(pp, gp, ["-i"], 8)
+        rows.append({"explore": x.get("count"), "ref_exit": x.get("error", 0)})
+        if i % 100 == 0:
+            print(i, flush=True)
+    return {"graphs": {"c1": {"n": g.n, "m": g.m, "counts": rows}}}
 
 
 def section_dataset():

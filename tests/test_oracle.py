@@ -404,6 +404,21 @@ def test_golden_every_seven_vertex_class():
         assert O.count(sge.build_plan(p, g, sge.MODE_IEP), off, nbr) == want, i
 
 
+def test_reference_exploration_mode_on_seven_vertex_classes():
+    """The reference's -i -t 8 on the 853 seven-vertex classes: it crashes on the
+    same 83 classes as its WCOJ mode plus K7 (a single-bag plan with threads),
+    and its count differs from its WCOJ count on 69 of the 769 it completes."""
+    base = load_golden("classes7")
+    expl = load_golden("classes7_explore")
+    pats = class_patterns(base)
+    b = base["graphs"]["c1"]["counts"]
+    e = expl["graphs"]["c1"]["counts"]
+    wf = {i for i, r in enumerate(b) if r["ref_exit"]}
+    ef = {i for i, r in enumerate(e) if r["ref_exit"]}
+    assert ef - wf == {0} and wf <= ef and pats[0].info().single_bag
+    assert sum(1 for i, r in enumerate(e) if i not in ef and r["explore"] != b[i]["wcoj"]) == 69
+
+
 def test_seven_vertex_brute_force_counts_reproduce():
     """The fixture's brute-force counts, recomputed for the 16 cheapest classes
     (crashes and -u divergences both)."""
