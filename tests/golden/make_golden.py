@@ -3,7 +3,7 @@ from its own sources by oracle/ref.mk into oracle/_ref/, on this repo's
 synthetic graphs.  Run in a container that has /root/reference and the built
 binary:  python tests/golden/make_golden.py [section ...]
 
-Sections: c1, c2, c3, c3_modes, rmat7, rmat7_iep, rmat7_explore, queries, queries_c2, classes, classes_explore, classes7, classes7_explore, dataset.  Output: tests/golden/<section>.json
+Sections: c1, c2, c3, c3_modes, rmat7, rmat7_iep, rmat7_explore, queries, queries_c2, queries_explore, classes, classes_explore, classes7, classes7_explore, dataset.  Output: tests/golden/<section>.json
 (inputs are named by generator + parameters; counts are the reference's
 "Number of matches" lines; "bags" its "num bags" line).
 """
@@ -189,6 +189,28 @@ def section_queries_c2():
                    "wcoj": run_ref(path, gp, [], t), "iep": run_ref(path, gp, ["-u"], t)}
             res["patterns"].append(rec)
             print(rec["file"], rec["wcoj"].get("count"), rec["iep"].get("count"), flush=True)
+    return res
+
+
+def section_queries_explore():
+    """The reference's query files on C1 and C2 in exploration mode (-i, 8
+    threads).  Single-bag files are skipped: the reference ignores -i with one
+    thread (main.cpp:86) and crashes on single-bag plans with more."""
+    res = {}
+    for name, g in (("c1", W.graph_c1(False)), ("c2", W.graph_c2(False))):
+        gp = graph_file(name, g)
+        rows = []
+        for d in ["light_query", "huge_query"]:
+            for f in sorted(os.listdir(os.path.join(REF_DATA, d))):
+                path = os.path.join(REF_DATA, d, f)
+                p = sge.Pattern.load(path)
+                rec = {"file": f"{d}/{f}"}
+                if not p.info().single_bag:
+                    x = run_ref(path, gp, ["-i"], 8)
+                    rec.update({"explore": x.get("count"), "ref_exit": x.get("error", 0)})
+                rows.append(rec)
+                print(name, rec, flush=True)
+        res[name] = {"n": g.n, "m": g.m, "patterns": rows}
     return res
 
 

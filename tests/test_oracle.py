@@ -339,6 +339,18 @@ REF_CLASS_EXPLORE_DIFFERS = {"c1": {8, 57, 96, 101, 102, 126, 128},
                              "cl60": {8, 30, 56, 57, 96, 101, 102, 126, 128}}
 
 
+def test_reference_exploration_mode_on_query_files():
+    """The reference's -i -t 8 on its 11 multi-bag query files (single-bag files
+    need one thread, where it ignores -i) equals its WCOJ count on C1 and C2."""
+    x = load_golden("queries_explore")
+    for name, f in (("c1", "queries"), ("c2", "queries_c2")):
+        w = {r["file"]: r["wcoj"]["count"] for r in load_golden(f)["patterns"]}
+        run = [r for r in x[name]["patterns"] if "explore" in r]
+        assert len(run) == 11
+        for r in run:
+            assert not r["ref_exit"] and r["explore"] == w[r["file"]], (name, r["file"])
+
+
 def test_reference_exploration_mode_on_small_classes():
     base = load_golden("classes")
     expl = load_golden("classes_explore")
