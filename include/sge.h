@@ -203,10 +203,12 @@ typedef struct sge_tuning {
                                  work item may spend before its open candidate ranges go to the
                                  next pass (0 = default 4096, 1 << 30 = off) */
     uint32_t bitmap_ratio;    /* the same ratio when F is staged as an LDS bitmap (0 = stage_max) */
-    uint32_t node_cap_k;      /* stop a counting call after about node_cap_k * 1024 wave steps (as
-                                 donate_budget counts them) and fail with SGE_ELIMIT (0 = no cap;
-                                 the batched executor ignores it).  For timing orders of unknown
-                                 cost. */
+    uint32_t node_cap_k;      /* stop a counting call (plain or batched) after about node_cap_k * 1024
+                                 wave steps (as donate_budget counts them) and fail with SGE_ELIMIT
+                                 (0 = no cap).  Each wavefront checks the cap between search-tree
+                                 nodes and between sibling batches of a last-level count, so a capped
+                                 call always ends with its kernels exiting normally.  For timing
+                                 orders of unknown cost and for bounding runs on large graphs. */
     uint32_t head_items;      /* the heaviest head_items root subtasks of each plan are queue
                                  items of their own, the rest come in chunks (0 = one per
                                  resident wavefront, 1 << 30 = none) */
@@ -238,9 +240,20 @@ void sge_tuning_default(sge_tuning *t);
 int sge_count(sge_graph *g, const sge_plan *plans, uint32_t nplans, uint32_t flags, const sge_tuning *tune,
               uint64_t *counts, sge_stats *stats, sge_stats *per_plan_stats);
 /* Count over the root-task subrange [task_begin, task_end) of one plan (counts are
- * additive over disjoint ranges).  Used for sampled work logging. */
+ * additive over disjoint ranges).  Used for sampled work logging.  Root tasks are
+ * the directed data edges in CSR order, the reference's edge list
+ * (CandidateSpace::getEdges, relation/candidate_space.cpp:787-799) that
+ * parSharedJoin splits over its threads (matching/parallel.cpp:1414-1458). */
 int sge_count_range(sge_graph *g, const sge_plan *plan, uint64_t task_begin, uint64_t task_end,
                     const sge_tuning *tune, uint64_t *count, sge_stats *stats);
+/* Count plans[i] over its own root-task range [task_begin[i], task_end[i]):
+ * one launch per plan, or all plans in one launch with SGE_COUNT_BATCHED.
+ * Per-plan ranges let a pattern set run on bounded root subsets of a graph
+ * whose whole job is out of reach (partial counts depend on the plan's order:
+ * compare executors on the same plans). */
+int sge_count_ranges(sge_graph *g, const sge_plan *plans, uint32_t nplans, const uint64_t *task_begin,
+                     const uint64_t *task_end, uint32_t flags, const sge_tuning *tune, uint64_t *counts,
+                     sge_stats *stats, sge_stats *per_plan);
 /* Count over the strided root-task sample {offset, offset+stride, ...} of each
  * plan, one launch per plan (the plain executor); per-plan stats.  Used to log
  * (order, measured work) pairs for the learned cost model. */
@@ -271,12 +284,18 @@ int sge_trie_shape(const sge_trie *t, uint32_t *depth, uint64_t *tuples, uint64_
 int sge_trie_copy(const sge_trie *t, uint32_t *values, uint32_t *child);
 void sge_trie_destroy(sge_trie *t);
 
-/* Batched executor captured once in a hipGraph, replayed per run. */
+/* Batched executor captured once in a hipGraph, replayed per run.  A batch
+ * borrows its graph's device arrays: sge_graph_destroy on a graph with live
+ * batches only marks it, and the last sge_batch_destroy releases it. */
 typedef struct sge_batch sge_batch;
 int sge_batch_create(sge_graph *g, const sge_plan *plans, uint32_t nplans, const sge_tuning *tune, sge_batch **out);
 /* Same over the strided root-edge sample {offset, offset+stride, ...}. */
 int sge_batch_create_sampled(sge_graph *g, const sge_plan *plans, uint32_t nplans, uint64_t stride, uint64_t offset,
                              const sge_tuning *tune, sge_batch **out);
+/* Same with plan i over its own root-task range [task_begin[i], task_end[i]) (as sge_count_ranges). */
+int sge_batch_create_ranges(sge_graph *g, const sge_plan *plans, uint32_t nplans, const uint64_t *task_begin,
+                            const uint64_t *task_end, const sge_tuning *tune, sge_batch **out);
+/* SGE_ELIMIT when the tuning's node_cap_k stopped the run (counts partial). */
 int sge_batch_run(sge_batch *b, uint64_t *counts, sge_stats *stats);
 void sge_batch_destroy(sge_batch *b);
 

@@ -215,8 +215,16 @@ extern "C" int sge_graph_write(const sge_graph *g, const char *path, int binary)
     SGE_API_END
 }
 
-extern "C" void sge_graph_destroy(sge_graph *g) {
-    if (!g) return;
+void sge_graph_release(sge_graph *g) {
     sge_graph_device_free(g);
     delete g;
+}
+
+extern "C" void sge_graph_destroy(sge_graph *g) {
+    if (!g) return;
+    if (g->batches) {  // a live batch still reads the device arrays: the last batch releases the graph
+        g->destroyed = true;
+        return;
+    }
+    sge_graph_release(g);
 }

@@ -148,4 +148,11 @@ struct sge_graph {
     void *split_cache = nullptr;  // executor-owned: per-shape subtask prefix arrays on the device
     sge::GraphSummary summary;
     bool summary_ready = false;
+    // batches borrow the graph's device arrays: sge_graph_destroy defers the
+    // release while any batch of it is alive, and the last one releases it
+    uint32_t batches = 0;
+    bool destroyed = false;
 };
+
+// Free a graph's device mirror and the graph itself (api.cpp).
+void sge_graph_release(sge_graph *g);

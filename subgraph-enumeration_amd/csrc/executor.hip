@@ -117,6 +117,8 @@ struct DPlan {
     // split[i+1] - split[i] processes slice ci of the level-2 candidates.
     const unsigned long long *split;
     unsigned long long nsub;
+    // this plan's root tasks: CSR positions task_begin + i * stride, i < ntask
+    unsigned long long task_begin, ntask;
 };
 
 struct PlanOut {
@@ -131,7 +133,7 @@ struct KParams {
     uint64_t nnz;
     const DPlan *plans;
     uint32_t nplans;
-    uint64_t task_begin, task_end, stride;  // tasks are CSR positions task_begin + i*stride < task_end
+    uint64_t stride;  // root tasks of plan p are CSR positions p.task_begin + i * stride, i < p.ntask
     uint32_t chunk;
     uint32_t head;    // the heaviest `head` subtasks of each plan are one queue item each
     uint64_t nitems;
@@ -528,6 +530,19 @@ __device__ __forceinline__ bool term_has(const WaveLds &w, const TermLds &t, con
 // |bc|a + 2|abc|.  u64 arithmetic is exact modulo 2^64 and the true count is
 // below 2^64.
 
+// Work cap: report `since` wave steps once at least NODE_STEP have gathered;
+// true when the call's total has reached the cap (every wave then stops at its
+// next check, so a capped call ends with a normal kernel exit).
+template <bool CAP>
+__device__ __forceinline__ bool cap_reached(const KParams &P, uint32_t &since) {
+    if (!CAP || since < NODE_STEP) return false;
+    unsigned long long tot = 0;
+    if (lane_id() == 0) tot = atomicAdd(P.node_ctr, (unsigned long long)since);
+    tot = __shfl(tot, 0);
+    since = 0;
+    return tot >= P.node_cap;
+}
+
 // Whether a global bitmap of a large F (len elements) pays for its build and
 // clear (about 2 len / 64 wave steps): the siblings' rows short enough to
 // probe F (|row| <= 2 len) save about log2(len) loads per element.  Estimated
@@ -553,10 +568,13 @@ __device__ __forceinline__ bool gbm_pays(const KParams &P, const WaveLds &w, int
 // depend on the sibling, with every bound/check from levels below the sibling
 // applied; per (sibling, term) the count is |F ∩ N(c) ∩ range(c)| (or a range
 // size), computed with the flattened probe loop.
-// Returns the wave steps spent (64-lane iterations of the entry setup and the
-// flattened probes), the unit of the donation budget and the node cap.
-__device__ __forceinline__ uint32_t count_last(const KParams &P, WaveLds &w, TermLds *terms, uint32_t *wb, uint32_t *gb,
-                                               Acc &acc) {
+// The wave steps it spends (64-lane iterations of the entry setup and the
+// flattened probes), the unit of the donation budget and the work cap, are
+// added to `spent` and `since`; the cap is checked after every sibling batch.
+// Returns true when the cap stopped it (its count is then partial).
+template <bool CAP>
+__device__ __forceinline__ bool count_last(const KParams &P, WaveLds &w, TermLds *terms, uint32_t *wb, uint32_t *gb,
+                                           Acc &acc, uint32_t &spent, uint32_t &since) {
     const DPlan &pl = w.pl;
     const int s = pl.sib;
     const int lane = lane_id();
@@ -677,7 +695,17 @@ __device__ __forceinline__ uint32_t count_last(const KParams &P, WaveLds &w, Ter
     const uint32_t per_block = SIB_BLOCK / K;     // siblings per batch
     unsigned long long local = 0;                 // per-lane partial
     uint32_t steps = 1;
+    bool stopped = false;
     for (uint32_t sb = 0; sb < nsib; sb += per_block) {
+        if (CAP) {
+            spent += steps;
+            since += steps;
+            steps = 0;
+            if (cap_reached<CAP>(P, since)) {
+                stopped = true;
+                break;
+            }
+        }
         const uint32_t nb = min(per_block, nsib - sb);
         const uint32_t ne = nb * K;  // entries: sibling i, term j -> i*K + j
         // phase 1: per-entry ranges and work (lane per entry)
@@ -834,7 +862,9 @@ __device__ __forceinline__ uint32_t count_last(const KParams &P, WaveLds &w, Ter
         }
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");
     }
-    return steps;
+    spent += steps;
+    since += steps;
+    return stopped;
 }
 
 __device__ void flush(const KParams &P, uint32_t plan, Acc &acc) {
@@ -931,10 +961,10 @@ __device__ __forceinline__ void donate(const KParams &P, WaveLds &w, uint32_t pl
 // Depth-first search from level l0 whose iteration range [pos, iend) is set,
 // images of levels < l0 fixed.  Levels below `stop` are enumerated; `stop`
 // is counted for all its siblings at once (or emitted when enumerating).
-template <bool ENUM>
-__device__ __forceinline__ void run_dfs(const KParams &P, WaveLds &w, TermLds *terms, uint32_t *wb, uint32_t *gb, Acc &acc,
-                                        uint32_t plan, int l0,
-                        int stop) {
+// Returns true when the work cap stopped the search (the wave then exits).
+template <bool ENUM, bool CAP>
+__device__ __forceinline__ bool run_dfs(const KParams &P, WaveLds &w, TermLds *terms, uint32_t *wb, uint32_t *gb,
+                                        Acc &acc, uint32_t plan, int l0, int stop) {
     const DPlan &pl = w.pl;
     uint32_t spent = 0, since = 0;  // wave steps since the last donation / node-cap report
     int d = l0;
@@ -950,10 +980,8 @@ __device__ __forceinline__ void run_dfs(const KParams &P, WaveLds &w, TermLds *t
                     if (descend) trie_enter(w, d, x, i);
                     emit_last(P, w, wb, acc);
                 }
-            } else {
-                const uint32_t st = count_last(P, w, terms, wb, gb, acc);
-                spent += st;
-                since += st;
+            } else if (count_last<CAP>(P, w, terms, wb, gb, acc, spent, since)) {
+                return true;  // work cap reached (reported by count_last)
             }
             --d;
             continue;
@@ -988,18 +1016,13 @@ __device__ __forceinline__ void run_dfs(const KParams &P, WaveLds &w, TermLds *t
                 donate(P, w, plan, l0, d, stop);
                 spent = 0;
             }
-            if (P.node_cap && since >= NODE_STEP) {
-                unsigned long long tot = 0;
-                if (lane_id() == 0) tot = atomicAdd(P.node_ctr, (unsigned long long)since);
-                tot = __shfl(tot, 0);
-                since = 0;
-                if (tot >= P.node_cap) return;  // the host reports the call as stopped
-            }
+            if (cap_reached<CAP>(P, since)) return true;  // the host reports the call as stopped
         } else {
             --d;
         }
     }
-    if (P.node_cap && since && lane_id() == 0) atomicAdd(P.node_ctr, (unsigned long long)since);
+    if (CAP && since && lane_id() == 0) atomicAdd(P.node_ctr, (unsigned long long)since);
+    return false;
 }
 
 // Resident wavefronts per SIMD the register allocation is held to: the
@@ -1012,7 +1035,9 @@ __device__ __forceinline__ void run_dfs(const KParams &P, WaveLds &w, TermLds *t
 #endif
 static_assert((sizeof(WaveLds) + MAX_TERMS * sizeof(TermLds)) * 4 * SGE_WAVES_PER_SIMD <= 160 * 1024,
               "the resident wavefronts' LDS state must fit one CU");
-template <bool ENUM>
+// CAP: the call has a work cap (node_cap != 0); the uncapped kernel carries no
+// cap accounting at all.
+template <bool ENUM, bool CAP>
 __global__ void __launch_bounds__(256, SGE_WAVES_PER_SIMD) wcoj_kernel(KParams P) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     const int lane = lane_id();
@@ -1027,11 +1052,11 @@ __global__ void __launch_bounds__(256, SGE_WAVES_PER_SIMD) wcoj_kernel(KParams P
     if (lane == 0)
         for (int k = 0; k < A_NUM; ++k) w.accv[k] = 0;
     uint32_t cur_plan = NONE;
-    const uint64_t ntask = (P.task_end - P.task_begin + P.stride - 1) / P.stride;
     const bool cont = P.in_items != nullptr;
     const uint64_t nitems = cont ? min((uint64_t)*P.in_count, P.item_cap) : P.nitems;
     unsigned long long t_item = __builtin_amdgcn_s_memtime();  // start of the current item
-    uint32_t fetched = 0;
+    uint32_t fetched = 15;  // the work cap is checked before the first item and every 16th after it
+    uint32_t since = 0;     // wave steps of root tasks counted at the first level (not yet reported)
     while (true) {
         {
             // charge the time since the last item started to that item's plan
@@ -1043,7 +1068,7 @@ __global__ void __launch_bounds__(256, SGE_WAVES_PER_SIMD) wcoj_kernel(KParams P
         if (lane == 0) item = atomicAdd(P.queue, 1ull);
         item = __shfl(item, 0);
         if (item >= nitems) break;
-        if (P.node_cap && (++fetched & 15u) == 0) {  // every 16th item: has the work cap been reached?
+        if (CAP && (++fetched & 15u) == 0) {  // every 16th item: has the work cap been reached?
             unsigned long long spent = 0;
             if (lane == 0) spent = __hip_atomic_load(P.node_ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             if (__shfl(spent, 0) >= P.node_cap) break;
@@ -1080,7 +1105,7 @@ __global__ void __launch_bounds__(256, SGE_WAVES_PER_SIMD) wcoj_kernel(KParams P
             const uint32_t rb = it[2], re = it[3];
             w.pos[l] = max(rb, w.cand[l].b);
             w.iend[l] = min(re, w.cand[l].e);
-            run_dfs<ENUM>(P, w, terms, wb, gb, acc, plan, l, stop);
+            if (run_dfs<ENUM, CAP>(P, w, terms, wb, gb, acc, plan, l, stop)) break;  // work cap reached
             continue;
         }
         // ---- root pass: chunk of subtasks of one plan
@@ -1093,6 +1118,7 @@ __global__ void __launch_bounds__(256, SGE_WAVES_PER_SIMD) wcoj_kernel(KParams P
         }
         const DPlan &pl = w.pl;
         if (pl.k < 2) continue;  // never launched by the host; no edge tasks exist for it
+        const uint64_t ntask = pl.ntask;
         const uint64_t nsub = pl.split ? pl.nsub : ntask;
         // heavy-first: item 0 covers the highest CSR positions; the first
         // `head` items are single subtasks, the rest chunks of P.chunk
@@ -1107,7 +1133,8 @@ __global__ void __launch_bounds__(256, SGE_WAVES_PER_SIMD) wcoj_kernel(KParams P
             hi_u = nsub - head - c2 * P.chunk;
             lo_u = hi_u > P.chunk ? hi_u - P.chunk : 0;
         }
-        for (uint64_t u = lo_u; u < hi_u; ++u) {
+        bool halt = false;  // the work cap stopped a search: this wave exits
+        for (uint64_t u = lo_u; u < hi_u && !halt; ++u) {
             uint64_t ti = u, ci = 0, nch = 1;
             if (pl.split) {
                 // sampled root edge owning subtask u: last i with split[i] <= u
@@ -1120,7 +1147,7 @@ __global__ void __launch_bounds__(256, SGE_WAVES_PER_SIMD) wcoj_kernel(KParams P
                 ci = u - pl.split[a];
                 nch = pl.split[a + 1] - pl.split[a];
             }
-            const uint64_t t = P.task_begin + ti * P.stride;
+            const uint64_t t = pl.task_begin + ti * P.stride;
             uint32_t v0 = P.src[t], v1 = P.nbr[t];
             if ((pl.lower[1] & 1u) && !(v1 > v0)) continue;
             if ((pl.upper[1] & 1u) && !(v1 < v0)) continue;
@@ -1139,7 +1166,8 @@ __global__ void __launch_bounds__(256, SGE_WAVES_PER_SIMD) wcoj_kernel(KParams P
                 explore_after(P, w, wb, 1, acc);
             }
             if (!ENUM && pl.sib == 1) {
-                count_last(P, w, terms, wb, gb, acc);
+                uint32_t spent = 0;
+                halt = count_last<CAP>(P, w, terms, wb, gb, acc, spent, since);
                 continue;
             }
             if (ENUM && pl.k == 3) {
@@ -1154,9 +1182,11 @@ __global__ void __launch_bounds__(256, SGE_WAVES_PER_SIMD) wcoj_kernel(KParams P
                 w.pos[2] = c2.b + (uint32_t)(len * ci / nch);
                 w.iend[2] = c2.b + (uint32_t)(len * (ci + 1) / nch);
             }
-            run_dfs<ENUM>(P, w, terms, wb, gb, acc, plan, 2, stop);
+            halt = run_dfs<ENUM, CAP>(P, w, terms, wb, gb, acc, plan, 2, stop);
         }
+        if (halt) break;
     }
+    if (CAP && since && lane == 0) atomicAdd(P.node_ctr, (unsigned long long)since);
     if (cur_plan != NONE) acc.add(A_CYCLES, __builtin_amdgcn_s_memtime() - t_item);
     if (cur_plan != NONE) flush(P, cur_plan, acc);
 }
@@ -1327,6 +1357,24 @@ int validate_plan(const sge_plan &p) {
             sge::set_error("plan: bad ready level");
             return SGE_EINVAL;
         }
+        // exploration forms a level's parent intersection after level ready[i]:
+        // every parent must be matched by then
+        if (p.mode == SGE_MODE_EXPLORE && i >= 2 && p.ready[i] < 31u - __builtin_clz(p.parent[i])) {
+            sge::set_error("plan: ready level precedes a parent (exploration would read an unmatched image)");
+            return SGE_EINVAL;
+        }
+        // a reused set must contain this level's candidates: its parents and
+        // bounds are this level's, and each level it must differ from is
+        // excluded here too (by a check, a bound or adjacency)
+        if (p.reuse[i] >= 0) {
+            const uint32_t r = (uint32_t)p.reuse[i];
+            const uint32_t excl = p.check[i] | p.lower[i] | p.upper[i] | p.parent[i];
+            if ((p.parent[r] & ~p.parent[i]) || (p.lower[r] & ~p.lower[i]) || (p.upper[r] & ~p.upper[i]) ||
+                (p.check[r] & ~excl)) {
+                sge::set_error("plan: reuse level's set is not a superset of this level's candidates");
+                return SGE_EINVAL;
+            }
+        }
     }
     if (p.mode == SGE_MODE_IEP && p.tail >= 2) {
         if (p.tail >= p.k || p.k - p.tail < 2) {
@@ -1398,9 +1446,9 @@ int launch_cfg(const sge_tuning *t, bool enumerate, LaunchCfg &cfg) {
     if (per_cu <= 0) {
         int occ = 0;
         if (enumerate)
-            HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, wcoj_kernel<true>, cfg.block, cfg.lds));
+            HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, wcoj_kernel<true, false>, cfg.block, cfg.lds));
         else
-            HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, wcoj_kernel<false>, cfg.block, cfg.lds));
+            HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, wcoj_kernel<false, false>, cfg.block, cfg.lds));
         per_cu = std::max(1, occ);
     }
     cfg.grid = c.num_cu * per_cu;
@@ -1502,9 +1550,20 @@ __global__ void reset_kernel(PlanOut *out, uint32_t n, unsigned long long *queue
     }
 }
 
+// The kernel variant for a launch: enumeration or counting, with or without
+// the work cap's accounting.
+void launch_wcoj(bool enumerate, const LaunchCfg &cfg, hipStream_t st, const KParams &kp) {
+    if (enumerate)
+        wcoj_kernel<true, false><<<cfg.grid, cfg.block, cfg.lds, st>>>(kp);
+    else if (kp.node_cap)
+        wcoj_kernel<false, true><<<cfg.grid, cfg.block, cfg.lds, st>>>(kp);
+    else
+        wcoj_kernel<false, false><<<cfg.grid, cfg.block, cfg.lds, st>>>(kp);
+}
+
 // One counting launch over `np` device plans (already uploaded at `dplans`).
-int run_launch(sge_graph *g, const DPlan *dplans, uint32_t np, PlanOut *dout, uint64_t tb, uint64_t te,
-               uint64_t stride, uint64_t max_sub, const LaunchCfg &cfg, bool enumerate, uint32_t *emb, uint64_t emb_cap, unsigned long long *emb_n,
+int run_launch(sge_graph *g, const DPlan *dplans, uint32_t np, PlanOut *dout, uint64_t stride, uint64_t max_sub,
+               const LaunchCfg &cfg, bool enumerate, uint32_t *emb, uint64_t emb_cap, unsigned long long *emb_n,
                float *ms) {
     DeviceCtx &c = dctx();
     KParams kp;
@@ -1516,8 +1575,6 @@ int run_launch(sge_graph *g, const DPlan *dplans, uint32_t np, PlanOut *dout, ui
     kp.nnz = g->h.nnz;
     kp.plans = dplans;
     kp.nplans = np;
-    kp.task_begin = tb;
-    kp.task_end = te;
     kp.stride = stride ? stride : 1;
     kp.chunk = cfg.chunk;
     kp.head = cfg.head;
@@ -1559,10 +1616,7 @@ int run_launch(sge_graph *g, const DPlan *dplans, uint32_t np, PlanOut *dout, ui
         kp.out_items = c.items[cur];
         kp.out_count = c.icount + cur;
         if (pass == 0 && kp.nitems == 0) break;
-        if (enumerate)
-            wcoj_kernel<true><<<cfg.grid, cfg.block, cfg.lds, c.stream>>>(kp);
-        else
-            wcoj_kernel<false><<<cfg.grid, cfg.block, cfg.lds, c.stream>>>(kp);
+        launch_wcoj(enumerate, cfg, c.stream, kp);
         HIP_TRY(hipGetLastError());
     }
     HIP_TRY(hipEventRecord(c.ev1, c.stream));
@@ -1655,14 +1709,30 @@ int get_split(sge_graph *g, const sge_plan &p, uint64_t tb, uint64_t te, uint64_
     return SGE_OK;
 }
 
-int upload_plans(sge_graph *g, const sge_plan *plans, uint32_t n, bool enumerate, uint64_t tb, uint64_t te,
+// Root tasks of each plan: CSR positions tb[i] + j * stride < te[i] (clamped to
+// the graph's nnz), as (task_begin, ntask).
+struct TaskRanges {
+    std::vector<uint64_t> begin, ntask;
+    TaskRanges(const sge_graph *g, uint32_t n, const uint64_t *tb, const uint64_t *te, uint64_t stride)
+        : begin(n), ntask(n) {
+        for (uint32_t i = 0; i < n; ++i) {
+            uint64_t e = std::min<uint64_t>(te[i], g->h.nnz), b = std::min<uint64_t>(tb[i], e);
+            begin[i] = b;
+            ntask[i] = (e - b + stride - 1) / stride;
+        }
+    }
+};
+
+// Validate and upload plans with their task ranges (and split prefixes); nsub[i]
+// = the root subtasks of plan i.
+int upload_plans(sge_graph *g, const sge_plan *plans, uint32_t n, bool enumerate, const TaskRanges &tr,
                  uint64_t stride, uint32_t span, std::vector<uint64_t> &nsub,
                  const std::vector<sge::BagTrie> *tries = nullptr) {
     DeviceCtx &c = dctx();
     std::vector<DPlan> dp(n);
     nsub.assign(n, 0);
-    uint64_t ntask = (te - tb + stride - 1) / stride;
     for (uint32_t i = 0; i < n; ++i) {
+        const uint64_t tb = tr.begin[i], ntask = tr.ntask[i], te = tb + ntask * stride;
         int rc = validate_plan(plans[i]);
         if (rc) return rc;
         sge_plan p = plans[i];
@@ -1688,6 +1758,8 @@ int upload_plans(sge_graph *g, const sge_plan *plans, uint32_t n, bool enumerate
         }
         dp[i].split = sp;
         dp[i].nsub = ns;
+        dp[i].task_begin = tb;
+        dp[i].ntask = ntask;
         nsub[i] = sp ? ns : ntask;
     }
     const BufLayout L = buf_layout(dp.data(), n);
@@ -1736,15 +1808,16 @@ int build_bag_trie(sge_graph *g, const sge_plan &plan, sge::BagTrie &t) {
     DeviceCtx &c = dctx();
     sge_plan sub;
     sge::bag_subplan(plan, sub);
-    const uint64_t nnz = g->h.nnz;
+    const uint64_t zero = 0, nnz = g->h.nnz;
+    const TaskRanges all(g, 1, &zero, &nnz, 1);
     std::vector<uint64_t> nsub;
-    int rc = upload_plans(g, &sub, 1, false, 0, nnz, 1, span_of(nullptr), nsub);
+    int rc = upload_plans(g, &sub, 1, false, all, 1, span_of(nullptr), nsub);
     if (rc) return rc;
     LaunchCfg cfg;
     rc = launch_cfg(nullptr, false, cfg);
     if (rc) return rc;
     float ms = 0;
-    rc = run_launch(g, c.plans, 1, c.out, 0, nnz, 1, nsub[0], cfg, false, nullptr, 0, nullptr, &ms);
+    rc = run_launch(g, c.plans, 1, c.out, 1, nsub[0], cfg, false, nullptr, 0, nullptr, &ms);
     if (rc) return rc;
     PlanOut o;
     HIP_TRY(hipMemcpy(&o, c.out, sizeof(o), hipMemcpyDeviceToHost));
@@ -1762,12 +1835,12 @@ int build_bag_trie(sge_graph *g, const sge_plan &plan, sge::BagTrie &t) {
     HIP_TRY(hipMalloc(&d_n, sizeof(unsigned long long)));
     fr.b = d_n;
     if (N) {
-        rc = upload_plans(g, &sub, 1, true, 0, nnz, 1, span_of(nullptr), nsub);
+        rc = upload_plans(g, &sub, 1, true, all, 1, span_of(nullptr), nsub);
         if (rc) return rc;
         LaunchCfg ecfg;
         rc = launch_cfg(nullptr, true, ecfg);
         if (rc) return rc;
-        rc = run_launch(g, c.plans, 1, c.out, 0, nnz, 1, nsub[0], ecfg, true, rows, N, d_n, &ms);
+        rc = run_launch(g, c.plans, 1, c.out, 1, nsub[0], ecfg, true, rows, N, d_n, &ms);
         if (rc) return rc;
         unsigned long long got = 0;
         HIP_TRY(hipMemcpy(&got, d_n, sizeof(got), hipMemcpyDeviceToHost));
@@ -1859,11 +1932,13 @@ extern "C" void sge_graph_device_free(sge_graph *g) {
     free_split_cache(g);
 }
 
+// Count plans[i] over its root tasks tb[i] + j * stride < te[i] (one launch per
+// plan, or all plans in one launch with SGE_COUNT_BATCHED).
 static int count_impl(sge_graph *g, const sge_plan *plans, uint32_t nplans, uint32_t flags, const sge_tuning *tune,
-                      uint64_t tb, uint64_t te, uint64_t stride, uint64_t *counts, sge_stats *stats,
+                      const uint64_t *tb, const uint64_t *te, uint64_t stride, uint64_t *counts, sge_stats *stats,
                       sge_stats *per_plan) {
     auto t0 = std::chrono::steady_clock::now();
-    if (!g || !plans || !counts || nplans == 0) {
+    if (!g || !plans || !counts || nplans == 0 || !tb || !te) {
         sge::set_error("sge_count: null argument or no plans");
         return SGE_EINVAL;
     }
@@ -1871,9 +1946,8 @@ static int count_impl(sge_graph *g, const sge_plan *plans, uint32_t nplans, uint
     if (rc) return rc;
     std::lock_guard<std::mutex> lk(dmutex());
     DeviceCtx &c = dctx();
-    if (te > g->h.nnz) te = g->h.nnz;
-    if (tb > te) tb = te;
     if (stride == 0) stride = 1;
+    const TaskRanges tr(g, nplans, tb, te, stride);
     if (stats) std::memset(stats, 0, sizeof(*stats));
     if (per_plan) std::memset(per_plan, 0, sizeof(sge_stats) * nplans);
     TrieSet tries;
@@ -1885,7 +1959,7 @@ static int count_impl(sge_graph *g, const sge_plan *plans, uint32_t nplans, uint
     if (per_plan)
         for (uint32_t i = 0; i < nplans; ++i) per_plan[i].trie_ms = trie_each[i];
     std::vector<uint64_t> nsub;
-    rc = upload_plans(g, plans, nplans, false, tb, te, stride, span_of(tune), nsub, &tries.t);
+    rc = upload_plans(g, plans, nplans, false, tr, stride, span_of(tune), nsub, &tries.t);
     if (rc) return rc;
     LaunchCfg cfg;
     rc = launch_cfg(tune, false, cfg);
@@ -1898,7 +1972,7 @@ static int count_impl(sge_graph *g, const sge_plan *plans, uint32_t nplans, uint
     for (uint32_t i = 0; i < nplans; ++i) any_k1 = any_k1 || plans[i].k < 2;
     if ((flags & SGE_COUNT_BATCHED) && !any_k1) {
         float t = 0;
-        rc = run_launch(g, c.plans, nplans, c.out, tb, te, stride, max_sub, cfg, false, nullptr, 0, nullptr, &t);
+        rc = run_launch(g, c.plans, nplans, c.out, stride, max_sub, cfg, false, nullptr, 0, nullptr, &t);
         if (rc == SGE_ELIMIT) limited = true;
         else if (rc) return rc;
         HIP_TRY(hipMemcpy(host.data(), c.out, sizeof(PlanOut) * nplans, hipMemcpyDeviceToHost));
@@ -1913,8 +1987,7 @@ static int count_impl(sge_graph *g, const sge_plan *plans, uint32_t nplans, uint
                 HIP_TRY(hipMemsetAsync(c.out + i, 0, sizeof(PlanOut), c.stream));
                 continue;
             }
-            rc = run_launch(g, c.plans + i, 1, c.out + i, tb, te, stride, nsub[i], cfg, false, nullptr, 0, nullptr,
-                            &ms[i]);
+            rc = run_launch(g, c.plans + i, 1, c.out + i, stride, nsub[i], cfg, false, nullptr, 0, nullptr, &ms[i]);
             if (rc == SGE_ELIMIT) {  // keep going: the other plans get their own cap
                 limited = true;
                 continue;
@@ -1948,7 +2021,21 @@ extern "C" int sge_count(sge_graph *g, const sge_plan *plans, uint32_t nplans, u
         sge::set_error("sge_count: null graph");
         return SGE_EINVAL;
     }
-    return count_impl(g, plans, nplans, flags, tune, 0, g->h.nnz, 1, counts, stats, per_plan);
+    std::vector<uint64_t> tb(nplans, 0), te(nplans, g->h.nnz);
+    return count_impl(g, plans, nplans, flags, tune, tb.data(), te.data(), 1, counts, stats, per_plan);
+    SGE_API_END
+}
+
+extern "C" int sge_count_ranges(sge_graph *g, const sge_plan *plans, uint32_t nplans, const uint64_t *task_begin,
+                                const uint64_t *task_end, uint32_t flags, const sge_tuning *tune, uint64_t *counts,
+                                sge_stats *stats, sge_stats *per_plan) {
+    SGE_API_BEGIN
+    for (uint32_t i = 0; plans && i < nplans; ++i)
+        if (plans[i].k == 1) {
+            sge::set_error("sge_count_ranges: single-vertex patterns have no edge tasks");
+            return SGE_EINVAL;
+        }
+    return count_impl(g, plans, nplans, flags, tune, task_begin, task_end, 1, counts, stats, per_plan);
     SGE_API_END
 }
 
@@ -1959,7 +2046,7 @@ extern "C" int sge_count_range(sge_graph *g, const sge_plan *plan, uint64_t task
         sge::set_error("sge_count_range: single-vertex patterns have no edge tasks");
         return SGE_EINVAL;
     }
-    return count_impl(g, plan, 1, 0, tune, task_begin, task_end, 1, count, stats, nullptr);
+    return count_impl(g, plan, 1, 0, tune, &task_begin, &task_end, 1, count, stats, nullptr);
     SGE_API_END
 }
 
@@ -1975,7 +2062,8 @@ extern "C" int sge_count_sampled(sge_graph *g, const sge_plan *plans, uint32_t n
             sge::set_error("sge_count_sampled: single-vertex patterns have no edge tasks");
             return SGE_EINVAL;
         }
-    return count_impl(g, plans, nplans, 0, tune, offset, g->h.nnz, stride, counts, nullptr, per_plan);
+    std::vector<uint64_t> tb(nplans, offset), te(nplans, g->h.nnz);
+    return count_impl(g, plans, nplans, 0, tune, tb.data(), te.data(), stride, counts, nullptr, per_plan);
     SGE_API_END
 }
 
@@ -2010,7 +2098,8 @@ extern "C" int sge_enumerate(sge_graph *g, const sge_plan *plan, uint32_t *out, 
     std::lock_guard<std::mutex> lk(dmutex());
     DeviceCtx &c = dctx();
     std::vector<uint64_t> nsub;
-    rc = upload_plans(g, plan, 1, true, 0, g->h.nnz, 1, span_of(nullptr), nsub);
+    const uint64_t zero = 0, nnz = g->h.nnz;
+    rc = upload_plans(g, plan, 1, true, TaskRanges(g, 1, &zero, &nnz, 1), 1, span_of(nullptr), nsub);
     if (rc) return rc;
     LaunchCfg cfg;
     rc = launch_cfg(nullptr, true, cfg);
@@ -2023,7 +2112,7 @@ extern "C" int sge_enumerate(sge_graph *g, const sge_plan *plan, uint32_t *out, 
     HIP_TRY(hipMalloc(&d_n, sizeof(unsigned long long)));
     fr.b = d_n;
     float t = 0;
-    rc = run_launch(g, c.plans, 1, c.out, 0, g->h.nnz, 1, nsub[0], cfg, true, d_emb, cap, d_n, &t);
+    rc = run_launch(g, c.plans, 1, c.out, 1, nsub[0], cfg, true, d_emb, cap, d_n, &t);
     if (rc) return rc;
     unsigned long long n = 0;
     HIP_TRY(hipMemcpy(&n, d_n, sizeof(n), hipMemcpyDeviceToHost));
@@ -2104,7 +2193,7 @@ extern "C" void sge_trie_destroy(sge_trie *t) { delete t; }
 
 // ---------------------------------------------------------------- batches
 struct sge_batch {
-    sge_graph *g = nullptr;
+    sge_graph *g = nullptr;  // borrowed: the graph is kept alive until the batch is destroyed
     std::vector<sge_plan> plans;
     TrieSet tries;        // bag tries the plans read (built at creation)
     double trie_ms = 0;
@@ -2114,7 +2203,7 @@ struct sge_batch {
     uint32_t *scratch = nullptr;
     uint32_t *gbm = nullptr;  // global bitmap regions, all zero between uses
     uint32_t *items[2] = {nullptr, nullptr};
-    unsigned long long *icount = nullptr;
+    unsigned long long *icount = nullptr;  // [0], [1] item counts; [2] work-cap counter
     KParams kp;           // root-pass parameters (continuation passes derive from it)
     LaunchCfg cfg;
     int last_out = 0;     // item buffer written by the last captured pass
@@ -2134,31 +2223,21 @@ static void enqueue_cont_pass(sge_batch *b, int in) {
     kp.out_items = b->items[in ^ 1];
     kp.out_count = b->icount + (in ^ 1);
     reset_pass_kernel<<<1, 64, 0, b->stream>>>(b->queue, b->icount + (in ^ 1));
-    wcoj_kernel<false><<<b->cfg.grid, b->cfg.block, b->cfg.lds, b->stream>>>(kp);
+    launch_wcoj(false, b->cfg, b->stream, kp);
 }
 
-extern "C" int sge_batch_create_sampled(sge_graph *g, const sge_plan *plans, uint32_t nplans, uint64_t stride,
-                                        uint64_t offset, const sge_tuning *tune, sge_batch **out);
-
-extern "C" int sge_batch_create(sge_graph *g, const sge_plan *plans, uint32_t nplans, const sge_tuning *tune,
-                                sge_batch **out) {
-    SGE_API_BEGIN
-    return sge_batch_create_sampled(g, plans, nplans, 1, 0, tune, out);
-    SGE_API_END
-}
-
-extern "C" int sge_batch_create_sampled(sge_graph *g, const sge_plan *plans, uint32_t nplans, uint64_t stride,
-                                        uint64_t offset, const sge_tuning *tune, sge_batch **out) {
-    SGE_API_BEGIN
+// A batch over plans[i]'s root tasks tb[i] + j * stride < te[i].
+static int batch_create(sge_graph *g, const sge_plan *plans, uint32_t nplans, const uint64_t *tb, const uint64_t *te,
+                        uint64_t stride, const sge_tuning *tune, sge_batch **out) {
     if (stride == 0) stride = 1;
-    if (!g || !plans || !nplans || !out) {
+    if (!g || !plans || !nplans || !out || !tb || !te) {
         sge::set_error("sge_batch_create: bad arguments");
         return SGE_EINVAL;
     }
     int rc = sge_graph_to_device(g);
     if (rc) return rc;
     std::lock_guard<std::mutex> lk(dmutex());
-    DeviceCtx &c = dctx();
+    const TaskRanges tr(g, nplans, tb, te, stride);
     std::vector<DPlan> dp(nplans);
     for (uint32_t i = 0; i < nplans; ++i) {
         rc = validate_plan(plans[i]);
@@ -2182,20 +2261,23 @@ extern "C" int sge_batch_create_sampled(sge_graph *g, const sge_plan *plans, uin
     for (uint32_t i = 0; i < nplans; ++i) {
         const unsigned long long *sp = nullptr;
         unsigned long long ns = 0;
-        if (offset < g->h.nnz) {
-            rc = get_split(g, plans[i], offset, g->h.nnz, stride, span_of(tune), &sp, &ns);
+        if (tr.ntask[i]) {
+            rc = get_split(g, plans[i], tr.begin[i], tr.begin[i] + tr.ntask[i] * stride, stride, span_of(tune), &sp,
+                           &ns);
             if (rc) return rc;
         }
         dp[i].split = sp;
         dp[i].nsub = ns;
-        uint64_t ntask = offset < g->h.nnz ? (g->h.nnz - offset + stride - 1) / stride : 0;
-        max_sub = std::max<uint64_t>(max_sub, sp ? ns : ntask);
+        dp[i].task_begin = tr.begin[i];
+        dp[i].ntask = tr.ntask[i];
+        max_sub = std::max<uint64_t>(max_sub, sp ? ns : tr.ntask[i]);
     }
     sge_batch *b = new sge_batch();
     b->g = g;
     b->plans.assign(plans, plans + nplans);
     b->tries.t.swap(tries.t);
     b->trie_ms = trie_ms;
+    ++g->batches;
     KParams kp;
     std::memset(&kp, 0, sizeof(kp));
     uint32_t bstride = ((g->h.max_degree + 64 + 63) / 64) * 64;
@@ -2221,7 +2303,7 @@ extern "C" int sge_batch_create_sampled(sge_graph *g, const sge_plan *plans, uin
     for (int q = 0; q < 2; ++q)
         if ((e = hipMalloc(&b->items[q], sizeof(uint32_t) * ITEM_WORDS * ITEM_CAP)) != hipSuccess)
             return fail(e, "items");
-    if ((e = hipMalloc(&b->icount, sizeof(unsigned long long) * 2)) != hipSuccess) return fail(e, "item counts");
+    if ((e = hipMalloc(&b->icount, sizeof(unsigned long long) * 4)) != hipSuccess) return fail(e, "item counts");
     if ((e = hipMemcpy(b->dplans, dp.data(), sizeof(DPlan) * nplans, hipMemcpyHostToDevice)) != hipSuccess)
         return fail(e, "plan upload");
     if ((e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking)) != hipSuccess) return fail(e, "stream");
@@ -2234,8 +2316,6 @@ extern "C" int sge_batch_create_sampled(sge_graph *g, const sge_plan *plans, uin
     kp.nnz = g->h.nnz;
     kp.plans = b->dplans;
     kp.nplans = nplans;
-    kp.task_begin = std::min<uint64_t>(offset, g->h.nnz);
-    kp.task_end = g->h.nnz;
     kp.stride = stride;
     kp.chunk = cfg.chunk;
     kp.head = cfg.head;
@@ -2249,6 +2329,8 @@ extern "C" int sge_batch_create_sampled(sge_graph *g, const sge_plan *plans, uin
     kp.probe_ratio = cfg.probe_ratio;
     kp.bitmap_ratio = cfg.bitmap_ratio;
     kp.gbm_min = cfg.gbm_min;
+    kp.node_cap = cfg.node_cap;
+    kp.node_ctr = b->icount + 2;
     kp.out_items = b->items[0];
     kp.out_count = b->icount;
     b->kp = kp;
@@ -2256,10 +2338,10 @@ extern "C" int sge_batch_create_sampled(sge_graph *g, const sge_plan *plans, uin
     // capture: resets (fill kernels, no memset nodes), the root pass and a fixed
     // number of continuation passes that read their item counts on the device
     if ((e = hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal)) != hipSuccess) return fail(e, "capture");
-    reset_kernel<<<(nplans + 255) / 256, 256, 0, b->stream>>>(b->dout, nplans, b->queue, nullptr, nullptr);
+    reset_kernel<<<(nplans + 255) / 256, 256, 0, b->stream>>>(b->dout, nplans, b->queue, nullptr, kp.node_ctr);
     reset_pass_kernel<<<1, 64, 0, b->stream>>>(b->queue, b->icount);
     if (kp.nitems) {
-        wcoj_kernel<false><<<cfg.grid, cfg.block, cfg.lds, b->stream>>>(kp);
+        launch_wcoj(false, cfg, b->stream, kp);
         int in = 0;
         for (int pass = 0; pass < CAPTURED_PASSES; ++pass) {
             enqueue_cont_pass(b, in);
@@ -2271,9 +2353,32 @@ extern "C" int sge_batch_create_sampled(sge_graph *g, const sge_plan *plans, uin
     }
     if ((e = hipStreamEndCapture(b->stream, &b->graph)) != hipSuccess) return fail(e, "end capture");
     if ((e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0)) != hipSuccess) return fail(e, "instantiate");
-    (void)c;
     *out = b;
     return SGE_OK;
+}
+
+extern "C" int sge_batch_create(sge_graph *g, const sge_plan *plans, uint32_t nplans, const sge_tuning *tune,
+                                sge_batch **out) {
+    SGE_API_BEGIN
+    if (!g) return batch_create(g, plans, nplans, nullptr, nullptr, 1, tune, out);
+    std::vector<uint64_t> tb(nplans, 0), te(nplans, g->h.nnz);
+    return batch_create(g, plans, nplans, tb.data(), te.data(), 1, tune, out);
+    SGE_API_END
+}
+
+extern "C" int sge_batch_create_sampled(sge_graph *g, const sge_plan *plans, uint32_t nplans, uint64_t stride,
+                                        uint64_t offset, const sge_tuning *tune, sge_batch **out) {
+    SGE_API_BEGIN
+    if (!g) return batch_create(g, plans, nplans, nullptr, nullptr, stride, tune, out);
+    std::vector<uint64_t> tb(nplans, offset), te(nplans, g->h.nnz);
+    return batch_create(g, plans, nplans, tb.data(), te.data(), stride, tune, out);
+    SGE_API_END
+}
+
+extern "C" int sge_batch_create_ranges(sge_graph *g, const sge_plan *plans, uint32_t nplans, const uint64_t *task_begin,
+                                       const uint64_t *task_end, const sge_tuning *tune, sge_batch **out) {
+    SGE_API_BEGIN
+    return batch_create(g, plans, nplans, task_begin, task_end, 1, tune, out);
     SGE_API_END
 }
 
@@ -2287,13 +2392,19 @@ extern "C" int sge_batch_run(sge_batch *b, uint64_t *counts, sge_stats *stats) {
     uint32_t n = (uint32_t)b->plans.size();
     HIP_TRY(hipEventRecord(b->ev0, b->stream));
     HIP_TRY(hipGraphLaunch(b->exec, b->stream));
-    // items left after the captured passes: drain them eagerly
+    // items left after the captured passes: drain them eagerly, unless the work
+    // cap has stopped the call
     int in = b->last_out;
+    bool limited = false;
     for (;;) {
-        unsigned long long left = 0;
-        HIP_TRY(hipMemcpyAsync(&left, b->icount + in, sizeof(left), hipMemcpyDeviceToHost, b->stream));
+        unsigned long long hc[3] = {0, 0, 0};  // item counts, work spent
+        HIP_TRY(hipMemcpyAsync(hc, b->icount, sizeof(hc), hipMemcpyDeviceToHost, b->stream));
         HIP_TRY(hipStreamSynchronize(b->stream));
-        if (left == 0) break;
+        if (b->kp.node_cap && hc[2] >= b->kp.node_cap) {
+            limited = true;
+            break;
+        }
+        if (hc[in] == 0) break;
         enqueue_cont_pass(b, in);
         HIP_TRY(hipGetLastError());
         in ^= 1;
@@ -2312,6 +2423,10 @@ extern "C" int sge_batch_run(sge_batch *b, uint64_t *counts, sge_stats *stats) {
     }
     for (uint32_t i = 0; i < n; ++i) counts[i] = host[i].count * (b->plans[i].multiplier ? b->plans[i].multiplier : 1);
     if (stats) stats->wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (limited) {
+        sge::set_error("work cap reached: counts are partial (stats cover the work done)");
+        return SGE_ELIMIT;
+    }
     return SGE_OK;
     SGE_API_END
 }
@@ -2331,7 +2446,10 @@ extern "C" void sge_batch_destroy(sge_batch *b) {
     for (int q = 0; q < 2; ++q)
         if (b->items[q]) (void)hipFree(b->items[q]);
     if (b->icount) (void)hipFree(b->icount);
+    sge_graph *g = b->g;
     delete b;
+    // the last batch of a graph its owner already destroyed frees it
+    if (g && --g->batches == 0 && g->destroyed) sge_graph_release(g);
 }
 
 extern "C" int sge_device_info(int *num_cu, int *num_xcd_hint, char *name, size_t cap) {

@@ -127,6 +127,8 @@ _SIGS = {
                             C.POINTER(Stats), C.POINTER(Stats)]),
     "sge_count_range": (C.c_int, [C.c_void_p, C.POINTER(Plan), C.c_uint64, C.c_uint64, C.POINTER(Tuning),
                                   C.POINTER(C.c_uint64), C.POINTER(Stats)]),
+    "sge_count_ranges": (C.c_int, [C.c_void_p, C.POINTER(Plan), C.c_uint32, C.c_void_p, C.c_void_p, C.c_uint32,
+                                   C.POINTER(Tuning), C.c_void_p, C.POINTER(Stats), C.POINTER(Stats)]),
     "sge_num_tasks": (C.c_int, [C.c_void_p, C.POINTER(Plan), C.POINTER(C.c_uint64)]),
     "sge_count_sampled": (C.c_int, [C.c_void_p, C.POINTER(Plan), C.c_uint32, C.c_uint64, C.c_uint64,
                                     C.POINTER(Tuning), C.c_void_p, C.POINTER(Stats)]),
@@ -148,6 +150,8 @@ _SIGS = {
                                    C.POINTER(C.c_void_p)]),
     "sge_batch_create_sampled": (C.c_int, [C.c_void_p, C.POINTER(Plan), C.c_uint32, C.c_uint64, C.c_uint64,
                                            C.POINTER(Tuning), C.POINTER(C.c_void_p)]),
+    "sge_batch_create_ranges": (C.c_int, [C.c_void_p, C.POINTER(Plan), C.c_uint32, C.c_void_p, C.c_void_p,
+                                          C.POINTER(Tuning), C.POINTER(C.c_void_p)]),
     "sge_batch_run": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Stats)]),
     "sge_batch_destroy": (None, [C.c_void_p]),
     "sge_device_info": (C.c_int, [C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_char_p, C.c_size_t]),
@@ -453,6 +457,26 @@ def count_range(graph: Graph, plan: Plan, begin: int, end: int, tuning: Optional
     return int(c.value), st
 
 
+def count_ranges(graph: Graph, plans, begins, ends, batched: bool = False, tuning: Optional[Tuning] = None):
+    """Each plan over its own root-task range [begins[i], ends[i]) (CSR positions);
+    plain (one launch per plan) or batched.  Returns counts, Stats, per-plan Stats."""
+    if isinstance(plans, Plan):
+        plans = [plans]
+    n = len(plans)
+    arr = (Plan * n)(*plans)
+    b = np.ascontiguousarray(begins, dtype=np.uint64)
+    e = np.ascontiguousarray(ends, dtype=np.uint64)
+    if len(b) != n or len(e) != n:
+        raise ValueError("count_ranges: one range per plan")
+    counts = np.zeros(n, dtype=np.uint64)
+    st = Stats()
+    pp = (Stats * n)()
+    _check(lib().sge_count_ranges(graph._h, arr, n, _ptr(b), _ptr(e), COUNT_BATCHED if batched else 0,
+                                  C.byref(tuning) if tuning else None, _ptr(counts), C.byref(st), pp),
+           "sge_count_ranges")
+    return [int(x) for x in counts], st, [pp[i] for i in range(n)]
+
+
 def count_sampled(graph: Graph, plans, stride: int, offset: int = 0, tuning: Optional[Tuning] = None):
     """Plain executor over the strided root sample; returns counts and per-plan Stats."""
     if isinstance(plans, Plan):
@@ -583,17 +607,28 @@ def canonicalize(pattern: Pattern, rows: np.ndarray) -> np.ndarray:
 
 
 class Batch:
-    """Batched executor: all plans in one persistent launch, captured in a hipGraph."""
+    """Batched executor: all plans in one persistent launch, captured in a hipGraph.
+
+    Root tasks: the strided sample {offset, offset + stride, ...}, or with
+    `ranges=(begins, ends)` plan i over its own range [begins[i], ends[i])."""
 
     def __init__(self, graph: Graph, plans: Sequence[Plan], tuning: Optional[Tuning] = None, stride: int = 1,
-                 offset: int = 0):
+                 offset: int = 0, ranges=None):
         self.graph = graph
         self.n = len(plans)
         self._plans = (Plan * self.n)(*plans)
         self._h = C.c_void_p()
-        _check(lib().sge_batch_create_sampled(graph._h, self._plans, self.n, stride, offset,
-                                              C.byref(tuning) if tuning else None, C.byref(self._h)),
-               "sge_batch_create_sampled")
+        tun = C.byref(tuning) if tuning else None
+        if ranges is not None:
+            b = np.ascontiguousarray(ranges[0], dtype=np.uint64)
+            e = np.ascontiguousarray(ranges[1], dtype=np.uint64)
+            if len(b) != self.n or len(e) != self.n:
+                raise ValueError("Batch: one range per plan")
+            _check(lib().sge_batch_create_ranges(graph._h, self._plans, self.n, _ptr(b), _ptr(e), tun,
+                                                 C.byref(self._h)), "sge_batch_create_ranges")
+        else:
+            _check(lib().sge_batch_create_sampled(graph._h, self._plans, self.n, stride, offset, tun,
+                                                  C.byref(self._h)), "sge_batch_create_sampled")
 
     def run(self):
         counts = np.zeros(self.n, dtype=np.uint64)
