@@ -1,29 +1,41 @@
 """Benchmark: exact matches/s over a pattern set, batched vs plain GPU executor.
 
-Default workload: the 16 connected 6-vertex patterns with 11-13 edges over the
-Chung-Lu graph n=7,115, m=100,762 (gamma 2.1, expected degree capped at 1,100,
-seed 1), int32 degree-ordered CSR resident in HBM, IEP mode, full exact counts.
+Default workload (`--workload c4`): the 78 connected 7-vertex patterns with
+15-19 edges over R-MAT scale 20 (edge factor 16, a,b,c,d = .57,.19,.19,.05,
+seed 4; n = 1,048,576, m = 15,702,114, int32 degree-ordered CSR resident in
+HBM), IEP mode.  The whole job holds 10^17 or more matches per pattern, so
+each pattern runs over its own stated range of root edges (CSR positions),
+recorded with the oracle's count in tests/golden/large_c4.json; `--span S`
+multiplies every range's length by S (S = 1: the counts are checked against
+the oracle's).
 
-One step = one pass of the batched executor over the whole pattern set (one
-persistent launch plus its continuation passes, replayed from a captured
-hipGraph) with the learned orders and the autotuned launch parameters.
-`value` = total matches / step time.  The plain executor (one launch per
-pattern, cost-model orders, library-default launch parameters, which suit it
-better than the batch's tuned ones) is timed on the same patterns; both must
-return identical counts.  The CPU baseline is the reference's graph_mining_system
-(built from its own sources into oracle/_ref/) run with all host cores on the
-same graph, patterns in set order within a time budget; the oracle stands in
-where that binary is absent.
+One step = one pass over all 78 ranges.  Executors, each warmed up W times and
+then timed over K steps (setup outside every timer):
+  plain           one launch per pattern (cost-model orders, library defaults)
+  batched_cost    one batched launch (hipGraph replay), the same plans: counts
+                  must equal plain's
+  batched         one batched launch with the learned orders: `value` = its
+                  matches / step time.  On a root subset the learned orders
+                  count a different part of each pattern's job (the root edge
+                  maps to the plan's first two pattern vertices and the
+                  symmetry rules follow the order), so its counts are checked
+                  against the plain executor run with the same learned plans.
+The roofline of the batched run's wcoj_kernel comes from the kernel's own byte
+counters and the HIP-event time on the launch stream.  The CPU baseline is the
+oracle (a C restatement of the reference's join, all host threads) on the same
+ranges, patterns in set order within a time budget, printed beside the GPU
+plain executor's rate on the same patterns.
 
-The 78-pattern 7-vertex set over R-MAT scale 20 (`--workload rmat20`) is not
-the default: its patterns have ~1e17-1e19 matches each and single root edges
-carry ~1e13, beyond exact enumeration in a benchmark window; it runs only on a
-root-edge sample (`--stride`).
+The C3 block (`c3_whole_job`) is the largest configuration that runs whole and
+exact: the 16 six-vertex patterns over the Chung-Lu graph n=7,115 m=100,762
+(gamma 2.1, expected degree capped at 1,100, seed 1), batched learned-order
+against plain, counts equal, and the reference's graph_mining_system (built
+from its own sources into oracle/_ref/) on all host cores beside it.
 
-The same 78 patterns over R-MAT scale 8 (seed 7, the graph of the reference's
-golden counts in tests/golden/rmat7.json) run exactly with `--workload rmat8`.
+Other whole-job workloads: `--workload sixv|house|rmat8` (the C3 pass alone,
+house + K5-e over the Chung-Lu graph, the 78 patterns over R-MAT scale 8).
 
-    python bench.py [--steps K] [--warmup W] [--workload sixv|house|rmat8|rmat20] [--no-cpu]
+    python bench.py [--steps K] [--warmup W] [--workload c4|sixv|house|rmat8] [--span S] [--no-c3] [--no-cpu]
 """
 import argparse
 import json
@@ -41,29 +53,13 @@ import sge  # noqa: E402
 import sge.workloads as W  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+C4_SUBSETS = os.path.join(ROOT, "tests", "golden", "large_c4.json")
+LEARNED = os.path.join(ROOT, "subgraph-enumeration_amd", "sge", "models", "order_mlp.npz")
+METRIC = "exact matches/sec (batched learned-order executor, whole pattern set)"
 
 
 def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
-
-
-def workload(name):
-    c2 = "chung_lu(n=7115, m=100762, gamma=2.1, cap=1100, seed=1)"
-    if name in ("sixv", "c3"):
-        return ("chunglu7115_sixvertex16", lambda: W.graph_c2(), W.six_vertex_set,
-                {"graph": c2, "patterns": "16 x 6-vertex, 11-13 edges"})
-    if name in ("house", "c2house"):
-        return ("chunglu7115_house_k5e", lambda: W.graph_c2(), lambda: [W.named("house"), W.named("near5clique")],
-                {"graph": c2, "patterns": "house, K5-e"})
-    if name in ("rmat8", "rmat8_78"):
-        return ("rmat8_sevenvertex78", lambda: W.graph_rmat(8, 7, device_build=True), W.seven_vertex_set,
-                {"graph": "rmat(scale=8, ef=16, a=.57, b=.19, c=.19, seed=7)",
-                 "patterns": "78 x 7-vertex, 15-19 edges"})
-    if name in ("rmat20", "c4"):
-        return ("rmat20_sevenvertex78", lambda: W.graph_c4(device_build=True), W.seven_vertex_set,
-                {"graph": "rmat(scale=20, ef=16, a=.57, b=.19, c=.19, seed=4)",
-                 "patterns": "78 x 7-vertex, 15-19 edges"})
-    raise SystemExit(f"unknown workload {name}")
 
 
 def load_traffic(kernel_workload):
@@ -73,73 +69,291 @@ def load_traffic(kernel_workload):
         return None
     try:
         with open(path) as f:
-            data = json.load(f)
-        return data.get(kernel_workload)
-    except Exception:
+            return json.load(f).get(kernel_workload)
+    except (OSError, ValueError):
         return None
 
 
-def cpu_baseline(graph, pats, plans, stride, budget_s=30.0):
-    """The reference binary (oracle/_ref/graph_mining_system) with all host cores
-    on the same graph, patterns in set order until the budget is spent (each run
-    limited to the budget left); the oracle port on the same root sample when
-    the binary is absent."""
+def learned_plans(g, pats, plans_cost, mode):
+    try:
+        from sge import learned as LM
+    except ImportError:
+        return plans_cost, False
+    if not os.path.exists(LEARNED):
+        return plans_cost, False
+    model = LM.load(LEARNED)
+    return [LM.choose_plan(model, p, g, mode, fallback=pc)[0] for p, pc in zip(pats, plans_cost)], True
+
+
+def timed_steps(run, warmup, steps, label):
+    """run() -> (counts, wall-independent kernel ms); W warmups, then K timed steps."""
+    counts = None
+    for i in range(warmup):
+        counts, kms = run()
+        log(f"{label} warmup {i}: kernel {kms:.1f} ms")
+    walls, kernels = [], []
+    for i in range(steps):
+        t0 = time.perf_counter()
+        c, kms = run()
+        walls.append((time.perf_counter() - t0) * 1000)
+        kernels.append(kms)
+        if counts is not None and c != counts:
+            log(f"{label}: counts changed between steps")
+            sys.exit(3)
+        counts = c
+        log(f"{label} step {i}: {walls[-1]:.1f} ms (kernel {kms:.1f} ms)")
+    return counts, float(np.mean(walls)), float(np.mean(kernels))
+
+
+def plain_runner(g, plans, ranges=None, stride=1):
+    def run():
+        if ranges is not None:
+            c, st, pp = sge.count_ranges(g, plans, ranges[0], ranges[1], batched=False)
+        else:
+            c, pp = sge.count_sampled(g, plans, stride)
+        run.per_plan = pp
+        return c, sum(s.kernel_ms for s in pp)
+    return run
+
+
+def batch_runner(batch):
+    def run():
+        c, st = batch.run()
+        run.stats = st
+        return c, st.kernel_ms
+    return run
+
+
+def roofline(st, kernel_ms, wname):
+    achieved = st.bytes / (kernel_ms / 1000.0) / 1e9 if kernel_ms > 0 else 0.0
+    model = st.model_bytes / (kernel_ms / 1000.0) / 1e9 if kernel_ms > 0 else 0.0
+    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": load_traffic(wname), "kernel": "wcoj_kernel<false,false>",
+            "algorithmic_bytes_per_launch": st.bytes, "kernel_ms": kernel_ms,
+            "per_pair_model": {"bytes_per_launch": st.model_bytes, "achieved": model, "frac": model / HBM_PEAK_GBS,
+                               "note": "4(|A'|+|B'|) + 4|out| + 8 per CSR row for every pairwise intersection; "
+                                       "`achieved` charges the last level's probes as executed (8 B each)"}}
+
+
+def cpu_oracle(g, plans, ranges, budget_s=20.0):
+    """The oracle on all host threads over the same per-pattern ranges, patterns in
+    set order until the budget is spent (each pattern's range is bounded)."""
+    from oracle import oracle as O
+
+    off, nbr = g.csr()
+    off, nbr = np.ascontiguousarray(off, np.uint32), np.ascontiguousarray(nbr, np.uint32)
+    threads = os.cpu_count() or 1
+    tot, t_all, done = 0, 0.0, []
+    t_start = time.time()
+    for i, pl in enumerate(plans):
+        if time.time() - t_start > budget_s:
+            break
+        t0 = time.perf_counter()
+        tot += O.count(pl, off, nbr, threads, int(ranges[0][i]), int(ranges[1][i]))
+        t_all += time.perf_counter() - t0
+        done.append(i)
+    return {"value": tot / t_all if t_all > 0 else None, "unit": "matches/s", "cores": threads, "kind": "port",
+            "sample": f"oracle (C restatement of the reference's join, {threads} threads) over the root-edge "
+                      f"ranges of patterns 0..{done[-1] if done else -1} of {len(plans)} (set order, "
+                      f"{budget_s:.0f}s budget): {tot} matches in {t_all:.2f}s"}, done, tot
+
+
+def cpu_reference(graph, pats, budget_s=30.0):
+    """The reference binary (oracle/_ref/graph_mining_system) with all host cores,
+    patterns in set order until the budget is spent; None where it is absent."""
     import subprocess
     import tempfile
 
     from oracle import oracle as O
 
+    if not os.path.exists(O.REF_BIN):
+        return None, []
     cores = os.cpu_count() or 1
-    if os.path.exists(O.REF_BIN) and stride == 1:
-        tmp = tempfile.mkdtemp(prefix="sge_bench_")
-        gpath = os.path.join(tmp, "g.txt")
-        graph.write(gpath)
-        total, t_exec, done = 0, 0.0, []
-        t_start = time.time()
-        for i, p in enumerate(pats):
-            left = budget_s - (time.time() - t_start)
-            if left <= 1:
-                break
-            qpath = os.path.join(tmp, f"q{i}.txt")
-            with open(qpath, "w") as f:
-                e = p.edges()
-                f.write(f"{p.k} {len(e)}\n" + "".join(f"{a} {b}\n" for a, b in e))
-            try:
-                r = subprocess.run([O.REF_BIN, "-q", qpath, "-d", gpath, "-u", "-t", str(cores)],
-                                   capture_output=True, text=True, timeout=left)
-            except subprocess.TimeoutExpired:
-                break
-            cnt = exe = None
-            for line in r.stdout.splitlines():
-                if line.startswith("Number of matches:"):
-                    cnt = int(line.split(":")[1])
-                if line.startswith("Execution Time:"):
-                    exe = float(line.split(":")[1])
-            if cnt is None or exe is None:
-                break
-            total += cnt
-            t_exec += exe
-            done.append(i)
-        if done:
-            return {"value": total / t_exec, "unit": "matches/s", "cores": cores, "kind": "reference",
-                    "sample": f"reference graph_mining_system -u -t {cores} on patterns {done} of {len(pats)} "
-                              f"(set order, {budget_s:.0f}s budget), {total} matches in {t_exec:.2f}s of its "
-                              f"'Execution Time'"}
-    off, nbr = graph.csr()
-    threads = min(cores, 64)
-    total_count, total_t, done = 0, 0.0, 0
+    tmp = tempfile.mkdtemp(prefix="sge_bench_")
+    gpath = os.path.join(tmp, "g.txt")
+    graph.write(gpath)
+    total, t_exec, done = 0, 0.0, []
     t_start = time.time()
-    for pl in plans:
-        if time.time() - t_start > budget_s:
+    for i, p in enumerate(pats):
+        left = budget_s - (time.time() - t_start)
+        if left <= 1:
             break
-        t0 = time.perf_counter()
-        total_count += O.count_strided(pl, off, nbr, stride, 0, threads)
-        total_t += time.perf_counter() - t0
-        done += 1
-    return {"value": total_count / total_t if total_t > 0 else None, "unit": "matches/s", "cores": threads,
-            "kind": "port",
-            "sample": f"oracle on the first {done} of {len(plans)} patterns over the 1/{stride} root-edge "
-                      f"sample, {total_count} matches in {total_t:.2f}s"}
+        qpath = os.path.join(tmp, f"q{i}.txt")
+        with open(qpath, "w") as f:
+            e = p.edges()
+            f.write(f"{p.k} {len(e)}\n" + "".join(f"{a} {b}\n" for a, b in e))
+        try:
+            r = subprocess.run([O.REF_BIN, "-q", qpath, "-d", gpath, "-u", "-t", str(cores)],
+                               capture_output=True, text=True, timeout=left)
+        except subprocess.TimeoutExpired:
+            break
+        cnt = exe = None
+        for line in r.stdout.splitlines():
+            if line.startswith("Number of matches:"):
+                cnt = int(line.split(":")[1])
+            if line.startswith("Execution Time:"):
+                exe = float(line.split(":")[1])
+        if cnt is None or exe is None:
+            break
+        total += cnt
+        t_exec += exe
+        done.append(i)
+    if not done:
+        return None, []
+    return {"value": total / t_exec, "unit": "matches/s", "cores": cores, "kind": "reference",
+            "sample": f"reference graph_mining_system -u -t {cores} on patterns {done} of {len(pats)} "
+                      f"(set order, {budget_s:.0f}s budget), {total} matches in {t_exec:.2f}s of its "
+                      f"'Execution Time'"}, done
+
+
+def gpu_rate_on(per_plan, counts, done):
+    """The plain executor's rate (kernel time) on the patterns a CPU baseline covered."""
+    if not done:
+        return None
+    ms = sum(per_plan[i].kernel_ms for i in done)
+    return {"value": sum(counts[i] for i in done) / (ms / 1000.0) if ms > 0 else None, "unit": "matches/s",
+            "executor": "plain", "patterns": done, "kernel_ms": ms}
+
+
+def c4_subsets(args):
+    wname = "rmat20_sevenvertex78_subsets"
+    with open(C4_SUBSETS) as f:
+        doc = json.load(f)
+    t0 = time.time()
+    g = W.graph_c4(device_build=True)
+    log(f"graph n={g.n} m={g.m} maxdeg={g.stats.max_degree} built on the GPU in {time.time() - t0:.1f}s")
+    assert (g.n, g.m) == (doc["n"], doc["m"])
+    pats = W.seven_vertex_set()
+    subs = doc["subsets"]
+    t0 = time.time()
+    plans_cost = [sge.build_plan(pats[e["pattern"]], g, sge.MODE_IEP, e["plan"]["order"]) for e in subs]
+    for pl, e in zip(plans_cost, subs):
+        assert list(pl.order)[:pl.k] == e["plan"]["order"]
+    plan_s = time.time() - t0
+    t0 = time.time()
+    plans_fast, learned = learned_plans(g, [pats[e["pattern"]] for e in subs], plans_cost, sge.MODE_IEP)
+    learn_s = time.time() - t0
+    changed = sum(list(a.order)[:a.k] != list(b.order)[:b.k] for a, b in zip(plans_fast, plans_cost))
+    log(f"78 cost-model plans in {plan_s:.2f}s; learned orders in {learn_s:.2f}s ({changed} changed)")
+    span = max(1, args.span)
+    begins = np.array([e["begin"] for e in subs], dtype=np.uint64)
+    ends = np.array([e["begin"] + (e["end"] - e["begin"]) * span for e in subs], dtype=np.uint64)
+    ranges = (begins, ends)
+    root_edges = int((ends - begins).sum())
+
+    plain = plain_runner(g, plans_cost, ranges)
+    plain_counts, plain_ms, plain_kms = timed_steps(plain, args.warmup, args.steps, "plain")
+    if span == 1 and plain_counts != [e["count"] for e in subs]:
+        log("COUNT MISMATCH: plain executor against the oracle's counts")
+        sys.exit(3)
+    bc = sge.Batch(g, plans_cost, ranges=ranges)
+    bcost_counts, bcost_ms, bcost_kms = timed_steps(batch_runner(bc), args.warmup, args.steps, "batched_cost")
+    bc.close()
+    if bcost_counts != plain_counts:
+        log("COUNT MISMATCH between plain and batched executors (same plans)")
+        sys.exit(3)
+    b = sge.Batch(g, plans_fast, ranges=ranges)
+    brun = batch_runner(b)
+    counts, step_ms, kernel_ms = timed_steps(brun, args.warmup, args.steps, "batched")
+    st = brun.stats
+    b.close()
+    if changed:
+        check, _, _ = sge.count_ranges(g, plans_fast, begins, ends, batched=False)
+        if check != counts:
+            log("COUNT MISMATCH between plain and batched executors (learned plans)")
+            sys.exit(3)
+    total = sum(counts)
+    res = {
+        "metric": METRIC, "value": total / (step_ms / 1000.0), "unit": "matches/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": step_ms, "higher_is_better": True,
+        "scaling": "strong", "vs_baseline": None, "dtype": "u32 ids / u64 counts", "data": "synthetic",
+        "config": {"workload": f"{wname} (78 x 7-vertex, 15-19 edges, over R-MAT scale 20 seed 4; per-pattern "
+                               f"root-edge ranges of tests/golden/large_c4.json x{span}: {root_edges} root edges "
+                               f"of {2 * g.m})",
+                   "mode": "iep", "graph": doc["graph"], "n": g.n, "m": g.m, "patterns": len(subs),
+                   "span": span, "root_edges": root_edges, "learned_order": learned, "learned_changed": changed,
+                   "tuning": "library defaults"},
+        "pattern_set_ms": {"plain": plain_ms, "plain_kernel": plain_kms, "batched_cost_order": bcost_ms,
+                           "batched_cost_order_kernel": bcost_kms, "batched": step_ms, "batched_kernel": kernel_ms,
+                           "speedup_vs_plain": plain_ms / step_ms,
+                           "speedup_vs_plain_kernel": plain_kms / kernel_ms if kernel_ms > 0 else None,
+                           "speedup_batching_same_plans": plain_ms / bcost_ms,
+                           "note": "plain and batched_cost run the same plans and return the same counts "
+                                   "(= the oracle's at span 1); batched runs the learned orders, whose counts "
+                                   "over a root subset differ by plan and are checked against plain with the "
+                                   "same plans"},
+        "total_matches": total, "total_matches_cost_order": sum(plain_counts),
+        "counts_checked": "plain == batched_cost" + (" == oracle fixture" if span == 1 else "")
+                          + ("; batched == plain(learned plans)" if changed else ""),
+        "intersections_per_s": st.intersections / (kernel_ms / 1000.0) if kernel_ms > 0 else None,
+        "planning_s": plan_s, "learned_choice_s": learn_s,
+        "roofline": roofline(st, kernel_ms, wname),
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu:
+        cpu, done, _ = cpu_oracle(g, plans_cost, (begins, np.array([e["end"] for e in subs], np.uint64)))
+        res["cpu_baseline"] = cpu
+        if span == 1:
+            res["cpu_baseline"]["gpu_same_sample"] = gpu_rate_on(plain.per_plan, plain_counts, done)
+    return res
+
+
+def whole_job(args, name):
+    c2 = "chung_lu(n=7115, m=100762, gamma=2.1, cap=1100, seed=1)"
+    table = {
+        "sixv": ("chunglu7115_sixvertex16", lambda: W.graph_c2(), W.six_vertex_set, c2, "16 x 6-vertex, 11-13 edges"),
+        "house": ("chunglu7115_house_k5e", lambda: W.graph_c2(), lambda: [W.named("house"), W.named("near5clique")],
+                  c2, "house, K5-e"),
+        "rmat8": ("rmat8_sevenvertex78", lambda: W.graph_rmat(8, 7, device_build=True), W.seven_vertex_set,
+                  "rmat(scale=8, ef=16, a=.57, b=.19, c=.19, seed=7)", "78 x 7-vertex, 15-19 edges"),
+    }
+    if name not in table:
+        raise SystemExit(f"unknown workload {name}")
+    wname, gfn, pfn, gdesc, pdesc = table[name]
+    mode = {"wcoj": sge.MODE_WCOJ, "explore": sge.MODE_EXPLORE, "iep": sge.MODE_IEP}[args.mode]
+    g = gfn()
+    g.to_device()
+    pats = pfn()
+    t0 = time.time()
+    plans_cost = [sge.build_plan(p, g, mode) for p in pats]
+    plan_s = time.time() - t0
+    plans_fast, learned = learned_plans(g, pats, plans_cost, mode)
+    from sge import autotune
+
+    tuning = autotune.tuned(wname)
+    tuning_desc = {k: getattr(tuning, k) for k, _ in sge.Tuning._fields_} if tuning else "library defaults"
+    plain = plain_runner(g, plans_cost)
+    plain_counts, plain_ms, plain_kms = timed_steps(plain, args.warmup, args.steps, f"{name} plain")
+    b = sge.Batch(g, plans_fast, tuning=tuning)
+    brun = batch_runner(b)
+    counts, step_ms, kernel_ms = timed_steps(brun, args.warmup, args.steps, f"{name} batched")
+    st = brun.stats
+    b.close()
+    if counts != plain_counts:
+        log("COUNT MISMATCH between plain and batched executors")
+        sys.exit(3)
+    total = sum(counts)
+    res = {
+        "metric": METRIC, "value": total / (step_ms / 1000.0), "unit": "matches/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": step_ms, "higher_is_better": True,
+        "scaling": "strong", "vs_baseline": None, "dtype": "u32 ids / u64 counts", "data": "synthetic",
+        "config": {"workload": wname, "mode": args.mode, "graph": gdesc, "patterns_desc": pdesc, "n": g.n, "m": g.m,
+                   "patterns": len(pats), "learned_order": learned, "tuning": tuning_desc, "whole_job": True},
+        "pattern_set_ms": {"plain": plain_ms, "plain_kernel": plain_kms, "batched": step_ms,
+                           "batched_kernel": kernel_ms, "speedup_vs_plain": plain_ms / step_ms,
+                           "speedup_vs_plain_kernel": plain_kms / kernel_ms if kernel_ms > 0 else None},
+        "total_matches": total, "counts_checked": "plain == batched (whole job: counts do not depend on the order)",
+        "intersections_per_s": st.intersections / (kernel_ms / 1000.0) if kernel_ms > 0 else None,
+        "planning_s": plan_s,
+        "roofline": roofline(st, kernel_ms, wname),
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu and mode == sge.MODE_IEP:
+        cpu, done = cpu_reference(g, pats)
+        if cpu is not None:
+            cpu["gpu_same_sample"] = gpu_rate_on(plain.per_plan, plain_counts, done)
+        res["cpu_baseline"] = cpu
+    return res
 
 
 def main():
@@ -147,112 +361,28 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="sixv")
+    ap.add_argument("--workload", default="c4")
     ap.add_argument("--mode", default="iep", choices=["wcoj", "explore", "iep"])
-    ap.add_argument("--stride", type=int, default=None,
-                    help="root-edge sample: every stride-th directed edge (default 1 = the whole job)")
+    ap.add_argument("--span", type=int, default=1, help="c4: multiply every root-edge range's length by S")
+    ap.add_argument("--no-c3", action="store_true", help="c4: leave out the C3 whole-job block")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--calibrate", action="store_true",
                     help="also run the known-size copy kernel (scales rocprofv3 FETCH_SIZE / WRITE_SIZE)")
-    ap.add_argument("--learned", default=os.path.join(ROOT, "subgraph-enumeration_amd", "sge", "models",
-                                                      "order_mlp.npz"))
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is design-only",
                           "n_gpus": args.gpus}))
         sys.exit(2)
-    mode = {"wcoj": sge.MODE_WCOJ, "explore": sge.MODE_EXPLORE, "iep": sge.MODE_IEP}[args.mode]
-    wname, gfn, pfn, cfg = workload(args.workload)
-    stride = args.stride or 1
-    t0 = time.time()
-    g = gfn()
-    log(f"graph n={g.n} m={g.m} maxdeg={g.stats.max_degree} built in {time.time() - t0:.1f}s")
-    g.to_device()
     if args.calibrate:
         nbytes = 256 << 20
         ms = sge.calibration_copy(nbytes)
         log(f"calibration copy: {nbytes} bytes read + written in {ms:.3f} ms ({2 * nbytes / ms / 1e6:.0f} GB/s)")
-    pats = pfn()
-    t0 = time.time()
-    plans_cost = [sge.build_plan(p, g, mode) for p in pats]
-    plan_s = time.time() - t0
-    plans_fast = plans_cost
-    learned = False
-    try:
-        from sge import learned as LM
-
-        if os.path.exists(args.learned):
-            model = LM.load(args.learned)
-            plans_fast = [LM.choose_plan(model, p, g, mode, fallback=pc)[0] for p, pc in zip(pats, plans_cost)]
-            learned = True
-    except ImportError:
-        pass
-    log(f"{len(pats)} plans in {plan_s:.2f}s (learned order: {learned})")
-    from sge import autotune
-
-    tuning = autotune.tuned(wname)
-    tuning_desc = {k: getattr(tuning, k) for k, _ in sge.Tuning._fields_} if tuning else "library defaults"
-    log(f"tuning: {tuning_desc}")
-
-    # plain executor: cost-model order, one launch per pattern
-    plain_ms = None
-    plain_counts = None
-    if not args.no_plain:
-        t0 = time.perf_counter()
-        # library defaults: better for one launch per pattern than the batch's tuned values
-        plain_counts, pst = sge.count_sampled(g, plans_cost, stride)
-        plain_ms = (time.perf_counter() - t0) * 1000
-        log(f"plain: {plain_ms:.1f} ms, kernel {sum(s.kernel_ms for s in pst):.1f} ms, total {sum(plain_counts)}")
-
-    batch = sge.Batch(g, plans_fast, tuning=tuning, stride=stride)
-    for i in range(args.warmup):
-        counts, st = batch.run()
-        log(f"warmup {i}: {st.kernel_ms:.1f} ms")
-    times, kms, stats = [], [], None
-    for i in range(args.steps):
-        t0 = time.perf_counter()
-        counts, st = batch.run()
-        times.append(time.perf_counter() - t0)
-        kms.append(st.kernel_ms)
-        stats = st
-        log(f"step {i}: {times[-1] * 1000:.1f} ms (kernel {st.kernel_ms:.1f} ms)")
-    if plain_counts is not None and plain_counts != counts:
-        log("COUNT MISMATCH between plain and batched executors")
-        sys.exit(3)
-    total = sum(counts)
-    step_s = float(np.mean(times))
-    kernel_s = float(np.mean(kms)) / 1000.0
-    achieved = stats.bytes / kernel_s / 1e9 if kernel_s > 0 else 0.0
-    res = {
-        "metric": "exact matches/sec (batched executor, whole pattern set)",
-        "value": total / step_s,
-        "unit": "matches/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": step_s * 1000,
-        "higher_is_better": True,
-        "scaling": "strong",
-        "vs_baseline": None,
-        "dtype": "u32 ids / u64 counts",
-        "data": "synthetic",
-        "config": {"workload": wname, "mode": args.mode, "learned_order": learned, "tuning": tuning_desc, **cfg,
-                   "n": g.n, "m": g.m, "patterns": len(pats), "root_edge_stride": stride,
-                   "root_edges": (2 * g.m + stride - 1) // stride},
-        "pattern_set_ms": {"batched": step_s * 1000, "plain": plain_ms,
-                           "speedup_vs_plain": (plain_ms / (step_s * 1000)) if plain_ms else None},
-        "total_matches": total,
-        "intersections_per_s": stats.intersections / kernel_s if kernel_s > 0 else None,
-        "planning_s": plan_s,
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": load_traffic(wname),
-                     "kernel": "wcoj_kernel<false>", "algorithmic_bytes_per_launch": stats.bytes,
-                     "kernel_ms": kernel_s * 1000},
-        "cpu_baseline": None,
-    }
-    if not args.no_cpu:
-        res["cpu_baseline"] = cpu_baseline(g, pats, plans_cost, stride)
+    if args.workload in ("c4", "rmat20"):
+        res = c4_subsets(args)
+        if not args.no_c3:
+            res["c3_whole_job"] = whole_job(args, "sixv")
+    else:
+        res = whole_job(args, args.workload)
     print(json.dumps(res))
 
 

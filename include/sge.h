@@ -229,6 +229,9 @@ typedef struct sge_stats {
                                  independent of how well the launch was balanced) */
     double trie_ms;           /* host wall time spent materialising bag tries (sge_batch_run:
                                  the one-off build at sge_batch_create) */
+    uint64_t model_bytes;     /* list bytes of the per-pair model, 4(|A'|+|B'|) + 4|out| + 8 per CSR
+                                 row for every pairwise intersection, whichever list drove it
+                                 (`bytes` charges the last level's probes as executed: 8 per probe) */
 } sge_stats;
 
 void sge_tuning_default(sge_tuning *t);

@@ -69,7 +69,7 @@ constexpr int MAX_TERMS = 7;    // inclusion-exclusion terms of a 3-level indepe
 constexpr int NBUF_MAX = 2 * MAXK + 1 + MAX_TERMS;  // scratch buffers per wave at most (see BufLayout)
 constexpr uint32_t NODE_STEP = 1024;  // work-cap accounting granularity per wavefront (wave steps)
 #ifndef SGE_FSTAGE
-#define SGE_FSTAGE 400
+#define SGE_FSTAGE 396
 #endif
 constexpr int FSTAGE = SGE_FSTAGE;  // LDS words per wave for staging the terms' F sets (bitmaps or copies)
 #ifndef SGE_SEARCH_STAGE
@@ -124,7 +124,8 @@ struct DPlan {
 struct PlanOut {
     unsigned long long count, inter, bytes, nodes, tasks;
     unsigned long long cycles;  // wavefront clock cycles spent on the plan's work items
-    unsigned long long pad[2];
+    unsigned long long model;   // list bytes of the per-pair model: 4(|A'|+|B'|) + 4|out| + 8 per row
+    unsigned long long pad;
 };
 
 struct KParams {
@@ -194,7 +195,7 @@ struct WaveLds {
     uint32_t pos[MAXK];  // next candidate position (absolute index into cand[d].ptr)
     uint32_t iend[MAXK]; // end of this wave's iteration range at each level
     uint32_t tpos[MAXK]; // bag levels: absolute trie value index of phi[level]
-    unsigned long long accv[6];  // statistics counters (Acc), lane 0 only
+    unsigned long long accv[7];  // statistics counters (Acc), lane 0 only
     Src cand[MAXK];      // candidate set per level (whole set: reuse reads it)
     Src pre[MAXK + 1];   // exploration: parent intersections formed early ([MAXK] = F)
     Src srcs[MAXK + 2];  // sources of the intersection being formed
@@ -263,9 +264,14 @@ __device__ __forceinline__ uint64_t choosec(uint64_t n, uint32_t k) {
 }
 
 // Per-wave statistics counters (matches, pairwise intersections, algorithmic
-// bytes, search-tree nodes, root tasks, clock cycles), kept in the wave's LDS and
-// updated by lane 0 only, so they occupy no registers.
-enum { A_COUNT, A_INTER, A_BYTES, A_NODES, A_TASKS, A_CYCLES, A_NUM };
+// bytes of the executed strategy, search-tree nodes, root tasks, clock cycles,
+// list bytes of the per-pair model), kept in the wave's LDS and updated by lane
+// 0 only, so they occupy no registers.  The two byte counts differ only at the
+// last level: the strategy charges 8 bytes per probe it runs, the model
+// 4(|A'|+|B'|) per (sibling, term) pair whatever the probe direction.
+enum { A_COUNT, A_INTER, A_BYTES, A_NODES, A_TASKS, A_CYCLES, A_MODEL, A_NUM };
+static_assert(A_NUM == sizeof(((WaveLds *)0)->accv) / sizeof(unsigned long long), "one LDS slot per counter");
+static_assert(A_NUM <= 8, "PlanOut holds the counters in Acc order");
 struct Acc {
     unsigned long long *c;  // WaveLds::accv
     __device__ __forceinline__ void add(int k, unsigned long long v) const {
@@ -316,6 +322,7 @@ __device__ __forceinline__ uint32_t wave_intersect(WaveLds &w, int ns, int nchk,
         w.cur[i] = w.srcs[i].b;
     }
     acc.add(A_BYTES, bytes);
+    acc.add(A_MODEL, bytes);
     acc.add(A_INTER, (ns > 1) ? (ns - 1) : 0);
     const guint *D = w.srcs[drv].ptr;
     const uint32_t db = w.srcs[drv].b, de = w.srcs[drv].e;
@@ -360,6 +367,7 @@ __device__ __forceinline__ uint32_t wave_intersect(WaveLds &w, int ns, int nchk,
         total += __popcll(mask);
     }
     acc.add(A_BYTES, 4ull * total);
+    acc.add(A_MODEL, 4ull * total);
     wave_sync();  // other lanes read the survivors next
     return total;
 }
@@ -428,6 +436,7 @@ __device__ int gather_sources(const KParams &P, WaveLds &w, int d, uint32_t pmas
         clamp_range(s.ptr, s.b, s.e, has_lo, lo, has_hi, hi);
         w.srcs[ns++] = s;
         acc.add(A_BYTES, 8);
+        acc.add(A_MODEL, 8);
     }
     for (uint32_t m = pmask; m; m &= m - 1) {
         uint32_t v = w.phi[__builtin_ctz(m)];
@@ -438,6 +447,7 @@ __device__ int gather_sources(const KParams &P, WaveLds &w, int d, uint32_t pmas
         clamp_range(s.ptr, s.b, s.e, has_lo, lo, has_hi, hi);
         w.srcs[ns++] = s;
         acc.add(A_BYTES, 8);
+        acc.add(A_MODEL, 8);
     }
     return ns;
 }
@@ -453,6 +463,7 @@ __device__ Src form_set(WaveLds &w, int ns, int nchk, bool may_alias, uint32_t *
     }
     if (ns == 1 && may_alias) {
         acc.add(A_BYTES, 4ull * (w.srcs[0].e - w.srcs[0].b));
+        acc.add(A_MODEL, 4ull * (w.srcs[0].e - w.srcs[0].b));
         return w.srcs[0];
     }
     Src r;
@@ -776,6 +787,10 @@ __device__ __forceinline__ bool count_last(const KParams &P, WaveLds &w, TermLds
                 w.ent[ei] = make_uint4(carry + incl - work, dstart, tl, th);
             }
             carry += __shfl(incl, WAVE - 1);
+            // model bytes: both clamped lists of every probed (sibling, term) pair
+            uint32_t mw = work + (th - tl);
+            for (int off = WAVE / 2; off > 0; off >>= 1) mw += __shfl_xor(mw, off);
+            acc.add(A_MODEL, 4ull * mw);
         }
         if (lane == 0) w.pre_sum[0] = 0;
         wave_sync();
@@ -783,6 +798,7 @@ __device__ __forceinline__ bool count_last(const KParams &P, WaveLds &w, TermLds
         steps += (ne + W + WAVE - 1) / WAVE;
         acc.add(A_INTER, ne);
         acc.add(A_BYTES, 8ull * nb);
+        acc.add(A_MODEL, 8ull * nb);
         // phase 2: flattened probes, a window of 64 per round.  Lane x's entry is
         // the last entry starting at or before u0 + x: entries e0+1.. (at most 64,
         // as ne <= SIB_BLOCK <= 64) mark their start slot in the window, and each
@@ -1542,7 +1558,7 @@ __global__ void reset_pass_kernel(unsigned long long *queue, unsigned long long 
 __global__ void reset_kernel(PlanOut *out, uint32_t n, unsigned long long *queue, unsigned long long *emb_n,
                              unsigned long long *cap_ctr) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) out[i] = PlanOut{0, 0, 0, 0, 0, 0, {0, 0}};
+    if (i < n) out[i] = PlanOut{0, 0, 0, 0, 0, 0, 0, 0};
     if (i == 0) {
         queue[0] = 0;
         if (emb_n) *emb_n = 0;
@@ -1784,6 +1800,7 @@ void fill_stats(const PlanOut &o, double ms, sge_stats *s) {
     s->tasks += o.tasks;
     s->nodes += o.nodes;
     s->wave_cycles += o.cycles;
+    s->model_bytes += o.model;
     s->kernel_ms += ms;
 }
 
@@ -2508,3 +2525,4 @@ extern "C" int sge_calibration_copy(uint64_t bytes, double *ms) {
     return SGE_OK;
     SGE_API_END
 }
+

@@ -77,7 +77,7 @@ class Tuning(C.Structure):
 class Stats(C.Structure):
     _fields_ = [("intersections", C.c_uint64), ("bytes", C.c_uint64), ("tasks", C.c_uint64),
                 ("nodes", C.c_uint64), ("kernel_ms", C.c_double), ("wall_ms", C.c_double),
-                ("wave_cycles", C.c_uint64), ("trie_ms", C.c_double)]
+                ("wave_cycles", C.c_uint64), ("trie_ms", C.c_double), ("model_bytes", C.c_uint64)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
