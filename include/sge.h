@@ -164,6 +164,14 @@ typedef struct sge_plan {
 int sge_plan_build(const sge_pattern *p, const sge_graph *g, uint32_t mode, const uint32_t *order, sge_plan *out);
 /* Cost-model estimate of one order (lower is better). */
 int sge_plan_cost(const sge_pattern *p, const sge_graph *g, uint32_t mode, const uint32_t *order, double *cost);
+/* The reference's sampled cost model for one order (computeOrderCost,
+ * plan/optimizer.cpp:319-439, with cardinalities from cardEstimateUnlabeled,
+ * plan/estimator.cpp:562-661: SAMPLE_SIZE 40000, SAMPLE_PORTION 4), for a
+ * plan whose one bag holds every pattern vertex, under the plan's symmetry
+ * bounds; deterministic for a seed.  The fallback and judge of the learned
+ * order model.  g is required. */
+int sge_plan_cost_sampled(const sge_pattern *p, const sge_graph *g, uint32_t mode, const uint32_t *order,
+                          uint64_t seed, double *cost);
 /* All connected orders of the pattern (k! bounded), deduplicated up to automorphism; returns count in *n. */
 int sge_plan_orders(const sge_pattern *p, uint32_t *orders, uint32_t cap, uint32_t *n);
 /* All connected orders ranked as the planner ranks them for `mode` (longest

@@ -130,6 +130,8 @@ struct GraphSummary {
     bool valid = false;
 };
 GraphSummary summarize(const HostGraph *g);
+// estimator.cpp: the reference's sampled cost model of one plan's order
+double sampled_order_cost(const Pattern &p, const HostGraph &g, const sge_plan &pl, uint64_t seed);
 double order_cost(const Pattern &p, const GraphSummary &s, const int *order, const std::vector<std::vector<int>> &aut,
                   uint32_t mode);
 int build_plan(const Pattern &p, const HostGraph *g, uint32_t mode, const uint32_t *order, sge_plan &out);

@@ -110,6 +110,8 @@ _SIGS = {
     "sge_canonicalize_embeddings": (C.c_int, [C.POINTER(CPattern), C.c_void_p, C.c_uint64]),
     "sge_plan_build": (C.c_int, [C.POINTER(CPattern), C.c_void_p, C.c_uint32, C.c_void_p, C.POINTER(Plan)]),
     "sge_plan_cost": (C.c_int, [C.POINTER(CPattern), C.c_void_p, C.c_uint32, C.c_void_p, C.POINTER(C.c_double)]),
+    "sge_plan_cost_sampled": (C.c_int, [C.POINTER(CPattern), C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint64,
+                                        C.POINTER(C.c_double)]),
     "sge_plan_orders": (C.c_int, [C.POINTER(CPattern), C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32)]),
     "sge_plan_rank_orders": (C.c_int, [C.POINTER(CPattern), C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p,
                                        C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32)]),
@@ -347,6 +349,17 @@ def order_cost(pattern: Pattern, graph: Optional[Graph], order: Sequence[int], m
     c = C.c_double()
     _check(lib().sge_plan_cost(C.byref(pattern.c), graph._h if graph else None, mode, _ptr(o), C.byref(c)),
            "sge_plan_cost")
+    return c.value
+
+
+def order_cost_sampled(pattern: Pattern, graph: Graph, order: Sequence[int], mode: int = MODE_WCOJ,
+                       seed: int = 1) -> float:
+    """The reference's sampled cost model (computeOrderCost over sampled
+    cardinalities) of one order on `graph`, under the order's symmetry bounds."""
+    o = np.ascontiguousarray(order, dtype=np.uint32)
+    c = C.c_double()
+    _check(lib().sge_plan_cost_sampled(C.byref(pattern.c), graph._h, mode, _ptr(o), seed, C.byref(c)),
+           "sge_plan_cost_sampled")
     return c.value
 
 

@@ -363,3 +363,82 @@ def test_bench_refuses_more_than_one_gpu():
     assert r.returncode == 2
     line = json.loads(r.stdout.strip().splitlines()[-1])
     assert line["n_gpus"] == 2 and "one GPU" in line["error"]
+
+
+def _complete_graph(n):
+    return sge.Graph.from_edges(n, [(i, j) for i in range(n) for j in range(i + 1, n)])
+
+
+def _asymmetric(k, lo, hi):
+    return [p for p in sge.sample_patterns(k, lo, hi, 400, 7) if p.info().aut_size == 1][:2]
+
+
+def _ref_sampled_card(N, r):
+    """cardEstimateUnlabeled (plan/estimator.cpp:562-661) on K_N for an r-vertex
+    prefix without symmetry bounds, in closed form.  Every node at depth d has
+    N - d - 1 children, so sampling does not matter, only the scaling: all N
+    roots are taken (N < SAMPLE_SIZE); depth d >= 1 keeps k_d = max(1, (N-d)//4)
+    of its N - d candidates (SAMPLE_PORTION 4; the SAMPLE_SIZE budget never binds
+    at this size); the last level is counted, not sampled.  A finished subtree
+    is folded into its parent's estimate scaled by the PARENT level's ratio
+    (N-d)/k_d (ratio 1 at the root), so the deepest sampled level's own ratio is
+    never applied: est[j] = N * prod_{d=1}^{j-1} (N-d) * k_j for the prefix of
+    j+1 vertices below the last, and N * prod_{d=1}^{r-3} (N-d) * k_{r-2} * (N-r+1)
+    for the whole prefix; the estimate is the largest of them."""
+    k = [0] + [max(1, (N - d) // 4) for d in range(1, r)]
+    est = [float(N)]
+    for j in range(1, r - 1):
+        est.append(N * float(math.prod(N - d for d in range(1, j))) * k[j])
+    est.append(N * float(math.prod(N - d for d in range(1, r - 2))) * k[r - 2] * (N - r + 1))
+    return max(est)
+
+
+def test_sampled_cost_model_closed_form_on_complete_graph():
+    """The reference's sampled cost model (computeOrderCost over cardEstimateUnlabeled
+    cardinalities) in closed form on K_N for asymmetric patterns (no symmetry
+    bounds): card(1) = N, card(2) = N(N-1) directed edges, card(r >= 3) as
+    _ref_sampled_card derives it (the reference's estimator scales a subtree by
+    its parent level's sampling ratio, so the deepest sampled level is left
+    unscaled; with exact scaling card(r) would be N!/(N-r)!).  listSize is the
+    average degree N-1 at level 1, and at level i >= 2 the average degree times
+    the backward neighbours not covered by the widest earlier level (>= 2)
+    whose backward-neighbour list is a prefix of level i's (orderedSubset)."""
+    N = 14
+    assert 40000 // N >= 4 * 3 * 3 * 2 * 2 * 2  # the sample budget never binds below depth 6
+    g = _complete_graph(N)
+    avg = N - 1
+    for p in _asymmetric(6, 6, 9) + _asymmetric(7, 7, 12):
+        adj = {(a, b) for a, b in p.edges()} | {(b, a) for a, b in p.edges()}
+        for order in p.orders()[:8]:
+            k = len(order)
+            par = [[j for j in range(i) if (order[i], order[j]) in adj] for i in range(k)]
+            want = 0.0
+            for i in range(1, k):
+                card = float(N) if i == 1 else (float(N * (N - 1)) if i == 2 else _ref_sampled_card(N, i))
+                if i == 1:
+                    ls = avg
+                else:
+                    cover = []
+                    for j in range(2, i):
+                        if len(par[j]) > max(1, len(cover)) and par[i][:len(par[j])] == par[j]:
+                            cover = par[j]
+                    ls = avg * (len(par[i]) - len(cover))
+                want += card * ls
+            got = sge.order_cost_sampled(p, g, order, sge.MODE_WCOJ)
+            assert got == pytest.approx(want, rel=1e-9), (p.edges(), order)
+
+
+def test_sampled_cost_model_is_deterministic_and_bounds_halve_the_pair():
+    """A seed gives one cost (the reference draws from std::random_device); with a
+    symmetry bound between the first two levels the two-vertex card is m, not 2m,
+    and level 1's list is half the average degree."""
+    g = W.graph_c1()
+    p = W.named("house")
+    for order in p.orders()[:4]:
+        a = sge.order_cost_sampled(p, g, order, sge.MODE_IEP, seed=3)
+        assert a == sge.order_cost_sampled(p, g, order, sge.MODE_IEP, seed=3)
+    tri = W.named("triangle")  # every pair is bounded: cost = n*avg/2 + m*(2 uncovered parents)*(avg/2)
+    n, nnz = g.n, 2 * g.m
+    avg = nnz / n
+    got = sge.order_cost_sampled(tri, g, [0, 1, 2], sge.MODE_WCOJ)
+    assert got == pytest.approx(n * avg / 2 + (nnz / 2) * (2 * avg / 2), rel=1e-12)
