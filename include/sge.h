@@ -310,6 +310,25 @@ int sge_batch_create_ranges(sge_graph *g, const sge_plan *plans, uint32_t nplans
 int sge_batch_run(sge_batch *b, uint64_t *counts, sge_stats *stats);
 void sge_batch_destroy(sge_batch *b);
 
+/* Allocation-free count.  No counterpart in the reference: for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  The same counts as sge_count with
+ * SGE_COUNT_BATCHED, with every device buffer the kernels need beyond the
+ * inputs and outputs taken from one static per-device array of 64 MiB that the
+ * library holds until the process ends (control words, per-plan outputs and
+ * descriptors, continuation items, and as many wavefronts' scratch slabs as fit
+ * -- fewer at high max degree, so large graphs run on part of the GPU).  It
+ * calls no hipMalloc/hipFree.  Requirements: the graph already resident on the
+ * current device (sge_graph_to_device or SGE_GRAPH_BUILD_DEVICE), at most 256
+ * plans, no bag; large last-level sets are searched instead of staged as global
+ * bitmaps and hub root edges are not split.  SGE_ERANGE when one wavefront's
+ * slab exceeds the array (max degree above about 1.2 million for a 7-vertex IEP
+ * set) or nplans is above 256.  Every call on a GPU reuses that GPU's array and
+ * leaves the last call's contents in it: calls that overlap on one GPU race.
+ * stats (may be NULL): kernel_ms is the call's wall time. */
+int sge_count_noalloc(sge_graph *g, const sge_plan *plans, uint32_t nplans, const sge_tuning *tune,
+                      uint64_t *counts, sge_stats *stats);
+
 /* ------------------------------------------------------ learned cost model
  * 64 -> 128 -> 128 -> 1 ReLU MLP; hidden layers on BF16 MFMA with FP32
  * accumulation, FP32 master weights trained by Adam.  No reference counterpart:

@@ -2526,3 +2526,221 @@ extern "C" int sge_calibration_copy(uint64_t bytes, double *ms) {
     SGE_API_END
 }
 
+// ------------------------------------------------------ allocation-free count
+// sge_count_noalloc: the batched count with every device buffer the kernels
+// need beyond the caller's inputs (the graph's device CSR) and outputs (the
+// counts) carved out of one file-scope device array, `scratch`, of
+// NOALLOC_SCRATCH_BYTES.  Each GPU the library runs on holds its own copy
+// until the process ends; its address on a device is taken once, with
+// hipGetSymbolAddress, on the first call with that device current (the
+// array's name is a host-side stand-in, never passed to a kernel).  Layout:
+//   [0, 64)                   queue, item counts [2], work-cap counter
+//   PlanOut[NOALLOC_MAX_PLANS], DPlan[NOALLOC_MAX_PLANS]
+//   two continuation-item buffers of NOALLOC_ITEMS items
+//   the wavefronts' scratch slabs (as many wavefronts as fit, up to the
+//   resident maximum)
+// No hipMalloc/hipFree, no library allocation, no memset node: control words
+// and counters are cleared by fill kernels.  Every call on a GPU shares that
+// GPU's array and leaves the last call's contents in it; calls that overlap on
+// one GPU race (this library serialises its own calls with a mutex).
+constexpr size_t NOALLOC_SCRATCH_BYTES = 64ull << 20;
+constexpr uint32_t NOALLOC_MAX_PLANS = 256;
+constexpr uint64_t NOALLOC_ITEMS = 1ull << 12;
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+struct NoallocLayout {
+    size_t out, plans, items0, items1, slabs;
+};
+
+constexpr size_t align256(size_t x) { return (x + 255) / 256 * 256; }
+
+constexpr NoallocLayout noalloc_layout() {
+    NoallocLayout L{};
+    L.out = 256;
+    L.plans = align256(L.out + sizeof(PlanOut) * NOALLOC_MAX_PLANS);
+    L.items0 = align256(L.plans + sizeof(DPlan) * NOALLOC_MAX_PLANS);
+    L.items1 = align256(L.items0 + sizeof(uint32_t) * ITEM_WORDS * NOALLOC_ITEMS);
+    L.slabs = align256(L.items1 + sizeof(uint32_t) * ITEM_WORDS * NOALLOC_ITEMS);
+    return L;
+}
+static_assert(noalloc_layout().slabs < NOALLOC_SCRATCH_BYTES / 2, "the slabs get most of the array");
+
+// The array's device address on each device, taken once per device at its
+// first call (under dmutex()).
+unsigned char *noalloc_base(int dev) {
+    static unsigned char *ptr[NOALLOC_MAX_DEVICES] = {};
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return nullptr;
+    if (!ptr[dev]) {
+        void *p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) {
+            sge::set_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+            return nullptr;
+        }
+        ptr[dev] = static_cast<unsigned char *>(p);
+    }
+    return ptr[dev];
+}
+
+}  // namespace
+
+extern "C" int sge_count_noalloc(sge_graph *g, const sge_plan *plans, uint32_t nplans, const sge_tuning *tune,
+                                 uint64_t *counts, sge_stats *stats) {
+    SGE_API_BEGIN
+    auto t0 = std::chrono::steady_clock::now();
+    if (!g || !plans || !counts || nplans == 0) {
+        sge::set_error("sge_count_noalloc: null argument or no plans");
+        return SGE_EINVAL;
+    }
+    if (nplans > NOALLOC_MAX_PLANS) {
+        sge::set_error("sge_count_noalloc: at most " + std::to_string(NOALLOC_MAX_PLANS) + " plans per call");
+        return SGE_ERANGE;
+    }
+    int dev = -1;
+    HIP_TRY(hipGetDevice(&dev));
+    if (!g->d_off || g->device != dev) {
+        sge::set_error("sge_count_noalloc: the graph is not resident on the current device (sge_graph_to_device "
+                       "or SGE_GRAPH_BUILD_DEVICE first; uploading it would allocate)");
+        return SGE_EINVAL;
+    }
+    std::lock_guard<std::mutex> lk(dmutex());
+    unsigned char *base = noalloc_base(dev);
+    if (!base) return SGE_EDEVICE;
+    constexpr NoallocLayout L = noalloc_layout();
+    // launch shape: the uncapped or capped counting kernel, as many resident
+    // wavefronts as the slab region holds
+    sge_tuning def;
+    sge_tuning_default(&def);
+    if (!tune) tune = &def;
+    LaunchCfg cfg;
+    cfg.block = (int)tune->block_threads;
+    if (cfg.block < 64 || cfg.block > 256 || cfg.block % 64) {
+        sge::set_error("tuning: block_threads must be 64, 128, 192 or 256");
+        return SGE_EINVAL;
+    }
+    cfg.chunk = tune->chunk ? tune->chunk : 4;
+    cfg.budget = tune->donate_budget ? (tune->donate_budget >= (1u << 30) ? 0u : tune->donate_budget) : 4096u;
+    cfg.probe_ratio = tune->stage_max ? tune->stage_max : 2u;
+    cfg.bitmap_ratio = tune->bitmap_ratio ? tune->bitmap_ratio : cfg.probe_ratio;
+    cfg.node_cap = (unsigned long long)tune->node_cap_k * NODE_STEP;
+    cfg.lds = (size_t)(cfg.block / WAVE) * (sizeof(WaveLds) + MAX_TERMS * sizeof(TermLds));
+    int num_cu = 0, occ = 0;
+    HIP_TRY(hipDeviceGetAttribute(&num_cu, hipDeviceAttributeMultiprocessorCount, dev));
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, wcoj_kernel<false, false>, cfg.block, cfg.lds));
+    const uint64_t resident = (uint64_t)num_cu * std::max(1, occ) * (cfg.block / WAVE);
+    // plans: validated, converted and uploaded one at a time (no host buffers)
+    DPlan *dplans = reinterpret_cast<DPlan *>(base + L.plans);
+    uint32_t kmax = 2, terms = 1;
+    bool explore = false;
+    for (uint32_t i = 0; i < nplans; ++i) {
+        int rc = validate_plan(plans[i]);
+        if (rc) return rc;
+        if (plans[i].k < 2 || plans[i].bag) {
+            sge::set_error("sge_count_noalloc: plans need >= 2 vertices and no bag (a bag trie is built in "
+                           "allocated memory)");
+            return SGE_EINVAL;
+        }
+        DPlan d = to_dplan(plans[i]);
+        d.split = nullptr;  // no root-edge split: its prefix arrays would be allocated
+        d.nsub = 0;
+        d.task_begin = 0;
+        d.ntask = g->h.nnz;
+        kmax = std::max(kmax, d.k);
+        terms = std::max(terms, d.nterms);
+        explore = explore || d.mode == SGE_MODE_EXPLORE;
+        HIP_TRY(hipMemcpy(dplans + i, &d, sizeof(DPlan), hipMemcpyHostToDevice));
+    }
+    const uint32_t nbuf = explore ? 2 * kmax + 1 + terms : kmax + terms;
+    KParams kp;
+    std::memset(&kp, 0, sizeof(kp));
+    kp.buf_stride = ((g->h.max_degree + 64 + 63) / 64) * 64;
+    kp.wave_stride = (uint64_t)kp.buf_stride * nbuf;
+    if (explore)
+        set_layout(kp, kmax, 2 * kmax, 2 * kmax + 1);
+    else
+        set_layout(kp, kmax, kmax, kmax);
+    const uint64_t fit = (NOALLOC_SCRATCH_BYTES - L.slabs) / (kp.wave_stride * sizeof(uint32_t));
+    if (fit == 0) {
+        sge::set_error("sge_count_noalloc: one wavefront's scratch slab (" +
+                       std::to_string(kp.wave_stride * sizeof(uint32_t)) + " bytes at max degree " +
+                       std::to_string(g->h.max_degree) + ") exceeds the " +
+                       std::to_string((NOALLOC_SCRATCH_BYTES - L.slabs) >> 20) + " MiB slab region");
+        return SGE_ERANGE;
+    }
+    const uint64_t waves = std::min(resident, fit);
+    if (waves < (uint64_t)(cfg.block / WAVE)) cfg.block = WAVE;  // fewer wavefronts than one workgroup holds
+    cfg.lds = (size_t)(cfg.block / WAVE) * (sizeof(WaveLds) + MAX_TERMS * sizeof(TermLds));
+    cfg.grid = (int)(waves / (uint64_t)(cfg.block / WAVE));
+    cfg.head = tune->head_items ? (tune->head_items >= (1u << 30) ? 0u : tune->head_items) : (uint32_t)waves;
+    unsigned long long *ctl = reinterpret_cast<unsigned long long *>(base);  // [0] queue, [1..2] items, [3] cap
+    PlanOut *dout = reinterpret_cast<PlanOut *>(base + L.out);
+    kp.off = (const uint32_t *)g->d_off;
+    kp.nbr = (const uint32_t *)g->d_nbr;
+    kp.src = (const uint32_t *)g->d_src;
+    kp.n = g->h.n;
+    kp.nnz = g->h.nnz;
+    kp.plans = dplans;
+    kp.nplans = nplans;
+    kp.stride = 1;
+    kp.chunk = cfg.chunk;
+    kp.head = cfg.head;
+    kp.nitems = items_per_plan(g->h.nnz, cfg) * nplans;
+    kp.queue = ctl;
+    kp.scratch = reinterpret_cast<uint32_t *>(base + L.slabs);
+    kp.gbm = nullptr;  // large F sets are probed by search instead of a global bitmap
+    kp.out = dout;
+    kp.item_cap = NOALLOC_ITEMS;
+    kp.budget = cfg.budget;
+    kp.probe_ratio = cfg.probe_ratio;
+    kp.bitmap_ratio = cfg.bitmap_ratio;
+    kp.node_cap = cfg.node_cap;
+    kp.node_ctr = ctl + 3;
+    uint32_t *items[2] = {reinterpret_cast<uint32_t *>(base + L.items0), reinterpret_cast<uint32_t *>(base + L.items1)};
+    hipStream_t st = nullptr;  // the null stream: nothing to create
+    reset_kernel<<<(nplans + 255) / 256, 256, 0, st>>>(dout, nplans, ctl, nullptr, kp.node_ctr);
+    reset_pass_kernel<<<1, 64, 0, st>>>(ctl, ctl + 1);
+    HIP_TRY(hipGetLastError());
+    int cur = 0;
+    bool limited = false;
+    for (int pass = 0;; ++pass) {
+        if (pass > 0) {
+            unsigned long long hc[3] = {0, 0, 0};  // item counts, work spent
+            HIP_TRY(hipMemcpy(hc, ctl + 1, sizeof(hc), hipMemcpyDeviceToHost));
+            if (kp.node_cap && hc[2] >= kp.node_cap) {
+                limited = true;
+                break;
+            }
+            if (hc[cur] == 0) break;
+            kp.in_items = items[cur];
+            kp.in_count = ctl + 1 + cur;
+            cur ^= 1;
+            reset_pass_kernel<<<1, 64, 0, st>>>(ctl, ctl + 1 + cur);
+        }
+        kp.out_items = items[cur];
+        kp.out_count = ctl + 1 + cur;
+        if (pass == 0 && kp.nitems == 0) break;
+        launch_wcoj(false, cfg, st, kp);
+        HIP_TRY(hipGetLastError());
+    }
+    HIP_TRY(hipStreamSynchronize(st));
+    if (stats) std::memset(stats, 0, sizeof(*stats));
+    for (uint32_t i = 0; i < nplans; ++i) {
+        PlanOut o;
+        HIP_TRY(hipMemcpy(&o, dout + i, sizeof(o), hipMemcpyDeviceToHost));
+        counts[i] = o.count * (plans[i].multiplier ? plans[i].multiplier : 1);
+        fill_stats(o, 0.0, stats);
+    }
+    if (stats) {
+        stats->wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        stats->kernel_ms = stats->wall_ms;
+    }
+    if (limited) {
+        sge::set_error("work cap reached: counts are partial (stats cover the work done)");
+        return SGE_ELIMIT;
+    }
+    return SGE_OK;
+    SGE_API_END
+}

@@ -131,6 +131,8 @@ _SIGS = {
                                   C.POINTER(C.c_uint64), C.POINTER(Stats)]),
     "sge_count_ranges": (C.c_int, [C.c_void_p, C.POINTER(Plan), C.c_uint32, C.c_void_p, C.c_void_p, C.c_uint32,
                                    C.POINTER(Tuning), C.c_void_p, C.POINTER(Stats), C.POINTER(Stats)]),
+    "sge_count_noalloc": (C.c_int, [C.c_void_p, C.POINTER(Plan), C.c_uint32, C.POINTER(Tuning), C.c_void_p,
+                                    C.POINTER(Stats)]),
     "sge_num_tasks": (C.c_int, [C.c_void_p, C.POINTER(Plan), C.POINTER(C.c_uint64)]),
     "sge_count_sampled": (C.c_int, [C.c_void_p, C.POINTER(Plan), C.c_uint32, C.c_uint64, C.c_uint64,
                                     C.POINTER(Tuning), C.c_void_p, C.POINTER(Stats)]),
@@ -488,6 +490,21 @@ def count_ranges(graph: Graph, plans, begins, ends, batched: bool = False, tunin
                                   C.byref(tuning) if tuning else None, _ptr(counts), C.byref(st), pp),
            "sge_count_ranges")
     return [int(x) for x in counts], st, [pp[i] for i in range(n)]
+
+
+def count_noalloc(graph: Graph, plans, tuning: Optional[Tuning] = None):
+    """All plans in one launch with no device allocation: every buffer comes from
+    the library's static per-device array (the graph must already be on the
+    device).  Returns counts and Stats."""
+    if isinstance(plans, Plan):
+        plans = [plans]
+    n = len(plans)
+    arr = (Plan * n)(*plans)
+    counts = np.zeros(n, dtype=np.uint64)
+    st = Stats()
+    _check(lib().sge_count_noalloc(graph._h, arr, n, C.byref(tuning) if tuning else None, _ptr(counts),
+                                   C.byref(st)), "sge_count_noalloc")
+    return [int(x) for x in counts], st
 
 
 def count_sampled(graph: Graph, plans, stride: int, offset: int = 0, tuning: Optional[Tuning] = None):
