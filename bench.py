@@ -125,14 +125,19 @@ def batch_runner(batch):
 
 
 def roofline(st, kernel_ms, wname):
-    achieved = st.bytes / (kernel_ms / 1000.0) / 1e9 if kernel_ms > 0 else 0.0
-    model = st.model_bytes / (kernel_ms / 1000.0) / 1e9 if kernel_ms > 0 else 0.0
+    """achieved = the per-pair list-byte model (4(|A'|+|B'|) + 4|out| + 8 per CSR
+    row for every pairwise intersection of clamped sorted lists) over the kernel
+    time; `executed` = the bytes the kernel's strategy touches (it probes the
+    longer list of a last-level pair by search or bitmap instead of reading it)."""
+    s = kernel_ms / 1000.0
+    achieved = st.model_bytes / s / 1e9 if s > 0 else 0.0
+    executed = st.bytes / s / 1e9 if s > 0 else 0.0
     return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": load_traffic(wname), "kernel": "wcoj_kernel<false,false>",
-            "algorithmic_bytes_per_launch": st.bytes, "kernel_ms": kernel_ms,
-            "per_pair_model": {"bytes_per_launch": st.model_bytes, "achieved": model, "frac": model / HBM_PEAK_GBS,
-                               "note": "4(|A'|+|B'|) + 4|out| + 8 per CSR row for every pairwise intersection; "
-                                       "`achieved` charges the last level's probes as executed (8 B each)"}}
+            "algorithmic_bytes_per_launch": st.model_bytes, "kernel_ms": kernel_ms,
+            "executed": {"bytes_per_launch": st.bytes, "achieved": executed, "frac": executed / HBM_PEAK_GBS,
+                         "note": "bytes the kernel's strategy touches: the last level's probes at 8 B each "
+                                 "(driver element + probed element) instead of both whole lists"}}
 
 
 def cpu_oracle(g, plans, ranges, budget_s=20.0):
@@ -241,23 +246,25 @@ def c4_subsets(args):
     ranges = (begins, ends)
     root_edges = int((ends - begins).sum())
 
+    plain_counts = plain_ms = plain_kms = bcost_counts = bcost_ms = bcost_kms = None
     plain = plain_runner(g, plans_cost, ranges)
-    plain_counts, plain_ms, plain_kms = timed_steps(plain, args.warmup, args.steps, "plain")
-    if span == 1 and plain_counts != [e["count"] for e in subs]:
-        log("COUNT MISMATCH: plain executor against the oracle's counts")
-        sys.exit(3)
-    bc = sge.Batch(g, plans_cost, ranges=ranges)
-    bcost_counts, bcost_ms, bcost_kms = timed_steps(batch_runner(bc), args.warmup, args.steps, "batched_cost")
-    bc.close()
-    if bcost_counts != plain_counts:
-        log("COUNT MISMATCH between plain and batched executors (same plans)")
-        sys.exit(3)
+    if not args.no_plain:
+        plain_counts, plain_ms, plain_kms = timed_steps(plain, args.warmup, args.steps, "plain")
+        if span == 1 and plain_counts != [e["count"] for e in subs]:
+            log("COUNT MISMATCH: plain executor against the oracle's counts")
+            sys.exit(3)
+        bc = sge.Batch(g, plans_cost, ranges=ranges)
+        bcost_counts, bcost_ms, bcost_kms = timed_steps(batch_runner(bc), args.warmup, args.steps, "batched_cost")
+        bc.close()
+        if bcost_counts != plain_counts:
+            log("COUNT MISMATCH between plain and batched executors (same plans)")
+            sys.exit(3)
     b = sge.Batch(g, plans_fast, ranges=ranges)
     brun = batch_runner(b)
     counts, step_ms, kernel_ms = timed_steps(brun, args.warmup, args.steps, "batched")
     st = brun.stats
     b.close()
-    if changed:
+    if changed and not args.no_plain:
         check, _, _ = sge.count_ranges(g, plans_fast, begins, ends, batched=False)
         if check != counts:
             log("COUNT MISMATCH between plain and batched executors (learned plans)")
@@ -275,16 +282,17 @@ def c4_subsets(args):
                    "tuning": "library defaults"},
         "pattern_set_ms": {"plain": plain_ms, "plain_kernel": plain_kms, "batched_cost_order": bcost_ms,
                            "batched_cost_order_kernel": bcost_kms, "batched": step_ms, "batched_kernel": kernel_ms,
-                           "speedup_vs_plain": plain_ms / step_ms,
-                           "speedup_vs_plain_kernel": plain_kms / kernel_ms if kernel_ms > 0 else None,
-                           "speedup_batching_same_plans": plain_ms / bcost_ms,
+                           "speedup_vs_plain": plain_ms / step_ms if plain_ms else None,
+                           "speedup_vs_plain_kernel": plain_kms / kernel_ms if plain_kms and kernel_ms > 0 else None,
+                           "speedup_batching_same_plans": plain_ms / bcost_ms if plain_ms else None,
                            "note": "plain and batched_cost run the same plans and return the same counts "
                                    "(= the oracle's at span 1); batched runs the learned orders, whose counts "
                                    "over a root subset differ by plan and are checked against plain with the "
                                    "same plans"},
-        "total_matches": total, "total_matches_cost_order": sum(plain_counts),
-        "counts_checked": "plain == batched_cost" + (" == oracle fixture" if span == 1 else "")
-                          + ("; batched == plain(learned plans)" if changed else ""),
+        "total_matches": total, "total_matches_cost_order": sum(plain_counts) if plain_counts else None,
+        "counts_checked": ("not checked (--no-plain)" if args.no_plain else
+                           "plain == batched_cost" + (" == oracle fixture" if span == 1 else "")
+                           + ("; batched == plain(learned plans)" if changed else "")),
         "intersections_per_s": st.intersections / (kernel_ms / 1000.0) if kernel_ms > 0 else None,
         "planning_s": plan_s, "learned_choice_s": learn_s,
         "roofline": roofline(st, kernel_ms, wname),
@@ -293,7 +301,7 @@ def c4_subsets(args):
     if not args.no_cpu:
         cpu, done, _ = cpu_oracle(g, plans_cost, (begins, np.array([e["end"] for e in subs], np.uint64)))
         res["cpu_baseline"] = cpu
-        if span == 1:
+        if span == 1 and plain_counts:
             res["cpu_baseline"]["gpu_same_sample"] = gpu_rate_on(plain.per_plan, plain_counts, done)
     return res
 
@@ -323,13 +331,15 @@ def whole_job(args, name):
     tuning = autotune.tuned(wname)
     tuning_desc = {k: getattr(tuning, k) for k, _ in sge.Tuning._fields_} if tuning else "library defaults"
     plain = plain_runner(g, plans_cost)
-    plain_counts, plain_ms, plain_kms = timed_steps(plain, args.warmup, args.steps, f"{name} plain")
+    plain_counts = plain_ms = plain_kms = None
+    if not args.no_plain:
+        plain_counts, plain_ms, plain_kms = timed_steps(plain, args.warmup, args.steps, f"{name} plain")
     b = sge.Batch(g, plans_fast, tuning=tuning)
     brun = batch_runner(b)
     counts, step_ms, kernel_ms = timed_steps(brun, args.warmup, args.steps, f"{name} batched")
     st = brun.stats
     b.close()
-    if counts != plain_counts:
+    if plain_counts is not None and counts != plain_counts:
         log("COUNT MISMATCH between plain and batched executors")
         sys.exit(3)
     total = sum(counts)
@@ -340,9 +350,11 @@ def whole_job(args, name):
         "config": {"workload": wname, "mode": args.mode, "graph": gdesc, "patterns_desc": pdesc, "n": g.n, "m": g.m,
                    "patterns": len(pats), "learned_order": learned, "tuning": tuning_desc, "whole_job": True},
         "pattern_set_ms": {"plain": plain_ms, "plain_kernel": plain_kms, "batched": step_ms,
-                           "batched_kernel": kernel_ms, "speedup_vs_plain": plain_ms / step_ms,
-                           "speedup_vs_plain_kernel": plain_kms / kernel_ms if kernel_ms > 0 else None},
-        "total_matches": total, "counts_checked": "plain == batched (whole job: counts do not depend on the order)",
+                           "batched_kernel": kernel_ms, "speedup_vs_plain": plain_ms / step_ms if plain_ms else None,
+                           "speedup_vs_plain_kernel": plain_kms / kernel_ms if plain_kms and kernel_ms > 0 else None},
+        "total_matches": total,
+        "counts_checked": ("not checked (--no-plain)" if plain_counts is None else
+                           "plain == batched (whole job: counts do not depend on the order)"),
         "intersections_per_s": st.intersections / (kernel_ms / 1000.0) if kernel_ms > 0 else None,
         "planning_s": plan_s,
         "roofline": roofline(st, kernel_ms, wname),
@@ -350,7 +362,7 @@ def whole_job(args, name):
     }
     if not args.no_cpu and mode == sge.MODE_IEP:
         cpu, done = cpu_reference(g, pats)
-        if cpu is not None:
+        if cpu is not None and plain_counts is not None:
             cpu["gpu_same_sample"] = gpu_rate_on(plain.per_plan, plain_counts, done)
         res["cpu_baseline"] = cpu
     return res
@@ -366,6 +378,7 @@ def main():
     ap.add_argument("--span", type=int, default=1, help="c4: multiply every root-edge range's length by S")
     ap.add_argument("--no-c3", action="store_true", help="c4: leave out the C3 whole-job block")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true", help="time the batched executor only (profiling runs)")
     ap.add_argument("--calibrate", action="store_true",
                     help="also run the known-size copy kernel (scales rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()

@@ -119,24 +119,54 @@ std::vector<int> ri_order(const Pattern &p, const std::vector<int> &vertices) {
     return order;
 }
 
+// Candidates of one sampled level: a clamped CSR row minus the matched
+// vertices it holds (a level with one parent: nothing is copied), or the
+// materialised intersection of several clamped rows.  The set is the one
+// generateCandidates (join.cpp:137-178) forms for an unlabeled candidate
+// space; only its representation is lazy.
+struct Cands {
+    bool lazy = false;
+    const uint32_t *b = nullptr, *e = nullptr;  // lazy: the clamped row
+    std::vector<uint32_t> skip;                  // lazy: sorted row offsets of matched vertices
+    std::vector<uint32_t> mat;                   // materialised set
+    uint32_t size() const { return lazy ? (uint32_t)(e - b) - (uint32_t)skip.size() : (uint32_t)mat.size(); }
+    uint32_t at(uint32_t i) const {  // i-th element of the set
+        if (!lazy) return mat[i];
+        uint32_t raw = i;
+        for (uint32_t s : skip) {
+            if (s <= raw) ++raw;
+            else break;
+        }
+        return b[raw];
+    }
+};
+
 // sampleKElements (utility/utils.cpp:56-68): k distinct positions drawn
 // uniformly, kept in increasing position order.
-void sample_k(const std::vector<uint32_t> &src, uint32_t len, uint32_t k, Rng &rng, std::vector<uint32_t> &dst) {
+void sample_k(const Cands &src, uint32_t k, Rng &rng, std::vector<uint32_t> &dst) {
+    const uint32_t len = src.size();
     std::set<uint32_t> idx;
     while (idx.size() < k) idx.insert((uint32_t)rng.below(len));
     dst.clear();
-    for (uint32_t i : idx) dst.push_back(src[i]);
+    for (uint32_t i : idx) dst.push_back(src.at(i));
 }
 
-// Clamped, injective intersection of the parents' rows (generateCandidates,
-// join.cpp:137-178, for an unlabeled candidate space).
+void take_all(const Cands &src, std::vector<uint32_t> &dst) {
+    dst.clear();
+    for (uint32_t i = 0; i < src.size(); ++i) dst.push_back(src.at(i));
+}
+
+// Clamped, injective intersection of the parents' rows: the shortest row
+// drives, every other row is searched from the previous hit (galloping, as
+// ComputeCNGallopingAVX2 does for skewed pairs, compute_set_intersection.cpp:
+// 346-392); the result is the same set as a merge.
 void candidates_of(const HostGraph &g, const Pattern &p, const std::vector<int> &ord, int d, Rules rules,
-                   const std::vector<uint32_t> &phi, const std::vector<char> &visited, std::vector<uint32_t> &out,
-                   std::vector<uint32_t> &tmp) {
+                   const std::vector<uint32_t> &phi, const std::vector<char> &visited, Cands &out) {
     const int u = ord[d];
     bool has_lo = false, has_hi = false;
     uint32_t lo = 0, hi = 0xffffffffu;
-    std::vector<std::pair<const uint32_t *, const uint32_t *>> rows;
+    std::pair<const uint32_t *, const uint32_t *> rows[SGE_MAX_K];
+    int nr = 0;
     for (int j = 0; j < d; ++j) {
         const int u2 = ord[j];
         if (rule(rules, u2, u)) {
@@ -154,43 +184,62 @@ void candidates_of(const HostGraph &g, const Pattern &p, const std::vector<int> 
             const uint32_t *b = g.nbr.data() + g.off[v], *e = g.nbr.data() + g.off[v + 1];
             if (has_lo) b = std::upper_bound(b, e, lo);
             if (has_hi) e = std::lower_bound(b, e, hi);
-            rows.push_back({b, e});
+            rows[nr++] = {b, e};
         }
-    out.clear();
-    if (rows.empty()) return;
-    std::sort(rows.begin(), rows.end(), [](const auto &a, const auto &b) { return a.second - a.first < b.second - b.first; });
-    out.assign(rows[0].first, rows[0].second);
-    for (size_t r = 1; r < rows.size() && !out.empty(); ++r) {
-        tmp.clear();
-        std::set_intersection(out.begin(), out.end(), rows[r].first, rows[r].second, std::back_inserter(tmp));
-        out.swap(tmp);
+    out.mat.clear();
+    out.skip.clear();
+    out.lazy = false;
+    if (nr == 0) return;
+    std::sort(rows, rows + nr, [](const auto &a, const auto &b) { return a.second - a.first < b.second - b.first; });
+    if (nr == 1) {
+        out.lazy = true;
+        out.b = rows[0].first;
+        out.e = rows[0].second;
+        for (int j = 0; j < d; ++j) {  // matched vertices inside the row are not candidates
+            const uint32_t v = phi[ord[j]];
+            const uint32_t *q = std::lower_bound(out.b, out.e, v);
+            if (q != out.e && *q == v) out.skip.push_back((uint32_t)(q - out.b));
+        }
+        std::sort(out.skip.begin(), out.skip.end());
+        return;
     }
-    size_t w = 0;
-    for (uint32_t x : out)
-        if (!visited[x]) out[w++] = x;
-    out.resize(w);
+    const uint32_t *cur[SGE_MAX_K];
+    for (int r = 1; r < nr; ++r) cur[r] = rows[r].first;
+    for (const uint32_t *x = rows[0].first; x != rows[0].second; ++x) {
+        bool in = true;
+        for (int r = 1; r < nr && in; ++r) {
+            cur[r] = std::lower_bound(cur[r], rows[r].second, *x);
+            in = cur[r] != rows[r].second && *cur[r] == *x;
+        }
+        if (in && !visited[*x]) out.mat.push_back(*x);
+    }
 }
 
 // cardEstimateUnlabeled (estimator.cpp:562-661) along `ord` (>= 3 vertices).
 double card_sampled(const HostGraph &g, const Pattern &p, const std::vector<int> &ord, Rules rules, uint64_t seed) {
     const int L = (int)ord.size();
     Rng rng(seed);
-    std::vector<std::vector<uint32_t>> cand(L), total(L);
+    std::vector<std::vector<uint32_t>> cand(L);
+    std::vector<Cands> total(L);
     std::vector<uint32_t> candn(L, 0), totaln(L, 0), pos(L, 0), nsamp(L, 0), used(L, 0);
     std::vector<std::vector<double>> est(L);
     for (int i = 0; i < L; ++i) est[i].assign(L - i, 0.0);
-    std::vector<uint32_t> phi(SGE_MAX_K, 0), tmp;
+    std::vector<uint32_t> phi(SGE_MAX_K, 0);
     std::vector<char> visited(g.n, 0);
-    // depth 0: every vertex is a candidate, SAMPLE_SIZE of them sampled
-    std::vector<uint32_t> all(g.n);
-    for (uint32_t v = 0; v < g.n; ++v) all[v] = v;
+    // depth 0: every vertex is a candidate, SAMPLE_SIZE of them sampled; the
+    // sample budgets are u32 as in the reference (numSamples / numUsed)
+    nsamp[0] = SAMPLE_SIZE;
     totaln[0] = g.n;
     candn[0] = std::min<uint32_t>(g.n, SAMPLE_SIZE);
-    if (candn[0] < g.n)
-        sample_k(all, g.n, candn[0], rng, cand[0]);
-    else
-        cand[0] = all;
     if (candn[0] == 0) return 1.0;
+    {
+        std::set<uint32_t> idx;
+        if (candn[0] < g.n)
+            while (idx.size() < candn[0]) idx.insert((uint32_t)rng.below(g.n));
+        else
+            for (uint32_t v = 0; v < g.n; ++v) idx.insert(v);
+        cand[0].assign(idx.begin(), idx.end());
+    }
     nsamp[1] = SAMPLE_SIZE / candn[0];
     int depth = 0;
     while (depth >= 0) {
@@ -202,8 +251,8 @@ double card_sampled(const HostGraph &g, const Pattern &p, const std::vector<int>
             used[depth] = 0;
             std::fill(est[depth].begin(), est[depth].end(), 0.0);
             pos[depth] = 0;
-            candidates_of(g, p, ord, depth, rules, phi, visited, total[depth], tmp);
-            totaln[depth] = (uint32_t)total[depth].size();
+            candidates_of(g, p, ord, depth, rules, phi, visited, total[depth]);
+            totaln[depth] = total[depth].size();
             if (depth == L - 1) {
                 --depth;
                 est[depth][0] += 1.0;
@@ -214,9 +263,9 @@ double card_sampled(const HostGraph &g, const Pattern &p, const std::vector<int>
                 uint32_t k = std::min(nsamp[depth], totaln[depth] / SAMPLE_PORTION);
                 if (totaln[depth] != 0 && k == 0) k = 1;
                 if (k < totaln[depth])
-                    sample_k(total[depth], totaln[depth], k, rng, cand[depth]);
+                    sample_k(total[depth], k, rng, cand[depth]);
                 else
-                    cand[depth] = total[depth];
+                    take_all(total[depth], cand[depth]);
                 candn[depth] = k;
                 nsamp[depth + 1] = k ? nsamp[depth] / k : 0;
             }
