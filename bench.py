@@ -82,7 +82,7 @@ def learned_plans(g, pats, plans_cost, mode):
     if not os.path.exists(LEARNED):
         return plans_cost, False
     model = LM.load(LEARNED)
-    return [LM.choose_plan(model, p, g, mode, fallback=pc)[0] for p, pc in zip(pats, plans_cost)], True
+    return [plan for plan, _ in LM.choose_plans(model, pats, g, mode, plans_cost)], True
 
 
 def timed_steps(run, warmup, steps, label):

@@ -6,10 +6,12 @@ executor measures for a (pattern, order) -- wavefront clock cycles, summed over
 the wavefronts that ran it -- relative to the pattern's other orders, from the order's features
 (sge_order_features: plan structure, degree moments, analytic cost, Knuth
 random-walk estimates over sampled root edges).  It only reorders: counts never
-change.  The analytic cost model (plan/optimizer.cpp:441-577 optCostOrder,
-restated in the planner) stays the fallback: when the network's choice costs
-more than `threshold` bits above the cost model's own choice by the cost
-model's estimate, the cost-model order is kept.
+change.  The planner's own choice (its analytic cost model) is the starting
+point, and the reference's sampled cost model (computeOrderCost over
+cardEstimateUnlabeled cardinalities, plan/optimizer.cpp:319-439 and
+plan/estimator.cpp:562-661, restated as sge_plan_cost_sampled) is the fallback
+and the judge: when it puts the network's choice more than `judge_bits` above
+the planner's order, the planner's order is kept.
 
 Training data are (order, measured work) pairs logged by the plain executor on
 a strided root-edge sample (sge_count_sampled), for patterns disjoint from the
@@ -23,8 +25,10 @@ from typing import Dict, List, Optional, Sequence
 
 import numpy as np
 
-from . import (MLP, Graph, Pattern, Plan, build_plan, count_sampled, order_cost, order_features, rank_orders,
-               time_capped, work_steps_bound)
+from concurrent.futures import ThreadPoolExecutor
+
+from . import (MLP, Graph, Pattern, Plan, build_plan, count_sampled, order_cost, order_cost_sampled, order_features,
+               rank_orders, time_capped, work_steps_bound)
 
 FEATURE_SAMPLES = 512
 CAPPED_PENALTY = 2.0  # bits added to the time of an order stopped by the work cap
@@ -83,6 +87,7 @@ class LearnedOrder:
         self.sd = np.ones(64, np.float32)
         self.t_sd = 1.0
         self.threshold = 3.0  # bits of analytic cost the learned order may exceed the cost model's by
+        self.judge_bits = 1.0  # bits of the sampled (reference) cost model the learned order may exceed it by
         self.margin = 0.25    # predicted bits of improvement needed to leave the cost model's order
         self.sizes: Optional[List[int]] = None  # pattern sizes trained on (None: no restriction)
         self.history: Dict[str, list] = {}
@@ -136,7 +141,7 @@ class LearnedOrder:
     def save(self, path: str):
         np.savez(path, params=self.mlp.params(), mu=self.mu, sd=self.sd,
                  meta=np.array(json.dumps({"t_sd": self.t_sd, "threshold": self.threshold, "margin": self.margin,
-                                           "sizes": self.sizes})))
+                                           "judge_bits": self.judge_bits, "sizes": self.sizes})))
 
     @classmethod
     def load(cls, path: str) -> "LearnedOrder":
@@ -148,6 +153,7 @@ class LearnedOrder:
         m.t_sd = meta["t_sd"]
         m.threshold = meta["threshold"]
         m.margin = meta.get("margin", m.margin)
+        m.judge_bits = meta.get("judge_bits", m.judge_bits)
         m.sizes = meta.get("sizes")
         return m
 
@@ -165,39 +171,88 @@ def load(path: str) -> LearnedOrder:
     return LearnedOrder.load(path)
 
 
+JUDGE_SEED = 1  # seed of the sampled cost model's sampling (one seed: one judgement)
+
+
+def _proposal(model: LearnedOrder, pattern: Pattern, graph: Graph, mode: int, fallback: Plan, top: int):
+    """The network's pick among the planner's `top` candidate orders, or None
+    (the planner's order) when the pick is not predicted `margin` bits better
+    or the pattern size is outside the training sizes."""
+    if model.sizes and pattern.k not in model.sizes:
+        return _proposal_from(model, pattern, fallback, None, None)
+    cands, X = _features(pattern, graph, mode, top)
+    return _proposal_from(model, pattern, fallback, cands, X)
+
+
+def _judge(model: LearnedOrder, pattern: Pattern, graph: Graph, mode: int, fallback: Plan, order, info):
+    """The reference's sampled cost model judges the pick: more than
+    `judge_bits` above the planner's order -> the planner's plan."""
+    c_new = order_cost_sampled(pattern, graph, order, mode, JUDGE_SEED)
+    c_old = order_cost_sampled(pattern, graph, info["cost_order"], mode, JUDGE_SEED)
+    info["judge_bits_over"] = float(np.log2(max(c_new, 1.0)) - np.log2(max(c_old, 1.0)))
+    info["judge_costs"] = [c_old, c_new]
+    if info["judge_bits_over"] > model.judge_bits:
+        info["fallback"] = True
+        return fallback, info
+    return build_plan(pattern, graph, mode, order), info
+
+
 def choose_plan(model: LearnedOrder, pattern: Pattern, graph: Graph, mode: int, fallback: Optional[Plan] = None,
                 top: int = 12):
-    """Order with the lowest predicted work among the cost model's `top`
+    """Order with the lowest predicted work among the planner's `top`
     candidates, taken only when the network predicts it at least `margin` bits
-    below the cost model's own order.  The cost model stays the fallback: its
-    plan is kept when the learned order's analytic cost exceeds the cost-model
-    order's by more than the model's threshold (bits), i.e. when the two
-    disagree that much."""
+    below the planner's own order and the reference's sampled cost model does
+    not put it more than `judge_bits` above that order (the fallback)."""
     if fallback is None:
         fallback = build_plan(pattern, graph, mode)
+    order, info = _proposal(model, pattern, graph, mode, fallback, top)
+    if order is None:
+        return fallback, info
+    return _judge(model, pattern, graph, mode, fallback, order, info)
+
+
+def choose_plans(model: LearnedOrder, patterns: Sequence[Pattern], graph: Graph, mode: int,
+                 fallbacks: Sequence[Plan], top: int = 12, threads: int = 16):
+    """choose_plan over a pattern set: feature sampling and the sampled cost
+    model (host code, the GIL released in the library) run on `threads` threads;
+    the network's forward passes stay on the caller's thread."""
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        feats = list(ex.map(lambda p: ((None, None) if model.sizes and p.k not in model.sizes
+                                       else _features(p, graph, mode, top)), patterns))
+        props = []
+        for p, fb, (cands, X) in zip(patterns, fallbacks, feats):
+            props.append(_proposal_from(model, p, fb, cands, X))
+        judged = list(ex.map(lambda a: (a[1] if a[0] is None else
+                                        _judge(model, a[2], graph, mode, a[3], a[0], a[1])),
+                             [(o, inf, p, fb) for (o, inf), p, fb in zip(props, patterns, fallbacks)]))
+    out = []
+    for (o, inf), fb, j in zip(props, fallbacks, judged):
+        out.append((fb, inf) if o is None else j)
+    return out
+
+
+def _features(pattern: Pattern, graph: Graph, mode: int, top: int):
+    cands = candidate_orders(pattern, graph, mode, top)
+    return cands, np.stack([order_features(pattern, graph, o, mode, FEATURE_SAMPLES, 11) for o in cands])
+
+
+def _proposal_from(model: LearnedOrder, pattern: Pattern, fallback: Plan, cands, X):
+    cost_order = list(fallback.order)[: pattern.k]
+    info = {"pred_rel_log2": 0.0, "cost_order": cost_order, "learned_order": cost_order, "fallback": False,
+            "judge_bits_over": 0.0}
     if model.sizes and pattern.k not in model.sizes:
         # outside the pattern sizes the network was trained on its ranking is
         # unfounded (5-vertex house in IEP mode: a 3.1x slower order)
-        order = list(fallback.order)[: pattern.k]
-        return fallback, {"pred_rel_log2": 0.0, "cost_order": order, "learned_order": order, "fallback": True,
-                          "cost_bits_over": 0.0, "reason": "pattern size outside the training sizes"}
-    cands = candidate_orders(pattern, graph, mode, top)
-    X = np.stack([order_features(pattern, graph, o, mode, FEATURE_SAMPLES, 11) for o in cands])
+        info.update(fallback=True, reason="pattern size outside the training sizes")
+        return None, info
     pred = model.predict_rel_log2(X)
     best = int(np.argmin(pred))
-    cost_order = list(fallback.order)[: pattern.k]
     ci = cands.index(cost_order) if cost_order in cands else None
-    learned_cost = order_cost(pattern, graph, cands[best], mode)
-    info = {"pred_rel_log2": float(pred[best] - (pred[ci] if ci is not None else 0.0)),
-            "cost_order": cost_order, "learned_order": list(cands[best]), "fallback": False,
-            "cost_bits_over": float(np.log2(max(learned_cost, 1.0)) - np.log2(max(fallback.cost, 1.0)))}
+    info["pred_rel_log2"] = float(pred[best] - (pred[ci] if ci is not None else 0.0))
     if ci is not None and pred[ci] - pred[best] < model.margin:
-        info["learned_order"] = cost_order  # not a clear enough improvement
-        return fallback, info
-    if info["cost_bits_over"] > model.threshold:
-        info["fallback"] = True
-        return fallback, info
-    return build_plan(pattern, graph, mode, cands[best]), info
+        return None, info
+    info["learned_order"] = list(cands[best])
+    return list(cands[best]), info
 
 
 def evaluate(model: "LearnedOrder", graph: Graph, patterns: Sequence[Pattern], mode: int, stride: int = 1,
@@ -250,7 +305,7 @@ def log_pairs(graph: Graph, patterns: Sequence[Pattern], mode: int, stride: int 
     stopped by its work cap is logged CAPPED_PENALTY bits above its
     (lower-bound) time; a pattern whose pilot order is stopped is skipped."""
     rng = np.random.default_rng(seed)
-    X, y, c, cap, grp = [], [], [], [], []
+    X, y, c, cap, grp, ords = [], [], [], [], [], []
     t0 = time.time()
     for i, p in enumerate(patterns):
         orders = candidate_orders(p, graph, mode, orders_per_pattern // 2, rng)[:orders_per_pattern]
@@ -260,6 +315,7 @@ def log_pairs(graph: Graph, patterns: Sequence[Pattern], mode: int, stride: int 
             continue
         ms, by, capped, used = r
         for o, t, cp in zip(orders, ms, capped):
+            ords.append(list(o) + [255] * (8 - len(o)))
             X.append(order_features(p, graph, o, mode, FEATURE_SAMPLES, 11))
             y.append(np.log2(max(t, 1.0)) + (CAPPED_PENALTY if cp else 0.0))
             c.append(np.log2(max(order_cost(p, graph, o, mode), 1.0)))
@@ -269,7 +325,7 @@ def log_pairs(graph: Graph, patterns: Sequence[Pattern], mode: int, stride: int 
             f"projected Gcycles {float(np.min(ms)) / 1e9:.1f}..{float(np.max(ms)) / 1e9:.1f}, capped {int(np.sum(capped))}, "
             f"{time.time() - t0:.0f}s")
     return (np.array(X, np.float32), np.array(y, np.float64), np.array(c, np.float64), np.array(cap, bool),
-            np.array(grp, np.int64))
+            np.array(grp, np.int64), np.array(ords, np.uint8).reshape(-1, 8))
 
 
 def fit_model(X: np.ndarray, y: np.ndarray, groups: np.ndarray, epochs: int = 300, seed: int = 0, log=print):
@@ -284,7 +340,7 @@ def fit_model(X: np.ndarray, y: np.ndarray, groups: np.ndarray, epochs: int = 30
 def train(graph: Graph, patterns: Sequence[Pattern], mode: int, stride: int = 1024, orders_per_pattern: int = 8,
           epochs: int = 300, seed: int = 0, log=print):
     """Log (order, measured work) pairs with the plain executor and fit the MLP."""
-    X, y, c, cap, grp = log_pairs(graph, patterns, mode, stride, orders_per_pattern, seed, log)
+    X, y, c, cap, grp, _ = log_pairs(graph, patterns, mode, stride, orders_per_pattern, seed, log)
     model = fit_model(X, y, grp, epochs, seed, log)
     model.sizes = sorted({p.k for p in patterns})
     return model, (X, y, c, grp)

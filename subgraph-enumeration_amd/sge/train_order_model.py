@@ -36,6 +36,16 @@ import sge.workloads as W  # noqa: E402
 from sge import learned as L  # noqa: E402
 
 
+# graphs (order, work) pairs are logged on: the Chung-Lu benchmark graph and
+# R-MAT graphs of the benchmark's parameters at other scales and seeds (none of
+# them the graph of a benchmark configuration the model is evaluated on)
+LOG_GRAPHS = {
+    "c2": lambda: W.graph_c2(),
+    "rmat12": lambda: W.graph_rmat(12, 11, device_build=True),
+    "rmat14": lambda: W.graph_rmat(14, 12, device_build=True),
+}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", default=os.path.join(HERE, "models", "order_mlp.npz"))
@@ -46,6 +56,8 @@ def main():
     ap.add_argument("--mode", default="iep", choices=["wcoj", "explore", "iep"])
     ap.add_argument("--epochs", type=int, default=300)
     ap.add_argument("--log", default=None)
+    ap.add_argument("--graph", default="c2", choices=sorted(LOG_GRAPHS),
+                    help="data graph the (order, work) pairs are logged on")
     ap.add_argument("--log-only", default=None, help="log (order, work) pairs to this .npz and stop")
     ap.add_argument("--first", type=int, default=0, help="first training pattern to log")
     ap.add_argument("--count", type=int, default=None, help="number of training patterns to log")
@@ -67,6 +79,8 @@ def main():
     g = W.graph_c2()
     g.to_device()
     log(f"graph n={g.n} m={g.m} in {time.time() - t0:.1f}s")
+    lg = LOG_GRAPHS[args.graph]()
+    lg.to_device()
     if args.eval_only:
         model = L.load(args.out)
         rows = L.evaluate(model, g, W.six_vertex_set(), mode, args.eval_stride, log=log)
@@ -94,10 +108,11 @@ def main():
         log(f"loaded {len(X)} logged orders of {len(np.unique(grp))} patterns from {args.data}")
     else:
         sel = train_p[args.first:args.first + args.count if args.count else None]
-        X, y, c, cap, grp = L.log_pairs(g, sel, mode, stride=args.stride, orders_per_pattern=args.orders,
-                                        seed=args.first, log=log)
+        X, y, c, cap, grp, ords = L.log_pairs(lg, sel, mode, stride=args.stride, orders_per_pattern=args.orders,
+                                              seed=args.first, log=log)
         if args.log_only:
-            np.savez(args.log_only, X=X, y=y, c=c, capped=cap, group=grp)
+            np.savez(args.log_only, X=X, y=y, c=c, capped=cap, group=grp, orders=ords,
+                     graph=np.array(args.graph), first=np.array(args.first))
             log(f"wrote {len(X)} orders to {args.log_only}")
             return
     model = L.fit_model(X, y, grp, epochs=args.epochs, log=log)

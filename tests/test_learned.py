@@ -69,6 +69,38 @@ def test_gpu_forward_and_gradients_match_reference():
 
 
 @pytest.mark.gpu
+def test_gpu_adam_steps_match_numpy_update():
+    """Two Adam steps on the GPU (adam_kernel) against the textbook update in
+    NumPy from the same gradients: m = b1 m + (1-b1) g, v = b2 v + (1-b2) g^2,
+    p -= lr (m / (1 - b1^t)) / (sqrt(v / (1 - b2^t)) + eps), with b1 = 0.9,
+    b2 = 0.999, eps = 1e-8; the BF16 weight copies the forward pass reads are
+    refreshed from the new master weights."""
+    rng = np.random.default_rng(7)
+    m = sge.MLP(seed=1)
+    p = rand_params(8)
+    m.set_params(p)
+    b1, b2, eps, lr = 0.9, 0.999, 1e-8, 3e-3
+    mom = np.zeros_like(p, np.float64)
+    vel = np.zeros_like(p, np.float64)
+    want = p.astype(np.float64)
+    for step in (1, 2):
+        X = rng.normal(size=(48, 64)).astype(np.float32)
+        t = rng.normal(size=48).astype(np.float32)
+        g, _ = m.gradients(X, t)  # the gradients train_step applies (same batch, same weights)
+        g = g.astype(np.float64)
+        m.train_step(X, t, lr)
+        mom = b1 * mom + (1 - b1) * g
+        vel = b2 * vel + (1 - b2) * g * g
+        want = want - lr * (mom / (1 - b1 ** step)) / (np.sqrt(vel / (1 - b2 ** step)) + eps)
+        got = m.params().astype(np.float64)
+        assert np.abs(got - want).max() <= 1e-5 * max(1.0, np.abs(want).max()), step
+        # the forward pass reads BF16 copies refreshed from the updated weights
+        yr, _ = R.forward(got.astype(np.float32), X)
+        y = m.forward(X)
+        assert np.allclose(y, yr, rtol=2e-2, atol=2e-2 * np.abs(yr).max()), step
+
+
+@pytest.mark.gpu
 def test_gpu_training_reduces_loss():
     rng = np.random.default_rng(5)
     X = rng.normal(size=(512, 64)).astype(np.float32)
