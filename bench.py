@@ -34,8 +34,12 @@ from its own sources into oracle/_ref/) on all host cores beside it.
 
 Other whole-job workloads: `--workload sixv|house|rmat8` (the C3 pass alone,
 house + K5-e over the Chung-Lu graph, the 78 patterns over R-MAT scale 8).
+Measurement sweeps that print their own JSON: `--workload bags` (plans with and
+without a materialised bag) and `--workload graph_build` (host against GPU CSR
+build).
 
-    python bench.py [--steps K] [--warmup W] [--workload c4|sixv|house|rmat8] [--span S] [--no-c3] [--no-cpu]
+    python bench.py [--steps K] [--warmup W] [--workload c4|sixv|house|rmat8|bags|graph_build] [--span S]
+                    [--no-c3] [--no-cpu]
 """
 import argparse
 import json
@@ -246,6 +250,10 @@ def c4_subsets(args):
     ranges = (begins, ends)
     root_edges = int((ends - begins).sum())
 
+    from sge import autotune
+
+    tuning = autotune.tuned(wname)  # batched executors only; plain keeps the library defaults
+    tuning_desc = {k: getattr(tuning, k) for k, _ in sge.Tuning._fields_} if tuning else "library defaults"
     plain_counts = plain_ms = plain_kms = bcost_counts = bcost_ms = bcost_kms = None
     plain = plain_runner(g, plans_cost, ranges)
     if not args.no_plain:
@@ -253,13 +261,13 @@ def c4_subsets(args):
         if span == 1 and plain_counts != [e["count"] for e in subs]:
             log("COUNT MISMATCH: plain executor against the oracle's counts")
             sys.exit(3)
-        bc = sge.Batch(g, plans_cost, ranges=ranges)
+        bc = sge.Batch(g, plans_cost, tuning=tuning, ranges=ranges)
         bcost_counts, bcost_ms, bcost_kms = timed_steps(batch_runner(bc), args.warmup, args.steps, "batched_cost")
         bc.close()
         if bcost_counts != plain_counts:
             log("COUNT MISMATCH between plain and batched executors (same plans)")
             sys.exit(3)
-    b = sge.Batch(g, plans_fast, ranges=ranges)
+    b = sge.Batch(g, plans_fast, tuning=tuning, ranges=ranges)
     brun = batch_runner(b)
     counts, step_ms, kernel_ms = timed_steps(brun, args.warmup, args.steps, "batched")
     st = brun.stats
@@ -279,7 +287,7 @@ def c4_subsets(args):
                                f"of {2 * g.m})",
                    "mode": "iep", "graph": doc["graph"], "n": g.n, "m": g.m, "patterns": len(subs),
                    "span": span, "root_edges": root_edges, "learned_order": learned, "learned_changed": changed,
-                   "tuning": "library defaults"},
+                   "tuning": tuning_desc},
         "pattern_set_ms": {"plain": plain_ms, "plain_kernel": plain_kms, "batched_cost_order": bcost_ms,
                            "batched_cost_order_kernel": bcost_kms, "batched": step_ms, "batched_kernel": kernel_ms,
                            "speedup_vs_plain": plain_ms / step_ms if plain_ms else None,
@@ -368,6 +376,72 @@ def whole_job(args, name):
     return res
 
 
+def bag_times(args):
+    """Plans with and without a materialised bag (plain executor, one launch per
+    plan): for each pattern the cost model's plan without a bag and with every
+    valid bag of at most 4 vertices; a bag's time is its trie build (enumerate
+    the bag relation, radix sort, per-depth scans) plus the join.  Counts must
+    be identical.  Sets: the 16 six-vertex patterns over the Chung-Lu graph
+    n=7,115 m=100,762, and every fourth of the 78 seven-vertex patterns over
+    R-MAT scale 12 (seed 4), both in IEP mode."""
+    def best_time(g, plan):
+        best = None
+        for _ in range(max(1, args.steps)):
+            c, st = sge.count(g, plan)
+            tot = st.kernel_ms + st.trie_ms
+            if best is None or tot < best[0]:
+                best = (tot, c[0], st.trie_ms)
+        return best
+
+    out = {"workload": "bags", "sets": {}}
+    for label, g, pats in (("chunglu7115_sixvertex16", W.graph_c2(), W.six_vertex_set()),
+                           ("rmat12_sevenvertex_every4th", W.graph_rmat(12, 4), W.seven_vertex_set()[::4])):
+        g.to_device()
+        res = []
+        for i, p in enumerate(pats):
+            plan = sge.build_plan(p, g, sge.MODE_IEP)
+            base = best_time(g, plan)
+            rows = []
+            for b in sge.valid_bags(plan):
+                if bin(b).count("1") > 4:
+                    continue
+                try:
+                    r = best_time(g, sge.set_bag(plan, b))
+                except sge.SgeError as e:  # relation above the tuple cap
+                    rows.append({"bag": b, "error": str(e)})
+                    continue
+                if r[1] != base[1]:
+                    log(f"{label} pattern {i} bag {b}: count {r[1]} != {base[1]}")
+                    sys.exit(3)
+                rows.append({"bag": b, "ms": r[0], "trie_ms": r[2]})
+            timed = [r for r in rows if "ms" in r]
+            best = min(timed, key=lambda r: r["ms"]) if timed else None
+            log(f"{label} pattern {i}: no bag {base[0]:.3f} ms | best bag {best} of {len(rows)}")
+            res.append({"pattern": i, "no_bag_ms": base[0], "count": base[1], "bags": rows})
+        out["sets"][label] = res
+    return out
+
+
+def graph_build_times(args):
+    """The CSR build on the host and on the GPU for R-MAT scales 18, 20, 21
+    (seed 4): both deduplicate, drop self-loops, relabel by ascending degree
+    and build ascending rows; the arrays must be identical.  Edge generation
+    (host) is not part of either figure."""
+    out = {"workload": "graph_build", "scales": []}
+    for scale in (18, 20, 21):
+        h = sge.Graph.rmat(scale, 16, seed=4)
+        d = sge.Graph.rmat(scale, 16, seed=4, device_build=True)
+        same = all(np.array_equal(a, b) for a, b in zip(h.csr(), d.csr())) and np.array_equal(h.perm(), d.perm())
+        log(f"rmat scale {scale}: n {h.n} m {h.m} | host build {h.stats.build_ms:.1f} ms | device build "
+            f"{d.stats.build_ms:.1f} ms | identical arrays: {same}")
+        if not same:
+            sys.exit(3)
+        out["scales"].append({"scale": scale, "n": h.n, "m": h.m, "host_ms": h.stats.build_ms,
+                              "device_ms": d.stats.build_ms, "identical": same})
+        del h, d
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -394,6 +468,10 @@ def main():
         res = c4_subsets(args)
         if not args.no_c3:
             res["c3_whole_job"] = whole_job(args, "sixv")
+    elif args.workload == "bags":
+        res = bag_times(args)
+    elif args.workload == "graph_build":
+        res = graph_build_times(args)
     else:
         res = whole_job(args, args.workload)
     print(json.dumps(res))

@@ -242,6 +242,59 @@ __device__ __forceinline__ bool contains(PT a, uint32_t lo, uint32_t hi, uint32_
     return p < hi && a[p] == key;
 }
 
+#ifndef SGE_PROBE_ARITY
+#define SGE_PROBE_ARITY 1
+#endif
+// Membership by a search of fan-out 4 while the range is long: three
+// independent pivot loads per step (issued together, one round trip) cut the
+// range to a quarter, so a search of a row in L2 takes about half the
+// dependent round trips of a binary search; the last few elements are searched
+// as before.
+template <typename PT>
+__device__ __forceinline__ uint32_t lower_bound_wide(PT a, uint32_t lo, uint32_t hi, uint32_t key) {
+    if (SGE_PROBE_ARITY == 4) {
+        // invariant: a[lo - 1] < key (or lo is the start) and a[hi] >= key (or hi is the end)
+        while (hi - lo > 16) {
+            const uint32_t q = (hi - lo) >> 2, m1 = lo + q, m2 = m1 + q, m3 = m2 + q;
+            const uint32_t v1 = a[m1], v2 = a[m2], v3 = a[m3];
+            if (key <= v1) {
+                hi = m1;
+            } else if (key <= v2) {
+                lo = m1 + 1;
+                hi = m2;
+            } else if (key <= v3) {
+                lo = m2 + 1;
+                hi = m3;
+            } else {
+                lo = m3 + 1;
+            }
+        }
+    }
+    return lower_bound(a, lo, hi, key);
+}
+
+template <typename PT>
+__device__ __forceinline__ bool contains_wide(PT a, uint32_t lo, uint32_t hi, uint32_t key) {
+    if (SGE_PROBE_ARITY == 4) {
+        while (hi - lo > 16) {
+            const uint32_t q = (hi - lo) >> 2, m1 = lo + q, m2 = m1 + q, m3 = m2 + q;
+            const uint32_t v1 = a[m1], v2 = a[m2], v3 = a[m3];
+            if (key < v1) {
+                hi = m1;
+            } else if (key < v2) {
+                lo = m1;
+                hi = m2;
+            } else if (key < v3) {
+                lo = m2;
+                hi = m3;
+            } else {
+                lo = m3;
+            }
+        }
+    }
+    return contains(a, lo, hi, key);
+}
+
 // Clamp [b,e) of a sorted list to the open interval (lo, hi) (setBeginPos /
 // setEndPos).  Every lane computes the same result.
 template <typename PT>
@@ -349,12 +402,12 @@ __device__ __forceinline__ uint32_t wave_intersect(WaveLds &w, int ns, int nchk,
                     p = c0 + pos;
                     in = w.fst[pos] == x;  // pos < n: fst[n - 1] >= x
                 } else {
-                    p = lower_bound(a, c0 + n, e, x);
+                    p = lower_bound_wide(a, c0 + n, e, x);
                     in = p < e && a[p] == x;
                 }
                 wave_sync();  // the next source is staged over these words
             } else {
-                p = lower_bound(a, c0, e, x);
+                p = lower_bound_wide(a, c0, e, x);
                 in = p < e && a[p] == x;
             }
             ok = ok && in;
@@ -532,7 +585,7 @@ __device__ __forceinline__ bool term_has(const WaveLds &w, const TermLds &t, con
         return (word >> (r & 31u)) & 1u;
     }
     if (t.flags & 8u) return contains(&w.fst[t.st_off], tlo - t.F.b, thi - t.F.b, x);
-    return contains(t.F.ptr, tlo, thi, x);
+    return contains_wide(t.F.ptr, tlo, thi, x);
 }
 
 // Inclusion-exclusion over set partitions of the tail (phase 3 of count_last):
@@ -836,7 +889,7 @@ __device__ __forceinline__ bool count_last(const KParams &P, WaveLds &w, TermLds
                     hit = term_has(w, t, gb, x, en.z, en.w);
                 } else {
                     uint32_t x = t.F.ptr[en.y + off];
-                    hit = contains(P.nbr, en.z, en.w, x);
+                    hit = contains_wide(P.nbr, en.z, en.w, x);
                 }
                 if (hit) {
                     if (per_entry) atomicAdd(&w.cnt[a], 1u);

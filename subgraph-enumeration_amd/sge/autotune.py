@@ -35,8 +35,8 @@ def _tuning(d: Dict[str, int]) -> Tuning:
     return t
 
 
-def _time(graph: Graph, plans: Sequence[Plan], stride: int, d: Dict[str, int], reps: int = 2):
-    b = Batch(graph, plans, _tuning(d), stride=stride)
+def _time(graph: Graph, plans: Sequence[Plan], stride: int, d: Dict[str, int], reps: int = 2, ranges=None):
+    b = Batch(graph, plans, _tuning(d), stride=stride, ranges=ranges)
     try:
         counts, st = b.run()  # warm
         best = float("inf")
@@ -50,7 +50,10 @@ def _time(graph: Graph, plans: Sequence[Plan], stride: int, d: Dict[str, int], r
 
 
 def tune(graph: Graph, plans: Sequence[Plan], stride: int, rounds: int = 1, log=print,
-         cache: Optional[str] = None, key: Optional[str] = None, budget_s: float = 120.0) -> Dict[str, int]:
+         cache: Optional[str] = None, key: Optional[str] = None, budget_s: float = 120.0,
+         ranges=None) -> Dict[str, int]:
+    """Coordinate descent from the library defaults; `ranges` = per-plan root
+    ranges (begins, ends) instead of a strided sample."""
     if cache and key and os.path.exists(cache):
         with open(cache) as f:
             data = json.load(f)
@@ -58,7 +61,7 @@ def tune(graph: Graph, plans: Sequence[Plan], stride: int, rounds: int = 1, log=
             return data[key]["params"]
     cur = {k: v[0] for k, v in SPACE.items()}
     t0 = time.time()
-    best_ms, ref = _time(graph, plans, stride, cur)
+    best_ms, ref = _time(graph, plans, stride, cur, ranges=ranges)
     log(f"[autotune] start {cur}: {best_ms:.2f} ms")
     trace = [{"params": dict(cur), "ms": best_ms}]
     for _ in range(rounds):
@@ -67,7 +70,7 @@ def tune(graph: Graph, plans: Sequence[Plan], stride: int, rounds: int = 1, log=
                 if v == cur[name] or time.time() - t0 > budget_s:
                     continue
                 trial = dict(cur, **{name: v})
-                ms, counts = _time(graph, plans, stride, trial)
+                ms, counts = _time(graph, plans, stride, trial, ranges=ranges)
                 if counts != ref:
                     raise RuntimeError(f"autotune: counts changed with {trial}")
                 trace.append({"params": trial, "ms": ms})
@@ -96,37 +99,57 @@ def tuned(key: str, cache: str = CACHE) -> Optional[Tuning]:
 
 
 def main():
-    """Tune the batched executor for the 16 six-vertex patterns over the
-    Chung-Lu graph (n=7,115, m=100,762), IEP mode, whole job, with the orders
-    the batch runs (the learned model's when it is present), and cache the
-    result under the workload's name."""
+    """Tune the batched executor for a benchmark workload with the orders the
+    batch runs (the learned model's when it is present) and cache the result
+    under the workload's name:
+      sixv  the 16 six-vertex patterns over the Chung-Lu graph (n=7,115,
+            m=100,762), IEP mode, whole job
+      c4    the 78 seven-vertex patterns over R-MAT scale 20 (seed 4), IEP
+            mode, over the per-pattern root ranges of tests/golden/large_c4.json"""
     import argparse
+
+    import numpy as np
 
     from . import MODE_IEP, build_plan
     from . import workloads as W
 
     ap = argparse.ArgumentParser()
+    ap.add_argument("--workload", default="sixv", choices=["sixv", "c4"])
     ap.add_argument("--stride", type=int, default=1)
     ap.add_argument("--rounds", type=int, default=1)
     ap.add_argument("--budget-s", type=float, default=240.0)
     args = ap.parse_args()
-    g = W.graph_c2()
-    g.to_device()
-    plans = [build_plan(p, g, MODE_IEP) for p in W.six_vertex_set()]
+    ranges = None
+    if args.workload == "sixv":
+        key = "chunglu7115_sixvertex16"
+        g = W.graph_c2()
+        g.to_device()
+        pats = W.six_vertex_set()
+        plans = [build_plan(p, g, MODE_IEP) for p in pats]
+    else:
+        key = "rmat20_sevenvertex78_subsets"
+        root = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+        with open(os.path.join(root, "tests", "golden", "large_c4.json")) as f:
+            subs = json.load(f)["subsets"]
+        g = W.graph_c4(device_build=True)
+        all7 = W.seven_vertex_set()
+        pats = [all7[e["pattern"]] for e in subs]
+        plans = [build_plan(p, g, MODE_IEP, e["plan"]["order"]) for p, e in zip(pats, subs)]
+        ranges = (np.array([e["begin"] for e in subs], np.uint64), np.array([e["end"] for e in subs], np.uint64))
     model_path = os.path.join(os.path.dirname(CACHE), "order_mlp.npz")
     if os.path.exists(model_path):
         from . import learned as L
 
         model = L.load(model_path)
-        plans = [L.choose_plan(model, p, g, MODE_IEP, fallback=c)[0] for p, c in zip(W.six_vertex_set(), plans)]
+        plans = [pl for pl, _ in L.choose_plans(model, pats, g, MODE_IEP, plans)]
     if os.path.exists(CACHE):
         with open(CACHE) as f:
             data = json.load(f)
-        data.pop("chunglu7115_sixvertex16", None)
+        data.pop(key, None)
         with open(CACHE, "w") as f:
             json.dump(data, f, indent=1)
-    tune(g, plans, args.stride, args.rounds, log=lambda *a: print(*a, flush=True), cache=CACHE,
-         key="chunglu7115_sixvertex16", budget_s=args.budget_s)
+    tune(g, plans, args.stride, args.rounds, log=lambda *a: print(*a, flush=True), cache=CACHE, key=key,
+         budget_s=args.budget_s, ranges=ranges)
 
 
 if __name__ == "__main__":

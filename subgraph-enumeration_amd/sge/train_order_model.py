@@ -46,6 +46,55 @@ LOG_GRAPHS = {
 }
 
 
+# evaluation sets: the whole job of each, exact, with the same executor for
+# both choices; the Chung-Lu graph is one the pairs are logged on (its
+# patterns are not), R-MAT scale 8 (seed 7) is a graph no pair is logged on
+EVAL_SETS = {
+    "c3": ("chung_lu(n=7115, m=100762, gamma=2.1, cap=1100, seed=1), 16 x 6-vertex", lambda: W.graph_c2(),
+           W.six_vertex_set),
+    "rmat8": ("rmat(scale=8, ef=16, seed=7), 78 x 7-vertex", lambda: W.graph_rmat(8, 7, device_build=True),
+              W.seven_vertex_set),
+}
+
+
+def evaluate_sets(model, names, mode, stride, log):
+    """Per evaluation set: the learned choice against the planner's choice (whole
+    job), and the judges' ranking quality: Spearman correlation of measured time
+    with the network's prediction, the planner's analytic cost and the
+    reference's sampled cost model over each pattern's two orders."""
+    out = {}
+    for name in names:
+        desc, gfn, pfn = EVAL_SETS[name]
+        g = gfn()
+        g.to_device()
+        pats = pfn()
+        rows = L.evaluate(model, g, pats, mode, stride, log=log)
+        t_cost = sum(r["cost_ms"] for r in rows)
+        t_learn = sum(r["learned_ms"] for r in rows)
+        ratios = [r["learned_ms"] / max(r["cost_ms"], 1e-6) for r in rows]
+        changed = [r for r in rows if r["learned_order"] != r["cost_order"] and not r["fallback"]]
+        # on the changed patterns: does each judge order the two choices as measured?
+        agree = {"network": 0, "analytic": 0, "sampled": 0}
+        for r in changed:
+            p = pats[r["pattern"]]
+            faster_learned = r["learned_ms"] < r["cost_ms"]
+            agree["network"] += int((r["pred_rel_log2"] < 0) == faster_learned)
+            ca = sge.order_cost(p, g, r["learned_order"], mode) < sge.order_cost(p, g, r["cost_order"], mode)
+            agree["analytic"] += int(ca == faster_learned)
+            cs = (sge.order_cost_sampled(p, g, r["learned_order"], mode, L.JUDGE_SEED) <
+                  sge.order_cost_sampled(p, g, r["cost_order"], mode, L.JUDGE_SEED))
+            agree["sampled"] += int(cs == faster_learned)
+        out[name] = {"set": desc, "eval_stride": stride, "total_ms": {"cost_model": t_cost, "learned": t_learn,
+                                                                     "speedup": t_cost / t_learn if t_learn else None},
+                     "max_slowdown": max(ratios), "changed": len(changed),
+                     "fallbacks": int(sum(r["fallback"] for r in rows)), "capped": int(sum(r["capped"] for r in rows)),
+                     "judges_agree_with_measured_on_changed": agree, "per_pattern": rows}
+        log(f"[eval {name}] cost-model {t_cost:.1f} ms, learned {t_learn:.1f} ms "
+            f"({t_cost / t_learn:.2f}x), worst pattern {max(ratios):.2f}x slower, {len(changed)} changed, "
+            f"judges agree {agree}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", default=os.path.join(HERE, "models", "order_mlp.npz"))
@@ -62,6 +111,7 @@ def main():
     ap.add_argument("--first", type=int, default=0, help="first training pattern to log")
     ap.add_argument("--count", type=int, default=None, help="number of training patterns to log")
     ap.add_argument("--data", default=None, help="comma-separated .npz files of logged pairs to fit from")
+    ap.add_argument("--eval", default="c3,rmat8", help="evaluation sets: " + ", ".join(sorted(EVAL_SETS)))
     ap.add_argument("--eval-only", action="store_true",
                     help="evaluate the model at --out on the benchmark set and rewrite its report; no training")
     args = ap.parse_args()
@@ -83,19 +133,11 @@ def main():
     lg.to_device()
     if args.eval_only:
         model = L.load(args.out)
-        rows = L.evaluate(model, g, W.six_vertex_set(), mode, args.eval_stride, log=log)
         rep_path = os.path.splitext(args.out)[0] + "_report.json"
         report = json.load(open(rep_path)) if os.path.exists(rep_path) else {}
-        t_cost = sum(r["cost_ms"] for r in rows)
-        t_learn = sum(r["learned_ms"] for r in rows)
-        report.update({"eval_stride": args.eval_stride, "per_pattern": rows,
-                       "eval_total_ms": {"cost_model": t_cost, "learned": t_learn,
-                                         "speedup": t_cost / t_learn if t_learn else None},
-                       "fallbacks": int(sum(r["fallback"] for r in rows)),
-                       "capped": int(sum(r["capped"] for r in rows))})
+        report["eval"] = evaluate_sets(model, args.eval.split(","), mode, args.eval_stride, log)
         with open(rep_path, "w") as f:
             json.dump(report, f, indent=1)
-        log("total cost-model %.1f ms, learned %.1f ms" % (t_cost, t_learn))
         return
     bench_codes = [p.info().canon for p in W.six_vertex_set()]
     train_p = W.training_set(args.patterns) + sge.sample_patterns(6, 8, 10, 80, 5, bench_codes)
@@ -105,8 +147,10 @@ def main():
         y = np.concatenate([d["y"] for d in parts])
         c = np.concatenate([d["c"] for d in parts])
         grp = np.concatenate([d["group"] + 1_000_000 * k for k, d in enumerate(parts)])
+        graphs = [str(d["graph"]) if "graph" in d.files else "c2" for d in parts]
         log(f"loaded {len(X)} logged orders of {len(np.unique(grp))} patterns from {args.data}")
     else:
+        graphs = [args.graph]
         sel = train_p[args.first:args.first + args.count if args.count else None]
         X, y, c, cap, grp, ords = L.log_pairs(lg, sel, mode, stride=args.stride, orders_per_pattern=args.orders,
                                               seed=args.first, log=log)
@@ -123,23 +167,15 @@ def main():
            "train_corr_rel": float(np.corrcoef(pred, rel)[0, 1]),
            "cost_model_corr_rel": float(np.corrcoef(L.relative_targets(c, grp), rel)[0, 1])}
     log("fit", fit)
-    # evaluate on the benchmark set
-    rows = L.evaluate(model, g, W.six_vertex_set(), mode, args.eval_stride, log=log)
-    t_cost = sum(r["cost_ms"] for r in rows)
-    t_learn = sum(r["learned_ms"] for r in rows)
-    report = {"graph": "chung_lu(n=7115, m=100762, gamma=2.1, cap=1100, seed=1)", "mode": args.mode,
-              "train_patterns": len(train_p), "train_orders": int(len(X)), "train_stride": args.stride,
-              "eval_stride": args.eval_stride, "fit": fit, "loss_first_last": [model.history["loss"][0],
-                                                                              model.history["loss"][-1]],
-              "eval_total_ms": {"cost_model": t_cost, "learned": t_learn,
-                                "speedup": t_cost / t_learn if t_learn else None},
-              "fallbacks": int(sum(r["fallback"] for r in rows)), "capped": int(sum(r["capped"] for r in rows)),
-              "per_pattern": rows,
-              "wall_s": time.time() - t0}
     model.save(args.out)
+    report = {"log_graphs": sorted(set(graphs)), "mode": args.mode, "train_patterns": len(train_p),
+              "train_orders": int(len(X)), "train_stride": args.stride, "fit": fit,
+              "loss_first_last": [model.history["loss"][0], model.history["loss"][-1]],
+              "best_epoch": model.history["best_epoch"],
+              "eval": evaluate_sets(model, args.eval.split(","), mode, args.eval_stride, log),
+              "wall_s": time.time() - t0}
     with open(os.path.splitext(args.out)[0] + "_report.json", "w") as f:
         json.dump(report, f, indent=1)
-    log("total cost-model %.1f ms, learned %.1f ms" % (t_cost, t_learn))
 
 
 if __name__ == "__main__":
