@@ -224,11 +224,6 @@ typedef struct sge_tuning {
                                  this many elements, is staged as a bitmap in a per-wave global
                                  region when its siblings' rows will probe it (0 = default 2048,
                                  1 << 30 = off) */
-    uint32_t steal;           /* 1: waves whose queue runs out take the continuation items other
-                                 waves donate within the same pass (one wave in 8 polls for them)
-                                 instead of leaving them all to the next pass; 0 (default): off.
-                                 Pays where a few heavy items make long pass tails; counts never
-                                 change. */
 } sge_tuning;
 
 typedef struct sge_stats {

@@ -170,25 +170,11 @@ struct KParams {
     uint32_t bitmap_ratio;             // the same when F is staged as a bitmap
     unsigned long long node_cap;       // stop after about this many wave steps of search work (0 = no cap)
     unsigned long long *node_ctr;      // wave steps spent so far, reported in blocks of >= NODE_STEP
-    // in-pass work stealing: a wave whose queue is exhausted takes this pass's
-    // continuation items as other waves write them (claimed through `steal`),
-    // until no wave is working (`active` = 0); `serial` numbers the passes of
-    // the control block, and an item's stamp word holds its pass's serial
-    // (~serial once taken in that pass)
-    unsigned long long *steal;
-    uint32_t *active;
-    const uint32_t *serial;
-    uint32_t steal_on;
 };
 
 // continuation item: plan, level l, absolute candidate range [b, e) of level l,
-// images of levels 0..l-1, and in its last word the stamp: the serial of the
-// pass that wrote it, stored after the other words (~serial once a wave took
-// it in that pass).  A slot whose stamp is not the previous pass's serial
-// holds no item for the next pass (never written, taken, or stale).
+// images of levels 0..l-1
 constexpr int ITEM_WORDS = 4 + MAXK;
-constexpr int ITEM_STAMP = ITEM_WORDS - 1;
-static_assert(4 + (MAXK - 2) <= ITEM_STAMP, "items hold at most k - 2 images before the stamp");
 
 // list descriptor: [ptr + b, ptr + e)
 typedef const __attribute__((address_space(1))) uint32_t guint;
@@ -1019,17 +1005,18 @@ __device__ __forceinline__ void donate(const KParams &P, WaveLds &w, uint32_t pl
     unsigned long long base = 0;
     if (lane == 0) base = atomicAdd(P.out_count, (unsigned long long)n);
     base = __shfl(base, 0);
-    // no room: the reserved slots stay unstamped (no item), the wave keeps the work
-    if (base + n > P.item_cap) return;
-    uint32_t serial = 0;  // this pass's serial (constant during the pass)
-    if (lane == 0) serial = __hip_atomic_load(P.serial, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    serial = __shfl(serial, 0);
+    if (base + n > P.item_cap) {
+        // no room: mark the reserved slots that exist as empty, keep the work
+        for (uint64_t slot = base; slot < P.item_cap && slot < base + n; ++slot)
+            if (lane == 0) P.out_items[slot * ITEM_WORDS] = NONE;
+        return;
+    }
     uint64_t slot = base;
     for (int j = l0; j <= d && j < stop; ++j) {
         uint32_t b = give_from(w, j, d), e = w.iend[j];
         if (b >= e) continue;
         uint32_t *it = P.out_items + slot * ITEM_WORDS;
-        if (lane < ITEM_STAMP) {
+        if (lane < ITEM_WORDS) {
             uint32_t v;
             if (lane == 0) v = plan;
             else if (lane == 1) v = (uint32_t)j;
@@ -1038,9 +1025,6 @@ __device__ __forceinline__ void donate(const KParams &P, WaveLds &w, uint32_t pl
             else v = (lane - 4 < j) ? w.phi[lane - 4] : 0u;
             it[lane] = v;
         }
-        // the item's words are visible at the device before its stamp
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        if (lane == 0) __hip_atomic_store(it + ITEM_STAMP, serial, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         w.iend[j] = b;  // this wave keeps [pos, b)
         ++slot;
     }
@@ -1113,118 +1097,6 @@ __device__ __forceinline__ bool run_dfs(const KParams &P, WaveLds &w, TermLds *t
     return false;
 }
 
-// Run one continuation item: lane q < ITEM_WORDS holds its word q in `v`.
-// Rebuilds the path (images of levels 0..l-1, their candidate sets, trie
-// positions and exploration's early intersections, all deterministic) and
-// searches level l's range.  Returns true when the work cap stopped it.
-template <bool ENUM, bool CAP>
-__device__ __forceinline__ bool run_item(const KParams &P, WaveLds &w, TermLds *terms, uint32_t *wb, uint32_t *gb,
-                                         Acc &acc, uint32_t &cur_plan, uint32_t v) {
-    const uint32_t plan = __builtin_amdgcn_readlane(v, 0);
-    if (plan >= P.nplans) return false;
-    if (plan != cur_plan) {
-        if (cur_plan != NONE) flush(P, cur_plan, acc);
-        cur_plan = plan;
-        load_plan(P, w, plan);
-    }
-    const DPlan &pl = w.pl;
-    const int l = (int)__builtin_amdgcn_readlane(v, 1);
-    const int stop = ENUM ? (int)pl.k - 2 : (int)pl.sib;
-    if (l < 2 || l >= stop) return false;  // never written by donate()
-    const uint32_t rb = __builtin_amdgcn_readlane(v, 2), re = __builtin_amdgcn_readlane(v, 3);
-    if (lane_id() >= 4 && lane_id() < 4 + l) w.phi[lane_id() - 4] = v;
-    wave_sync();
-    if (pl.mode == SGE_MODE_EXPLORE && !ENUM) {
-        explore_after(P, w, wb, 0, acc);
-        explore_after(P, w, wb, 1, acc);
-    }
-    // trie nodes of the path's bag levels (all present: the path was live)
-    bool live = true;
-    if ((pl.bag & 1u) && !trie_enter(w, 0, w.phi[0], NONE)) live = false;
-    if (live && (pl.bag & 2u) && !trie_enter(w, 1, w.phi[1], NONE)) live = false;
-    for (int j = 2; j <= l && live; ++j) {
-        materialize(P, w, wb, j, acc);
-        if (j < l && ((pl.bag >> j) & 1u) && !trie_enter(w, j, w.phi[j], NONE)) live = false;
-        if (j < l && pl.mode == SGE_MODE_EXPLORE && !ENUM) explore_after(P, w, wb, j, acc);
-    }
-    if (!live) return false;
-    w.pos[l] = max(rb, w.cand[l].b);
-    w.iend[l] = min(re, w.cand[l].e);
-    return run_dfs<ENUM, CAP>(P, w, terms, wb, gb, acc, plan, l, stop);
-}
-
-// Stealing is kept cheap for the waves still working: the device-scope atomics
-// an idle wave polls (steal cursor, item count, active count) live at one
-// address each, and on a multi-XCD part such loads queue at that address, so
-// only one wave in STEAL_STRIDE steals (the others exit when their queue runs
-// out), each polls about every 15 us, and gives up after STEAL_IDLE_LIMIT
-// polls without an item (about 20 ms; its pass's later items go to the next
-// pass).  A claimed slot's stamp is awaited at most 256 short sleeps.  Every
-// wait ends.  (Measured: every idle wave polling every 2 us for up to 0.5 s
-// made one-launch-per-pattern runs over R-MAT scale 20 subsets time out.)
-constexpr uint32_t STEAL_STRIDE = 8;
-constexpr uint32_t STEAL_IDLE_LIMIT = 1300;
-constexpr uint32_t STEAL_STAMP_LIMIT = 256;
-
-// In-pass work stealing (counting kernels), one attempt: a wave whose queue is
-// exhausted takes a continuation item another wave wrote in this pass.  A slot
-// is claimed by advancing `steal` (compare-and-swap: every slot is claimed
-// once), its stamp is awaited (bounded) and swapped from this pass's serial to
-// ~serial, and the item's words land in `v` (lane q: word q).  A slot whose
-// stamp does not come in time is left to the next pass, which runs every slot
-// still stamped with this pass's serial.  Returns 1 with an item, 0 to try
-// again, -1 when stealing is over: no wave is working (no item can come any
-// more), the work cap is reached, or the idle limit has passed.
-template <bool CAP>
-__device__ __forceinline__ int steal_step(const KParams &P, uint32_t serial, uint32_t &idle, uint32_t &v) {
-    const int lane = lane_id();
-    unsigned long long a = 0, s = 0, c = 0, spent = 0;
-    if (lane == 0) {
-        // `active` first: once it reads 0, every item written in this pass is counted in out_count
-        a = __hip_atomic_load(P.active, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-        s = __hip_atomic_load(P.steal, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-        c = __hip_atomic_load(P.out_count, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-        if (CAP) spent = __hip_atomic_load(P.node_ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    a = __shfl(a, 0);
-    s = __shfl(s, 0);
-    c = min(__shfl(c, 0), P.item_cap);
-    if (CAP && __shfl(spent, 0) >= P.node_cap) return -1;
-    if (s < c) {
-        int got = 0;
-        if (lane == 0) {
-            unsigned long long exp = s;
-            got = __hip_atomic_compare_exchange_strong(P.steal, &exp, s + 1, __ATOMIC_ACQ_REL, __ATOMIC_RELAXED,
-                                                       __HIP_MEMORY_SCOPE_AGENT);
-        }
-        if (!__shfl(got, 0)) return 0;
-        uint32_t *it = P.out_items + s * ITEM_WORDS;
-        int take = 0;
-        for (uint32_t t = 0; t < STEAL_STAMP_LIMIT; ++t) {
-            uint32_t st = 0;
-            if (lane == 0) st = __hip_atomic_load(it + ITEM_STAMP, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-            if (__shfl(st, 0) == serial) {
-                if (lane == 0) {
-                    uint32_t exp = serial;
-                    take = __hip_atomic_compare_exchange_strong(it + ITEM_STAMP, &exp, ~serial, __ATOMIC_ACQ_REL,
-                                                                __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                }
-                take = __shfl(take, 0);
-                break;
-            }
-            __builtin_amdgcn_s_sleep(8);
-        }
-        if (!take) return 0;  // left for the next pass
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        v = lane < ITEM_STAMP ? __hip_atomic_load(it + lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0u;
-        idle = 0;
-        return 1;
-    }
-    if (a == 0 || ++idle >= STEAL_IDLE_LIMIT) return -1;
-    for (int z = 0; z < 4; ++z) __builtin_amdgcn_s_sleep(127);  // ~15 us between polls
-    return 0;
-}
-
 // Resident wavefronts per SIMD the register allocation is held to: the
 // hardware maximum, 8 (64 VGPRs).  The search is latency bound (dependent loads
 // of binary searches and LDS round trips) and every extra resident wavefront
@@ -1236,10 +1108,8 @@ __device__ __forceinline__ int steal_step(const KParams &P, uint32_t serial, uin
 static_assert((sizeof(WaveLds) + MAX_TERMS * sizeof(TermLds)) * 4 * SGE_WAVES_PER_SIMD <= 160 * 1024,
               "the resident wavefronts' LDS state must fit one CU");
 // CAP: the call has a work cap (node_cap != 0); the uncapped kernel carries no
-// cap accounting at all.  STEAL: waves whose queue runs out take other waves'
-// continuation items within the pass (steal_step); a separate instantiation,
-// so the kernels without it keep their smaller register footprint.
-template <bool ENUM, bool CAP, bool STEAL = false>
+// cap accounting at all.
+template <bool ENUM, bool CAP>
 __global__ void __launch_bounds__(256, SGE_WAVES_PER_SIMD) wcoj_kernel(KParams P) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     const int lane = lane_id();
@@ -1256,16 +1126,9 @@ __global__ void __launch_bounds__(256, SGE_WAVES_PER_SIMD) wcoj_kernel(KParams P
     uint32_t cur_plan = NONE;
     const bool cont = P.in_items != nullptr;
     const uint64_t nitems = cont ? min((uint64_t)*P.in_count, P.item_cap) : P.nitems;
-    uint32_t serial = 0;  // this pass's serial: items of the previous pass carry serial - 1
-    if (lane == 0) serial = __hip_atomic_load(P.serial, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    serial = __builtin_amdgcn_readfirstlane(__shfl(serial, 0));
-    const bool steal = STEAL && !ENUM && P.steal_on && P.out_items;
-    if (steal && lane == 0) __hip_atomic_fetch_add(P.active, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
     unsigned long long t_item = __builtin_amdgcn_s_memtime();  // start of the current item
     uint32_t fetched = 15;  // the work cap is checked before the first item and every 16th after it
     uint32_t since = 0;     // wave steps of root tasks counted at the first level (not yet reported)
-    bool stealing = false;  // the queue is exhausted: items come from other waves (steal_step)
-    uint32_t idle = 0;
     while (true) {
         {
             // charge the time since the last item started to that item's plan
@@ -1273,44 +1136,48 @@ __global__ void __launch_bounds__(256, SGE_WAVES_PER_SIMD) wcoj_kernel(KParams P
             if (cur_plan != NONE) acc.add(A_CYCLES, now - t_item);
             t_item = now;
         }
-        uint32_t v = 0;  // a continuation item to run: lane q holds its word q
         unsigned long long item = 0;
-        if (!stealing) {
-            if (lane == 0) item = atomicAdd(P.queue, 1ull);
-            item = __shfl(item, 0);
-            if (item >= nitems) {
-                if (!steal) break;
-                stealing = true;  // this wave stops working
-                if (lane == 0) __hip_atomic_fetch_sub(P.active, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-                if (gwave % STEAL_STRIDE) break;  // one wave in STEAL_STRIDE takes other waves' items
-                continue;
-            }
-            if (CAP && (++fetched & 15u) == 0) {  // every 16th item: has the work cap been reached?
-                unsigned long long spent = 0;
-                if (lane == 0) spent = __hip_atomic_load(P.node_ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                if (__shfl(spent, 0) >= P.node_cap) break;
-            }
-            if (cont) {
-                // continuation item of the previous pass (stamped with its
-                // serial; not taken there)
-                const uint32_t *it = P.in_items + item * ITEM_WORDS;
-                v = lane < ITEM_WORDS ? it[lane] : 0u;
-                if (__builtin_amdgcn_readlane(v, ITEM_STAMP) != serial - 1u) continue;
-            }
-        } else {
-            const int r = steal_step<CAP>(P, serial, idle, v);
-            if (r < 0) break;
-            if (r == 0) {
-                t_item = __builtin_amdgcn_s_memtime();  // idle time is charged to no plan
-                continue;
-            }
+        if (lane == 0) item = atomicAdd(P.queue, 1ull);
+        item = __shfl(item, 0);
+        if (item >= nitems) break;
+        if (CAP && (++fetched & 15u) == 0) {  // every 16th item: has the work cap been reached?
+            unsigned long long spent = 0;
+            if (lane == 0) spent = __hip_atomic_load(P.node_ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (__shfl(spent, 0) >= P.node_cap) break;
         }
-        if (cont || stealing) {
+        if (cont) {
             // ---- continuation item: rebuild the path, then search its range
-            if (stealing && lane == 0) __hip_atomic_fetch_add(P.active, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-            const bool halt = run_item<ENUM, CAP>(P, w, terms, wb, gb, acc, cur_plan, v);
-            if (stealing && lane == 0) __hip_atomic_fetch_sub(P.active, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-            if (halt) break;  // work cap reached
+            const uint32_t *it = P.in_items + item * ITEM_WORDS;
+            const uint32_t plan = it[0];
+            if (plan >= P.nplans) continue;  // empty slot
+            if (plan != cur_plan) {
+                if (cur_plan != NONE) flush(P, cur_plan, acc);
+                cur_plan = plan;
+                load_plan(P, w, plan);
+            }
+            const DPlan &pl = w.pl;
+            const int l = (int)it[1];
+            const int stop = ENUM ? (int)pl.k - 2 : (int)pl.sib;
+            if (l < 2 || l >= stop) continue;  // never written by donate()
+            for (int q = 0; q < l; ++q) w.phi[q] = it[4 + q];
+            if (pl.mode == SGE_MODE_EXPLORE && !ENUM) {
+                explore_after(P, w, wb, 0, acc);
+                explore_after(P, w, wb, 1, acc);
+            }
+            // trie nodes of the path's bag levels (all present: the path was live)
+            bool live = true;
+            if ((pl.bag & 1u) && !trie_enter(w, 0, w.phi[0], NONE)) live = false;
+            if (live && (pl.bag & 2u) && !trie_enter(w, 1, w.phi[1], NONE)) live = false;
+            for (int j = 2; j <= l && live; ++j) {
+                materialize(P, w, wb, j, acc);
+                if (j < l && ((pl.bag >> j) & 1u) && !trie_enter(w, j, w.phi[j], NONE)) live = false;
+                if (j < l && pl.mode == SGE_MODE_EXPLORE && !ENUM) explore_after(P, w, wb, j, acc);
+            }
+            if (!live) continue;
+            const uint32_t rb = it[2], re = it[3];
+            w.pos[l] = max(rb, w.cand[l].b);
+            w.iend[l] = min(re, w.cand[l].e);
+            if (run_dfs<ENUM, CAP>(P, w, terms, wb, gb, acc, plan, l, stop)) break;  // work cap reached
             continue;
         }
         // ---- root pass: chunk of subtasks of one plan
@@ -1391,8 +1258,6 @@ __global__ void __launch_bounds__(256, SGE_WAVES_PER_SIMD) wcoj_kernel(KParams P
         }
         if (halt) break;
     }
-    // a wave stopped by the work cap before its queue ran out is still counted as working
-    if (steal && !stealing && lane == 0) __hip_atomic_fetch_sub(P.active, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
     if (CAP && since && lane == 0) atomicAdd(P.node_ctr, (unsigned long long)since);
     if (cur_plan != NONE) acc.add(A_CYCLES, __builtin_amdgcn_s_memtime() - t_item);
     if (cur_plan != NONE) flush(P, cur_plan, acc);
@@ -1426,7 +1291,6 @@ struct DeviceCtx {
 };
 
 constexpr uint64_t ITEM_CAP = 1ull << 21;  // continuation items per pass buffer (96 MB)
-constexpr int ICOUNT_WORDS = 8;            // u64 words of a control block (item counts, cap, steal state)
 
 DeviceCtx &dctx() {
     static DeviceCtx c;
@@ -1456,11 +1320,7 @@ int ensure_ctx() {
     HIP_TRY(hipEventCreate(&c.ev1));
     HIP_TRY(hipMalloc(&c.items[0], sizeof(uint32_t) * ITEM_WORDS * ITEM_CAP));
     HIP_TRY(hipMalloc(&c.items[1], sizeof(uint32_t) * ITEM_WORDS * ITEM_CAP));
-    HIP_TRY(hipMalloc(&c.icount, sizeof(unsigned long long) * ICOUNT_WORDS));
-    // item stamps and the pass serial start at zero: no slot is an item before it is written
-    HIP_TRY(hipMemset(c.items[0], 0, sizeof(uint32_t) * ITEM_WORDS * ITEM_CAP));
-    HIP_TRY(hipMemset(c.items[1], 0, sizeof(uint32_t) * ITEM_WORDS * ITEM_CAP));
-    HIP_TRY(hipMemset(c.icount, 0, sizeof(unsigned long long) * ICOUNT_WORDS));
+    HIP_TRY(hipMalloc(&c.icount, sizeof(unsigned long long) * 4));
     c.init = true;
     return SGE_OK;
 }
@@ -1634,7 +1494,6 @@ struct LaunchCfg {
     uint32_t bitmap_ratio = 2;
     unsigned long long node_cap = 0;
     uint32_t gbm_min = 2048;
-    bool steal = false;
 };
 
 int launch_cfg(const sge_tuning *t, bool enumerate, LaunchCfg &cfg) {
@@ -1654,7 +1513,6 @@ int launch_cfg(const sge_tuning *t, bool enumerate, LaunchCfg &cfg) {
     cfg.bitmap_ratio = t->bitmap_ratio ? t->bitmap_ratio : cfg.probe_ratio;
     cfg.node_cap = (unsigned long long)t->node_cap_k * NODE_STEP;
     cfg.gbm_min = t->global_bitmap_min ? t->global_bitmap_min : 2048u;
-    cfg.steal = t->steal != 0;
     cfg.lds = (size_t)(cfg.block / WAVE) * (sizeof(WaveLds) + MAX_TERMS * sizeof(TermLds));
     int per_cu = (int)t->blocks_per_cu;
     if (per_cu <= 0) {
@@ -1746,26 +1604,11 @@ int ensure_scratch(const sge_graph *g, const LaunchCfg &cfg, KParams &kp) {
     return SGE_OK;
 }
 
-// Start a pass: clear the queue, the output item count and the steal state,
-// and advance the control block's pass serial (icount: [0], [1] item counts,
-// [2] work-cap counter, [3] steal cursor, [4] active waves (u32), [5] pass
-// serial (u32)).
-__global__ void reset_pass_kernel(unsigned long long *queue, unsigned long long *count, unsigned long long *icount) {
+__global__ void reset_pass_kernel(unsigned long long *queue, unsigned long long *count) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         queue[0] = 0;
         *count = 0;
-        icount[3] = 0;
-        reinterpret_cast<uint32_t *>(icount + 4)[0] = 0;
-        reinterpret_cast<uint32_t *>(icount + 5)[0] += 1;
     }
-}
-
-// The stealing fields of a launch from its control block.
-void set_steal(KParams &kp, unsigned long long *icount, bool on) {
-    kp.steal = icount + 3;
-    kp.active = reinterpret_cast<uint32_t *>(icount + 4);
-    kp.serial = reinterpret_cast<const uint32_t *>(icount + 5);
-    kp.steal_on = on ? 1u : 0u;
 }
 
 __global__ void reset_kernel(PlanOut *out, uint32_t n, unsigned long long *queue, unsigned long long *emb_n,
@@ -1780,17 +1623,12 @@ __global__ void reset_kernel(PlanOut *out, uint32_t n, unsigned long long *queue
 }
 
 // The kernel variant for a launch: enumeration or counting, with or without
-// the work cap's accounting, with or without in-pass stealing.
+// the work cap's accounting.
 void launch_wcoj(bool enumerate, const LaunchCfg &cfg, hipStream_t st, const KParams &kp) {
     if (enumerate)
         wcoj_kernel<true, false><<<cfg.grid, cfg.block, cfg.lds, st>>>(kp);
     else if (kp.node_cap)
-        if (kp.steal_on)
-            wcoj_kernel<false, true, true><<<cfg.grid, cfg.block, cfg.lds, st>>>(kp);
-        else
-            wcoj_kernel<false, true><<<cfg.grid, cfg.block, cfg.lds, st>>>(kp);
-    else if (kp.steal_on)
-        wcoj_kernel<false, false, true><<<cfg.grid, cfg.block, cfg.lds, st>>>(kp);
+        wcoj_kernel<false, true><<<cfg.grid, cfg.block, cfg.lds, st>>>(kp);
     else
         wcoj_kernel<false, false><<<cfg.grid, cfg.block, cfg.lds, st>>>(kp);
 }
@@ -1827,9 +1665,8 @@ int run_launch(sge_graph *g, const DPlan *dplans, uint32_t np, PlanOut *dout, ui
     kp.node_cap = cfg.node_cap;
     kp.gbm_min = cfg.gbm_min;
     kp.node_ctr = c.icount + 2;  // read back with the item counts, in one copy per pass
-    set_steal(kp, c.icount, cfg.steal && !enumerate);
     reset_kernel<<<(np + 255) / 256, 256, 0, c.stream>>>(dout, np, c.queue, emb_n, kp.node_ctr);
-    reset_pass_kernel<<<1, 64, 0, c.stream>>>(c.queue, c.icount, c.icount);
+    reset_pass_kernel<<<1, 64, 0, c.stream>>>(c.queue, c.icount);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(c.ev0, c.stream));
     int cur = 0;
@@ -1846,7 +1683,7 @@ int run_launch(sge_graph *g, const DPlan *dplans, uint32_t np, PlanOut *dout, ui
             kp.in_items = c.items[cur];
             kp.in_count = c.icount + cur;
             cur ^= 1;
-            reset_pass_kernel<<<1, 64, 0, c.stream>>>(c.queue, c.icount + cur, c.icount);
+            reset_pass_kernel<<<1, 64, 0, c.stream>>>(c.queue, c.icount + cur);
         }
         kp.out_items = c.items[cur];
         kp.out_count = c.icount + cur;
@@ -2458,7 +2295,7 @@ static void enqueue_cont_pass(sge_batch *b, int in) {
     kp.in_count = b->icount + in;
     kp.out_items = b->items[in ^ 1];
     kp.out_count = b->icount + (in ^ 1);
-    reset_pass_kernel<<<1, 64, 0, b->stream>>>(b->queue, b->icount + (in ^ 1), b->icount);
+    reset_pass_kernel<<<1, 64, 0, b->stream>>>(b->queue, b->icount + (in ^ 1));
     launch_wcoj(false, b->cfg, b->stream, kp);
 }
 
@@ -2539,14 +2376,7 @@ static int batch_create(sge_graph *g, const sge_plan *plans, uint32_t nplans, co
     for (int q = 0; q < 2; ++q)
         if ((e = hipMalloc(&b->items[q], sizeof(uint32_t) * ITEM_WORDS * ITEM_CAP)) != hipSuccess)
             return fail(e, "items");
-    if ((e = hipMalloc(&b->icount, sizeof(unsigned long long) * ICOUNT_WORDS)) != hipSuccess)
-        return fail(e, "item counts");
-    // item stamps and the pass serial start at zero: no slot is an item before it is written
-    for (int q = 0; q < 2; ++q)
-        if ((e = hipMemset(b->items[q], 0, sizeof(uint32_t) * ITEM_WORDS * ITEM_CAP)) != hipSuccess)
-            return fail(e, "items");
-    if ((e = hipMemset(b->icount, 0, sizeof(unsigned long long) * ICOUNT_WORDS)) != hipSuccess)
-        return fail(e, "item counts");
+    if ((e = hipMalloc(&b->icount, sizeof(unsigned long long) * 4)) != hipSuccess) return fail(e, "item counts");
     if ((e = hipMemcpy(b->dplans, dp.data(), sizeof(DPlan) * nplans, hipMemcpyHostToDevice)) != hipSuccess)
         return fail(e, "plan upload");
     if ((e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking)) != hipSuccess) return fail(e, "stream");
@@ -2574,7 +2404,6 @@ static int batch_create(sge_graph *g, const sge_plan *plans, uint32_t nplans, co
     kp.gbm_min = cfg.gbm_min;
     kp.node_cap = cfg.node_cap;
     kp.node_ctr = b->icount + 2;
-    set_steal(kp, b->icount, cfg.steal);
     kp.out_items = b->items[0];
     kp.out_count = b->icount;
     b->kp = kp;
@@ -2583,7 +2412,7 @@ static int batch_create(sge_graph *g, const sge_plan *plans, uint32_t nplans, co
     // number of continuation passes that read their item counts on the device
     if ((e = hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal)) != hipSuccess) return fail(e, "capture");
     reset_kernel<<<(nplans + 255) / 256, 256, 0, b->stream>>>(b->dout, nplans, b->queue, nullptr, kp.node_ctr);
-    reset_pass_kernel<<<1, 64, 0, b->stream>>>(b->queue, b->icount, b->icount);
+    reset_pass_kernel<<<1, 64, 0, b->stream>>>(b->queue, b->icount);
     if (kp.nitems) {
         launch_wcoj(false, cfg, b->stream, kp);
         int in = 0;
@@ -2925,11 +2754,10 @@ extern "C" int sge_count_noalloc(sge_graph *g, const sge_plan *plans, uint32_t n
     kp.bitmap_ratio = cfg.bitmap_ratio;
     kp.node_cap = cfg.node_cap;
     kp.node_ctr = ctl + 3;
-    set_steal(kp, ctl + 1, tune->steal != 0);  // control block: ctl + 1 (item counts, cap, steal, active, serial)
     uint32_t *items[2] = {reinterpret_cast<uint32_t *>(base + L.items0), reinterpret_cast<uint32_t *>(base + L.items1)};
     hipStream_t st = nullptr;  // the null stream: nothing to create
     reset_kernel<<<(nplans + 255) / 256, 256, 0, st>>>(dout, nplans, ctl, nullptr, kp.node_ctr);
-    reset_pass_kernel<<<1, 64, 0, st>>>(ctl, ctl + 1, ctl + 1);
+    reset_pass_kernel<<<1, 64, 0, st>>>(ctl, ctl + 1);
     HIP_TRY(hipGetLastError());
     int cur = 0;
     bool limited = false;
@@ -2945,7 +2773,7 @@ extern "C" int sge_count_noalloc(sge_graph *g, const sge_plan *plans, uint32_t n
             kp.in_items = items[cur];
             kp.in_count = ctl + 1 + cur;
             cur ^= 1;
-            reset_pass_kernel<<<1, 64, 0, st>>>(ctl, ctl + 1 + cur, ctl + 1);
+            reset_pass_kernel<<<1, 64, 0, st>>>(ctl, ctl + 1 + cur);
         }
         kp.out_items = items[cur];
         kp.out_count = ctl + 1 + cur;

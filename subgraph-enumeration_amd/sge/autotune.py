@@ -24,7 +24,6 @@ SPACE = {  # first value = the library default, the starting point
     "donate_budget": [4096, 1024, 16384],
     "stage_max": [2, 1, 4, 8],
     "bitmap_ratio": [2, 1, 4, 8],
-    "steal": [0, 1],
 }
 CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "models", "tuning.json")
 
