@@ -209,7 +209,7 @@ typedef struct sge_tuning {
     uint32_t donate_budget;   /* wave steps (one per search-tree node plus one per 64 candidates it
                                  forms, plus the 64-lane iterations of each last-level count) one
                                  work item may spend before its open candidate ranges go to the
-                                 next pass (0 = default 4096, 1 << 30 = off) */
+                                 next pass (0 = default 1024, 1 << 30 = off) */
     uint32_t bitmap_ratio;    /* the same ratio when F is staged as an LDS bitmap (0 = stage_max) */
     uint32_t node_cap_k;      /* stop a counting call (plain or batched) after about node_cap_k * 1024
                                  wave steps (as donate_budget counts them) and fail with SGE_ELIMIT
@@ -224,6 +224,8 @@ typedef struct sge_tuning {
                                  this many elements, is staged as a bitmap in a per-wave global
                                  region when its siblings' rows will probe it (0 = default 2048,
                                  1 << 30 = off) */
+    uint32_t donate_split;    /* a donated candidate range goes to the next pass as at most this many
+                                 items of near-equal length (0 = default 64, at most 256) */
 } sge_tuning;
 
 typedef struct sge_stats {

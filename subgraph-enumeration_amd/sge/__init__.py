@@ -71,7 +71,8 @@ class Tuning(C.Structure):
     _fields_ = [("block_threads", C.c_uint32), ("chunk", C.c_uint32), ("stage_max", C.c_uint32),
                 ("blocks_per_cu", C.c_uint32), ("split_span", C.c_uint32), ("donate_budget", C.c_uint32),
                 ("bitmap_ratio", C.c_uint32), ("node_cap_k", C.c_uint32),
-                ("head_items", C.c_uint32), ("global_bitmap_min", C.c_uint32)]
+                ("head_items", C.c_uint32), ("global_bitmap_min", C.c_uint32),
+                ("donate_split", C.c_uint32)]
 
 
 class Stats(C.Structure):

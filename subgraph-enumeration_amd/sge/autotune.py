@@ -21,7 +21,8 @@ SPACE = {  # first value = the library default, the starting point
     "block_threads": [256, 128, 64],
     "chunk": [4, 2, 8],
     "split_span": [32, 16, 64],
-    "donate_budget": [4096, 1024, 16384],
+    "donate_budget": [1024, 4096, 256, 16384],
+    "donate_split": [64, 16, 256],
     "stage_max": [2, 1, 4, 8],
     "bitmap_ratio": [2, 1, 4, 8],
 }

@@ -145,6 +145,26 @@ def test_tuning_variants_same_counts():
         assert sge.count(g, plans, batched=False, tuning=t)[0] == want
 
 
+def test_donation_split_same_counts():
+    """Donated ranges split into several items (small budgets make every
+    search donate) give the same counts in all three modes, batched and plain."""
+    g = W.graph_c2()
+    pats = [W.named("house"), W.named("near5clique")] + W.six_vertex_set()[:3]
+    for mode in (sge.MODE_WCOJ, sge.MODE_EXPLORE, sge.MODE_IEP):
+        plans = [sge.build_plan(p, g, mode) for p in pats]
+        want, _ = sge.count(g, plans)
+        for split, budget in ((2, 16), (16, 64), (256, 16), (64, 4096)):
+            t = sge.default_tuning()
+            t.donate_split, t.donate_budget = split, budget
+            say("donation split", mode, split, budget)
+            assert sge.count(g, plans, batched=True, tuning=t)[0] == want, (mode, split, budget)
+            assert sge.count(g, plans, batched=False, tuning=t)[0] == want, (mode, split, budget)
+    t = sge.default_tuning()
+    t.donate_split = 257
+    with pytest.raises(sge.SgeError):
+        sge.count(g, plans, tuning=t)
+
+
 def test_probe_direction_ratios_same_counts():
     """Which side of a last-level term drives the probes (LDS-staged F or the
     sibling's CSR row) must not change a count."""

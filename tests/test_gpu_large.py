@@ -108,13 +108,15 @@ def test_c4_subsets_plain_batched_oracle(c4):
 
 
 def test_c4_subsets_tuning_variants(c4):
-    """Donation, split and probe-ratio knobs at their extremes on hub-row searches."""
+    """Donation (budget and split), root-split and probe-ratio knobs at their
+    extremes on hub-row searches."""
     g, doc, plans = c4
     want = [e["count"] for e in doc["subsets"]]
     b = [e["begin"] for e in doc["subsets"]]
     e = [e["end"] for e in doc["subsets"]]
     for knobs in ({"donate_budget": 64, "split_span": 8}, {"stage_max": 64, "bitmap_ratio": 64, "chunk": 1},
-                  {"global_bitmap_min": 16, "head_items": 1 << 30}):
+                  {"global_bitmap_min": 16, "head_items": 1 << 30}, {"donate_split": 64, "donate_budget": 256},
+                  {"donate_split": 256, "donate_budget": 1024}):
         t = capped()
         for k, v in knobs.items():
             setattr(t, k, v)
