@@ -142,7 +142,8 @@ def main():
         from . import learned as L
 
         model = L.load(model_path)
-        plans = [pl for pl, _ in L.choose_plans(model, pats, g, MODE_IEP, plans)]
+        # on root subsets only the orders that count the same embeddings there
+        plans = [pl for pl, _ in L.choose_plans(model, pats, g, MODE_IEP, plans, keep_root=ranges is not None)]
     if os.path.exists(CACHE):
         with open(CACHE) as f:
             data = json.load(f)

@@ -34,11 +34,76 @@ FEATURE_SAMPLES = 512
 CAPPED_PENALTY = 2.0  # bits added to the time of an order stopped by the work cap
 
 
-def candidate_orders(pattern: Pattern, graph: Optional[Graph], mode: int, top: int = 16, rng=None) -> List[List[int]]:
+def automorphisms(pattern: Pattern) -> List[List[int]]:
+    """Every automorphism of the pattern (g[u] = image of u), by backtracking."""
+    k = pattern.k
+    adj = [0] * k
+    for a, b in pattern.edges():
+        adj[a] |= 1 << b
+        adj[b] |= 1 << a
+    out, g, used = [], [0] * k, [False] * k
+
+    def extend(u):
+        if u == k:
+            out.append(list(g))
+            return
+        for v in range(k):
+            if used[v] or bin(adj[v]).count("1") != bin(adj[u]).count("1"):
+                continue
+            if any(((adj[u] >> w) & 1) != ((adj[v] >> g[w]) & 1) for w in range(u)):
+                continue
+            g[u], used[v] = v, True
+            extend(u + 1)
+            used[v] = False
+
+    extend(0)
+    return out
+
+
+def symmetry_prefix(pattern: Pattern, order: Sequence[int], aut=None) -> int:
+    """Length of the order's prefix that fixes its symmetry rules and root edge:
+    the planner's rules come from the stabiliser chain along the order (a rule
+    at level i for every other vertex in the orbit of order[i] under the
+    automorphisms fixing order[:i]), so once the stabiliser is trivial no rule
+    follows; at least the root edge (2 levels)."""
+    G = automorphisms(pattern) if aut is None else aut
+    j = 0
+    while j < pattern.k and len(G) > 1:
+        G = [g for g in G if g[order[j]] == order[j]]
+        j += 1
+    return max(2, j)
+
+
+def root_preserving(pattern: Pattern, order: Sequence[int], candidates: Sequence[Sequence[int]]) -> List[List[int]]:
+    """The candidates that, up to an automorphism, start with `order`'s symmetry
+    prefix, each relabelled so that it starts with exactly that prefix.  Such an
+    order maps a root edge to the same two pattern vertices and has the same
+    symmetry rules, so over any set of root edges it counts exactly the
+    embeddings `order` counts (a count over a root subset is comparable)."""
+    aut = automorphisms(pattern)
+    P = symmetry_prefix(pattern, order, aut)
+    out, seen = [], set()
+    for o in candidates:
+        for g in aut:
+            if all(g[o[q]] == order[q] for q in range(P)):
+                t = [g[x] for x in o]
+                if tuple(t) not in seen:
+                    seen.add(tuple(t))
+                    out.append(t)
+                break
+    return out
+
+
+def candidate_orders(pattern: Pattern, graph: Optional[Graph], mode: int, top: int = 16, rng=None,
+                     keep_root_of: Optional[Sequence[int]] = None) -> List[List[int]]:
     """Connected orders (deduplicated up to automorphism) ranked as the planner
     ranks them for `mode`; the `top` best, and with rng up to `top` others drawn
-    from ranks [top, 4 * top) -- the neighbourhood the model has to choose in."""
+    from ranks [top, 4 * top) -- the neighbourhood the model has to choose in.
+    With keep_root_of, only the orders that count the same embeddings as that
+    order over every root subset (root_preserving), in rank order."""
     orders, costs, tails = rank_orders(pattern, graph, mode)
+    if keep_root_of is not None:
+        orders = root_preserving(pattern, list(keep_root_of), orders)
     pick = list(range(min(top, len(orders))))
     hi = min(len(orders), 4 * top)
     if rng is not None and hi > top:
@@ -174,13 +239,14 @@ def load(path: str) -> LearnedOrder:
 JUDGE_SEED = 1  # seed of the sampled cost model's sampling (one seed: one judgement)
 
 
-def _proposal(model: LearnedOrder, pattern: Pattern, graph: Graph, mode: int, fallback: Plan, top: int):
+def _proposal(model: LearnedOrder, pattern: Pattern, graph: Graph, mode: int, fallback: Plan, top: int,
+              keep_root: bool = False):
     """The network's pick among the planner's `top` candidate orders, or None
     (the planner's order) when the pick is not predicted `margin` bits better
     or the pattern size is outside the training sizes."""
     if model.sizes and pattern.k not in model.sizes:
         return _proposal_from(model, pattern, fallback, None, None)
-    cands, X = _features(pattern, graph, mode, top)
+    cands, X = _features(pattern, graph, mode, top, _root_of(fallback, pattern) if keep_root else None)
     return _proposal_from(model, pattern, fallback, cands, X)
 
 
@@ -198,27 +264,31 @@ def _judge(model: LearnedOrder, pattern: Pattern, graph: Graph, mode: int, fallb
 
 
 def choose_plan(model: LearnedOrder, pattern: Pattern, graph: Graph, mode: int, fallback: Optional[Plan] = None,
-                top: int = 12):
+                top: int = 12, keep_root: bool = False):
     """Order with the lowest predicted work among the planner's `top`
     candidates, taken only when the network predicts it at least `margin` bits
     below the planner's own order and the reference's sampled cost model does
-    not put it more than `judge_bits` above that order (the fallback)."""
+    not put it more than `judge_bits` above that order (the fallback).
+    keep_root: choose only among the orders that count the same embeddings as
+    the fallback's order over any root subset (root_preserving)."""
     if fallback is None:
         fallback = build_plan(pattern, graph, mode)
-    order, info = _proposal(model, pattern, graph, mode, fallback, top)
+    order, info = _proposal(model, pattern, graph, mode, fallback, top, keep_root)
     if order is None:
         return fallback, info
     return _judge(model, pattern, graph, mode, fallback, order, info)
 
 
 def choose_plans(model: LearnedOrder, patterns: Sequence[Pattern], graph: Graph, mode: int,
-                 fallbacks: Sequence[Plan], top: int = 12, threads: int = 16):
+                 fallbacks: Sequence[Plan], top: int = 12, threads: int = 16, keep_root: bool = False):
     """choose_plan over a pattern set: feature sampling and the sampled cost
     model (host code, the GIL released in the library) run on `threads` threads;
     the network's forward passes stay on the caller's thread."""
     with ThreadPoolExecutor(max_workers=threads) as ex:
-        feats = list(ex.map(lambda p: ((None, None) if model.sizes and p.k not in model.sizes
-                                       else _features(p, graph, mode, top)), patterns))
+        feats = list(ex.map(lambda a: ((None, None) if model.sizes and a[0].k not in model.sizes
+                                       else _features(a[0], graph, mode, top,
+                                                      _root_of(a[1], a[0]) if keep_root else None)),
+                            list(zip(patterns, fallbacks))))
         props = []
         for p, fb, (cands, X) in zip(patterns, fallbacks, feats):
             props.append(_proposal_from(model, p, fb, cands, X))
@@ -231,8 +301,12 @@ def choose_plans(model: LearnedOrder, patterns: Sequence[Pattern], graph: Graph,
     return out
 
 
-def _features(pattern: Pattern, graph: Graph, mode: int, top: int):
-    cands = candidate_orders(pattern, graph, mode, top)
+def _root_of(plan: Plan, pattern: Pattern) -> List[int]:
+    return list(plan.order)[: pattern.k]
+
+
+def _features(pattern: Pattern, graph: Graph, mode: int, top: int, keep_root_of=None):
+    cands = candidate_orders(pattern, graph, mode, top, keep_root_of=keep_root_of)
     return cands, np.stack([order_features(pattern, graph, o, mode, FEATURE_SAMPLES, 11) for o in cands])
 
 

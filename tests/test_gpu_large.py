@@ -125,6 +125,31 @@ def test_c4_subsets_tuning_variants(c4):
         assert got == want, knobs
 
 
+def test_c4_learned_orders_count_the_same_subsets(c4):
+    """The learned model, choosing only among orders that keep each cost-model
+    order's root edge and symmetry rules, changes orders and leaves every
+    subset count equal to the oracle's (the bench's learned-vs-plain timing
+    rests on this)."""
+    from sge import learned as L
+
+    g, doc, plans = c4
+    want = [e["count"] for e in doc["subsets"]]
+    b = [e["begin"] for e in doc["subsets"]]
+    e = [e["end"] for e in doc["subsets"]]
+    model = L.load(os.path.join(os.path.dirname(sge.__file__), "models", "order_mlp.npz"))
+    pats = [W.seven_vertex_set()[x["pattern"]] for x in doc["subsets"]]
+    say("c4 learned root-preserving orders")
+    learned = [pl for pl, _ in L.choose_plans(model, pats, g, sge.MODE_IEP, plans, keep_root=True)]
+    changed = sum(list(a.order) != list(c.order) for a, c in zip(learned, plans))
+    say(f"c4 learned: {changed} of {len(plans)} orders changed; batched, capped")
+    assert changed >= 5
+    got, _, _ = sge.count_ranges(g, learned, b, e, batched=True, tuning=capped())
+    assert got == want
+    say("c4 learned: plain, capped")
+    got, _, _ = sge.count_ranges(g, learned, b, e, batched=False, tuning=capped())
+    assert got == want
+
+
 def test_c4_oracle_spot_checks(c4):
     g, doc, plans = c4
     oracle_spot_checks(g, plans, doc, [0, 3, 40, 77])

@@ -157,6 +157,45 @@ def test_root_ranges_are_additive():
     assert parts == whole
 
 
+def test_automorphisms_by_brute_force():
+    from sge import learned as L
+
+    for p in W.seven_vertex_set()[:20] + W.six_vertex_set():
+        e = {tuple(sorted(x)) for x in p.edges()}
+        brute = sorted(list(g) for g in itertools.permutations(range(p.k))
+                       if {tuple(sorted((g[a], g[b]))) for a, b in e} == e)
+        assert sorted(L.automorphisms(p)) == brute
+        assert len(brute) == p.info().aut_size
+
+
+def test_root_preserving_orders_count_the_same_root_subsets():
+    """An order that keeps the cost-model order's symmetry prefix (root edge
+    and rules) counts the same embeddings over every root-edge range, so the
+    learned order can be timed against the cost-model order on a root subset
+    with equal counts.  The control: orders that root the search elsewhere
+    give other counts on the same ranges."""
+    from sge import learned as L
+
+    g = W.graph_rmat(8, 7)
+    off, nbr = g.csr()
+    nnz = int(off[-1])
+    cuts = [(0, nnz // 7), (nnz // 3, nnz // 3 + 200), (nnz - 300, nnz)]
+    checked = differs = 0
+    for p in W.seven_vertex_set()[::6]:
+        cost = sge.build_plan(p, g, sge.MODE_IEP)
+        R = list(cost.order)[: p.k]
+        want = [O.count(cost, off, nbr, 8, a, b) for a, b in cuts]
+        for o in L.candidate_orders(p, g, sge.MODE_IEP, top=6, keep_root_of=R):
+            plan = sge.build_plan(p, g, sge.MODE_IEP, o)
+            assert [O.count(plan, off, nbr, 8, a, b) for a, b in cuts] == want, (p, R, o)
+            checked += 1
+        for o in L.candidate_orders(p, g, sge.MODE_IEP, top=4):
+            if o[:2] != R[:2] and {o[0], o[1]} != {R[0], R[1]}:
+                plan = sge.build_plan(p, g, sge.MODE_IEP, o)
+                differs += [O.count(plan, off, nbr, 8, a, b) for a, b in cuts] != want
+    assert checked >= 20 and differs >= 5
+
+
 # ---------------------------------------------------------------- reference goldens
 def test_golden_c1():
     gold = load_golden("c1")
