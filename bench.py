@@ -80,7 +80,7 @@ def load_traffic(kernel_workload):
         return None
 
 
-def learned_plans(g, pats, plans_cost, mode, keep_root=False):
+def learned_plans(g, pats, plans_cost, mode, keep_root=False, root_ranges=None):
     try:
         from sge import learned as LM
     except ImportError:
@@ -88,7 +88,8 @@ def learned_plans(g, pats, plans_cost, mode, keep_root=False):
     if not os.path.exists(LEARNED):
         return plans_cost, False
     model = LM.load(LEARNED)
-    return [plan for plan, _ in LM.choose_plans(model, pats, g, mode, plans_cost, keep_root=keep_root)], True
+    return [plan for plan, _ in LM.choose_plans(model, pats, g, mode, plans_cost, keep_root=keep_root,
+                                                root_ranges=root_ranges)], True
 
 
 def timed_steps(run, warmup, steps, label):
@@ -241,15 +242,15 @@ def c4_subsets(args):
     for pl, e in zip(plans_cost, subs):
         assert list(pl.order)[:pl.k] == e["plan"]["order"]
     plan_s = time.time() - t0
-    t0 = time.time()
-    plans_fast, learned = learned_plans(g, [pats[e["pattern"]] for e in subs], plans_cost, sge.MODE_IEP,
-                                        keep_root=True)
-    learn_s = time.time() - t0
-    changed = sum(list(a.order)[:a.k] != list(b.order)[:b.k] for a, b in zip(plans_fast, plans_cost))
-    log(f"78 cost-model plans in {plan_s:.2f}s; learned orders in {learn_s:.2f}s ({changed} changed)")
     span = max(1, args.span)
     begins = np.array([e["begin"] for e in subs], dtype=np.uint64)
     ends = np.array([e["begin"] + (e["end"] - e["begin"]) * span for e in subs], dtype=np.uint64)
+    t0 = time.time()
+    plans_fast, learned = learned_plans(g, [pats[e["pattern"]] for e in subs], plans_cost, sge.MODE_IEP,
+                                        keep_root=True, root_ranges=(begins, ends))
+    learn_s = time.time() - t0
+    changed = sum(list(a.order)[:a.k] != list(b.order)[:b.k] for a, b in zip(plans_fast, plans_cost))
+    log(f"78 cost-model plans in {plan_s:.2f}s; learned orders in {learn_s:.2f}s ({changed} changed)")
     ranges = (begins, ends)
     root_edges = int((ends - begins).sum())
 

@@ -355,6 +355,13 @@ void sge_mlp_destroy(sge_mlp *m);
  * estimate (nsamples random root edges, seeded) of nodes and list work per level. */
 int sge_order_features(const sge_pattern *p, const sge_graph *g, const uint32_t *order, uint32_t mode,
                        uint32_t nsamples, uint64_t seed, float *feat64);
+/* The same features with the random walks rooted in the CSR positions
+ * [begin, end) only: the task set of a run over a root-edge range
+ * (sge_count_range), whose work can rank orders differently from the whole
+ * job's.  The reference samples its cardinalities over the whole candidate
+ * space (plan/estimator.cpp:562-661); a range has no counterpart there. */
+int sge_order_features_range(const sge_pattern *p, const sge_graph *g, const uint32_t *order, uint32_t mode,
+                             uint32_t nsamples, uint64_t seed, uint64_t begin, uint64_t end, float *feat64);
 
 /* Streaming copy of `bytes` (read once, written once) timed on the device: the
  * known-size kernel that scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings. */

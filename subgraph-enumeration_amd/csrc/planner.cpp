@@ -626,22 +626,24 @@ extern "C" int sge_plan_text(const sge_plan *pl, char *buf, size_t cap) {
 // sampled cardinalities, plan/estimator.cpp:562-661).
 namespace sge {
 
-static void knuth_estimate(const HostGraph &g, const sge_plan &pl, uint32_t nsamples, uint64_t seed, double *card,
-                           double *work) {
+// Root edges are drawn from the CSR positions [rb, re) (the task set a run
+// covers: the whole graph, or a root-edge range), each weighted by re - rb.
+static void knuth_estimate(const HostGraph &g, const sge_plan &pl, uint32_t nsamples, uint64_t seed, uint64_t rb,
+                           uint64_t re, double *card, double *work) {
     int k = pl.k;
     for (int i = 0; i < SGE_MAX_K; ++i) card[i] = work[i] = 0;
-    if (g.nnz == 0 || k < 3) return;
+    if (g.nnz == 0 || k < 3 || re <= rb) return;
     Rng rng(seed);
     std::vector<uint32_t> cur, nxt;
     cur.reserve(g.max_degree + 1);
     nxt.reserve(g.max_degree + 1);
     for (uint32_t s = 0; s < nsamples; ++s) {
-        uint64_t e = rng.below(g.nnz);
+        uint64_t e = rb + rng.below(re - rb);
         uint32_t v0 = (uint32_t)(std::upper_bound(g.off.begin(), g.off.end(), (uint32_t)e) - g.off.begin()) - 1;
         uint32_t v1 = g.nbr[e];
         if ((pl.lower[1] & 1u) && !(v1 > v0)) continue;
         uint32_t phi[SGE_MAX_K] = {v0, v1};
-        double w = (double)g.nnz;
+        double w = (double)(re - rb);
         card[1] += w;
         for (int d = 2; d < k; ++d) {
             uint32_t lo = 0, hi = 0xffffffffu;
@@ -692,7 +694,7 @@ static void knuth_estimate(const HostGraph &g, const sge_plan &pl, uint32_t nsam
 }
 
 int order_features(const Pattern &p, const HostGraph *g, const uint32_t *order, uint32_t mode, uint32_t nsamples,
-                   uint64_t seed, float *f) {
+                   uint64_t seed, uint64_t rb, uint64_t re, float *f) {
     sge_plan pl;
     int rc = build_plan(p, g, mode, order, pl);
     if (rc) return rc;
@@ -722,7 +724,7 @@ int order_features(const Pattern &p, const HostGraph *g, const uint32_t *order, 
         f[base + 4] = (float)(std::log2(std::max(sz, 1e-6)) / 20.0);
     }
     double card[SGE_MAX_K], work[SGE_MAX_K];
-    if (g) knuth_estimate(*g, pl, nsamples, seed, card, work);
+    if (g) knuth_estimate(*g, pl, nsamples, seed, rb, re ? std::min<uint64_t>(re, g->nnz) : g->nnz, card, work);
     else for (int i = 0; i < SGE_MAX_K; ++i) card[i] = work[i] = 0;
     double total = 0;
     for (int i = 1; i < 8; ++i) {
@@ -748,6 +750,24 @@ extern "C" int sge_order_features(const sge_pattern *cp, const sge_graph *g, con
     }
     Pattern p;
     if (int rc = pattern_arg(cp, "sge_order_features", p)) return rc;
-    return order_features(p, g ? &g->h : nullptr, order, mode, nsamples, seed, feat);
+    return order_features(p, g ? &g->h : nullptr, order, mode, nsamples, seed, 0, 0, feat);
+    SGE_API_END
+}
+
+extern "C" int sge_order_features_range(const sge_pattern *cp, const sge_graph *g, const uint32_t *order,
+                                        uint32_t mode, uint32_t nsamples, uint64_t seed, uint64_t begin, uint64_t end,
+                                        float *feat) {
+    SGE_API_BEGIN
+    if (!order || !feat) {
+        set_error("sge_order_features_range: null argument");
+        return SGE_EINVAL;
+    }
+    if (g && (begin >= end || end > g->h.nnz)) {
+        set_error("sge_order_features_range: root range must satisfy begin < end <= number of directed edges");
+        return SGE_EINVAL;
+    }
+    Pattern p;
+    if (int rc = pattern_arg(cp, "sge_order_features_range", p)) return rc;
+    return order_features(p, g ? &g->h : nullptr, order, mode, nsamples, seed, begin, end, feat);
     SGE_API_END
 }

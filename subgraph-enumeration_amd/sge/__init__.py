@@ -139,6 +139,8 @@ _SIGS = {
                                     C.POINTER(Tuning), C.c_void_p, C.POINTER(Stats)]),
     "sge_order_features": (C.c_int, [C.POINTER(CPattern), C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32,
                                      C.c_uint64, C.c_void_p]),
+    "sge_order_features_range": (C.c_int, [C.POINTER(CPattern), C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32,
+                                           C.c_uint64, C.c_uint64, C.c_uint64, C.c_void_p]),
     "sge_mlp_num_params": (C.c_int, []),
     "sge_mlp_input_dim": (C.c_int, []),
     "sge_mlp_create": (C.c_int, [C.c_uint64, C.POINTER(C.c_void_p)]),
@@ -551,12 +553,36 @@ def time_capped(graph: Graph, plans, stride: int, node_cap_k: int, tuning: Optio
     return out
 
 
+def range_capped(graph: Graph, plan: Plan, begin: int, end: int, node_cap_k: int):
+    """Plain executor over root edges [begin, end) with a work cap (units of
+    1024 wave steps): (Stats, whether the cap stopped it)."""
+    t = Tuning()
+    lib().sge_tuning_default(C.byref(t))
+    t.node_cap_k = node_cap_k
+    arr = (Plan * 1)(plan)
+    b = np.array([begin], dtype=np.uint64)
+    e = np.array([end], dtype=np.uint64)
+    counts = np.zeros(1, dtype=np.uint64)
+    st = Stats()
+    pp = (Stats * 1)()
+    rc = lib().sge_count_ranges(graph._h, arr, 1, _ptr(b), _ptr(e), 0, C.byref(t), _ptr(counts), C.byref(st), pp)
+    if rc not in (0, ELIMIT):
+        _check(rc, "sge_count_ranges")
+    return pp[0], rc == ELIMIT
+
+
 def order_features(pattern: Pattern, graph: Optional[Graph], order, mode: int, nsamples: int = 1024,
-                   seed: int = 0) -> np.ndarray:
+                   seed: int = 0, root_range=None) -> np.ndarray:
+    """root_range = (begin, end): random walks rooted in those CSR positions only."""
     o = np.ascontiguousarray(np.asarray(order, dtype=np.uint32))
     f = np.zeros(64, dtype=np.float32)
-    _check(lib().sge_order_features(C.byref(pattern.c), graph._h if graph else None, _ptr(o), mode, nsamples, seed,
-                                    _ptr(f)), "sge_order_features")
+    if root_range is None:
+        _check(lib().sge_order_features(C.byref(pattern.c), graph._h if graph else None, _ptr(o), mode, nsamples,
+                                        seed, _ptr(f)), "sge_order_features")
+    else:
+        _check(lib().sge_order_features_range(C.byref(pattern.c), graph._h if graph else None, _ptr(o), mode,
+                                              nsamples, seed, int(root_range[0]), int(root_range[1]), _ptr(f)),
+               "sge_order_features_range")
     return f
 
 

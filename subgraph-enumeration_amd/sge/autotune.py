@@ -143,7 +143,8 @@ def main():
 
         model = L.load(model_path)
         # on root subsets only the orders that count the same embeddings there
-        plans = [pl for pl, _ in L.choose_plans(model, pats, g, MODE_IEP, plans, keep_root=ranges is not None)]
+        plans = [pl for pl, _ in L.choose_plans(model, pats, g, MODE_IEP, plans, keep_root=ranges is not None,
+                                                root_ranges=ranges)]
     if os.path.exists(CACHE):
         with open(CACHE) as f:
             data = json.load(f)

@@ -28,10 +28,17 @@ import numpy as np
 from concurrent.futures import ThreadPoolExecutor
 
 from . import (MLP, Graph, Pattern, Plan, build_plan, count_sampled, order_cost, order_cost_sampled, order_features,
-               rank_orders, time_capped, work_steps_bound)
+               range_capped, rank_orders, time_capped, work_steps_bound)
 
 FEATURE_SAMPLES = 512
 CAPPED_PENALTY = 2.0  # bits added to the time of an order stopped by the work cap
+# measured probe (the guard): a strided sample of at most 1/PROBE_MIN_STRIDE of
+# a whole job's root edges (about PROBE_EDGES of them), or the root edges next
+# to a run's range (as many, disjoint from it)
+PROBE_MIN_STRIDE = 8
+PROBE_EDGES = 16384
+PROBE_CAP_K = 1 << 16  # work cap of the probe's first plan (x1024 wave steps)
+PROBE_BITS = 0.2  # the learned order must measure at least this many bits less work
 
 
 def automorphisms(pattern: Pattern) -> List[List[int]]:
@@ -240,13 +247,13 @@ JUDGE_SEED = 1  # seed of the sampled cost model's sampling (one seed: one judge
 
 
 def _proposal(model: LearnedOrder, pattern: Pattern, graph: Graph, mode: int, fallback: Plan, top: int,
-              keep_root: bool = False):
+              keep_root: bool = False, root_range=None):
     """The network's pick among the planner's `top` candidate orders, or None
     (the planner's order) when the pick is not predicted `margin` bits better
     or the pattern size is outside the training sizes."""
     if model.sizes and pattern.k not in model.sizes:
         return _proposal_from(model, pattern, fallback, None, None)
-    cands, X = _features(pattern, graph, mode, top, _root_of(fallback, pattern) if keep_root else None)
+    cands, X = _features(pattern, graph, mode, top, _root_of(fallback, pattern) if keep_root else None, root_range)
     return _proposal_from(model, pattern, fallback, cands, X)
 
 
@@ -263,32 +270,80 @@ def _judge(model: LearnedOrder, pattern: Pattern, graph: Graph, mode: int, fallb
     return build_plan(pattern, graph, mode, order), info
 
 
+def probe_work(graph: Graph, plans: Sequence[Plan], root_range=None):
+    """Measured work (wavefront clock cycles) of each plan on a probe sample of
+    the run's task set that the run's own timing does not see in full: for a
+    whole job a strided sample (stride >= PROBE_MIN_STRIDE), for a root-edge
+    range the range next to it (disjoint from it, at most PROBE_EDGES long).
+    plans[0] runs first, under PROBE_CAP_K, and bounds the others by a work cap
+    of 4x its own; returns [(work, capped)], a capped plan's work a lower
+    bound."""
+    nnz = 2 * int(graph.m)
+    out = []
+    cap_k = PROBE_CAP_K
+    for i, pl in enumerate(plans):
+        if root_range is None:
+            stride = max(PROBE_MIN_STRIDE, nnz // PROBE_EDGES)
+            (st, lim), = time_capped(graph, pl, stride, cap_k)
+        else:
+            b, e = int(root_range[0]), int(root_range[1])
+            n = min(e - b, PROBE_EDGES)
+            pb, pe = (e, e + n) if e + n <= nnz else (max(0, b - n), b)
+            if pe <= pb:
+                pb, pe = b, e
+            st, lim = range_capped(graph, pl, pb, pe, cap_k)
+        out.append((float(st.wave_cycles), lim))
+        if i == 0:
+            cap_k = max(64, 4 * work_steps_bound(st) // 1024 + 1)
+    return out
+
+
+def _guard(graph: Graph, fallback: Plan, plan: Plan, info, root_range):
+    """Keep the learned plan only when the probe measures less work for it."""
+    (w_cost, _), (w_new, lim) = probe_work(graph, [fallback, plan], root_range)
+    info["probe_work"] = [w_cost, w_new]
+    if lim or np.log2(max(w_new, 1.0)) > np.log2(max(w_cost, 1.0)) - PROBE_BITS:
+        info["fallback"] = True
+        info["reason"] = "probe: no less measured work"
+        return fallback, info
+    return plan, info
+
+
 def choose_plan(model: LearnedOrder, pattern: Pattern, graph: Graph, mode: int, fallback: Optional[Plan] = None,
-                top: int = 12, keep_root: bool = False):
+                top: int = 12, keep_root: bool = False, root_range=None, probe: bool = True):
     """Order with the lowest predicted work among the planner's `top`
     candidates, taken only when the network predicts it at least `margin` bits
-    below the planner's own order and the reference's sampled cost model does
-    not put it more than `judge_bits` above that order (the fallback).
+    below the planner's own order, the reference's sampled cost model does
+    not put it more than `judge_bits` above that order (the fallback), and (with
+    probe) the measured probe (probe_work) confirms less work.
     keep_root: choose only among the orders that count the same embeddings as
-    the fallback's order over any root subset (root_preserving)."""
+    the fallback's order over any root subset (root_preserving); root_range =
+    (begin, end): the run covers those root edges only, and the features'
+    random walks start there."""
     if fallback is None:
         fallback = build_plan(pattern, graph, mode)
-    order, info = _proposal(model, pattern, graph, mode, fallback, top, keep_root)
+    order, info = _proposal(model, pattern, graph, mode, fallback, top, keep_root, root_range)
     if order is None:
         return fallback, info
-    return _judge(model, pattern, graph, mode, fallback, order, info)
+    plan, info = _judge(model, pattern, graph, mode, fallback, order, info)
+    if probe and plan is not fallback:
+        plan, info = _guard(graph, fallback, plan, info, root_range)
+    return plan, info
 
 
 def choose_plans(model: LearnedOrder, patterns: Sequence[Pattern], graph: Graph, mode: int,
-                 fallbacks: Sequence[Plan], top: int = 12, threads: int = 16, keep_root: bool = False):
+                 fallbacks: Sequence[Plan], top: int = 12, threads: int = 16, keep_root: bool = False,
+                 root_ranges=None, probe: bool = True):
     """choose_plan over a pattern set: feature sampling and the sampled cost
     model (host code, the GIL released in the library) run on `threads` threads;
-    the network's forward passes stay on the caller's thread."""
+    the network's forward passes stay on the caller's thread.  root_ranges =
+    (begins, ends): pattern i's run covers root edges [begins[i], ends[i])."""
+    rr = [None] * len(patterns) if root_ranges is None else list(zip(root_ranges[0], root_ranges[1]))
     with ThreadPoolExecutor(max_workers=threads) as ex:
         feats = list(ex.map(lambda a: ((None, None) if model.sizes and a[0].k not in model.sizes
                                        else _features(a[0], graph, mode, top,
-                                                      _root_of(a[1], a[0]) if keep_root else None)),
-                            list(zip(patterns, fallbacks))))
+                                                      _root_of(a[1], a[0]) if keep_root else None, a[2])),
+                            list(zip(patterns, fallbacks, rr))))
         props = []
         for p, fb, (cands, X) in zip(patterns, fallbacks, feats):
             props.append(_proposal_from(model, p, fb, cands, X))
@@ -296,8 +351,13 @@ def choose_plans(model: LearnedOrder, patterns: Sequence[Pattern], graph: Graph,
                                         _judge(model, a[2], graph, mode, a[3], a[0], a[1])),
                              [(o, inf, p, fb) for (o, inf), p, fb in zip(props, patterns, fallbacks)]))
     out = []
-    for (o, inf), fb, j in zip(props, fallbacks, judged):
-        out.append((fb, inf) if o is None else j)
+    for (o, inf), fb, j, r in zip(props, fallbacks, judged, rr):
+        if o is None:
+            out.append((fb, inf))
+        elif probe and j[0] is not fb:
+            out.append(_guard(graph, fb, j[0], j[1], r))  # GPU runs: on the caller's thread
+        else:
+            out.append(j)
     return out
 
 
@@ -305,9 +365,9 @@ def _root_of(plan: Plan, pattern: Pattern) -> List[int]:
     return list(plan.order)[: pattern.k]
 
 
-def _features(pattern: Pattern, graph: Graph, mode: int, top: int, keep_root_of=None):
+def _features(pattern: Pattern, graph: Graph, mode: int, top: int, keep_root_of=None, root_range=None):
     cands = candidate_orders(pattern, graph, mode, top, keep_root_of=keep_root_of)
-    return cands, np.stack([order_features(pattern, graph, o, mode, FEATURE_SAMPLES, 11) for o in cands])
+    return cands, np.stack([order_features(pattern, graph, o, mode, FEATURE_SAMPLES, 11, root_range) for o in cands])
 
 
 def _proposal_from(model: LearnedOrder, pattern: Pattern, fallback: Plan, cands, X):

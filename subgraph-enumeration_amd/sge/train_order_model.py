@@ -47,11 +47,14 @@ LOG_GRAPHS = {
 
 
 # evaluation sets: the whole job of each, exact, with the same executor for
-# both choices; the Chung-Lu graph is one the pairs are logged on (its
-# patterns are not), R-MAT scale 8 (seed 7) is a graph no pair is logged on
+# both choices; the Chung-Lu graph of seed 1 is one the pairs are logged on (its
+# patterns are not); the Chung-Lu graph of seed 9 (same parameters) and R-MAT
+# scale 8 (seed 7) are graphs no pair is logged on
 EVAL_SETS = {
     "c3": ("chung_lu(n=7115, m=100762, gamma=2.1, cap=1100, seed=1), 16 x 6-vertex", lambda: W.graph_c2(),
            W.six_vertex_set),
+    "chunglu9": ("chung_lu(n=7115, m=100762, gamma=2.1, cap=1100, seed=9), 16 x 6-vertex",
+                 lambda: sge.Graph.chung_lu(7115, 100762, 2.1, 1100.0, 9), W.six_vertex_set),
     "rmat8": ("rmat(scale=8, ef=16, seed=7), 78 x 7-vertex", lambda: W.graph_rmat(8, 7, device_build=True),
               W.seven_vertex_set),
 }
@@ -111,7 +114,7 @@ def main():
     ap.add_argument("--first", type=int, default=0, help="first training pattern to log")
     ap.add_argument("--count", type=int, default=None, help="number of training patterns to log")
     ap.add_argument("--data", default=None, help="comma-separated .npz files of logged pairs to fit from")
-    ap.add_argument("--eval", default="c3,rmat8", help="evaluation sets: " + ", ".join(sorted(EVAL_SETS)))
+    ap.add_argument("--eval", default="c3,chunglu9,rmat8", help="evaluation sets: " + ", ".join(sorted(EVAL_SETS)))
     ap.add_argument("--eval-only", action="store_true",
                     help="evaluate the model at --out on the benchmark set and rewrite its report; no training")
     args = ap.parse_args()

@@ -115,23 +115,47 @@ def test_gpu_training_reduces_loss():
 
 @pytest.mark.gpu
 def test_learned_order_gain_over_cost_model():
-    """The committed model (trained on patterns disjoint from these) chooses
-    orders for the 16 six-vertex patterns over the Chung-Lu graph; with the
-    cost-model fallback its choices must not be slower in total than the cost
-    model's, measured by the same executor on the whole job, and every choice
-    must give the same count."""
+    """The committed model (trained on (order, work) pairs of patterns disjoint
+    from these, logged on the Chung-Lu graph of seed 1 and on R-MAT scale 12)
+    chooses orders for the 16 six-vertex patterns over a Chung-Lu graph of the
+    same parameters but seed 9, a graph no pair was logged on.  Measured by the
+    same executor on the whole job (every changed choice gives the same count):
+    the learned choices must be at least 1.15x faster in total (1.21x
+    measured), and with the cost-model fallback, the sampled judge and the
+    measured probe no pattern may get more than 10% slower (timing noise)."""
     from sge import learned as L
 
     path = os.path.join(os.path.dirname(sge.__file__), "models", "order_mlp.npz")
     assert os.path.exists(path), "trained order model missing: python -m sge.train_order_model"
     model = L.load(path)
-    g = W.graph_c2()
+    g = sge.Graph.chung_lu(7115, 100762, 2.1, 1100.0, 9)
     rows = L.evaluate(model, g, W.six_vertex_set(), sge.MODE_IEP, stride=1, reps=2)
     t_cost = sum(r["cost_ms"] for r in rows)
     t_learn = sum(r["learned_ms"] for r in rows)
-    print(f"cost-model orders {t_cost:.2f} ms, learned orders {t_learn:.2f} ms")
+    ratios = [r["learned_ms"] / max(r["cost_ms"], 1e-3) for r in rows]
+    print(f"cost-model orders {t_cost:.2f} ms, learned orders {t_learn:.2f} ms ({t_cost / t_learn:.3f}x); "
+          f"per-pattern learned/cost {[round(x, 3) for x in ratios]}")
     assert not any(r["capped"] for r in rows)
-    assert t_learn <= 1.05 * t_cost
+    assert t_cost >= 1.15 * t_learn
+    assert max(ratios) <= 1.10
+
+
+@pytest.mark.gpu
+def test_learned_order_never_much_slower_on_rmat():
+    """On R-MAT scale 8 (seed 7; 78 seven-vertex patterns, a graph family the
+    C2 pairs do not cover and a graph no pair was logged on) the fallback,
+    judge and probe keep every learned choice within 10% of the cost-model
+    order (an earlier model without them made one pattern 11x slower)."""
+    from sge import learned as L
+
+    model = L.load(os.path.join(os.path.dirname(sge.__file__), "models", "order_mlp.npz"))
+    g = W.graph_rmat(8, 7)
+    rows = L.evaluate(model, g, W.seven_vertex_set(), sge.MODE_IEP, stride=1, reps=2)
+    ratios = [r["learned_ms"] / max(r["cost_ms"], 1e-3) for r in rows]
+    worst = int(np.argmax(ratios))
+    print(f"total cost {sum(r['cost_ms'] for r in rows):.1f} ms learned {sum(r['learned_ms'] for r in rows):.1f} ms; "
+          f"worst pattern {worst}: {ratios[worst]:.3f}")
+    assert max(ratios) <= 1.10
 
 
 @pytest.mark.gpu
