@@ -108,6 +108,9 @@ struct DPlan {
     // bit2: upper bound from s, bit3: must differ from s's image.
     uint32_t tail_kind, nterms;
     uint32_t t_par[MAX_TERMS], t_lower[MAX_TERMS], t_upper[MAX_TERMS], t_check[MAX_TERMS], t_sflags[MAX_TERMS];
+    // term j's F set starts from the candidate set of level (t_reuse >> 4j) & 15
+    // (15: from the parents' rows alone), a level whose constraints it implies
+    uint32_t t_reuse;
     // materialised bag: levels whose candidates come from the bag trie (the
     // trie list replaces the rows of bag parents; 0 = no bag)
     uint32_t bag;
@@ -209,7 +212,30 @@ struct WaveLds {
     uint32_t flag[SIB_BLOCK];  // 1: driver is the sibling's CSR row (target F), 0: driver F (target row)
     uint32_t mark[WAVE];       // phase 2: entry starting at each lane's probe slot (index + 1)
     uint32_t fst[FSTAGE];      // staged copies of small F sets (probed through LDS)
+#ifdef SGE_PHASE_PROFILE
+    unsigned long long ph[8];  // wavefront clock cycles per phase (PH_*), lane 0 only
+#endif
 };
+
+// Phase profile (builds with -DSGE_PHASE_PROFILE -DSGE_FSTAGE=380, so that the
+// counters fit the same LDS): wavefront clock cycles spent in each phase of the
+// search, summed over all waves and read back with sge_phase_profile_read.
+enum { PH_MAT, PH_EXPL, PH_TERMS, PH_STAGE, PH_P1, PH_P2, PH_P3, PH_ALL, PH_NUM };
+#ifdef SGE_PHASE_PROFILE
+__device__ unsigned long long g_phase[PH_NUM];
+#define PH_DECL unsigned long long _ph = __builtin_amdgcn_s_memtime();
+#define PH_MARK(w, i)                                                   \
+    do {                                                                \
+        const unsigned long long _n = __builtin_amdgcn_s_memtime();     \
+        if (__lane_id() == 0) (w).ph[i] += _n - _ph;                     \
+        _ph = _n;                                                       \
+    } while (0)
+#else
+#define PH_DECL
+#define PH_MARK(w, i) \
+    do {              \
+    } while (0)
+#endif
 
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
 
@@ -530,6 +556,7 @@ __device__ Src form_set(WaveLds &w, int ns, int nchk, bool may_alias, uint32_t *
 // Candidate set of level d (checks of level d applied now when the set is
 // materialised, or when it is read if it aliases a CSR row).
 __device__ __forceinline__ void materialize(const KParams &P, WaveLds &w, uint32_t *wb, int d, Acc &acc) {
+    PH_DECL
     const DPlan &pl = w.pl;
     bool has_lo, has_hi;
     uint32_t lo, hi;
@@ -542,11 +569,13 @@ __device__ __forceinline__ void materialize(const KParams &P, WaveLds &w, uint32
     w.pos[d] = c.b;
     w.iend[d] = c.e;
     acc.add(A_NODES, 1);
+    PH_MARK(w, PH_MAT);
 }
 
 // Exploration: form the parent intersection of every level whose last parent
 // is level d (and of the last level's sibling-independent part F), unclamped.
 __device__ __forceinline__ void explore_after(const KParams &P, WaveLds &w, uint32_t *wb, int d, Acc &acc) {
+    PH_DECL
     const DPlan &pl = w.pl;
     for (uint32_t j = d + 1; j <= pl.sib; ++j) {
         if (pl.ready[j] != (uint32_t)d || j < 2) continue;
@@ -557,6 +586,7 @@ __device__ __forceinline__ void explore_after(const KParams &P, WaveLds &w, uint
         int ns = gather_sources(P, w, pl.last, pl.f_parent, -1, false, false, 0, false, 0, acc);
         w.pre[MAXK] = form_set(w, ns, 0, true, buf_ptr(P, wb, P.pre_f), acc);
     }
+    PH_MARK(w, PH_EXPL);
 }
 
 // Per-term setup kept in LDS for a sibling batch.
@@ -643,6 +673,7 @@ __device__ __forceinline__ bool gbm_pays(const KParams &P, const WaveLds &w, int
 template <bool CAP>
 __device__ __forceinline__ bool count_last(const KParams &P, WaveLds &w, TermLds *terms, uint32_t *wb, uint32_t *gb,
                                            Acc &acc, uint32_t &spent, uint32_t &since) {
+    PH_DECL
     const DPlan &pl = w.pl;
     const int s = pl.sib;
     const int lane = lane_id();
@@ -681,9 +712,9 @@ __device__ __forceinline__ bool count_last(const KParams &P, WaveLds &w, TermLds
                 w.srcs[0] = sp;
                 ns = 1;
             } else {
-                int reuse = (K == 1) ? pl.reuse[pl.last] : -1;
-                if (reuse >= 0 && ((uint32_t)reuse >= (uint32_t)s || (pl.parent[reuse] & ~pl.t_par[j]))) reuse = -1;
-                ns = gather_sources(P, w, pl.last, pl.t_par[j], reuse, false, has_lo, lo, has_hi, hi, acc);
+                const uint32_t tr = (pl.t_reuse >> (4u * j)) & 15u;
+                ns = gather_sources(P, w, pl.last, pl.t_par[j], tr == 15u ? -1 : (int)tr, false, has_lo, lo, has_hi,
+                                    hi, acc);
             }
             int nchk = load_checks(w, pl.t_check[j]);
             t.F = form_set(w, ns, nchk, nchk == 0, buf_ptr(P, wb, P.term_base + j), acc);
@@ -692,6 +723,7 @@ __device__ __forceinline__ bool count_last(const KParams &P, WaveLds &w, TermLds
         terms[j] = t;
     }
     wave_sync();
+    PH_MARK(w, PH_TERMS);
     // ---- stage F sets in LDS so per-sibling probes hit LDS: a bitmap over
     // [min F, max F] when it fits, else a sorted copy when that fits, else (a
     // large F that the siblings' rows will probe) a bitmap in the wave's
@@ -743,6 +775,7 @@ __device__ __forceinline__ bool count_last(const KParams &P, WaveLds &w, TermLds
         if (used_g) __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");  // the bits are set before any probe
         wave_sync();
     }
+    PH_MARK(w, PH_STAGE);
     // ---- siblings
     Src sibs;
     uint32_t nsib;
@@ -851,6 +884,7 @@ __device__ __forceinline__ bool count_last(const KParams &P, WaveLds &w, TermLds
         }
         if (lane == 0) w.pre_sum[0] = 0;
         wave_sync();
+        PH_MARK(w, PH_P1);
         const uint32_t W = w.pre_sum[ne];
         steps += (ne + W + WAVE - 1) / WAVE;
         acc.add(A_INTER, ne);
@@ -900,6 +934,7 @@ __device__ __forceinline__ bool count_last(const KParams &P, WaveLds &w, TermLds
         }
         acc.add(A_BYTES, 4ull * W * 2);
         wave_sync();
+        PH_MARK(w, PH_P2);
         // phase 3: closed-form contributions (lane per sibling)
         for (uint32_t r0 = 0; r0 < nb; r0 += WAVE) {
             uint32_t i = r0 + lane;
@@ -922,6 +957,7 @@ __device__ __forceinline__ bool count_last(const KParams &P, WaveLds &w, TermLds
             }
         }
         wave_sync();
+        PH_MARK(w, PH_P3);
     }
     for (int off = WAVE / 2; off > 0; off >>= 1) local += __shfl_xor(local, off);
     acc.add(A_COUNT, local);
@@ -935,6 +971,7 @@ __device__ __forceinline__ bool count_last(const KParams &P, WaveLds &w, TermLds
         }
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");
     }
+    PH_MARK(w, PH_P3);
     spent += steps;
     since += steps;
     return stopped;
@@ -1132,6 +1169,11 @@ __global__ void __launch_bounds__(256, SGE_WAVES_PER_SIMD) wcoj_kernel(KParams P
     Acc acc{w.accv};
     if (lane == 0)
         for (int k = 0; k < A_NUM; ++k) w.accv[k] = 0;
+#ifdef SGE_PHASE_PROFILE
+    if (lane == 0)
+        for (int k = 0; k < PH_NUM; ++k) w.ph[k] = 0;
+    const unsigned long long t_kernel = __builtin_amdgcn_s_memtime();
+#endif
     uint32_t cur_plan = NONE;
     const bool cont = P.in_items != nullptr;
     const uint64_t nitems = cont ? min((uint64_t)*P.in_count, P.item_cap) : P.nitems;
@@ -1270,9 +1312,30 @@ __global__ void __launch_bounds__(256, SGE_WAVES_PER_SIMD) wcoj_kernel(KParams P
     if (CAP && since && lane == 0) atomicAdd(P.node_ctr, (unsigned long long)since);
     if (cur_plan != NONE) acc.add(A_CYCLES, __builtin_amdgcn_s_memtime() - t_item);
     if (cur_plan != NONE) flush(P, cur_plan, acc);
+#ifdef SGE_PHASE_PROFILE
+    if (lane == 0) {
+        w.ph[PH_ALL] = __builtin_amdgcn_s_memtime() - t_kernel;
+        for (int k = 0; k < PH_NUM; ++k) atomicAdd(&g_phase[k], w.ph[k]);
+    }
+#endif
 }
 
 }  // namespace
+
+#ifdef SGE_PHASE_PROFILE
+// Phase profile of a -DSGE_PHASE_PROFILE build: out[PH_NUM] = summed wavefront
+// clock cycles per phase since the last reset (materialise, exploration,
+// last-level terms, F staging, entry setup, flattened probes, closed forms,
+// whole kernel); reset != 0 zeroes the counters.
+extern "C" int sge_phase_profile_read(unsigned long long *out, int reset) {
+    if (hipMemcpyFromSymbol(out, HIP_SYMBOL(g_phase), sizeof(unsigned long long) * PH_NUM) != hipSuccess) return -1;
+    if (reset) {
+        static const unsigned long long zero[PH_NUM] = {};
+        if (hipMemcpyToSymbol(HIP_SYMBOL(g_phase), zero, sizeof(zero)) != hipSuccess) return -1;
+    }
+    return 0;
+}
+#endif
 
 // ============================================================================
 // Host side
@@ -1393,6 +1456,36 @@ DPlan to_dplan(const sge_plan &p) {
     }
     d.f_parent = d.parent[L] & ~(1u << s);
     d.f_ready = d.f_parent ? 31 - __builtin_clz(d.f_parent) : 0;
+    // A term's F formed from rows starts from the candidate set of an earlier
+    // level r < s (the reference's candidatesBefore, decomposition.h:15-57,
+    // applied per closed-form term) when r's parents are among the term's and
+    // every bound and check of r is one the term applies too: cand[r] then
+    // holds the intersection of those rows under a subset of the term's
+    // constraints, so F = cand[r] ∩ (the other rows) with the term's bounds and
+    // checks.  The most parents wins (ties: the later level).  Terms formed
+    // from two smaller terms' F sets, universes and bag plans take none.
+    d.t_reuse = 0xffffffffu;
+    if (!p.bag) {
+        for (uint32_t j = 0; j < d.nterms; ++j) {
+            const uint32_t par = d.t_par[j];
+            if (!par) continue;
+            const uint32_t B = j + 1, b1 = B & (~B + 1u);
+            if (d.tail_kind == 2 && B != b1 && d.t_par[b1 - 1] && d.t_par[(B ^ b1) - 1]) continue;
+            const uint32_t excl = d.t_check[j] | d.t_lower[j] | d.t_upper[j] | par;
+            int best = -1, bp = 1;
+            for (uint32_t r = 2; r < s; ++r) {
+                if ((d.parent[r] & par) != d.parent[r]) continue;
+                const int np = __builtin_popcount(d.parent[r]);
+                if (np < 2) continue;
+                if ((d.lower[r] & ~d.t_lower[j]) || (d.upper[r] & ~d.t_upper[j]) || (d.check[r] & ~excl)) continue;
+                if (np >= bp) {
+                    best = (int)r;
+                    bp = np;
+                }
+            }
+            if (best >= 0) d.t_reuse = (d.t_reuse & ~(15u << (4 * j))) | ((uint32_t)best << (4 * j));
+        }
+    }
     d.bag = p.bag;
     d.trie = nullptr;
     if (d.bag) {
