@@ -213,28 +213,41 @@ struct WaveLds {
     uint32_t mark[WAVE];       // phase 2: entry starting at each lane's probe slot (index + 1)
     uint32_t fst[FSTAGE];      // staged copies of small F sets (probed through LDS)
 #ifdef SGE_PHASE_PROFILE
-    unsigned long long ph[8];  // wavefront clock cycles per phase (PH_*), lane 0 only
+    unsigned long long ph[14];  // wavefront clock cycles per phase (PH_*), lane 0 only
+    unsigned long long ph_t;    // clock at the last phase switch
+    uint32_t ph_cur;            // current phase
 #endif
 };
 
-// Phase profile (builds with -DSGE_PHASE_PROFILE -DSGE_FSTAGE=380, so that the
-// counters fit the same LDS): wavefront clock cycles spent in each phase of the
-// search, summed over all waves and read back with sge_phase_profile_read.
-enum { PH_MAT, PH_EXPL, PH_TERMS, PH_STAGE, PH_P1, PH_P2, PH_P3, PH_ALL, PH_NUM };
+// Phase profile (builds with -DSGE_PHASE_PROFILE -DSGE_FSTAGE=348, so that the
+// counters fit the same LDS): every wavefront clock cycle is charged to the
+// phase the wave is in (phases nest: a function enters its phase and restores
+// the caller's on return), summed over all waves and read back with
+// sge_phase_profile_read.
+enum {
+    PH_OTHER, PH_MAT, PH_EXPL, PH_TERMS, PH_STAGE, PH_P1, PH_P2, PH_P3, PH_QUEUE, PH_ROOT, PH_DFS, PH_DONATE,
+    PH_ALL, PH_NUM
+};
 #ifdef SGE_PHASE_PROFILE
-__device__ unsigned long long g_phase[PH_NUM];
-#define PH_DECL unsigned long long _ph = __builtin_amdgcn_s_memtime();
-#define PH_MARK(w, i)                                                   \
-    do {                                                                \
-        const unsigned long long _n = __builtin_amdgcn_s_memtime();     \
-        if (__lane_id() == 0) (w).ph[i] += _n - _ph;                     \
-        _ph = _n;                                                       \
-    } while (0)
+__device__ unsigned long long g_phase[PH_NUM + 3];  // [PH_NUM]: queue fetches, [+1] waves, [+2] longest wave
+__device__ __forceinline__ uint32_t ph_switch(WaveLds &w, uint32_t i) {
+    uint32_t old = 0;
+    if (__lane_id() == 0) {
+        const unsigned long long now = __builtin_amdgcn_s_memtime();
+        old = w.ph_cur;
+        w.ph[old] += now - w.ph_t;
+        w.ph_t = now;
+        w.ph_cur = i;
+    }
+    return old;
+}
+#define PH_ENTER(w, i) const uint32_t _ph_prev = ph_switch(w, i)
+#define PH_SET(w, i) ph_switch(w, i)
+#define PH_LEAVE(w) ph_switch(w, _ph_prev)
 #else
-#define PH_DECL
-#define PH_MARK(w, i) \
-    do {              \
-    } while (0)
+#define PH_ENTER(w, i)
+#define PH_SET(w, i)
+#define PH_LEAVE(w)
 #endif
 
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
@@ -556,7 +569,7 @@ __device__ Src form_set(WaveLds &w, int ns, int nchk, bool may_alias, uint32_t *
 // Candidate set of level d (checks of level d applied now when the set is
 // materialised, or when it is read if it aliases a CSR row).
 __device__ __forceinline__ void materialize(const KParams &P, WaveLds &w, uint32_t *wb, int d, Acc &acc) {
-    PH_DECL
+    PH_ENTER(w, PH_MAT);
     const DPlan &pl = w.pl;
     bool has_lo, has_hi;
     uint32_t lo, hi;
@@ -569,13 +582,13 @@ __device__ __forceinline__ void materialize(const KParams &P, WaveLds &w, uint32
     w.pos[d] = c.b;
     w.iend[d] = c.e;
     acc.add(A_NODES, 1);
-    PH_MARK(w, PH_MAT);
+    PH_LEAVE(w);
 }
 
 // Exploration: form the parent intersection of every level whose last parent
 // is level d (and of the last level's sibling-independent part F), unclamped.
 __device__ __forceinline__ void explore_after(const KParams &P, WaveLds &w, uint32_t *wb, int d, Acc &acc) {
-    PH_DECL
+    PH_ENTER(w, PH_EXPL);
     const DPlan &pl = w.pl;
     for (uint32_t j = d + 1; j <= pl.sib; ++j) {
         if (pl.ready[j] != (uint32_t)d || j < 2) continue;
@@ -586,7 +599,7 @@ __device__ __forceinline__ void explore_after(const KParams &P, WaveLds &w, uint
         int ns = gather_sources(P, w, pl.last, pl.f_parent, -1, false, false, 0, false, 0, acc);
         w.pre[MAXK] = form_set(w, ns, 0, true, buf_ptr(P, wb, P.pre_f), acc);
     }
-    PH_MARK(w, PH_EXPL);
+    PH_LEAVE(w);
 }
 
 // Per-term setup kept in LDS for a sibling batch.
@@ -673,7 +686,7 @@ __device__ __forceinline__ bool gbm_pays(const KParams &P, const WaveLds &w, int
 template <bool CAP>
 __device__ __forceinline__ bool count_last(const KParams &P, WaveLds &w, TermLds *terms, uint32_t *wb, uint32_t *gb,
                                            Acc &acc, uint32_t &spent, uint32_t &since) {
-    PH_DECL
+    PH_ENTER(w, PH_TERMS);
     const DPlan &pl = w.pl;
     const int s = pl.sib;
     const int lane = lane_id();
@@ -723,7 +736,7 @@ __device__ __forceinline__ bool count_last(const KParams &P, WaveLds &w, TermLds
         terms[j] = t;
     }
     wave_sync();
-    PH_MARK(w, PH_TERMS);
+    PH_SET(w, PH_STAGE);
     // ---- stage F sets in LDS so per-sibling probes hit LDS: a bitmap over
     // [min F, max F] when it fits, else a sorted copy when that fits, else (a
     // large F that the siblings' rows will probe) a bitmap in the wave's
@@ -775,7 +788,7 @@ __device__ __forceinline__ bool count_last(const KParams &P, WaveLds &w, TermLds
         if (used_g) __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");  // the bits are set before any probe
         wave_sync();
     }
-    PH_MARK(w, PH_STAGE);
+    PH_SET(w, PH_P1);
     // ---- siblings
     Src sibs;
     uint32_t nsib;
@@ -884,7 +897,7 @@ __device__ __forceinline__ bool count_last(const KParams &P, WaveLds &w, TermLds
         }
         if (lane == 0) w.pre_sum[0] = 0;
         wave_sync();
-        PH_MARK(w, PH_P1);
+        PH_SET(w, PH_P2);
         const uint32_t W = w.pre_sum[ne];
         steps += (ne + W + WAVE - 1) / WAVE;
         acc.add(A_INTER, ne);
@@ -934,7 +947,7 @@ __device__ __forceinline__ bool count_last(const KParams &P, WaveLds &w, TermLds
         }
         acc.add(A_BYTES, 4ull * W * 2);
         wave_sync();
-        PH_MARK(w, PH_P2);
+        PH_SET(w, PH_P3);
         // phase 3: closed-form contributions (lane per sibling)
         for (uint32_t r0 = 0; r0 < nb; r0 += WAVE) {
             uint32_t i = r0 + lane;
@@ -957,8 +970,9 @@ __device__ __forceinline__ bool count_last(const KParams &P, WaveLds &w, TermLds
             }
         }
         wave_sync();
-        PH_MARK(w, PH_P3);
+        PH_SET(w, PH_P1);
     }
+    PH_SET(w, PH_P3);
     for (int off = WAVE / 2; off > 0; off >>= 1) local += __shfl_xor(local, off);
     acc.add(A_COUNT, local);
     if (used_g) {
@@ -971,7 +985,7 @@ __device__ __forceinline__ bool count_last(const KParams &P, WaveLds &w, TermLds
         }
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");
     }
-    PH_MARK(w, PH_P3);
+    PH_LEAVE(w);
     spent += steps;
     since += steps;
     return stopped;
@@ -1042,10 +1056,14 @@ __device__ __forceinline__ uint32_t pieces_of(const KParams &P, uint32_t b, uint
 }
 
 __device__ __forceinline__ void donate(const KParams &P, WaveLds &w, uint32_t plan, int l0, int d, int stop) {
+    PH_ENTER(w, PH_DONATE);
     const int lane = lane_id();
     uint32_t n = 0;
     for (int j = l0; j <= d && j < stop; ++j) n += pieces_of(P, give_from(w, j, d), w.iend[j]);
-    if (n == 0) return;
+    if (n == 0) {
+        PH_LEAVE(w);
+        return;
+    }
     unsigned long long base = 0;
     if (lane == 0) base = atomicAdd(P.out_count, (unsigned long long)n);
     base = __shfl(base, 0);
@@ -1053,6 +1071,7 @@ __device__ __forceinline__ void donate(const KParams &P, WaveLds &w, uint32_t pl
         // no room: mark the reserved slots that exist as empty, keep the work
         for (uint64_t slot = base; slot < P.item_cap && slot < base + n; ++slot)
             if (lane == 0) P.out_items[slot * ITEM_WORDS] = NONE;
+        PH_LEAVE(w);
         return;
     }
     uint64_t slot = base;
@@ -1074,6 +1093,7 @@ __device__ __forceinline__ void donate(const KParams &P, WaveLds &w, uint32_t pl
         }
         if (np) w.iend[j] = b;  // this wave keeps [pos, b)
     }
+    PH_LEAVE(w);
 }
 
 // Depth-first search from level l0 whose iteration range [pos, iend) is set,
@@ -1083,6 +1103,7 @@ __device__ __forceinline__ void donate(const KParams &P, WaveLds &w, uint32_t pl
 template <bool ENUM, bool CAP>
 __device__ __forceinline__ bool run_dfs(const KParams &P, WaveLds &w, TermLds *terms, uint32_t *wb, uint32_t *gb,
                                         Acc &acc, uint32_t plan, int l0, int stop) {
+    PH_ENTER(w, PH_DFS);
     const DPlan &pl = w.pl;
     uint32_t spent = 0, since = 0;  // wave steps since the last donation / node-cap report
     int d = l0;
@@ -1099,6 +1120,7 @@ __device__ __forceinline__ bool run_dfs(const KParams &P, WaveLds &w, TermLds *t
                     emit_last(P, w, wb, acc);
                 }
             } else if (count_last<CAP>(P, w, terms, wb, gb, acc, spent, since)) {
+                PH_LEAVE(w);
                 return true;  // work cap reached (reported by count_last)
             }
             --d;
@@ -1134,12 +1156,16 @@ __device__ __forceinline__ bool run_dfs(const KParams &P, WaveLds &w, TermLds *t
                 donate(P, w, plan, l0, d, stop);
                 spent = 0;
             }
-            if (cap_reached<CAP>(P, since)) return true;  // the host reports the call as stopped
+            if (cap_reached<CAP>(P, since)) {
+                PH_LEAVE(w);
+                return true;  // the host reports the call as stopped
+            }
         } else {
             --d;
         }
     }
     if (CAP && since && lane_id() == 0) atomicAdd(P.node_ctr, (unsigned long long)since);
+    PH_LEAVE(w);
     return false;
 }
 
@@ -1170,8 +1196,11 @@ __global__ void __launch_bounds__(256, SGE_WAVES_PER_SIMD) wcoj_kernel(KParams P
     if (lane == 0)
         for (int k = 0; k < A_NUM; ++k) w.accv[k] = 0;
 #ifdef SGE_PHASE_PROFILE
-    if (lane == 0)
-        for (int k = 0; k < PH_NUM; ++k) w.ph[k] = 0;
+    if (lane == 0) {
+        for (int k = 0; k <= PH_NUM; ++k) w.ph[k] = 0;
+        w.ph_cur = PH_OTHER;
+        w.ph_t = __builtin_amdgcn_s_memtime();
+    }
     const unsigned long long t_kernel = __builtin_amdgcn_s_memtime();
 #endif
     uint32_t cur_plan = NONE;
@@ -1187,15 +1216,20 @@ __global__ void __launch_bounds__(256, SGE_WAVES_PER_SIMD) wcoj_kernel(KParams P
             if (cur_plan != NONE) acc.add(A_CYCLES, now - t_item);
             t_item = now;
         }
+        PH_SET(w, PH_QUEUE);
         unsigned long long item = 0;
         if (lane == 0) item = atomicAdd(P.queue, 1ull);
         item = __shfl(item, 0);
+#ifdef SGE_PHASE_PROFILE
+        if (lane == 0) w.ph[PH_NUM] += 1;  // fetches
+#endif
         if (item >= nitems) break;
         if (CAP && (++fetched & 15u) == 0) {  // every 16th item: has the work cap been reached?
             unsigned long long spent = 0;
             if (lane == 0) spent = __hip_atomic_load(P.node_ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             if (__shfl(spent, 0) >= P.node_cap) break;
         }
+        PH_SET(w, PH_ROOT);
         if (cont) {
             // ---- continuation item: rebuild the path, then search its range
             const uint32_t *it = P.in_items + item * ITEM_WORDS;
@@ -1313,9 +1347,12 @@ __global__ void __launch_bounds__(256, SGE_WAVES_PER_SIMD) wcoj_kernel(KParams P
     if (cur_plan != NONE) acc.add(A_CYCLES, __builtin_amdgcn_s_memtime() - t_item);
     if (cur_plan != NONE) flush(P, cur_plan, acc);
 #ifdef SGE_PHASE_PROFILE
+    PH_SET(w, PH_OTHER);
     if (lane == 0) {
         w.ph[PH_ALL] = __builtin_amdgcn_s_memtime() - t_kernel;
-        for (int k = 0; k < PH_NUM; ++k) atomicAdd(&g_phase[k], w.ph[k]);
+        for (int k = 0; k <= PH_NUM; ++k) atomicAdd(&g_phase[k], w.ph[k]);
+        atomicAdd(&g_phase[PH_NUM + 1], 1ull);
+        atomicMax(&g_phase[PH_NUM + 2], w.ph[PH_ALL]);
     }
 #endif
 }
@@ -1324,13 +1361,16 @@ __global__ void __launch_bounds__(256, SGE_WAVES_PER_SIMD) wcoj_kernel(KParams P
 
 #ifdef SGE_PHASE_PROFILE
 // Phase profile of a -DSGE_PHASE_PROFILE build: out[PH_NUM] = summed wavefront
-// clock cycles per phase since the last reset (materialise, exploration,
+// clock cycles per phase since the last reset (other, materialise, exploration,
 // last-level terms, F staging, entry setup, flattened probes, closed forms,
-// whole kernel); reset != 0 zeroes the counters.
+// queue, root and continuation set-up, DFS iteration, donation, whole kernel),
+// out[PH_NUM] = queue fetches, out[PH_NUM + 1] = wavefronts, out[PH_NUM + 2] =
+// the longest wavefront's cycles; reset != 0 zeroes the counters.
 extern "C" int sge_phase_profile_read(unsigned long long *out, int reset) {
-    if (hipMemcpyFromSymbol(out, HIP_SYMBOL(g_phase), sizeof(unsigned long long) * PH_NUM) != hipSuccess) return -1;
+    if (hipMemcpyFromSymbol(out, HIP_SYMBOL(g_phase), sizeof(unsigned long long) * (PH_NUM + 3)) != hipSuccess)
+        return -1;
     if (reset) {
-        static const unsigned long long zero[PH_NUM] = {};
+        static const unsigned long long zero[PH_NUM + 3] = {};
         if (hipMemcpyToSymbol(HIP_SYMBOL(g_phase), zero, sizeof(zero)) != hipSuccess) return -1;
     }
     return 0;
@@ -2736,7 +2776,7 @@ constexpr size_t align256(size_t x) { return (x + 255) / 256 * 256; }
 
 constexpr NoallocLayout noalloc_layout() {
     NoallocLayout L{};
-    L.out = 256;
+    L.out = 256;  // the queue counter and the control words first
     L.plans = align256(L.out + sizeof(PlanOut) * NOALLOC_MAX_PLANS);
     L.items0 = align256(L.plans + sizeof(DPlan) * NOALLOC_MAX_PLANS);
     L.items1 = align256(L.items0 + sizeof(uint32_t) * ITEM_WORDS * NOALLOC_ITEMS);
@@ -2857,7 +2897,8 @@ extern "C" int sge_count_noalloc(sge_graph *g, const sge_plan *plans, uint32_t n
     cfg.lds = (size_t)(cfg.block / WAVE) * (sizeof(WaveLds) + MAX_TERMS * sizeof(TermLds));
     cfg.grid = (int)(waves / (uint64_t)(cfg.block / WAVE));
     cfg.head = tune->head_items ? (tune->head_items >= (1u << 30) ? 0u : tune->head_items) : (uint32_t)waves;
-    unsigned long long *ctl = reinterpret_cast<unsigned long long *>(base);  // [0] queue, [1..2] items, [3] cap
+    unsigned long long *queue = reinterpret_cast<unsigned long long *>(base);  // [0] queue counter
+    unsigned long long *ctl = queue + 1;  // [0..1] continuation item counts, [2] work-cap counter
     PlanOut *dout = reinterpret_cast<PlanOut *>(base + L.out);
     kp.off = (const uint32_t *)g->d_off;
     kp.nbr = (const uint32_t *)g->d_nbr;
@@ -2870,7 +2911,7 @@ extern "C" int sge_count_noalloc(sge_graph *g, const sge_plan *plans, uint32_t n
     kp.chunk = cfg.chunk;
     kp.head = cfg.head;
     kp.nitems = items_per_plan(g->h.nnz, cfg) * nplans;
-    kp.queue = ctl;
+    kp.queue = queue;
     kp.scratch = reinterpret_cast<uint32_t *>(base + L.slabs);
     kp.gbm = nullptr;  // large F sets are probed by search instead of a global bitmap
     kp.out = dout;
@@ -2880,30 +2921,30 @@ extern "C" int sge_count_noalloc(sge_graph *g, const sge_plan *plans, uint32_t n
     kp.probe_ratio = cfg.probe_ratio;
     kp.bitmap_ratio = cfg.bitmap_ratio;
     kp.node_cap = cfg.node_cap;
-    kp.node_ctr = ctl + 3;
+    kp.node_ctr = ctl + 2;
     uint32_t *items[2] = {reinterpret_cast<uint32_t *>(base + L.items0), reinterpret_cast<uint32_t *>(base + L.items1)};
     hipStream_t st = nullptr;  // the null stream: nothing to create
-    reset_kernel<<<(nplans + 255) / 256, 256, 0, st>>>(dout, nplans, ctl, nullptr, kp.node_ctr);
-    reset_pass_kernel<<<1, 64, 0, st>>>(ctl, ctl + 1);
+    reset_kernel<<<(nplans + 255) / 256, 256, 0, st>>>(dout, nplans, queue, nullptr, kp.node_ctr);
+    reset_pass_kernel<<<1, 64, 0, st>>>(queue, ctl);
     HIP_TRY(hipGetLastError());
     int cur = 0;
     bool limited = false;
     for (int pass = 0;; ++pass) {
         if (pass > 0) {
             unsigned long long hc[3] = {0, 0, 0};  // item counts, work spent
-            HIP_TRY(hipMemcpy(hc, ctl + 1, sizeof(hc), hipMemcpyDeviceToHost));
+            HIP_TRY(hipMemcpy(hc, ctl, sizeof(hc), hipMemcpyDeviceToHost));
             if (kp.node_cap && hc[2] >= kp.node_cap) {
                 limited = true;
                 break;
             }
             if (hc[cur] == 0) break;
             kp.in_items = items[cur];
-            kp.in_count = ctl + 1 + cur;
+            kp.in_count = ctl + cur;
             cur ^= 1;
-            reset_pass_kernel<<<1, 64, 0, st>>>(ctl, ctl + 1 + cur);
+            reset_pass_kernel<<<1, 64, 0, st>>>(queue, ctl + cur);
         }
         kp.out_items = items[cur];
-        kp.out_count = ctl + 1 + cur;
+        kp.out_count = ctl + cur;
         if (pass == 0 && kp.nitems == 0) break;
         launch_wcoj(false, cfg, st, kp);
         HIP_TRY(hipGetLastError());
