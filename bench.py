@@ -37,10 +37,12 @@ from its own sources into oracle/_ref/) on all host cores beside it.
 Other whole-job workloads: `--workload sixv|house|rmat8` (the C3 pass alone,
 house + K5-e over the Chung-Lu graph, the 78 patterns over R-MAT scale 8).
 Measurement sweeps that print their own JSON: `--workload bags` (plans with and
-without a materialised bag) and `--workload graph_build` (host against GPU CSR
-build).
+without a materialised bag), `--workload graph_build` (host against GPU CSR
+build) and `--workload phases` (wavefront cycles per search phase; needs the
+phase-profile library: `make -C subgraph-enumeration_amd libsge_phase.so`, then
+SGE_LIB=subgraph-enumeration_amd/libsge_phase.so).
 
-    python bench.py [--steps K] [--warmup W] [--workload c4|sixv|house|rmat8|bags|graph_build] [--span S]
+    python bench.py [--steps K] [--warmup W] [--workload c4|sixv|house|rmat8|bags|graph_build|phases] [--span S]
                     [--no-c3] [--no-cpu]
 """
 import argparse
@@ -423,6 +425,70 @@ def bag_times(args):
     return out
 
 
+PHASES = ["other", "materialize", "explore", "terms", "stage", "probe_setup", "probes", "closed_forms", "queue",
+          "root_setup", "dfs_iteration", "donate"]
+
+
+def phase_profile(args):
+    """Wavefront clock cycles per search phase of one batched pass (cost-model
+    plans), from a -DSGE_PHASE_PROFILE build of the executor: the Chung-Lu
+    whole job (16 six-vertex patterns) and four of its patterns alone, and the
+    R-MAT 20 and R-MAT 23 root-edge ranges.  `busy` = the phases' cycles over
+    (wavefronts x longest wavefront), all dispatches of the pass together."""
+    import ctypes as C
+
+    L = sge.lib()
+    if not hasattr(L, "sge_phase_profile_read"):
+        raise SystemExit("phases: load the phase-profile build (SGE_LIB=subgraph-enumeration_amd/libsge_phase.so)")
+    L.sge_phase_profile_read.restype = C.c_int
+    L.sge_phase_profile_read.argtypes = [C.c_void_p, C.c_int]
+    n = len(PHASES) + 1
+
+    def read():
+        a = np.zeros(n + 3, np.uint64)
+        if L.sge_phase_profile_read(a.ctypes.data, 1) != 0:
+            raise SystemExit("phases: sge_phase_profile_read failed")
+        return a.astype(np.float64)
+
+    out = {"workload": "phases", "runs": []}
+
+    def one(tag, g, plans, ranges=None, want=None):
+        b = sge.Batch(g, plans, ranges=ranges)
+        c, _ = b.run()  # warm-up
+        read()
+        c, st = b.run()
+        a = read()
+        b.close()
+        if want is not None and c != want:
+            log(f"phases {tag}: COUNT MISMATCH")
+            sys.exit(3)
+        tot = a[n - 1]
+        row = {"run": tag, "kernel_ms": st.kernel_ms,
+               "share": {p: a[i] / tot for i, p in enumerate(PHASES) if a[i] > 0},
+               "queue_fetches": a[n], "wavefronts": a[n + 1], "busy": tot / max(a[n + 1] * a[n + 2], 1.0)}
+        log(f"phases {tag}: {st.kernel_ms:.1f} ms " + ", ".join(f"{k} {100 * v:.1f}%" for k, v in row['share'].items()))
+        out["runs"].append(row)
+
+    g2 = W.graph_c2()
+    g2.to_device()
+    p2 = [sge.build_plan(p, g2, sge.MODE_IEP) for p in W.six_vertex_set()]
+    one("chunglu7115_sixvertex16 whole job", g2, p2)
+    for i in (1, 3, 7, 11):
+        one(f"chunglu7115 six-vertex pattern {i}", g2, [p2[i]])
+    for name, gfn, pfn in (("rmat20_sevenvertex78_subsets", lambda: W.graph_c4(device_build=True), W.seven_vertex_set),
+                           ("rmat23_sixvertex16_k5e_subsets", lambda: W.graph_c5(device_build=True),
+                            lambda: W.six_vertex_set() + [W.named("near5clique")])):
+        with open(os.path.join(ROOT, "tests", "golden", "large_" + ("c4" if "rmat20" in name else "c5") + ".json")) as f:
+            subs = json.load(f)["subsets"]
+        g = gfn()
+        pats = pfn()
+        plans = [sge.build_plan(pats[e["pattern"]], g, sge.MODE_IEP, e["plan"]["order"]) for e in subs]
+        ranges = (np.array([e["begin"] for e in subs], np.uint64), np.array([e["end"] for e in subs], np.uint64))
+        one(name, g, plans, ranges, [e["count"] for e in subs])
+        g.close()
+    return out
+
+
 def graph_build_times(args):
     """The CSR build on the host and on the GPU for R-MAT scales 18, 20, 21
     (seed 4): both deduplicate, drop self-loops, relabel by ascending degree
@@ -473,6 +539,8 @@ def main():
         res = bag_times(args)
     elif args.workload == "graph_build":
         res = graph_build_times(args)
+    elif args.workload == "phases":
+        res = phase_profile(args)
     else:
         res = whole_job(args, args.workload)
     print(json.dumps(res))
