@@ -146,6 +146,7 @@ struct sge_graph {
     double build_ms = 0;  // wall time of the CSR build
     // device mirror (executor.hip)
     void *d_off = nullptr, *d_nbr = nullptr, *d_src = nullptr;
+    void *d_skip = nullptr;  // skip index of the rows (executor.hip), built at the first launch
     int device = -1;
     void *split_cache = nullptr;  // executor-owned: per-shape subtask prefix arrays on the device
     sge::GraphSummary summary;

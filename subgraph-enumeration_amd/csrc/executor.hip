@@ -133,6 +133,8 @@ struct PlanOut {
 
 struct KParams {
     const uint32_t *off, *nbr, *src;
+    // skip index of the CSR: skip[i] = nbr[SKIP_G * i] (null: searches read the rows only)
+    const uint32_t *skip;
     uint32_t n;
     uint64_t nnz;
     const DPlan *plans;
@@ -345,6 +347,44 @@ __device__ __forceinline__ void clamp_range(PT a, uint32_t &b, uint32_t &e, bool
     if (e < b) e = b;
 }
 
+// Searches of long CSR rows go through the skip index first: skip[i] =
+// nbr[SKIP_G * i] for every SKIP_G-th CSR position, so a search of a row range
+// [b, e) is a binary search over the range's block starts in the skip array
+// (32x smaller, so the hub rows' blocks stay in L2) and then one within one
+// block of SKIP_G elements (one or two cache lines), instead of a binary
+// search whose middle steps each miss L2 on a 64K-element hub row.  Rows
+// shorter than SKIP_MIN are searched directly.
+constexpr uint32_t SKIP_G = 32;
+constexpr uint32_t SKIP_MIN = 1024;
+
+// first index in [b, e) of the CSR with nbr[i] >= key
+__device__ __forceinline__ uint32_t row_lower_bound(const KParams &P, uint32_t b, uint32_t e, uint32_t key) {
+    if (P.skip == nullptr || e - b < SKIP_MIN) return lower_bound(gl(P.nbr), b, e, key);
+    const uint32_t ib = (b + SKIP_G - 1) / SKIP_G, ie = (e - 1) / SKIP_G;  // blocks starting in [b, e)
+    const uint32_t j = lower_bound(gl(P.skip), ib, ie + 1, key);          // first block start >= key
+    const uint32_t lo = (j == ib) ? b : SKIP_G * (j - 1) + 1;             // nbr[SKIP_G (j-1)] < key
+    const uint32_t hi = (j <= ie) ? SKIP_G * j : e;                        // nbr[SKIP_G j] >= key
+    return lower_bound(gl(P.nbr), lo, hi, key);
+}
+
+// first index in [b, e) of the CSR with nbr[i] > key
+__device__ __forceinline__ uint32_t row_upper_bound(const KParams &P, uint32_t b, uint32_t e, uint32_t key) {
+    return key == 0xffffffffu ? e : row_lower_bound(P, b, e, key + 1);
+}
+
+__device__ __forceinline__ bool row_contains(const KParams &P, uint32_t b, uint32_t e, uint32_t key) {
+    const uint32_t p = row_lower_bound(P, b, e, key);
+    return p < e && P.nbr[p] == key;
+}
+
+// clamp_range for a CSR row range
+__device__ __forceinline__ void row_clamp(const KParams &P, uint32_t &b, uint32_t &e, bool has_lo, uint32_t lo,
+                                          bool has_hi, uint32_t hi) {
+    if (has_lo) b = row_upper_bound(P, b, e, lo);
+    if (has_hi) e = row_lower_bound(P, b, e, hi);
+    if (e < b) e = b;
+}
+
 __device__ __forceinline__ uint64_t choosec(uint64_t n, uint32_t k) {
     if ((uint64_t)k > n) return 0;
     if (k == 0) return 1;
@@ -404,7 +444,8 @@ __device__ __forceinline__ bool is_checked(const WaveLds &w, uint32_t mask, uint
 // elements are loaded into LDS with coalesced reads and searched there; elements
 // beyond them are searched in global memory.  Survivors are compacted with a wave
 // ballot + popcount prefix.  Returns the survivor count (uniform).
-__device__ __forceinline__ uint32_t wave_intersect(WaveLds &w, int ns, int nchk, uint32_t *out, Acc &acc) {
+__device__ __forceinline__ uint32_t wave_intersect(const KParams &P, WaveLds &w, int ns, int nchk, uint32_t *out,
+                                                   Acc &acc) {
     if (ns <= 0) return 0;
     int drv = 0;
     uint64_t bytes = 0;
@@ -429,6 +470,7 @@ __device__ __forceinline__ uint32_t wave_intersect(WaveLds &w, int ns, int nchk,
         for (int i = 0; i < ns; ++i) {
             if (i == drv) continue;
             const guint *a = w.srcs[i].ptr;
+            const bool row = a == gl(P.nbr);  // a CSR row: searched through the skip index
             const uint32_t e = w.srcs[i].e, c0 = w.cur[i];
             const uint32_t rem = e - c0;
             uint32_t p;
@@ -442,12 +484,12 @@ __device__ __forceinline__ uint32_t wave_intersect(WaveLds &w, int ns, int nchk,
                     p = c0 + pos;
                     in = w.fst[pos] == x;  // pos < n: fst[n - 1] >= x
                 } else {
-                    p = lower_bound_wide(a, c0 + n, e, x);
+                    p = row ? row_lower_bound(P, c0 + n, e, x) : lower_bound_wide(a, c0 + n, e, x);
                     in = p < e && a[p] == x;
                 }
                 wave_sync();  // the next source is staged over these words
             } else {
-                p = lower_bound_wide(a, c0, e, x);
+                p = row ? row_lower_bound(P, c0, e, x) : lower_bound_wide(a, c0, e, x);
                 in = p < e && a[p] == x;
             }
             ok = ok && in;
@@ -537,7 +579,7 @@ __device__ int gather_sources(const KParams &P, WaveLds &w, int d, uint32_t pmas
         s.ptr = gl(P.nbr);
         s.b = P.off[v];
         s.e = P.off[v + 1];
-        clamp_range(s.ptr, s.b, s.e, has_lo, lo, has_hi, hi);
+        row_clamp(P, s.b, s.e, has_lo, lo, has_hi, hi);
         w.srcs[ns++] = s;
         acc.add(A_BYTES, 8);
         acc.add(A_MODEL, 8);
@@ -547,7 +589,7 @@ __device__ int gather_sources(const KParams &P, WaveLds &w, int d, uint32_t pmas
 
 // Form a set from w.srcs: alias the single source when allowed, else
 // intersect into the scratch buffer `dst`.
-__device__ Src form_set(WaveLds &w, int ns, int nchk, bool may_alias, uint32_t *dst, Acc &acc) {
+__device__ Src form_set(const KParams &P, WaveLds &w, int ns, int nchk, bool may_alias, uint32_t *dst, Acc &acc) {
     if (ns <= 0) {
         Src z;
         z.ptr = gl(dst);
@@ -562,7 +604,7 @@ __device__ Src form_set(WaveLds &w, int ns, int nchk, bool may_alias, uint32_t *
     Src r;
     r.ptr = gl(dst);
     r.b = 0;
-    r.e = wave_intersect(w, ns, nchk, dst, acc);
+    r.e = wave_intersect(P, w, ns, nchk, dst, acc);
     return r;
 }
 
@@ -577,7 +619,7 @@ __device__ __forceinline__ void materialize(const KParams &P, WaveLds &w, uint32
     bool use_pre = pl.mode == SGE_MODE_EXPLORE;
     int ns = gather_sources(P, w, d, pl.parent[d], use_pre ? -1 : pl.reuse[d], use_pre, has_lo, lo, has_hi, hi, acc);
     int nchk = load_checks(w, pl.check[d]);
-    const Src c = form_set(w, ns, nchk, true, buf_ptr(P, wb, d), acc);
+    const Src c = form_set(P, w, ns, nchk, true, buf_ptr(P, wb, d), acc);
     w.cand[d] = c;
     w.pos[d] = c.b;
     w.iend[d] = c.e;
@@ -593,11 +635,11 @@ __device__ __forceinline__ void explore_after(const KParams &P, WaveLds &w, uint
     for (uint32_t j = d + 1; j <= pl.sib; ++j) {
         if (pl.ready[j] != (uint32_t)d || j < 2) continue;
         int ns = gather_sources(P, w, j, pl.parent[j], -1, false, false, 0, false, 0, acc);
-        w.pre[j] = form_set(w, ns, 0, true, buf_ptr(P, wb, P.ex_base + j), acc);
+        w.pre[j] = form_set(P, w, ns, 0, true, buf_ptr(P, wb, P.ex_base + j), acc);
     }
     if (pl.f_parent && pl.f_ready == (uint32_t)d) {
         int ns = gather_sources(P, w, pl.last, pl.f_parent, -1, false, false, 0, false, 0, acc);
-        w.pre[MAXK] = form_set(w, ns, 0, true, buf_ptr(P, wb, P.pre_f), acc);
+        w.pre[MAXK] = form_set(P, w, ns, 0, true, buf_ptr(P, wb, P.pre_f), acc);
     }
     PH_LEAVE(w);
 }
@@ -617,8 +659,8 @@ struct TermLds {
 // Membership of x in term t's F between F positions [tlo, thi) (a sibling's row
 // drove x, so its bounds already hold): one bit test for a bitmap, else a
 // binary search in the LDS copy or in F itself.
-__device__ __forceinline__ bool term_has(const WaveLds &w, const TermLds &t, const uint32_t *gb, uint32_t x,
-                                         uint32_t tlo, uint32_t thi) {
+__device__ __forceinline__ bool term_has(const KParams &P, const WaveLds &w, const TermLds &t, const uint32_t *gb,
+                                         uint32_t x, uint32_t tlo, uint32_t thi) {
     if (t.flags & 48u) {
         if (x < t.bm_lo || x > t.bm_hi) return false;
         uint32_t r = x - t.bm_lo;
@@ -629,6 +671,7 @@ __device__ __forceinline__ bool term_has(const WaveLds &w, const TermLds &t, con
         return (word >> (r & 31u)) & 1u;
     }
     if (t.flags & 8u) return contains(&w.fst[t.st_off], tlo - t.F.b, thi - t.F.b, x);
+    if (t.F.ptr == gl(P.nbr)) return row_contains(P, tlo, thi, x);  // F is a CSR row
     return contains_wide(t.F.ptr, tlo, thi, x);
 }
 
@@ -715,7 +758,7 @@ __device__ __forceinline__ bool count_last(const KParams &P, WaveLds &w, TermLds
         } else if (from_terms) {
             w.srcs[0] = terms[b1 - 1].F;
             w.srcs[1] = terms[(B ^ b1) - 1].F;
-            t.F = form_set(w, 2, 0, false, buf_ptr(P, wb, P.term_base + j), acc);
+            t.F = form_set(P, w, 2, 0, false, buf_ptr(P, wb, P.term_base + j), acc);
             acc.add(A_NODES, 1);
         } else {
             int ns;
@@ -730,7 +773,7 @@ __device__ __forceinline__ bool count_last(const KParams &P, WaveLds &w, TermLds
                                     hi, acc);
             }
             int nchk = load_checks(w, pl.t_check[j]);
-            t.F = form_set(w, ns, nchk, nchk == 0, buf_ptr(P, wb, P.term_base + j), acc);
+            t.F = form_set(P, w, ns, nchk, nchk == 0, buf_ptr(P, wb, P.term_base + j), acc);
             acc.add(A_NODES, 1);
         }
         terms[j] = t;
@@ -843,11 +886,11 @@ __device__ __forceinline__ bool count_last(const KParams &P, WaveLds &w, TermLds
                     if (s_upper) { chi_v = chi ? min(chi_v, c) : c; chi = true; }
                     if (t.sflags & 1u) {  // the sibling is a parent: intersect with its row
                         uint32_t rb = P.off[c], re = P.off[c + 1];
-                        clamp_range(P.nbr, rb, re, clo, clo_v, chi, chi_v);
+                        row_clamp(P, rb, re, clo, clo_v, chi, chi_v);
                         if (t.flags & 4u) {
                             cntv = re - rb;
                             for (uint32_t m = t.chk; m; m &= m - 1)
-                                if (contains(P.nbr, rb, re, w.phi[__builtin_ctz(m)])) --cntv;
+                                if (row_contains(P, rb, re, w.phi[__builtin_ctz(m)])) --cntv;
                         } else {
                             uint32_t fb = t.F.b, fe = t.F.e;
                             clamp_range(t.F.ptr, fb, fe, s_lower, c, s_upper, c);
@@ -934,10 +977,10 @@ __device__ __forceinline__ bool count_last(const KParams &P, WaveLds &w, TermLds
                 bool hit;
                 if (fl & 1u) {
                     uint32_t x = P.nbr[en.y + off];
-                    hit = term_has(w, t, gb, x, en.z, en.w);
+                    hit = term_has(P, w, t, gb, x, en.z, en.w);
                 } else {
                     uint32_t x = t.F.ptr[en.y + off];
-                    hit = contains_wide(P.nbr, en.z, en.w, x);
+                    hit = row_contains(P, en.z, en.w, x);
                 }
                 if (hit) {
                     if (per_entry) atomicAdd(&w.cnt[a], 1u);
@@ -1792,16 +1835,36 @@ void launch_wcoj(bool enumerate, const LaunchCfg &cfg, hipStream_t st, const KPa
         wcoj_kernel<false, false><<<cfg.grid, cfg.block, cfg.lds, st>>>(kp);
 }
 
+__global__ void skip_kernel(const uint32_t *__restrict__ nbr, uint64_t nblk, uint32_t *__restrict__ skip) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nblk) skip[i] = nbr[i * SKIP_G];
+}
+
+// The graph's skip index (skip[i] = nbr[SKIP_G i], i < ceil(nnz / SKIP_G)),
+// built on the device at the first launch that needs it and freed with the
+// device mirror; graphs without a row of SKIP_MIN elements get none.
+int ensure_skip(sge_graph *g, hipStream_t st) {
+    if (g->d_skip || g->h.max_degree < SKIP_MIN || !g->d_nbr) return SGE_OK;
+    const uint64_t nblk = (g->h.nnz + SKIP_G - 1) / SKIP_G;
+    HIP_TRY(hipMalloc(&g->d_skip, sizeof(uint32_t) * nblk));
+    skip_kernel<<<(unsigned)((nblk + 255) / 256), 256, 0, st>>>((const uint32_t *)g->d_nbr, nblk, (uint32_t *)g->d_skip);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(st));
+    return SGE_OK;
+}
+
 // One counting launch over `np` device plans (already uploaded at `dplans`).
 int run_launch(sge_graph *g, const DPlan *dplans, uint32_t np, PlanOut *dout, uint64_t stride, uint64_t max_sub,
                const LaunchCfg &cfg, bool enumerate, uint32_t *emb, uint64_t emb_cap, unsigned long long *emb_n,
                float *ms) {
     DeviceCtx &c = dctx();
+    if (int rc = ensure_skip(g, c.stream)) return rc;
     KParams kp;
     std::memset(&kp, 0, sizeof(kp));
     kp.off = (const uint32_t *)g->d_off;
     kp.nbr = (const uint32_t *)g->d_nbr;
     kp.src = (const uint32_t *)g->d_src;
+    kp.skip = (const uint32_t *)g->d_skip;
     kp.n = g->h.n;
     kp.nnz = g->h.nnz;
     kp.plans = dplans;
@@ -2161,7 +2224,8 @@ extern "C" void sge_graph_device_free(sge_graph *g) {
     if (g->d_off) (void)hipFree(g->d_off);
     if (g->d_nbr) (void)hipFree(g->d_nbr);
     if (g->d_src) (void)hipFree(g->d_src);
-    g->d_off = g->d_nbr = g->d_src = nullptr;
+    if (g->d_skip) (void)hipFree(g->d_skip);
+    g->d_off = g->d_nbr = g->d_src = g->d_skip = nullptr;
     free_split_cache(g);
 }
 
@@ -2542,9 +2606,14 @@ static int batch_create(sge_graph *g, const sge_plan *plans, uint32_t nplans, co
     if ((e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking)) != hipSuccess) return fail(e, "stream");
     if ((e = hipEventCreate(&b->ev0)) != hipSuccess) return fail(e, "event");
     if ((e = hipEventCreate(&b->ev1)) != hipSuccess) return fail(e, "event");
+    if (int rc = ensure_skip(g, b->stream)) {
+        sge_batch_destroy(b);
+        return rc;
+    }
     kp.off = (const uint32_t *)g->d_off;
     kp.nbr = (const uint32_t *)g->d_nbr;
     kp.src = (const uint32_t *)g->d_src;
+    kp.skip = (const uint32_t *)g->d_skip;
     kp.n = g->h.n;
     kp.nnz = g->h.nnz;
     kp.plans = b->dplans;
@@ -2903,6 +2972,7 @@ extern "C" int sge_count_noalloc(sge_graph *g, const sge_plan *plans, uint32_t n
     kp.off = (const uint32_t *)g->d_off;
     kp.nbr = (const uint32_t *)g->d_nbr;
     kp.src = (const uint32_t *)g->d_src;
+    kp.skip = (const uint32_t *)g->d_skip;  // built by an earlier launch, or none: this call allocates nothing
     kp.n = g->h.n;
     kp.nnz = g->h.nnz;
     kp.plans = dplans;
