@@ -434,7 +434,9 @@ def phase_profile(args):
     plans), from a -DSGE_PHASE_PROFILE build of the executor: the Chung-Lu
     whole job (16 six-vertex patterns) and four of its patterns alone, and the
     R-MAT 20 and R-MAT 23 root-edge ranges.  `busy` = the phases' cycles over
-    (wavefronts x longest wavefront), all dispatches of the pass together."""
+    (wavefronts x longest wavefront), all dispatches of the pass together;
+    `root_pass_busy` the same for the root dispatch alone (1 - its end-of-pass
+    idling)."""
     import ctypes as C
 
     L = sge.lib()
@@ -445,10 +447,10 @@ def phase_profile(args):
     n = len(PHASES) + 1
 
     def read():
-        a = np.zeros(n + 3, np.uint64)
+        a = np.zeros(n + 8, np.uint64)
         if L.sge_phase_profile_read(a.ctypes.data, 1) != 0:
             raise SystemExit("phases: sge_phase_profile_read failed")
-        return a.astype(np.float64)
+        return a
 
     out = {"workload": "phases", "runs": []}
 
@@ -457,7 +459,8 @@ def phase_profile(args):
         c, _ = b.run()  # warm-up
         read()
         c, st = b.run()
-        a = read()
+        raw = read()
+        a = raw.astype(np.float64)
         b.close()
         if want is not None and c != want:
             log(f"phases {tag}: COUNT MISMATCH")
@@ -465,8 +468,13 @@ def phase_profile(args):
         tot = a[n - 1]
         row = {"run": tag, "kernel_ms": st.kernel_ms,
                "share": {p: a[i] / tot for i, p in enumerate(PHASES) if a[i] > 0},
-               "queue_fetches": a[n], "wavefronts": a[n + 1], "busy": tot / max(a[n + 1] * a[n + 2], 1.0)}
-        log(f"phases {tag}: {st.kernel_ms:.1f} ms " + ", ".join(f"{k} {100 * v:.1f}%" for k, v in row['share'].items()))
+               "queue_fetches": a[n], "wavefronts": a[n + 1], "busy": tot / max(a[n + 1] * a[n + 2], 1.0),
+               "root_pass_busy": a[n + 2 + 1] / max(a[n + 2 + 3] * a[n + 2 + 2], 1.0),
+               "root_pass_longest_item": {"cycles": int(raw[n + 6] >> 32) << 8, "queue_index": int(raw[n + 6] & 0xffffffff),
+                                          "start_cycles": int(raw[n + 7] & 0xffffffff) << 8,
+                                          "longest_wave_cycles": int(raw[n + 4])}}
+        log(f"phases {tag}: {st.kernel_ms:.1f} ms " + ", ".join(f"{k} {100 * v:.1f}%" for k, v in row['share'].items())
+            + f"; root pass busy {100 * row['root_pass_busy']:.1f}%, longest item {row['root_pass_longest_item']}")
         out["runs"].append(row)
 
     g2 = W.graph_c2()

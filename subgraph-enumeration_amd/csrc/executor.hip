@@ -231,7 +231,11 @@ enum {
     PH_ALL, PH_NUM
 };
 #ifdef SGE_PHASE_PROFILE
-__device__ unsigned long long g_phase[PH_NUM + 3];  // [PH_NUM]: queue fetches, [+1] waves, [+2] longest wave
+// [PH_NUM]: queue fetches, [+1] waves, [+2] longest wave; root passes only: [+3]
+// summed wave cycles, [+4] longest wave, [+5] waves, [+6] the longest item
+// (cycles >> 8 in the high word, its queue index in the low word), [+7] the
+// same item's start (cycles >> 8 since the wave started) in the low word
+__device__ unsigned long long g_phase[PH_NUM + 8];
 __device__ __forceinline__ uint32_t ph_switch(WaveLds &w, uint32_t i) {
     uint32_t old = 0;
     if (__lane_id() == 0) {
@@ -722,13 +726,51 @@ __device__ __forceinline__ bool gbm_pays(const KParams &P, const WaveLds &w, int
 // The wave steps it spends (64-lane iterations of the entry setup and the
 // flattened probes), the unit of the donation budget and the work cap, are
 // added to `spent` and `since`; the cap is checked after every sibling batch.
-// A count is not split between waves: giving half of a hub's siblings to the
-// next pass re-forms its F sets there (measured on R-MAT scale 20 root subsets:
-// 4.7x slower).  Returns true when the cap stopped it (its count is then
-// partial).
+// A long count hands its remaining siblings to the next pass (when donation is
+// on): once the call has spent SIB_DONATE_AFTER donation budgets and at least
+// SIB_DONATE_MIN batches of siblings remain, the wave keeps the next batch and
+// writes the rest as up to `split` continuation items of level sib, each at
+// least SIB_DONATE_MIN batches long (each receiver re-forms the F sets once, so
+// small pieces would not pay: handing over half of a hub's siblings at every
+// budget was 4.7x slower on the R-MAT 20 ranges).  One call over a hub's
+// siblings had held a wavefront for the whole root pass there.  Returns true
+// when the cap stopped it (its count is then partial).
+constexpr uint32_t SIB_DONATE_AFTER = 16;
+constexpr uint32_t SIB_DONATE_MIN = 4;
+
+// Write [b, e) of level l's candidates as np continuation items of near-equal
+// length (plan, level, range, images of levels < l); false (nothing written)
+// when the output buffer is full.
+__device__ __forceinline__ bool give_range(const KParams &P, const WaveLds &w, uint32_t plan, int l, uint32_t b,
+                                           uint32_t e, uint32_t np) {
+    const int lane = lane_id();
+    unsigned long long base = 0;
+    if (lane == 0) base = atomicAdd(P.out_count, (unsigned long long)np);
+    base = __shfl(base, 0);
+    if (base + np > P.item_cap) {
+        for (uint64_t slot = base; slot < P.item_cap && slot < base + np; ++slot)
+            if (lane == 0) P.out_items[slot * ITEM_WORDS] = NONE;
+        return false;
+    }
+    const uint64_t len = e - b;
+    for (uint32_t q = 0; q < np; ++q) {
+        uint32_t *it = P.out_items + (base + q) * ITEM_WORDS;
+        if (lane < ITEM_WORDS) {
+            uint32_t v;
+            if (lane == 0) v = plan;
+            else if (lane == 1) v = (uint32_t)l;
+            else if (lane == 2) v = b + (uint32_t)(len * q / np);
+            else if (lane == 3) v = b + (uint32_t)(len * (q + 1) / np);
+            else v = (lane - 4 < l) ? w.phi[lane - 4] : 0u;
+            it[lane] = v;
+        }
+    }
+    return true;
+}
+
 template <bool CAP>
 __device__ __forceinline__ bool count_last(const KParams &P, WaveLds &w, TermLds *terms, uint32_t *wb, uint32_t *gb,
-                                           Acc &acc, uint32_t &spent, uint32_t &since) {
+                                           Acc &acc, uint32_t &spent, uint32_t &since, uint32_t plan) {
     PH_ENTER(w, PH_TERMS);
     const DPlan &pl = w.pl;
     const int s = pl.sib;
@@ -852,7 +894,8 @@ __device__ __forceinline__ bool count_last(const KParams &P, WaveLds &w, TermLds
     const uint32_t per_block = SIB_BLOCK / K;     // siblings per batch
     unsigned long long local = 0;                 // per-lane partial
     uint32_t steps = 1;
-    bool stopped = false;
+    uint32_t call_steps = 0;  // wave steps of this call's sibling batches
+    bool stopped = false, gave = false;
     for (uint32_t sb = 0; sb < nsib; sb += per_block) {
         if (CAP) {
             spent += steps;
@@ -862,6 +905,14 @@ __device__ __forceinline__ bool count_last(const KParams &P, WaveLds &w, TermLds
                 stopped = true;
                 break;
             }
+        }
+        if (!gave && s >= 2 && P.budget && P.out_items && call_steps >= SIB_DONATE_AFTER * P.budget &&
+            nsib - sb >= (SIB_DONATE_MIN + 1) * per_block) {
+            // keep the next batch, hand the rest to the next pass
+            const uint32_t gb0 = sibs.b + sb + per_block, ge = sibs.e;
+            const uint32_t np = max(1u, min(P.split, (ge - gb0) / (SIB_DONATE_MIN * per_block)));
+            if (give_range(P, w, plan, s, gb0, ge, np)) nsib = sb + per_block;
+            gave = true;
         }
         const uint32_t nb = min(per_block, nsib - sb);
         const uint32_t ne = nb * K;  // entries: sibling i, term j -> i*K + j
@@ -943,6 +994,7 @@ __device__ __forceinline__ bool count_last(const KParams &P, WaveLds &w, TermLds
         PH_SET(w, PH_P2);
         const uint32_t W = w.pre_sum[ne];
         steps += (ne + W + WAVE - 1) / WAVE;
+        call_steps += (ne + W + WAVE - 1) / WAVE;
         acc.add(A_INTER, ne);
         acc.add(A_BYTES, 8ull * nb);
         acc.add(A_MODEL, 8ull * nb);
@@ -1162,7 +1214,7 @@ __device__ __forceinline__ bool run_dfs(const KParams &P, WaveLds &w, TermLds *t
                     if (descend) trie_enter(w, d, x, i);
                     emit_last(P, w, wb, acc);
                 }
-            } else if (count_last<CAP>(P, w, terms, wb, gb, acc, spent, since)) {
+            } else if (count_last<CAP>(P, w, terms, wb, gb, acc, spent, since, plan)) {
                 PH_LEAVE(w);
                 return true;  // work cap reached (reported by count_last)
             }
@@ -1252,11 +1304,23 @@ __global__ void __launch_bounds__(256, SGE_WAVES_PER_SIMD) wcoj_kernel(KParams P
     unsigned long long t_item = __builtin_amdgcn_s_memtime();  // start of the current item
     uint32_t fetched = 15;  // the work cap is checked before the first item and every 16th after it
     uint32_t since = 0;     // wave steps of root tasks counted at the first level (not yet reported)
+#ifdef SGE_PHASE_PROFILE
+    unsigned long long ph_prev = 0, ph_long = 0, ph_long_start = 0;
+#endif
     while (true) {
         {
             // charge the time since the last item started to that item's plan
             const unsigned long long now = __builtin_amdgcn_s_memtime();
             if (cur_plan != NONE) acc.add(A_CYCLES, now - t_item);
+#ifdef SGE_PHASE_PROFILE
+            if (cur_plan != NONE) {
+                const unsigned long long d = (now - t_item) >> 8;
+                if ((d << 32 | (ph_prev & 0xffffffffull)) > ph_long) {
+                    ph_long = d << 32 | (ph_prev & 0xffffffffull);
+                    ph_long_start = d << 32 | (((t_item - t_kernel) >> 8) & 0xffffffffull);
+                }
+            }
+#endif
             t_item = now;
         }
         PH_SET(w, PH_QUEUE);
@@ -1265,6 +1329,7 @@ __global__ void __launch_bounds__(256, SGE_WAVES_PER_SIMD) wcoj_kernel(KParams P
         item = __shfl(item, 0);
 #ifdef SGE_PHASE_PROFILE
         if (lane == 0) w.ph[PH_NUM] += 1;  // fetches
+        ph_prev = item;
 #endif
         if (item >= nitems) break;
         if (CAP && (++fetched & 15u) == 0) {  // every 16th item: has the work cap been reached?
@@ -1286,7 +1351,7 @@ __global__ void __launch_bounds__(256, SGE_WAVES_PER_SIMD) wcoj_kernel(KParams P
             const DPlan &pl = w.pl;
             const int l = (int)it[1];
             const int stop = ENUM ? (int)pl.k - 2 : (int)pl.sib;
-            if (l < 2 || l >= stop) continue;  // never written by donate()
+            if (l < 2 || l > stop || (ENUM && l == stop)) continue;  // never written by donate() / count_last
             for (int q = 0; q < l; ++q) w.phi[q] = it[4 + q];
             if (pl.mode == SGE_MODE_EXPLORE && !ENUM) {
                 explore_after(P, w, wb, 0, acc);
@@ -1367,7 +1432,7 @@ __global__ void __launch_bounds__(256, SGE_WAVES_PER_SIMD) wcoj_kernel(KParams P
             }
             if (!ENUM && pl.sib == 1) {
                 uint32_t spent = 0;
-                halt = count_last<CAP>(P, w, terms, wb, gb, acc, spent, since);
+                halt = count_last<CAP>(P, w, terms, wb, gb, acc, spent, since, plan);
                 continue;
             }
             if (ENUM && pl.k == 3) {
@@ -1396,6 +1461,13 @@ __global__ void __launch_bounds__(256, SGE_WAVES_PER_SIMD) wcoj_kernel(KParams P
         for (int k = 0; k <= PH_NUM; ++k) atomicAdd(&g_phase[k], w.ph[k]);
         atomicAdd(&g_phase[PH_NUM + 1], 1ull);
         atomicMax(&g_phase[PH_NUM + 2], w.ph[PH_ALL]);
+        if (!cont) {
+            atomicAdd(&g_phase[PH_NUM + 3], w.ph[PH_ALL]);
+            atomicMax(&g_phase[PH_NUM + 4], w.ph[PH_ALL]);
+            atomicAdd(&g_phase[PH_NUM + 5], 1ull);
+            atomicMax(&g_phase[PH_NUM + 6], ph_long);
+            atomicMax(&g_phase[PH_NUM + 7], ph_long_start);
+        }
     }
 #endif
 }
@@ -1408,12 +1480,15 @@ __global__ void __launch_bounds__(256, SGE_WAVES_PER_SIMD) wcoj_kernel(KParams P
 // last-level terms, F staging, entry setup, flattened probes, closed forms,
 // queue, root and continuation set-up, DFS iteration, donation, whole kernel),
 // out[PH_NUM] = queue fetches, out[PH_NUM + 1] = wavefronts, out[PH_NUM + 2] =
-// the longest wavefront's cycles; reset != 0 zeroes the counters.
+// the longest wavefront's cycles, out[PH_NUM + 3 .. 5] = the same three for
+// root passes only (summed cycles, longest, wavefronts), out[PH_NUM + 6 .. 7] =
+// the root passes' longest item (cycles >> 8 : queue index, cycles >> 8 : start
+// within its wavefront); reset != 0 zeroes the counters.
 extern "C" int sge_phase_profile_read(unsigned long long *out, int reset) {
-    if (hipMemcpyFromSymbol(out, HIP_SYMBOL(g_phase), sizeof(unsigned long long) * (PH_NUM + 3)) != hipSuccess)
+    if (hipMemcpyFromSymbol(out, HIP_SYMBOL(g_phase), sizeof(unsigned long long) * (PH_NUM + 8)) != hipSuccess)
         return -1;
     if (reset) {
-        static const unsigned long long zero[PH_NUM + 3] = {};
+        static const unsigned long long zero[PH_NUM + 8] = {};
         if (hipMemcpyToSymbol(HIP_SYMBOL(g_phase), zero, sizeof(zero)) != hipSuccess) return -1;
     }
     return 0;
