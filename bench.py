@@ -447,7 +447,7 @@ def phase_profile(args):
     n = len(PHASES) + 1
 
     def read():
-        a = np.zeros(n + 8, np.uint64)
+        a = np.zeros(n + 10, np.uint64)
         if L.sge_phase_profile_read(a.ctypes.data, 1) != 0:
             raise SystemExit("phases: sge_phase_profile_read failed")
         return a
@@ -472,9 +472,11 @@ def phase_profile(args):
                "root_pass_busy": a[n + 2 + 1] / max(a[n + 2 + 3] * a[n + 2 + 2], 1.0),
                "root_pass_longest_item": {"cycles": int(raw[n + 6] >> 32) << 8, "queue_index": int(raw[n + 6] & 0xffffffff),
                                           "start_cycles": int(raw[n + 7] & 0xffffffff) << 8,
-                                          "longest_wave_cycles": int(raw[n + 4])}}
+                                          "longest_wave_cycles": int(raw[n + 4])},
+               "longest_f_formation_cycles": int(raw[n + 8]), "longest_count_cycles": int(raw[n + 9])}
         log(f"phases {tag}: {st.kernel_ms:.1f} ms " + ", ".join(f"{k} {100 * v:.1f}%" for k, v in row['share'].items())
-            + f"; root pass busy {100 * row['root_pass_busy']:.1f}%, longest item {row['root_pass_longest_item']}")
+            + f"; root pass busy {100 * row['root_pass_busy']:.1f}%, longest item {row['root_pass_longest_item']}, "
+            f"longest F formation {row['longest_f_formation_cycles']}, longest count {row['longest_count_cycles']}")
         out["runs"].append(row)
 
     g2 = W.graph_c2()

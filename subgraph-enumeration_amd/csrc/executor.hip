@@ -215,7 +215,8 @@ struct WaveLds {
     uint32_t mark[WAVE];       // phase 2: entry starting at each lane's probe slot (index + 1)
     uint32_t fst[FSTAGE];      // staged copies of small F sets (probed through LDS)
 #ifdef SGE_PHASE_PROFILE
-    unsigned long long ph[14];  // wavefront clock cycles per phase (PH_*), lane 0 only
+    unsigned long long ph[16];  // wavefront clock cycles per phase (PH_*), lane 0 only; [14], [15]: the
+                                // longest F-set formation and last-level count call
     unsigned long long ph_t;    // clock at the last phase switch
     uint32_t ph_cur;            // current phase
 #endif
@@ -235,7 +236,7 @@ enum {
 // summed wave cycles, [+4] longest wave, [+5] waves, [+6] the longest item
 // (cycles >> 8 in the high word, its queue index in the low word), [+7] the
 // same item's start (cycles >> 8 since the wave started) in the low word
-__device__ unsigned long long g_phase[PH_NUM + 8];
+__device__ unsigned long long g_phase[PH_NUM + 10];  // [+8], [+9]: longest F formation / count call
 __device__ __forceinline__ uint32_t ph_switch(WaveLds &w, uint32_t i) {
     uint32_t old = 0;
     if (__lane_id() == 0) {
@@ -726,17 +727,18 @@ __device__ __forceinline__ bool gbm_pays(const KParams &P, const WaveLds &w, int
 // The wave steps it spends (64-lane iterations of the entry setup and the
 // flattened probes), the unit of the donation budget and the work cap, are
 // added to `spent` and `since`; the cap is checked after every sibling batch.
-// A long count hands its remaining siblings to the next pass (when donation is
-// on): once the call has spent SIB_DONATE_AFTER donation budgets and at least
-// SIB_DONATE_MIN batches of siblings remain, the wave keeps the next batch and
-// writes the rest as up to `split` continuation items of level sib, each at
-// least SIB_DONATE_MIN batches long (each receiver re-forms the F sets once, so
-// small pieces would not pay: handing over half of a hub's siblings at every
-// budget was 4.7x slower on the R-MAT 20 ranges).  One call over a hub's
-// siblings had held a wavefront for the whole root pass there.  Returns true
-// when the cap stopped it (its count is then partial).
+// A long count hands siblings to the next pass (when donation is on): after the
+// entry set-up of a batch, the probes left in the call are estimated from the
+// batch's (per sibling, times the siblings left); at SIB_DONATE_AFTER donation
+// budgets of 64-lane rounds or more, the call is cut into pieces of about half
+// that many rounds (at most `split`), this wave keeps the first piece's share of
+// siblings (at least one) and writes the rest as continuation items of level
+// sib, once per call (each receiver re-forms the F sets once and may cut its
+// piece again: handing over half of a hub's siblings at every budget was 4.7x
+// slower on the R-MAT 20 ranges).  Single counts of hub siblings had held one
+// wavefront for the whole root pass there (91 M cycles).  Returns true when
+// the cap stopped it (its count is then partial).
 constexpr uint32_t SIB_DONATE_AFTER = 16;
-constexpr uint32_t SIB_DONATE_MIN = 4;
 
 // Write [b, e) of level l's candidates as np continuation items of near-equal
 // length (plan, level, range, images of levels < l); false (nothing written)
@@ -772,6 +774,9 @@ template <bool CAP>
 __device__ __forceinline__ bool count_last(const KParams &P, WaveLds &w, TermLds *terms, uint32_t *wb, uint32_t *gb,
                                            Acc &acc, uint32_t &spent, uint32_t &since, uint32_t plan) {
     PH_ENTER(w, PH_TERMS);
+#ifdef SGE_PHASE_PROFILE
+    const unsigned long long ph_t0 = __builtin_amdgcn_s_memtime();
+#endif
     const DPlan &pl = w.pl;
     const int s = pl.sib;
     const int lane = lane_id();
@@ -821,6 +826,9 @@ __device__ __forceinline__ bool count_last(const KParams &P, WaveLds &w, TermLds
         terms[j] = t;
     }
     wave_sync();
+#ifdef SGE_PHASE_PROFILE
+    if (lane_id() == 0) w.ph[14] = max(w.ph[14], __builtin_amdgcn_s_memtime() - ph_t0);
+#endif
     PH_SET(w, PH_STAGE);
     // ---- stage F sets in LDS so per-sibling probes hit LDS: a bitmap over
     // [min F, max F] when it fits, else a sorted copy when that fits, else (a
@@ -894,7 +902,6 @@ __device__ __forceinline__ bool count_last(const KParams &P, WaveLds &w, TermLds
     const uint32_t per_block = SIB_BLOCK / K;     // siblings per batch
     unsigned long long local = 0;                 // per-lane partial
     uint32_t steps = 1;
-    uint32_t call_steps = 0;  // wave steps of this call's sibling batches
     bool stopped = false, gave = false;
     for (uint32_t sb = 0; sb < nsib; sb += per_block) {
         if (CAP) {
@@ -906,18 +913,11 @@ __device__ __forceinline__ bool count_last(const KParams &P, WaveLds &w, TermLds
                 break;
             }
         }
-        if (!gave && s >= 2 && P.budget && P.out_items && call_steps >= SIB_DONATE_AFTER * P.budget &&
-            nsib - sb >= (SIB_DONATE_MIN + 1) * per_block) {
-            // keep the next batch, hand the rest to the next pass
-            const uint32_t gb0 = sibs.b + sb + per_block, ge = sibs.e;
-            const uint32_t np = max(1u, min(P.split, (ge - gb0) / (SIB_DONATE_MIN * per_block)));
-            if (give_range(P, w, plan, s, gb0, ge, np)) nsib = sb + per_block;
-            gave = true;
-        }
-        const uint32_t nb = min(per_block, nsib - sb);
-        const uint32_t ne = nb * K;  // entries: sibling i, term j -> i*K + j
+        uint32_t nb = min(per_block, nsib - sb);
+        uint32_t ne = nb * K;  // entries: sibling i, term j -> i*K + j
         // phase 1: per-entry ranges and work (lane per entry)
         uint32_t carry = 0;
+        uint32_t my_mw = 0;
         for (uint32_t r0 = 0; r0 < ne; r0 += WAVE) {
             uint32_t ei = r0 + lane;
             uint32_t work = 0, dstart = 0, tl = 0, th = 0;
@@ -984,17 +984,39 @@ __device__ __forceinline__ bool count_last(const KParams &P, WaveLds &w, TermLds
                 w.ent[ei] = make_uint4(carry + incl - work, dstart, tl, th);
             }
             carry += __shfl(incl, WAVE - 1);
-            // model bytes: both clamped lists of every probed (sibling, term) pair
-            uint32_t mw = work + (th - tl);
-            for (int off = WAVE / 2; off > 0; off >>= 1) mw += __shfl_xor(mw, off);
-            acc.add(A_MODEL, 4ull * mw);
+            // model bytes: both clamped lists of every probed (sibling, term)
+            // pair (entry ei = lane: ne <= SIB_BLOCK <= 64, one iteration)
+            my_mw = work + (th - tl);
         }
         if (lane == 0) w.pre_sum[0] = 0;
         wave_sync();
+        if (!gave && s >= 2 && P.budget && P.out_items && nsib - sb >= 2) {
+            // the probes this call has left, estimated from this batch
+            const uint64_t per_sib = max(1u, w.pre_sum[ne] / nb);
+            const uint64_t est = per_sib * (nsib - sb) / WAVE;  // rounds
+            const uint64_t unit = (uint64_t)SIB_DONATE_AFTER * P.budget;
+            if (est >= unit) {
+                gave = true;
+                // pieces of about unit / 2 rounds; this wave keeps as much
+                const uint32_t rem = nsib - sb;
+                const uint32_t np = (uint32_t)min<uint64_t>(min<uint64_t>(P.split, rem - 1), max<uint64_t>(1, est / (unit / 2)));
+                const uint32_t keep = min(nb, max(1u, rem / (np + 1)));
+                if (give_range(P, w, plan, s, sibs.b + sb + keep, sibs.e, np)) {
+                    // drop the given siblings' entries from this batch
+                    nb = keep;
+                    ne = keep * K;
+                    nsib = sb + keep;
+                }
+            }
+        }
+        {
+            uint32_t mw = (uint32_t)lane < ne ? my_mw : 0u;
+            for (int off = WAVE / 2; off > 0; off >>= 1) mw += __shfl_xor(mw, off);
+            acc.add(A_MODEL, 4ull * mw);
+        }
         PH_SET(w, PH_P2);
         const uint32_t W = w.pre_sum[ne];
         steps += (ne + W + WAVE - 1) / WAVE;
-        call_steps += (ne + W + WAVE - 1) / WAVE;
         acc.add(A_INTER, ne);
         acc.add(A_BYTES, 8ull * nb);
         acc.add(A_MODEL, 8ull * nb);
@@ -1068,6 +1090,9 @@ __device__ __forceinline__ bool count_last(const KParams &P, WaveLds &w, TermLds
         PH_SET(w, PH_P1);
     }
     PH_SET(w, PH_P3);
+#ifdef SGE_PHASE_PROFILE
+    if (lane_id() == 0) w.ph[15] = max(w.ph[15], __builtin_amdgcn_s_memtime() - ph_t0);
+#endif
     for (int off = WAVE / 2; off > 0; off >>= 1) local += __shfl_xor(local, off);
     acc.add(A_COUNT, local);
     if (used_g) {
@@ -1292,7 +1317,7 @@ __global__ void __launch_bounds__(256, SGE_WAVES_PER_SIMD) wcoj_kernel(KParams P
         for (int k = 0; k < A_NUM; ++k) w.accv[k] = 0;
 #ifdef SGE_PHASE_PROFILE
     if (lane == 0) {
-        for (int k = 0; k <= PH_NUM; ++k) w.ph[k] = 0;
+        for (int k = 0; k < 16; ++k) w.ph[k] = 0;
         w.ph_cur = PH_OTHER;
         w.ph_t = __builtin_amdgcn_s_memtime();
     }
@@ -1468,6 +1493,8 @@ __global__ void __launch_bounds__(256, SGE_WAVES_PER_SIMD) wcoj_kernel(KParams P
             atomicMax(&g_phase[PH_NUM + 6], ph_long);
             atomicMax(&g_phase[PH_NUM + 7], ph_long_start);
         }
+        atomicMax(&g_phase[PH_NUM + 8], w.ph[14]);
+        atomicMax(&g_phase[PH_NUM + 9], w.ph[15]);
     }
 #endif
 }
@@ -1483,12 +1510,13 @@ __global__ void __launch_bounds__(256, SGE_WAVES_PER_SIMD) wcoj_kernel(KParams P
 // the longest wavefront's cycles, out[PH_NUM + 3 .. 5] = the same three for
 // root passes only (summed cycles, longest, wavefronts), out[PH_NUM + 6 .. 7] =
 // the root passes' longest item (cycles >> 8 : queue index, cycles >> 8 : start
-// within its wavefront); reset != 0 zeroes the counters.
+// within its wavefront), out[PH_NUM + 8 .. 9] = the longest F-set formation and
+// the longest last-level count call; reset != 0 zeroes the counters.
 extern "C" int sge_phase_profile_read(unsigned long long *out, int reset) {
-    if (hipMemcpyFromSymbol(out, HIP_SYMBOL(g_phase), sizeof(unsigned long long) * (PH_NUM + 8)) != hipSuccess)
+    if (hipMemcpyFromSymbol(out, HIP_SYMBOL(g_phase), sizeof(unsigned long long) * (PH_NUM + 10)) != hipSuccess)
         return -1;
     if (reset) {
-        static const unsigned long long zero[PH_NUM + 8] = {};
+        static const unsigned long long zero[PH_NUM + 10] = {};
         if (hipMemcpyToSymbol(HIP_SYMBOL(g_phase), zero, sizeof(zero)) != hipSuccess) return -1;
     }
     return 0;
