@@ -699,6 +699,13 @@ __device__ __forceinline__ bool cap_reached(const KParams &P, uint32_t &since) {
     return tot >= P.node_cap;
 }
 
+// Probe direction for an F staged as a global bitmap: the sibling's row drives
+// while |row| <= SGE_GBM_RATIO |F| (a probe is then one L2 read, against a
+// binary search of the row per F element the other way).
+#ifndef SGE_GBM_RATIO
+#define SGE_GBM_RATIO 2
+#endif
+
 // Whether a global bitmap of a large F (len elements) pays for its build and
 // clear (about 2 len / 64 wave steps): the siblings' rows short enough to
 // probe F (|row| <= 2 len) save about log2(len) loads per element.  Estimated
@@ -712,7 +719,7 @@ __device__ __forceinline__ bool gbm_pays(const KParams &P, const WaveLds &w, int
     if ((uint32_t)lane < nsamp) {
         const uint32_t c = (s == 1) ? w.phi[1] : w.cand[s].ptr[b + lane];
         const uint32_t d = P.off[c + 1] - P.off[c];
-        if ((uint64_t)d <= 2ull * len) probes = d;
+        if ((uint64_t)d <= (uint64_t)SGE_GBM_RATIO * len) probes = d;
     }
     for (int off = WAVE / 2; off > 0; off >>= 1) probes += __shfl_xor(probes, off);
     const uint64_t est = probes * nsib / max(nsamp, 1u);
@@ -948,7 +955,8 @@ __device__ __forceinline__ bool count_last(const KParams &P, WaveLds &w, TermLds
                             // probes into a staged F cost LDS reads (one for a bitmap),
                             // probes into a row cost a global binary search
                             const uint64_t ratio =
-                                (t.flags & 48u) ? P.bitmap_ratio : ((t.flags & 8u) ? P.probe_ratio : 1u);
+                                (t.flags & 32u) ? (uint32_t)SGE_GBM_RATIO
+                                                : ((t.flags & 16u) ? P.bitmap_ratio : ((t.flags & 8u) ? P.probe_ratio : 1u));
                             if ((uint64_t)(re - rb) <= ratio * (uint64_t)(fe - fb)) {
                                 flag = 1;
                                 dstart = rb;
