@@ -186,10 +186,12 @@ int sge_plan_text(const sge_plan *plan, char *buf, size_t cap);
  * the plan's edges, bounds and injectivity checks among them (the reference's
  * bag tuples, join.cpp:309-421); the join then intersects trie child lists with
  * the rows of parents outside the bag (twoIntersection / multiIntersection,
- * join.cpp:9-135).  Counts never change.  Valid bags: >= 3 vertices, WCOJ or
- * IEP mode, every bag level below the closed-form level(s), the first two bag
- * levels adjacent and every later bag level adjacent to an earlier bag level
- * (SGE_EINVAL otherwise). */
+ * join.cpp:9-135).  In exploration mode the non-bag levels form their parent
+ * intersections early as usual and a bag level forms its set from the trie list
+ * when it is reached (the exploration nodeJoin / globalJoin, join.cpp:423-810).
+ * Counts never change.  Valid bags: >= 3 vertices, every bag level below the
+ * closed-form level(s), the first two bag levels adjacent and every later bag
+ * level adjacent to an earlier bag level (SGE_EINVAL otherwise). */
 int sge_plan_set_bag(sge_plan *plan, uint32_t vertex_mask);
 /* The bags of the pattern's minimal fractional hypertree decomposition (vertex
  * masks, as all_td.cpp:483-691 chooses them) that are valid bags of `plan`;

@@ -621,7 +621,9 @@ __device__ __forceinline__ void materialize(const KParams &P, WaveLds &w, uint32
     bool has_lo, has_hi;
     uint32_t lo, hi;
     level_bounds(w, pl.lower[d], pl.upper[d], has_lo, lo, has_hi, hi);
-    bool use_pre = pl.mode == SGE_MODE_EXPLORE;
+    // exploration: the parent intersection formed early (bag levels: formed now
+    // from the trie list and the non-bag parents' rows, like the other modes)
+    bool use_pre = pl.mode == SGE_MODE_EXPLORE && !((pl.bag >> d) & 1u);
     int ns = gather_sources(P, w, d, pl.parent[d], use_pre ? -1 : pl.reuse[d], use_pre, has_lo, lo, has_hi, hi, acc);
     int nchk = load_checks(w, pl.check[d]);
     const Src c = form_set(P, w, ns, nchk, true, buf_ptr(P, wb, d), acc);
@@ -638,7 +640,9 @@ __device__ __forceinline__ void explore_after(const KParams &P, WaveLds &w, uint
     PH_ENTER(w, PH_EXPL);
     const DPlan &pl = w.pl;
     for (uint32_t j = d + 1; j <= pl.sib; ++j) {
-        if (pl.ready[j] != (uint32_t)d || j < 2) continue;
+        // a bag level's candidates come from its trie list, which exists only
+        // once the previous bag level is matched: it is formed when it is reached
+        if (pl.ready[j] != (uint32_t)d || j < 2 || ((pl.bag >> j) & 1u)) continue;
         int ns = gather_sources(P, w, j, pl.parent[j], -1, false, false, 0, false, 0, acc);
         w.pre[j] = form_set(P, w, ns, 0, true, buf_ptr(P, wb, P.ex_base + j), acc);
     }

@@ -382,10 +382,6 @@ static uint32_t first_closed_level(const sge_plan &pl) {
 
 int bag_check(const sge_plan &pl, uint32_t B) {
     if (B == 0) return SGE_OK;
-    if (pl.mode == SGE_MODE_EXPLORE) {
-        set_error("bag: exploration mode forms parent intersections early and takes no bags");
-        return SGE_EINVAL;
-    }
     if (__builtin_popcount(B) < 3) {
         set_error("bag: a materialised bag needs at least 3 vertices");
         return SGE_EINVAL;

@@ -463,7 +463,7 @@ def test_bag_counts_match_oracle_c1():
     g = W.graph_c1()
     off, nbr = g.csr()
     for p in W.six_vertex_set() + W.seven_vertex_set()[::5] + small_patterns():
-        for mode in (sge.MODE_WCOJ, sge.MODE_IEP):
+        for mode in (sge.MODE_WCOJ, sge.MODE_EXPLORE, sge.MODE_IEP):
             plan = sge.build_plan(p, g, mode)
             want = O.count(plan, off, nbr)
             for b in spread(sge.valid_bags(plan), 4):
@@ -498,7 +498,7 @@ def test_bag_c2_goldens_plain_and_batched():
     b.close()
     gold2 = golden("c2")["patterns"]
     for name in ["near5clique", "house"]:
-        for mode, key in ((sge.MODE_WCOJ, "wcoj"), (sge.MODE_IEP, "iep")):
+        for mode, key in ((sge.MODE_WCOJ, "wcoj"), (sge.MODE_EXPLORE, "explore"), (sge.MODE_IEP, "iep")):
             plan = sge.build_plan(W.named(name), g, mode)
             for bag in [b for b in sge.valid_bags(plan) if bin(b).count("1") == 3]:
                 say("c2", name, key, "bag", bag)
@@ -506,14 +506,16 @@ def test_bag_c2_goldens_plain_and_batched():
 
 
 def test_bag_with_donation_and_tuning_variants():
-    """Continuation items rebuild their path's trie positions: tiny donation
-    budgets and unusual launch shapes give the same counts."""
+    """Continuation items rebuild their path's trie positions (and, in
+    exploration mode, the early parent intersections of the non-bag levels):
+    tiny donation budgets and unusual launch shapes give the same counts."""
     g = W.graph_c1()
     plans = []
     for p in W.six_vertex_set()[::2] + W.seven_vertex_set()[::11]:
-        plan = sge.build_plan(p, g, sge.MODE_WCOJ)
-        bags = spread(sge.valid_bags(plan), 2)
-        plans += [sge.set_bag(plan, b) for b in bags]
+        for mode in (sge.MODE_WCOJ, sge.MODE_EXPLORE):
+            plan = sge.build_plan(p, g, mode)
+            bags = spread(sge.valid_bags(plan), 2)
+            plans += [sge.set_bag(plan, b) for b in bags]
     want, _ = sge.count(g, [sge.set_bag(pl, 0) for pl in plans])
     for bt, ch, budget, span in [(64, 1, 16, 8), (256, 3, 64, 1 << 30), (128, 8, 1, 4)]:
         t = sge.default_tuning()
@@ -528,10 +530,11 @@ def test_bag_rmat8_seven_vertex_goldens():
     g = W.graph_rmat(8, 7)
     pats = W.seven_vertex_set()
     for rec in gold[::4]:
-        plan = sge.build_plan(pats[rec["index"]], g, sge.MODE_IEP)
-        for b in spread(smallest_bags(plan, 6), 2):
-            say("rmat8 bag", rec["index"], b)
-            assert sge.count(g, sge.set_bag(plan, b))[0][0] == rec["wcoj"]["count"], (rec["index"], b)
+        for mode in (sge.MODE_IEP, sge.MODE_EXPLORE):
+            plan = sge.build_plan(pats[rec["index"]], g, mode)
+            for b in spread(smallest_bags(plan, 6), 2):
+                say("rmat8 bag", rec["index"], mode, b)
+                assert sge.count(g, sge.set_bag(plan, b))[0][0] == rec["wcoj"]["count"], (rec["index"], mode, b)
 
 
 def test_bag_enumeration_unchanged():
@@ -551,7 +554,7 @@ def test_bag_empty_relation():
     g = sge.Graph.from_edges(16, edges)
     off, nbr = g.csr()
     house = W.named("house")
-    for mode in (sge.MODE_WCOJ, sge.MODE_IEP):
+    for mode in (sge.MODE_WCOJ, sge.MODE_EXPLORE, sge.MODE_IEP):
         plan = sge.set_bag(sge.build_plan(house, g, mode, [2, 3, 4, 0, 1]), [2, 3, 4])
         say("empty bag relation, mode", mode)
         t = sge.Trie(g, plan)
@@ -641,7 +644,7 @@ def test_bag_joins_on_query_files_c1():
     g = W.graph_c1()
     for rec in gold["patterns"]:
         p = sge.Pattern.from_edges(rec["k"], [tuple(e) for e in rec["edges"]])
-        for mode in (sge.MODE_WCOJ, sge.MODE_IEP):
+        for mode in (sge.MODE_WCOJ, sge.MODE_EXPLORE, sge.MODE_IEP):
             plan = sge.build_plan(p, g, mode)
             for b in sge.valid_bags(plan):
                 say("query file bag", rec["file"], mode, b)

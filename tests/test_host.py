@@ -311,8 +311,8 @@ def test_bag_validation():
             sge.set_bag(plan, bad)
     with pytest.raises(sge.SgeError):  # first two bag levels (vertices 1 and 2) not adjacent
         sge.set_bag(sge.build_plan(house, g, sge.MODE_WCOJ, [0, 1, 2, 3, 4]), [1, 2, 3])
-    with pytest.raises(sge.SgeError):  # exploration mode takes no bags
-        sge.set_bag(sge.build_plan(house, g, sge.MODE_EXPLORE, [2, 3, 4, 0, 1]), [2, 3, 4])
+    # exploration mode takes bags too
+    assert sge.set_bag(sge.build_plan(house, g, sge.MODE_EXPLORE, [2, 3, 4, 0, 1]), [2, 3, 4]).bag == 0b111
     # every bag the decomposition proposes is accepted
     for p in W.six_vertex_set():
         pl = sge.build_plan(p, g, sge.MODE_IEP)
