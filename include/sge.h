@@ -30,6 +30,7 @@ extern "C" {
 #endif
 
 #define SGE_MAX_K 8 /* pattern vertices; the reference's benchmark sets use <= 7 */
+#define SGE_MAX_BAGS 2 /* materialised bags per plan (sge_plan.bag) */
 
 /* error codes */
 #define SGE_OK 0
@@ -147,12 +148,16 @@ typedef struct sge_plan {
     uint32_t tail;                     /* IEP: trailing levels counted in closed form (0 = none) */
     uint32_t tail_kind;                /* 1: symmetric twin group, choosec (join.cpp:1532-1537);
                                           2: independent set, inclusion-exclusion over set partitions */
-    uint32_t bag;                      /* levels of one materialised bag (0 = none): its relation is
-                                          enumerated and built into a trie before the join, and each
-                                          bag level then reads its trie child list in place of the
-                                          rows of its bag parents (nodeJoin join.cpp:309-421, buildTrie
-                                          join.cpp:1713-1717, globalJoin join.cpp:1202-1420); set with
-                                          sge_plan_set_bag */
+    uint32_t bag;                      /* levels of up to two materialised bags (0 = none): bag 0's
+                                          level mask in bits 0-7, bag 1's in bits 8-15 (the bags of a
+                                          hypertree decomposition, HyperTree decomposition.h:59-123,
+                                          may share levels).  Each bag's relation is enumerated and
+                                          built into a trie before the join, and a bag level then
+                                          reads the trie child list of every bag holding it in place
+                                          of the rows of those bags' parents (nodeJoin
+                                          join.cpp:309-421, buildTrie join.cpp:1713-1717, globalJoin
+                                          join.cpp:1202-1420); set with sge_plan_set_bag and
+                                          sge_plan_add_bag */
     uint32_t num_bags;                 /* decomposition bag count (reporting / features) */
     uint64_t multiplier;               /* |Aut| when the reference counts embeddings (single bag), else 1 */
     uint64_t aut_size;
@@ -193,6 +198,12 @@ int sge_plan_text(const sge_plan *plan, char *buf, size_t cap);
  * closed-form level(s), the first two bag levels adjacent and every later bag
  * level adjacent to an earlier bag level (SGE_EINVAL otherwise). */
 int sge_plan_set_bag(sge_plan *plan, uint32_t vertex_mask);
+/* Add the pattern vertices in vertex_mask as a second bag of a plan that has
+ * exactly one (the reference's plans join two to five bags, HyperTree
+ * decomposition.h:59-123, globalJoin join.cpp:1202-1420): the same validity
+ * rules per bag, different from the first; the bags may share vertices, and a
+ * level in both intersects both trie lists. */
+int sge_plan_add_bag(sge_plan *plan, uint32_t vertex_mask);
 /* The bags of the pattern's minimal fractional hypertree decomposition (vertex
  * masks, as all_td.cpp:483-691 chooses them) that are valid bags of `plan`;
  * count in *n (at most cap written). */
@@ -282,7 +293,8 @@ int sge_num_tasks(const sge_graph *g, const sge_plan *plan, uint64_t *n);
  * total (may exceed cap -> SGE_ERANGE, nothing beyond cap written). */
 int sge_enumerate(sge_graph *g, const sge_plan *plan, uint32_t *out, uint64_t cap, uint64_t *n_out);
 
-/* Bag relation of plan->bag as a level-order trie on the device: the relation
+/* Relation of the plan's first bag (level mask plan->bag & 0xff) as a
+ * level-order trie on the device: the relation
  * is enumerated by the executor (the bag's levels in plan order), radix sorted
  * lexicographically, and grouped per depth (TrieLevel::buildTrieFromSortedMatchesBatch,
  * relation.cpp:340-379; TrieLevel relation.h:49-69).  Depth j holds nodes[j]

@@ -26,7 +26,9 @@
  *                         grouping by depth), flattened depth by depth (oracle_bag_trie)
  *   candidates (bag)      matching/join.cpp:1202-1300 (globalJoin: a level joined with a bag reads
  *                         the bag's current trie level in place of the rows of its bag parents and
- *                         descends into the matched value's children, join.cpp:244-264 setIters)
+ *                         descends into the matched value's children, join.cpp:244-264 setIters);
+ *                         a second bag (plan.bag bits 8-15) is joined the same way through its
+ *                         own trie, as the reference joins every bag of a multi-bag plan
  *
  * Plans are the flat sge_plan of include/sge.h: level i matches pattern vertex
  * order[i]; parent/lower/upper/check masks are over levels.
@@ -37,8 +39,9 @@
  * crashes or its -u mode disagrees with its WCOJ mode), K_n closed forms
  * n!/(n-k)!/|Aut|, brute force on random small graphs, the diamond formula
  * sum over edges of C(common neighbours, 2), mode/order/relabelling invariance,
- * std-library set operations for the primitives, and a hand-computed wrap case
- * for choosec.
+ * std-library set operations for the primitives, a hand-computed wrap case
+ * for choosec, and, for the bag joins, brute-force bag relations and one- and
+ * two-bag counts equal to the plain join's and the golden counts.
  */
 #include <stdint.h>
 #include <stdlib.h>
@@ -155,16 +158,23 @@ typedef struct {
     uint32_t *val, *child;
 } otrie_t;
 
+/* A plan holds up to two materialised bags: level masks in bits 0-7 (bag 0) and
+ * 8-15 (bag 1) of pl->bag. */
+#define NBAGS 2
+static uint32_t bag_levels(const sge_plan *pl, int b) { return (pl->bag >> (8 * b)) & 0xffu; }
+static uint32_t bag_all(const sge_plan *pl) { return (pl->bag | (pl->bag >> 8)) & 0xffu; }
+
 typedef struct {
     const sge_plan *pl;
-    const otrie_t *trie;       /* pl->bag's trie (NULL without a bag) */
-    uint32_t tpos[SGE_MAX_K];  /* bag levels: value index of phi[level] in the trie */
+    const otrie_t *trie[NBAGS];       /* each bag's trie (NULL: no such bag) */
+    uint32_t tpos[NBAGS][SGE_MAX_K];  /* bag levels: value index of phi[level] in the bag's trie */
     const uint32_t *off, *nbr;
     uint32_t n, maxdeg;
     uint32_t phi[SGE_MAX_K];
     uint32_t *buf[SGE_MAX_K];  /* candidate set per level */
     uint32_t len[SGE_MAX_K];
     uint32_t *tmp, *tmp2;
+    uint32_t tmpcap;           /* elements tmp / tmp2 hold, less one */
     uint32_t *emb;             /* enumeration output (may be NULL) */
     uint64_t emb_cap, *emb_n;
     uint64_t intersections;
@@ -193,16 +203,16 @@ static void bounds(const ctx_t *c, int d, uint32_t *lo_excl, int *has_lo, uint32
     *hi_excl = mn;
 }
 
-/* Trie list of bag level d: the root values for the first bag level, else the
- * children of the previous bag level's matched value. */
-static const uint32_t *bag_list(const ctx_t *c, int d, uint32_t *len, uint32_t *base) {
-    const otrie_t *t = c->trie;
-    uint32_t below = c->pl->bag & ((1u << d) - 1u), b, e;
+/* Trie list of level d in bag bi: the root values for the bag's first level,
+ * else the children of the bag's previous level's matched value. */
+static const uint32_t *bag_list(const ctx_t *c, int bi, int d, uint32_t *len, uint32_t *base) {
+    const otrie_t *t = c->trie[bi];
+    uint32_t below = bag_levels(c->pl, bi) & ((1u << d) - 1u), b, e;
     if (below == 0) {
         b = t->vb[0];
         e = t->vb[1];
     } else {
-        uint32_t j = (uint32_t)__builtin_popcount(below), a = c->tpos[31 - __builtin_clz(below)];
+        uint32_t j = (uint32_t)__builtin_popcount(below), a = c->tpos[bi][31 - __builtin_clz(below)];
         b = t->child[a + j - 1];
         e = t->child[a + j];
     }
@@ -211,20 +221,21 @@ static const uint32_t *bag_list(const ctx_t *c, int d, uint32_t *len, uint32_t *
     return t->val + b;
 }
 
-/* Record the trie position of phi[d] for bag level d; 0 when it is absent. */
-static int bag_enter(ctx_t *c, int d) {
+/* Record the trie position of phi[d] for level d of bag bi; 0 when it is absent. */
+static int bag_enter(ctx_t *c, int bi, int d) {
     uint32_t len, base;
-    const uint32_t *a = bag_list(c, d, &len, &base);
+    const uint32_t *a = bag_list(c, bi, d, &len, &base);
     uint32_t p = lower_bound_u32(a, 0, len, c->phi[d]);
-    c->tpos[d] = base + p;
+    c->tpos[bi][d] = base + p;
     return p < len && a[p] == c->phi[d];
 }
 
 /* reuse is off for bag levels and for levels that would reuse a bag level's set */
 static int reuse_of(const ctx_t *c, int d) {
     const sge_plan *pl = c->pl;
+    const uint32_t all = bag_all(pl);
     int r = pl->reuse[d];
-    if (pl->bag && r >= 0 && (((pl->bag >> d) & 1u) || ((pl->bag >> r) & 1u))) r = -1;
+    if (all && r >= 0 && (((all >> d) & 1u) || ((all >> r) & 1u))) r = -1;
     return r;
 }
 
@@ -250,12 +261,24 @@ static uint32_t candidates(ctx_t *c, int d, int materialise, uint32_t lower_mask
         srclen = c->len[r];
         lists &= ~pl->parent[r];
     }
-    if (c->trie && ((pl->bag >> d) & 1u)) {
-        /* the trie list satisfies every edge, bound and check among the bag's
-         * levels; rows of bag parents are not read (reuse is off here) */
-        uint32_t base;
-        src = bag_list(c, d, &srclen, &base);
-        if (pl->bag & ((1u << d) - 1u)) lists &= ~pl->bag;
+    /* a bag's trie list satisfies every edge, bound and check among the bag's
+     * levels; rows of bag parents are not read (reuse is off here); a level in
+     * two bags intersects both lists */
+    const uint32_t *tl2 = NULL;
+    uint32_t tl2len = 0;
+    for (int bi = 0; bi < NBAGS; ++bi) {
+        const uint32_t bm = bag_levels(pl, bi);
+        if (!c->trie[bi] || !((bm >> d) & 1u)) continue;
+        uint32_t base, len;
+        const uint32_t *l = bag_list(c, bi, d, &len, &base);
+        if (!src) {
+            src = l;
+            srclen = len;
+        } else {
+            tl2 = l;
+            tl2len = len;
+        }
+        if (bm & ((1u << d) - 1u)) lists &= ~bm;
     }
     /* generateCandidates: clamp every list, then chain pairwise intersections */
     uint32_t *cur = c->tmp, *nxt = c->tmp2;
@@ -268,6 +291,14 @@ static uint32_t candidates(ctx_t *c, int d, int materialise, uint32_t lower_mask
         memcpy(cur, src + b, (size_t)(e - b) * 4);
         curlen = e - b;
         first = 0;
+    }
+    if (tl2) {
+        uint32_t b = has_lo ? oracle_set_begin_pos(tl2, tl2len, mx) : 0;
+        uint32_t e = has_hi ? oracle_set_end_pos(tl2, tl2len, mn) : tl2len;
+        if (e < b) e = b;
+        curlen = oracle_intersect(cur, curlen, tl2 + b, e - b, nxt);
+        uint32_t *t = cur; cur = nxt; nxt = t;
+        c->intersections++;
     }
     for (j = 0; j < d; ++j) {
         if (!((lists >> j) & 1u)) continue;
@@ -366,20 +397,30 @@ static uint64_t dfs(ctx_t *c, int d) {
     candidates(c, d, 1, pl->lower[d], pl->upper[d], pl->check[d]);
     uint64_t sum = 0;
     uint32_t i;
-    int descend = c->trie && ((pl->bag >> d) & 1u) && (pl->bag >> (d + 1));
+    int descend[NBAGS], bi;
+    for (bi = 0; bi < NBAGS; ++bi) {
+        const uint32_t bm = bag_levels(pl, bi);
+        descend[bi] = c->trie[bi] && ((bm >> d) & 1u) && (bm >> (d + 1));
+    }
     for (i = 0; i < c->len[d]; ++i) {
         c->phi[d] = c->buf[d][i];
-        if (descend) bag_enter(c, d);
+        for (bi = 0; bi < NBAGS; ++bi)
+            if (descend[bi]) bag_enter(c, bi, d);
         sum += dfs(c, d + 1);
     }
     return sum;
 }
 
-/* Root edge (phi[0], phi[1]) of a plan with a bag: both ends must lie on a trie path. */
+/* Root edge (phi[0], phi[1]) of a plan with bags: both ends must lie on a path
+ * of every bag's trie that holds them. */
 static int bag_root_ok(ctx_t *c) {
-    if (!c->trie) return 1;
-    if ((c->pl->bag & 1u) && !bag_enter(c, 0)) return 0;
-    if ((c->pl->bag & 2u) && !bag_enter(c, 1)) return 0;
+    int bi;
+    for (bi = 0; bi < NBAGS; ++bi) {
+        const uint32_t bm = bag_levels(c->pl, bi);
+        if (!c->trie[bi]) continue;
+        if ((bm & 1u) && !bag_enter(c, bi, 0)) return 0;
+        if ((bm & 2u) && !bag_enter(c, bi, 1)) return 0;
+    }
     return 1;
 }
 
@@ -399,6 +440,7 @@ static int ctx_init(ctx_t *c, const sge_plan *pl, const uint32_t *off, const uin
     }
     c->tmp = (uint32_t *)malloc(((size_t)c->maxdeg + 1) * 4);
     c->tmp2 = (uint32_t *)malloc(((size_t)c->maxdeg + 1) * 4);
+    c->tmpcap = c->maxdeg;
     for (i = 0; i < 3; ++i) {
         c->ie[i] = (uint32_t *)malloc(((size_t)c->maxdeg + 1) * 4);
         if (!c->ie[i]) return -1;
@@ -406,13 +448,14 @@ static int ctx_init(ctx_t *c, const sge_plan *pl, const uint32_t *off, const uin
     return (c->tmp && c->tmp2) ? 0 : -1;
 }
 
-/* Attach a bag trie: the working lists must also hold the trie's root values
- * (up to n), which the first bag level copies before intersecting. */
-static void ctx_set_trie(ctx_t *c, const otrie_t *t) {
-    c->trie = t;
-    if (t->vb[1] > c->maxdeg) {
-        c->tmp = (uint32_t *)realloc(c->tmp, ((size_t)t->vb[1] + 1) * 4);
-        c->tmp2 = (uint32_t *)realloc(c->tmp2, ((size_t)t->vb[1] + 1) * 4);
+/* Attach bag bi's trie: the working lists must also hold the trie's root values
+ * (up to n), which the bag's first level copies before intersecting. */
+static void ctx_set_trie(ctx_t *c, int bi, const otrie_t *t) {
+    c->trie[bi] = t;
+    if (t->vb[1] > c->tmpcap) {
+        c->tmpcap = t->vb[1];
+        c->tmp = (uint32_t *)realloc(c->tmp, ((size_t)c->tmpcap + 1) * 4);
+        c->tmp2 = (uint32_t *)realloc(c->tmp2, ((size_t)c->tmpcap + 1) * 4);
     }
 }
 
@@ -440,13 +483,13 @@ uint64_t oracle_count(const sge_plan *pl, const uint32_t *off, const uint32_t *n
 
 /* ---- materialised bag: relation and trie */
 
-/* The bag's own plan (nodeJoin over the bag's attributes): its levels in plan
- * order, masks restricted to the bag, column i = bag level i. */
-static void bag_plan(const sge_plan *pl, sge_plan *sub) {
+/* The bag's own plan (nodeJoin over the bag's attributes): the levels of mask
+ * B in plan order, masks restricted to the bag, column i = bag level i. */
+static void bag_plan(const sge_plan *pl, uint32_t B, sge_plan *sub) {
     int lev[SGE_MAX_K], p = 0, i, j;
     memset(sub, 0, sizeof(*sub));
     for (i = 0; i < (int)pl->k; ++i)
-        if ((pl->bag >> i) & 1u) lev[p++] = i;
+        if ((B >> i) & 1u) lev[p++] = i;
     sub->k = (uint32_t)p;
     sub->mode = SGE_MODE_WCOJ;
     sub->multiplier = 1;
@@ -498,14 +541,15 @@ static void group_rows(const uint32_t *rows, uint32_t p, uint64_t start, uint64_
     }
 }
 
-/* Build pl->bag's trie over the CSR; 0 on success. */
-static int bag_trie_build(const sge_plan *pl, const uint32_t *off, const uint32_t *nbr, uint32_t n, otrie_t *t) {
+/* Build the trie of the bag with level mask B over the CSR; 0 on success. */
+static int bag_trie_build(const sge_plan *pl, uint32_t B, const uint32_t *off, const uint32_t *nbr, uint32_t n,
+                          otrie_t *t) {
     sge_plan sub;
     groups_t g;
     uint32_t j, p;
     memset(t, 0, sizeof(*t));
     memset(&g, 0, sizeof(g));
-    bag_plan(pl, &sub);
+    bag_plan(pl, B, &sub);
     p = sub.k;
     if (p < 3) return -1;
     uint64_t N = oracle_count(&sub, off, nbr, n, 0);
@@ -548,13 +592,40 @@ static void bag_trie_free(otrie_t *t) {
     memset(t, 0, sizeof(*t));
 }
 
-/* The trie of pl->bag: depth returned (-1 on failure), vb[SGE_MAX_K + 1] filled;
- * values / child copied when their capacities suffice (call with 0 to size). */
+/* The tries of every bag of the plan; 0 on success (nothing left allocated on failure). */
+static int bags_build(const sge_plan *pl, const uint32_t *off, const uint32_t *nbr, uint32_t n, otrie_t *t) {
+    int bi;
+    for (bi = 0; bi < NBAGS; ++bi) {
+        memset(&t[bi], 0, sizeof(t[bi]));
+        if (bag_levels(pl, bi) && bag_trie_build(pl, bag_levels(pl, bi), off, nbr, n, &t[bi])) {
+            while (--bi >= 0)
+                if (bag_levels(pl, bi)) bag_trie_free(&t[bi]);
+            return -1;
+        }
+    }
+    return 0;
+}
+
+static void bags_attach(ctx_t *c, const sge_plan *pl, const otrie_t *t) {
+    int bi;
+    for (bi = 0; bi < NBAGS; ++bi)
+        if (bag_levels(pl, bi)) ctx_set_trie(c, bi, &t[bi]);
+}
+
+static void bags_free(const sge_plan *pl, otrie_t *t) {
+    int bi;
+    for (bi = 0; bi < NBAGS; ++bi)
+        if (bag_levels(pl, bi)) bag_trie_free(&t[bi]);
+}
+
+/* The trie of the plan's first bag: depth returned (-1 on failure),
+ * vb[SGE_MAX_K + 1] filled; values / child copied when their capacities suffice
+ * (call with 0 to size). */
 int oracle_bag_trie(const sge_plan *pl, const uint32_t *off, const uint32_t *nbr, uint32_t n, uint32_t *vb,
                     uint32_t *values, uint64_t values_cap, uint32_t *child, uint64_t child_cap) {
     otrie_t t;
     uint32_t j;
-    if (!pl->bag || bag_trie_build(pl, off, nbr, n, &t)) return -1;
+    if (!bag_levels(pl, 0) || bag_trie_build(pl, bag_levels(pl, 0), off, nbr, n, &t)) return -1;
     for (j = 0; j <= SGE_MAX_K; ++j) vb[j] = t.vb[j];
     if (values && values_cap >= t.vb[t.p]) memcpy(values, t.val, (size_t)t.vb[t.p] * 4);
     if (child && child_cap >= (uint64_t)t.vb[t.p - 1] + t.p - 1)
@@ -570,9 +641,9 @@ uint64_t oracle_count_range(const sge_plan *pl, const uint32_t *off, const uint3
                             uint64_t tb, uint64_t te, int threads, uint64_t *intersections) {
     uint64_t total = 0, inter = 0;
     int k = (int)pl->k;
-    otrie_t trie;
+    otrie_t trie[NBAGS];
     if (k == 1) return (uint64_t)n * pl->multiplier;
-    if (pl->bag && bag_trie_build(pl, off, nbr, n, &trie)) return ~0ull;
+    if (bags_build(pl, off, nbr, n, trie)) return ~0ull;
 #ifdef _OPENMP
     if (threads > 0) omp_set_num_threads(threads);
 #endif
@@ -580,7 +651,7 @@ uint64_t oracle_count_range(const sge_plan *pl, const uint32_t *off, const uint3
     {
         ctx_t c;
         ctx_init(&c, pl, off, nbr, n);
-        if (pl->bag) ctx_set_trie(&c, &trie);
+        bags_attach(&c, pl, trie);
         uint64_t t;
 #pragma omp for schedule(dynamic, 64)
         for (t = tb; t < te; ++t) {
@@ -598,7 +669,7 @@ uint64_t oracle_count_range(const sge_plan *pl, const uint32_t *off, const uint3
         inter += c.intersections;
         ctx_free(&c);
     }
-    if (pl->bag) bag_trie_free(&trie);
+    bags_free(pl, trie);
     if (intersections) *intersections = inter;
     return total * pl->multiplier;
 }
@@ -608,9 +679,9 @@ uint64_t oracle_count_strided(const sge_plan *pl, const uint32_t *off, const uin
                               uint64_t stride, uint64_t offset, int threads) {
     uint64_t total = 0, nnz = off[n], ntask = offset < nnz ? (nnz - offset + stride - 1) / stride : 0, i;
     int k = (int)pl->k;
-    otrie_t trie;
+    otrie_t trie[NBAGS];
     if (k < 2) return 0;
-    if (pl->bag && bag_trie_build(pl, off, nbr, n, &trie)) return ~0ull;
+    if (bags_build(pl, off, nbr, n, trie)) return ~0ull;
 #ifdef _OPENMP
     if (threads > 0) omp_set_num_threads(threads);
 #endif
@@ -618,7 +689,7 @@ uint64_t oracle_count_strided(const sge_plan *pl, const uint32_t *off, const uin
     {
         ctx_t c;
         ctx_init(&c, pl, off, nbr, n);
-        if (pl->bag) ctx_set_trie(&c, &trie);
+        bags_attach(&c, pl, trie);
 #pragma omp for schedule(dynamic, 16)
         for (i = 0; i < ntask; ++i) {
             uint64_t t = offset + i * stride;
@@ -631,7 +702,7 @@ uint64_t oracle_count_strided(const sge_plan *pl, const uint32_t *off, const uin
         }
         ctx_free(&c);
     }
-    if (pl->bag) bag_trie_free(&trie);
+    bags_free(pl, trie);
     return total * pl->multiplier;
 }
 

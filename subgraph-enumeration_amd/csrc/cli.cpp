@@ -7,8 +7,10 @@
 // reference never writes; --order a,b,c,... fixes the matching order;
 // --bag a,b,c,... materialises those pattern vertices as one bag (its relation
 // built into a trie on the GPU before the join; Execution Time includes the
-// build, as the reference's includes its bag joins and tries); --bag auto takes
-// the first bag of the pattern's decomposition that is valid for the order;
+// build, as the reference's includes its bag joins and tries), a second --bag
+// adds a second bag; --bag auto takes the first bag of the pattern's
+// decomposition that is valid for the order and a second one when a valid pair
+// exists;
 // --device-build builds the data graph's CSR on the GPU.
 #include <chrono>
 #include <cstdio>
@@ -36,7 +38,7 @@ static void usage() {
               << "  --embeddings <path>  Write canonical embeddings (extension)\n"
               << "  --order a,b,...  Fixed matching order (extension)\n"
               << "  --bag a,b,...|auto  Materialise these pattern vertices (or the first valid\n"
-              << "                   decomposition bag) as one bag (extension)\n"
+              << "                   decomposition bags) as a bag; twice: two bags (extension)\n"
               << "  --device-build   Build the data graph's CSR on the GPU (extension)\n"
               << "  --help, -h       Show this help message\n"
               << std::endl;
@@ -48,7 +50,8 @@ static int die(const char *what) {
 }
 
 int main(int argc, char **argv) {
-    std::string q, d, r, emb, order_s, bag_s;
+    std::string q, d, r, emb, order_s;
+    std::vector<std::string> bag_s;
     bool explore = false, iep = false;
     uint32_t gflags = SGE_GRAPH_DEGREE_ORDER;
     int threads = 1;
@@ -66,7 +69,7 @@ int main(int argc, char **argv) {
         else if (a == "-t") threads = std::atoi(next().c_str());
         else if (a == "--embeddings") emb = next();
         else if (a == "--order") order_s = next();
-        else if (a == "--bag") bag_s = next();
+        else if (a == "--bag") bag_s.push_back(next());
         else if (a == "--device-build") gflags |= SGE_GRAPH_BUILD_DEVICE;
     }
     if (threads <= 0) threads = 1;  // atoi of a missing value -> 0 -> 1 (main.cpp:121)
@@ -98,24 +101,38 @@ int main(int argc, char **argv) {
     }
     sge_plan plan;
     if (sge_plan_build(&p, g, mode, order.empty() ? nullptr : order.data(), &plan)) return die("plan");
-    if (bag_s == "auto") {
+    if (bag_s.size() > 2 || (bag_s.size() == 2 && (bag_s[0] == "auto" || bag_s[1] == "auto"))) {
+        std::fprintf(stderr, "error: --bag: at most two bags, or one --bag auto\n");
+        return 1;
+    }
+    if (bag_s.size() == 1 && bag_s[0] == "auto") {
         uint32_t bags[16], nb = 0;
         if (sge_plan_bags(&p, &plan, bags, 16, &nb)) return die("bag");
         if (nb && sge_plan_set_bag(&plan, bags[0])) return die("bag");
-    } else if (!bag_s.empty()) {
-        std::stringstream ss(bag_s);
-        std::string tok;
-        uint32_t mask = 0;
-        while (std::getline(ss, tok, ',')) {
-            char *end = nullptr;
-            const unsigned long v = std::strtoul(tok.c_str(), &end, 10);
-            if (tok.empty() || *end || v >= p.k) {
-                std::fprintf(stderr, "error: --bag: '%s' is not a pattern vertex (0..%u)\n", tok.c_str(), p.k - 1);
-                return 1;
+        for (uint32_t i = 1; i < nb && i < 16; ++i) {
+            sge_plan two = plan;
+            if (sge_plan_add_bag(&two, bags[i]) == SGE_OK) {
+                plan = two;
+                break;
             }
-            mask |= 1u << v;
         }
-        if (sge_plan_set_bag(&plan, mask)) return die("bag");
+    } else {
+        for (size_t b = 0; b < bag_s.size(); ++b) {
+            std::stringstream ss(bag_s[b]);
+            std::string tok;
+            uint32_t mask = 0;
+            while (std::getline(ss, tok, ',')) {
+                char *end = nullptr;
+                const unsigned long v = std::strtoul(tok.c_str(), &end, 10);
+                if (tok.empty() || *end || v >= p.k) {
+                    std::fprintf(stderr, "error: --bag: '%s' is not a pattern vertex (0..%u)\n", tok.c_str(),
+                                 p.k - 1);
+                    return 1;
+                }
+                mask |= 1u << v;
+            }
+            if ((b == 0 ? sge_plan_set_bag(&plan, mask) : sge_plan_add_bag(&plan, mask))) return die("bag");
+        }
     }
     double plan_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     char text[4096];

@@ -137,7 +137,8 @@ double order_cost(const Pattern &p, const GraphSummary &s, const int *order, con
 int build_plan(const Pattern &p, const HostGraph *g, uint32_t mode, const uint32_t *order, sge_plan &out);
 std::vector<std::vector<int>> connected_orders(const Pattern &p, const std::vector<std::vector<int>> &aut);
 int bag_check(const sge_plan &pl, uint32_t level_mask);  // SGE_OK, or SGE_EINVAL with the reason
-void bag_subplan(const sge_plan &pl, sge_plan &sub);       // plan of pl.bag's relation (columns = bag levels)
+int bags_check(const sge_plan &pl);                        // every bag of pl.bag (bits 0-7, 8-15)
+void bag_subplan(const sge_plan &pl, uint32_t level_mask, sge_plan &sub);  // plan of one bag's relation
 
 }  // namespace sge
 

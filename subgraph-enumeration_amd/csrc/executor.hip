@@ -95,8 +95,10 @@ struct TrieDesc {
 struct DPlan {
     uint32_t k, tail, mode, last, sib;  // last: level counted in closed form; sib = last - 1
     uint32_t parent[MAXK], lower[MAXK], upper[MAXK], check[MAXK];
-    int32_t reuse[MAXK];
-    uint32_t ready[MAXK];
+    // bytes: rarely read, and every byte of the per-wave LDS copy is one
+    // fewer for the F stage
+    int8_t reuse[MAXK];
+    uint8_t ready[MAXK];
     // last-level decomposition (relative to the sibling level s)
     uint32_t f_parent;     // parents of `last` other than s
     uint32_t f_ready;      // exploration: level after which F's intersection is formed
@@ -111,10 +113,12 @@ struct DPlan {
     // term j's F set starts from the candidate set of level (t_reuse >> 4j) & 15
     // (15: from the parents' rows alone), a level whose constraints it implies
     uint32_t t_reuse;
-    // materialised bag: levels whose candidates come from the bag trie (the
-    // trie list replaces the rows of bag parents; 0 = no bag)
+    // materialised bags: levels whose candidates come from a bag trie (the
+    // trie list replaces the rows of the bag's parents); bag = the levels of
+    // any bag (0 = none), bagm[b] = bag b's levels
     uint32_t bag;
-    const TrieDesc *trie;
+    uint32_t bagm[SGE_MAX_BAGS];
+    const TrieDesc *trie[SGE_MAX_BAGS];
     // root-task split: split[i] = first subtask of sampled root edge i (prefix
     // over ntask + 1 entries); nsub = total subtasks.  Subtask (i, ci) of
     // split[i+1] - split[i] processes slice ci of the level-2 candidates.
@@ -200,7 +204,7 @@ struct WaveLds {
     uint32_t phi[MAXK];
     uint32_t pos[MAXK];  // next candidate position (absolute index into cand[d].ptr)
     uint32_t iend[MAXK]; // end of this wave's iteration range at each level
-    uint32_t tpos[MAXK]; // bag levels: absolute trie value index of phi[level]
+    uint32_t tpos[SGE_MAX_BAGS][MAXK];  // bag levels: absolute value index of phi[level] in each bag's trie
     unsigned long long accv[7];  // statistics counters (Acc), lane 0 only
     Src cand[MAXK];      // candidate set per level (whole set: reuse reads it)
     Src pre[MAXK + 1];   // exploration: parent intersections formed early ([MAXK] = F)
@@ -516,40 +520,54 @@ __device__ __forceinline__ uint32_t *buf_ptr(const KParams &P, uint32_t *wave_ba
     return wave_base + (uint64_t)i * P.buf_stride;
 }
 
-// Trie list of bag level d (unclamped): the first bag level reads the root
-// values, a later one the children of the previous bag level's trie node.
-__device__ __forceinline__ Src trie_list(const WaveLds &w, int d) {
-    const TrieDesc *T = w.pl.trie;
-    const uint32_t below = w.pl.bag & ((1u << d) - 1u);
+// Trie list of level d in bag b (unclamped): the bag's first level reads the
+// root values, a later one the children of the bag's previous level's node.
+__device__ __forceinline__ Src trie_list(const WaveLds &w, int b, int d) {
+    const TrieDesc *T = w.pl.trie[b];
+    const uint32_t below = w.pl.bagm[b] & ((1u << d) - 1u);
     Src s;
     s.ptr = gl(T->val);
     if (below == 0) {
         s.b = T->vb[0];
         s.e = T->vb[1];
     } else {
-        const uint32_t j = __popc(below);              // depth of level d
-        const uint32_t a = w.tpos[31 - __clz(below)];  // parent node (depth j - 1)
+        const uint32_t j = __popc(below);                 // depth of level d
+        const uint32_t a = w.tpos[b][31 - __clz(below)];  // parent node (depth j - 1)
         s.b = T->child[a + j - 1];
         s.e = T->child[a + j];
     }
     return s;
 }
 
-// Descend the trie at bag level d after phi[d] = x: record x's value index
-// (`idx` is its index in cand[d], NONE when unknown; when cand[d] is the trie
+// Descend bag b's trie at level d after phi[d] = x: record x's value index
+// (`idx` is its index in cand[d], NONE when unknown; when cand[d] is this trie
 // list itself that index is the value index).  Returns false when x is not in
 // the trie list, which only happens for the root levels (every other bag
-// level's candidates are a subset of its trie list).
-__device__ __forceinline__ bool trie_enter(WaveLds &w, int d, uint32_t x, uint32_t idx) {
-    const guint *val = gl(w.pl.trie->val);
+// level's candidates are a subset of its trie lists).
+__device__ __forceinline__ bool trie_enter(WaveLds &w, int b, int d, uint32_t x, uint32_t idx) {
+    const guint *val = gl(w.pl.trie[b]->val);
     if (idx != NONE && w.cand[d].ptr == val) {
-        w.tpos[d] = idx;
+        w.tpos[b][d] = idx;
         return true;
     }
-    const Src s = trie_list(w, d);
+    const Src s = trie_list(w, b, d);
     const uint32_t p = lower_bound(s.ptr, s.b, s.e, x);
-    w.tpos[d] = p;
+    w.tpos[b][d] = p;
     return p < s.e && val[p] == x;
+}
+
+// Descend every bag trie that holds level d and a later level (all = every bag
+// holding d, for root levels); false when x is missing from one of them.  Kept
+// out of line: only bag plans call it, and inlined at its five call sites it
+// cost bagless plans 3% (C2 six-vertex batch 432 -> 446 ms) through register
+// allocation in the DFS loop.
+__device__ __noinline__ bool trie_enter_all(WaveLds &w, int d, uint32_t x, uint32_t idx, bool all) {
+    bool ok = true;
+    for (int b = 0; b < SGE_MAX_BAGS; ++b) {
+        const uint32_t bm = w.pl.bagm[b];
+        if (((bm >> d) & 1u) && (all || (bm >> (d + 1))) && !trie_enter(w, b, d, x, idx)) ok = false;
+    }
+    return ok;
 }
 
 // Sources of the parent intersection of level d into w.srcs, bounds applied.
@@ -568,11 +586,13 @@ __device__ int gather_sources(const KParams &P, WaveLds &w, int d, uint32_t pmas
         clamp_range(s.ptr, s.b, s.e, has_lo, lo, has_hi, hi);
         w.srcs[ns++] = s;
     }
-    if ((w.pl.bag >> d) & 1u) {
+    for (int b = 0; ((w.pl.bag >> d) & 1u) && b < SGE_MAX_BAGS; ++b) {
+        const uint32_t bm = w.pl.bagm[b];
+        if (!((bm >> d) & 1u)) continue;
         // the trie list already satisfies every edge, bound and check among
-        // the bag's levels: the rows of bag parents are not read
-        Src s = trie_list(w, d);
-        if (w.pl.bag & ((1u << d) - 1u)) pmask &= ~w.pl.bag;
+        // the bag's levels: the rows of the bag's parents are not read
+        Src s = trie_list(w, b, d);
+        if (bm & ((1u << d) - 1u)) pmask &= ~bm;
         clamp_range(s.ptr, s.b, s.e, has_lo, lo, has_hi, hi);
         w.srcs[ns++] = s;
         acc.add(A_BYTES, 8);
@@ -1243,12 +1263,12 @@ __device__ __forceinline__ bool run_dfs(const KParams &P, WaveLds &w, TermLds *t
         if (d == stop) {
             if (ENUM) {
                 const Src c = w.cand[d];
-                const bool descend = ((pl.bag >> d) & 1u) && (pl.bag >> (d + 1));
+                const bool descend = (pl.bag >> d) & 1u;
                 for (uint32_t i = w.pos[d]; i < w.iend[d]; ++i) {
                     uint32_t x = c.ptr[i];
                     if (is_checked(w, pl.check[d], x)) continue;
                     w.phi[d] = x;
-                    if (descend) trie_enter(w, d, x, i);
+                    if (descend) trie_enter_all(w, d, x, i, false);
                     emit_last(P, w, wb, acc);
                 }
             } else if (count_last<CAP>(P, w, terms, wb, gb, acc, spent, since, plan)) {
@@ -1277,7 +1297,7 @@ __device__ __forceinline__ bool run_dfs(const KParams &P, WaveLds &w, TermLds *t
             w.phi[d] = x;
             // x is in d's trie list (a source of cand[d]); a miss would leave no
             // valid trie node to descend from, so it is skipped, never read
-            if (((pl.bag >> d) & 1u) && (pl.bag >> (d + 1)) && !trie_enter(w, d, x, p - 1)) continue;
+            if (((pl.bag >> d) & 1u) && !trie_enter_all(w, d, x, p - 1, false)) continue;
             if (pl.mode == SGE_MODE_EXPLORE && !ENUM) explore_after(P, w, wb, d, acc);
             materialize(P, w, wb, d + 1, acc);
             ++d;
@@ -1396,11 +1416,11 @@ __global__ void __launch_bounds__(256, SGE_WAVES_PER_SIMD) wcoj_kernel(KParams P
             }
             // trie nodes of the path's bag levels (all present: the path was live)
             bool live = true;
-            if ((pl.bag & 1u) && !trie_enter(w, 0, w.phi[0], NONE)) live = false;
-            if (live && (pl.bag & 2u) && !trie_enter(w, 1, w.phi[1], NONE)) live = false;
+            if ((pl.bag & 1u) && !trie_enter_all(w, 0, w.phi[0], NONE, true)) live = false;
+            if (live && (pl.bag & 2u) && !trie_enter_all(w, 1, w.phi[1], NONE, true)) live = false;
             for (int j = 2; j <= l && live; ++j) {
                 materialize(P, w, wb, j, acc);
-                if (j < l && ((pl.bag >> j) & 1u) && !trie_enter(w, j, w.phi[j], NONE)) live = false;
+                if (j < l && ((pl.bag >> j) & 1u) && !trie_enter_all(w, j, w.phi[j], NONE, true)) live = false;
                 if (j < l && pl.mode == SGE_MODE_EXPLORE && !ENUM) explore_after(P, w, wb, j, acc);
             }
             if (!live) continue;
@@ -1456,8 +1476,8 @@ __global__ void __launch_bounds__(256, SGE_WAVES_PER_SIMD) wcoj_kernel(KParams P
             w.phi[0] = v0;
             w.phi[1] = v1;
             // root levels of the bag: the edge must start a bag tuple
-            if ((pl.bag & 1u) && !trie_enter(w, 0, v0, NONE)) continue;
-            if ((pl.bag & 2u) && !trie_enter(w, 1, v1, NONE)) continue;
+            if ((pl.bag & 1u) && !trie_enter_all(w, 0, v0, NONE, true)) continue;
+            if ((pl.bag & 2u) && !trie_enter_all(w, 1, v1, NONE, true)) continue;
             if (ci == 0) acc.add(A_TASKS, 1);
             if (pl.k < 3) {  // an edge pattern counts its root tasks; k = 1 is counted on the host
                 if (pl.k == 2) acc.add(A_COUNT, 1);
@@ -1618,8 +1638,8 @@ DPlan to_dplan(const sge_plan &p) {
         d.lower[i] = p.lower[i];
         d.upper[i] = p.upper[i];
         d.check[i] = p.check[i];
-        d.reuse[i] = p.reuse[i];
-        d.ready[i] = p.ready[i];
+        d.reuse[i] = (int8_t)p.reuse[i];    // validated: -1 or a level below i
+        d.ready[i] = (uint8_t)p.ready[i];   // validated: below i
         d.order[i] = p.order[i];
     }
     uint32_t L = d.last, s = d.sib;
@@ -1684,8 +1704,12 @@ DPlan to_dplan(const sge_plan &p) {
             if (best >= 0) d.t_reuse = (d.t_reuse & ~(15u << (4 * j))) | ((uint32_t)best << (4 * j));
         }
     }
-    d.bag = p.bag;
-    d.trie = nullptr;
+    d.bag = 0;
+    for (int b = 0; b < SGE_MAX_BAGS; ++b) {
+        d.bagm[b] = (p.bag >> (8 * b)) & 0xffu;
+        d.bag |= d.bagm[b];
+        d.trie[b] = nullptr;
+    }
     if (d.bag) {
         // a bag level's set is not a superset of another level's candidates
         // (its trie list constrains that level alone), and a bag level's own
@@ -1780,7 +1804,7 @@ int validate_plan(const sge_plan &p) {
             return SGE_EINVAL;
         }
     }
-    return sge::bag_check(p, p.bag);
+    return sge::bags_check(p);
 }
 
 // Defaults of sge_tuning.donate_budget and donate_split, and the split's
@@ -2153,12 +2177,13 @@ int upload_plans(sge_graph *g, const sge_plan *plans, uint32_t n, bool enumerate
             p.bag = 0;  // embeddings are enumerated over CSR rows alone
         }
         dp[i] = to_dplan(p);
-        if (p.bag) {
-            if (!tries || !(*tries)[i].desc) {
+        for (int b = 0; b < SGE_MAX_BAGS; ++b) {
+            if (!dp[i].bagm[b]) continue;
+            if (!tries || !(*tries)[i * SGE_MAX_BAGS + b].desc) {
                 sge::set_error("plan has a bag but no trie was built for it");
                 return SGE_EINVAL;
             }
-            dp[i].trie = static_cast<const TrieDesc *>((*tries)[i].desc);
+            dp[i].trie[b] = static_cast<const TrieDesc *>((*tries)[i * SGE_MAX_BAGS + b].desc);
         }
         const unsigned long long *sp = nullptr;
         unsigned long long ns = 0;
@@ -2209,16 +2234,17 @@ struct DevFree {
     }
 };
 
-// Materialise plan.bag's relation and build its trie (nodeJoin over the bag,
+// Materialise the relation of the plan's bag with level mask B and build its
+// trie (nodeJoin over the bag,
 // join.cpp:309-421, then buildTrie, join.cpp:1713-1717): count the relation
 // with the bag's own plan, enumerate it into exactly that many rows, sort and
 // group on the device.  Uses the context's plan/output buffers: call before the
 // caller's own plans are uploaded.
-int build_bag_trie(sge_graph *g, const sge_plan &plan, sge::BagTrie &t) {
+int build_bag_trie(sge_graph *g, const sge_plan &plan, uint32_t B, sge::BagTrie &t) {
     auto t0 = std::chrono::steady_clock::now();
     DeviceCtx &c = dctx();
     sge_plan sub;
-    sge::bag_subplan(plan, sub);
+    sge::bag_subplan(plan, B, sub);
     const uint64_t zero = 0, nnz = g->h.nnz;
     const TaskRanges all(g, 1, &zero, &nnz, 1);
     std::vector<uint64_t> nsub;
@@ -2274,23 +2300,29 @@ int build_bag_trie(sge_graph *g, const sge_plan &plan, sge::BagTrie &t) {
     return SGE_OK;
 }
 
-// Tries of every plan that has a bag (entries of bagless plans stay empty).
+// Tries of every bag of every plan: entry i * SGE_MAX_BAGS + b holds bag b of
+// plan i (entries of absent bags stay empty); ms_each[i] sums plan i's bags.
 struct TrieSet {
     std::vector<sge::BagTrie> t;
     ~TrieSet() {
         for (auto &x : t) sge::trie_free(x);
     }
     int build(sge_graph *g, const sge_plan *plans, uint32_t n, double *ms_total, std::vector<double> *ms_each) {
-        t.assign(n, sge::BagTrie());
+        t.assign((size_t)n * SGE_MAX_BAGS, sge::BagTrie());
         if (ms_each) ms_each->assign(n, 0.0);
         for (uint32_t i = 0; i < n; ++i) {
             if (!plans[i].bag) continue;
             int rc = validate_plan(plans[i]);
             if (rc) return rc;
-            rc = build_bag_trie(g, plans[i], t[i]);
-            if (rc) return rc;
-            if (ms_total) *ms_total += t[i].build_ms;
-            if (ms_each) (*ms_each)[i] = t[i].build_ms;
+            for (int b = 0; b < SGE_MAX_BAGS; ++b) {
+                const uint32_t B = (plans[i].bag >> (8 * b)) & 0xffu;
+                if (!B) continue;
+                sge::BagTrie &x = t[(size_t)i * SGE_MAX_BAGS + b];
+                rc = build_bag_trie(g, plans[i], B, x);
+                if (rc) return rc;
+                if (ms_total) *ms_total += x.build_ms;
+                if (ms_each) (*ms_each)[i] += x.build_ms;
+            }
         }
         return SGE_OK;
     }
@@ -2562,7 +2594,7 @@ extern "C" int sge_trie_build(sge_graph *g, const sge_plan *plan, sge_trie **out
     if (rc) return rc;
     std::lock_guard<std::mutex> lk(dmutex());
     sge_trie *t = new sge_trie();
-    rc = build_bag_trie(g, *plan, t->t);
+    rc = build_bag_trie(g, *plan, plan->bag & 0xffu, t->t);
     if (rc) {
         delete t;
         return rc;
@@ -2665,7 +2697,8 @@ static int batch_create(sge_graph *g, const sge_plan *plans, uint32_t nplans, co
     rc = tries.build(g, plans, nplans, &trie_ms, nullptr);
     if (rc) return rc;
     for (uint32_t i = 0; i < nplans; ++i)
-        if (plans[i].bag) dp[i].trie = static_cast<const TrieDesc *>(tries.t[i].desc);
+        for (int b = 0; b < SGE_MAX_BAGS; ++b)
+            if (dp[i].bagm[b]) dp[i].trie[b] = static_cast<const TrieDesc *>(tries.t[(size_t)i * SGE_MAX_BAGS + b].desc);
     LaunchCfg cfg;
     rc = launch_cfg(tune, false, cfg);
     if (rc) return rc;

@@ -407,12 +407,30 @@ int bag_check(const sge_plan &pl, uint32_t B) {
     return SGE_OK;
 }
 
-// The bag relation's own plan: bag levels in plan order, masks restricted to
-// the bag and renumbered, no closed-form tail, output column i = bag level i.
-void bag_subplan(const sge_plan &pl, sge_plan &sub) {
+// Every bag of a plan (level masks in bits 0-7 and 8-15 of pl.bag): each a
+// valid bag, the second only beside a first and different from it, nothing
+// above bit 15.
+int bags_check(const sge_plan &pl) {
+    if (pl.bag >> (8 * SGE_MAX_BAGS)) {
+        set_error("bag: at most two bags per plan (level masks in bits 0-7 and 8-15)");
+        return SGE_EINVAL;
+    }
+    const uint32_t b0 = pl.bag & 0xffu, b1 = (pl.bag >> 8) & 0xffu;
+    if (b1 && (!b0 || b0 == b1)) {
+        set_error("bag: a second bag needs a first one and must differ from it");
+        return SGE_EINVAL;
+    }
+    if (int rc = bag_check(pl, b0)) return rc;
+    return bag_check(pl, b1);
+}
+
+// The bag relation's own plan: the levels of bag mask B in plan order, masks
+// restricted to the bag and renumbered, no closed-form tail, output column i =
+// bag level i.
+void bag_subplan(const sge_plan &pl, uint32_t B, sge_plan &sub) {
     std::memset(&sub, 0, sizeof(sub));
     int lev[SGE_MAX_K], p = 0;
-    for (uint32_t m = pl.bag; m; m &= m - 1) lev[p++] = __builtin_ctz(m);
+    for (uint32_t m = B; m; m &= m - 1) lev[p++] = __builtin_ctz(m);
     auto remap = [&](uint32_t mask) {
         uint32_t r = 0;
         for (int i = 0; i < p; ++i)
@@ -454,6 +472,32 @@ extern "C" int sge_plan_set_bag(sge_plan *plan, uint32_t vertex_mask) {
     int rc = bag_check(*plan, B);
     if (rc) return rc;
     plan->bag = B;
+    return SGE_OK;
+    SGE_API_END
+}
+
+extern "C" int sge_plan_add_bag(sge_plan *plan, uint32_t vertex_mask) {
+    SGE_API_BEGIN
+    if (!plan || plan->k < 1 || plan->k > SGE_MAX_K) {
+        set_error("sge_plan_add_bag: bad plan");
+        return SGE_EINVAL;
+    }
+    if (vertex_mask >> plan->k) {
+        set_error("sge_plan_add_bag: vertex mask names vertices beyond k");
+        return SGE_EINVAL;
+    }
+    if (!(plan->bag & 0xffu) || (plan->bag >> 8)) {
+        set_error("sge_plan_add_bag: the plan needs exactly one bag already (sge_plan_set_bag)");
+        return SGE_EINVAL;
+    }
+    uint32_t B = 0;
+    for (uint32_t i = 0; i < plan->k; ++i)
+        if ((vertex_mask >> plan->order[i]) & 1u) B |= 1u << i;
+    sge_plan trial = *plan;
+    trial.bag |= B << 8;
+    int rc = bags_check(trial);
+    if (rc) return rc;
+    plan->bag = trial.bag;
     return SGE_OK;
     SGE_API_END
 }
@@ -596,10 +640,12 @@ extern "C" int sge_plan_text(const sge_plan *pl, char *buf, size_t cap) {
             if ((pl->lower[j] >> i) & 1u) larger += std::to_string(pl->order[j]) + ",";
         if (!larger.empty()) s += std::to_string(pl->order[i]) + " < " + larger + "\n";
     }
-    if (pl->bag) {
+    for (int b = 0; b < SGE_MAX_BAGS; ++b) {
+        const uint32_t B = (pl->bag >> (8 * b)) & 0xffu;
+        if (!B) continue;
         s += "materialised bag:";
         for (uint32_t i = 0; i < pl->k; ++i)
-            if ((pl->bag >> i) & 1u) s += " " + std::to_string(pl->order[i]);
+            if ((B >> i) & 1u) s += " " + std::to_string(pl->order[i]);
         s += "\n";
     }
     if (s.size() + 1 > cap) {

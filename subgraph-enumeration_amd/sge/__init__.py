@@ -118,6 +118,7 @@ _SIGS = {
                                        C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32)]),
     "sge_plan_text": (C.c_int, [C.POINTER(Plan), C.c_char_p, C.c_size_t]),
     "sge_plan_set_bag": (C.c_int, [C.POINTER(Plan), C.c_uint32]),
+    "sge_plan_add_bag": (C.c_int, [C.POINTER(Plan), C.c_uint32]),
     "sge_plan_bags": (C.c_int, [C.POINTER(CPattern), C.POINTER(Plan), C.c_void_p, C.c_uint32,
                                 C.POINTER(C.c_uint32)]),
     "sge_trie_build": (C.c_int, [C.c_void_p, C.POINTER(Plan), C.POINTER(C.c_void_p)]),
@@ -378,6 +379,16 @@ def set_bag(plan: Plan, vertices) -> Plan:
     return q
 
 
+def add_bag(plan: Plan, vertices) -> Plan:
+    """Copy of `plan` (which has exactly one bag) with `vertices` as its second
+    materialised bag.  The two bags may share levels; each is joined through
+    its own trie.  Raises SgeError when the pair is not valid for the plan."""
+    mask = vertices if isinstance(vertices, int) else sum(1 << int(v) for v in vertices)
+    q = Plan.from_buffer_copy(plan)
+    _check(lib().sge_plan_add_bag(C.byref(q), mask), "sge_plan_add_bag")
+    return q
+
+
 def plan_bags(pattern: Pattern, plan: Plan) -> List[int]:
     """Vertex masks of the decomposition's bags that are valid bags of `plan`."""
     out = np.zeros(16, dtype=np.uint32)
@@ -395,6 +406,22 @@ def valid_bags(plan: Plan) -> List[int]:
         q = Plan.from_buffer_copy(plan)
         if lib().sge_plan_set_bag(C.byref(q), m) == 0:
             out.append(m)
+    return out
+
+
+def valid_bag_pairs(plan: Plan) -> List[tuple]:
+    """Every (first, second) pair of vertex masks that sge_plan_set_bag then
+    sge_plan_add_bag accept for `plan`, first < second."""
+    single = valid_bags(plan)
+    out = []
+    for i, a in enumerate(single):
+        q = Plan.from_buffer_copy(plan)
+        if lib().sge_plan_set_bag(C.byref(q), a) != 0:
+            continue
+        for b in single[i + 1:]:
+            r = Plan.from_buffer_copy(q)
+            if lib().sge_plan_add_bag(C.byref(r), b) == 0:
+                out.append((a, b))
     return out
 
 

@@ -386,6 +386,18 @@ def test_cli_drop_in(tmp_path):
             assert r.returncode == 0, r.stderr
             assert "materialised bag:" in r.stdout
             assert "Number of matches: %d" % gold["house"]["wcoj"]["count"] in r.stdout, (args, r.stdout)
+    # two --bag flags: two bags, each through its own trie
+    plan = sge.build_plan(ph, g, sge.MODE_WCOJ)
+    for a, b in sge.valid_bag_pairs(plan)[:2]:
+        args = ["--order", ",".join(str(v) for v in list(plan.order)[:plan.k])]
+        for m in (a, b):
+            args += ["--bag", ",".join(str(v) for v in range(ph.k) if (m >> v) & 1)]
+        say("cli two bags", args)
+        r = subprocess.run([cli, "-q", str(tmp_path / "house.txt"), "-d", dpath] + args, capture_output=True,
+                           text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        assert r.stdout.count("materialised bag:") == 2
+        assert "Number of matches: %d" % gold["house"]["wcoj"]["count"] in r.stdout, (args, r.stdout)
     # --bag auto: the first decomposition bag valid for the cost model's order
     for name in ["diamond", "house"]:
         say("cli bag auto", name)
@@ -581,6 +593,68 @@ def test_bag_rmat8_all_78_batched():
     got = b.run()[0]
     b.close()
     assert got == [r["wcoj"]["count"] for r in gold]
+
+
+def bag_pairs(plan, cap):
+    pairs = sge.valid_bag_pairs(plan)
+    return pairs[::max(1, len(pairs) // cap)][:cap]
+
+
+def test_two_bags_match_oracle_and_goldens():
+    """Two materialised bags per plan, each joined through its own trie: C1
+    counts equal the oracle's in all three modes (plain and batched), C2's
+    house and K5-e equal the reference's counts for three pairs each, and the
+    R-MAT 8 seven-vertex patterns equal the reference's WCOJ counts."""
+    g = W.graph_c1()
+    off, nbr = g.csr()
+    plans, want = [], []
+    for p in W.six_vertex_set()[::2] + W.seven_vertex_set()[::9]:
+        for mode in (sge.MODE_WCOJ, sge.MODE_EXPLORE, sge.MODE_IEP):
+            plan = sge.build_plan(p, g, mode)
+            base = O.count(plan, off, nbr)
+            for a, b in bag_pairs(plan, 2):
+                plans.append(sge.add_bag(sge.set_bag(plan, a), b))
+                want.append(base)
+    assert len(plans) >= 20
+    say("two bags c1 plain", len(plans))
+    got, st = sge.count(g, plans)
+    assert got == want and st.trie_ms > 0
+    say("two bags c1 batched")
+    assert sge.count(g, plans, batched=True)[0] == want
+    g2 = W.graph_c2()
+    gold2 = golden("c2")["patterns"]
+    for name in ["near5clique", "house"]:
+        for mode, key in ((sge.MODE_WCOJ, "wcoj"), (sge.MODE_EXPLORE, "explore"), (sge.MODE_IEP, "iep")):
+            plan = sge.build_plan(W.named(name), g2, mode)
+            for a, b in bag_pairs(plan, 3):
+                say("two bags c2", name, key, a, b)
+                assert sge.count(g2, sge.add_bag(sge.set_bag(plan, a), b))[0][0] == gold2[name][key]["count"]
+    gold = golden("rmat7")["patterns"]
+    g8 = W.graph_rmat(8, 7)
+    pats = W.seven_vertex_set()
+    for rec in gold[::6]:
+        plan = sge.build_plan(pats[rec["index"]], g8, sge.MODE_IEP)
+        for a, b in bag_pairs(plan, 1):
+            say("two bags rmat8", rec["index"], a, b)
+            assert sge.count(g8, sge.add_bag(sge.set_bag(plan, a), b))[0][0] == rec["wcoj"]["count"]
+
+
+def test_two_bags_with_donation_and_tuning_variants():
+    """Continuation items and level-2 root splits rebuild the positions in both
+    tries: tiny donation budgets and odd launch shapes give the same counts."""
+    g = W.graph_c1()
+    plans = []
+    for p in W.six_vertex_set()[1::3] + W.seven_vertex_set()[::13]:
+        for mode in (sge.MODE_WCOJ, sge.MODE_EXPLORE, sge.MODE_IEP):
+            plan = sge.build_plan(p, g, mode)
+            plans += [sge.add_bag(sge.set_bag(plan, a), b) for a, b in bag_pairs(plan, 2)]
+    want, _ = sge.count(g, [sge.set_bag(pl, 0) for pl in plans])
+    for bt, ch, budget, span in [(64, 1, 16, 8), (256, 3, 64, 1 << 30), (128, 8, 1, 4)]:
+        t = sge.default_tuning()
+        t.block_threads, t.chunk, t.donate_budget, t.split_span = bt, ch, budget, span
+        say("two bags tuning", bt, ch, budget, span)
+        assert sge.count(g, plans, batched=True, tuning=t)[0] == want
+        assert sge.count(g, plans, batched=False, tuning=t)[0] == want
 
 
 def test_global_bitmaps_forced_c2_goldens():

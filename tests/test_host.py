@@ -215,6 +215,9 @@ def test_cli_rejects_bad_arguments(tmp_path):
     for bag in ("2,3,33", "2,3,x", "2,,3", "5"):
         r = subprocess.run(base + ["--bag", bag], capture_output=True, text=True, timeout=60)
         assert r.returncode == 1 and "--bag" in r.stderr and "Number of matches" not in r.stdout, bag
+    for bags in (["2,3,4", "0,2,4", "0,2,3"], ["auto", "2,3,4"]):  # three bags; auto beside another
+        r = subprocess.run(base + sum((["--bag", x] for x in bags), []), capture_output=True, text=True, timeout=60)
+        assert r.returncode == 1 and "--bag" in r.stderr and "Number of matches" not in r.stdout, bags
     r = subprocess.run(base + ["-r", str(tmp_path / "missing" / "out.txt")], capture_output=True, text=True,
                        timeout=60)
     assert r.returncode == 1 and "result file" in r.stderr
@@ -313,6 +316,21 @@ def test_bag_validation():
         sge.set_bag(sge.build_plan(house, g, sge.MODE_WCOJ, [0, 1, 2, 3, 4]), [1, 2, 3])
     # exploration mode takes bags too
     assert sge.set_bag(sge.build_plan(house, g, sge.MODE_EXPLORE, [2, 3, 4, 0, 1]), [2, 3, 4]).bag == 0b111
+    # a second bag: its own level mask in bits 8-15, checked like the first
+    q2 = sge.add_bag(q, [2, 4, 0])
+    assert q2.bag == 0b111 | (0b1101 << 8) and q.bag == 0b111
+    assert sge.set_bag(q2, [2, 3, 4]).bag == 0b111  # set_bag replaces both
+    for first, second in ((plan, [2, 4, 0]),        # no first bag
+                          (q, [2, 3, 4]),          # the same bag twice
+                          (q2, [2, 3, 0]),         # a third bag
+                          (q, [3, 4, 0]),          # an invalid second bag
+                          (q, [2, 3, 1])):         # reaches the closed-form level
+        with pytest.raises(sge.SgeError):
+            sge.add_bag(first, second)
+    pairs = sge.valid_bag_pairs(plan)
+    assert ((1 << 0) | (1 << 2) | (1 << 4), (1 << 2) | (1 << 3) | (1 << 4)) in pairs
+    for a, b in pairs:
+        assert sge.add_bag(sge.set_bag(plan, a), b).bag >> 8
     # every bag the decomposition proposes is accepted
     for p in W.six_vertex_set():
         pl = sge.build_plan(p, g, sge.MODE_IEP)

@@ -594,6 +594,37 @@ def test_bag_join_counts_equal_plain_counts():
     assert checked >= 75
 
 
+def some_bag_pairs(plan, cap=3):
+    pairs = sge.valid_bag_pairs(plan)
+    return pairs[::max(1, len(pairs) // cap)][:cap]
+
+
+def test_two_bag_join_counts_equal_plain_counts():
+    """Two materialised bags (each joined through its own trie, shared levels
+    intersecting both trie lists) leave every count unchanged: the oracle's
+    two-bag join equals its plain join in all three modes, and the reference's
+    counts on the C1 goldens for every valid pair of house and K5-e."""
+    g = W.graph_c1()
+    off, nbr = g.csr()
+    gold = load_golden("c1")
+    checked = 0
+    for p in W.six_vertex_set()[::2] + W.seven_vertex_set()[::9]:
+        for mode in (sge.MODE_WCOJ, sge.MODE_EXPLORE, sge.MODE_IEP):
+            plan = sge.build_plan(p, g, mode)
+            base = O.count(plan, off, nbr)
+            for a, b in some_bag_pairs(plan):
+                q = sge.add_bag(sge.set_bag(plan, a), b)
+                assert O.count(q, off, nbr) == base, (p.edges(), mode, a, b)
+                checked += 1
+    for name in ("house", "near5clique"):
+        plan = sge.build_plan(W.named(name), g)
+        for a, b in sge.valid_bag_pairs(plan):
+            q = sge.add_bag(sge.set_bag(plan, a), b)
+            assert O.count(q, off, nbr) == gold["patterns"][name]["wcoj"]["count"], (name, a, b)
+            checked += 1
+    assert checked >= 40
+
+
 # The reference's IEP mode also overcounts huge_query/p8 on C1 (two triangles
 # joined by two edges): 64,398 against its own WCOJ count and brute force, 19,305.
 REF_IEP_OVERCOUNT_QUERIES = {"huge_query/p8.txt"}
