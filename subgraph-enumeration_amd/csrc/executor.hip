@@ -394,15 +394,37 @@ __device__ __forceinline__ void row_clamp(const KParams &P, uint32_t &b, uint32_
     if (e < b) e = b;
 }
 
+// x / d for d < 2^16 in three 32-bit divisions (a 64-bit division expands to a
+// long sequence whose temporaries raise the register pressure of the loops
+// around it): the high word, then the remainder with each 16-bit half of the
+// low word.
+__device__ __forceinline__ uint64_t div_small(uint64_t x, uint32_t d) {
+    const uint32_t hi = (uint32_t)(x >> 32), lo = (uint32_t)x;
+    const uint32_t qh = hi / d, rh = hi - qh * d;
+    const uint32_t t1 = (rh << 16) | (lo >> 16);
+    const uint32_t q1 = t1 / d, r1 = t1 - q1 * d;
+    const uint32_t t2 = (r1 << 16) | (lo & 0xffffu);
+    const uint32_t q2 = t2 / d;
+    return ((uint64_t)qh << 32) | ((uint64_t)q1 << 16) | q2;
+}
+
 __device__ __forceinline__ uint64_t choosec(uint64_t n, uint32_t k) {
     if ((uint64_t)k > n) return 0;
     if (k == 0) return 1;
-    uint64_t nom = n, den = 1;
+    uint64_t nom = n;
+    uint32_t den = 1;  // k! <= 8! < 2^16
     for (uint32_t i = 1; i < k; ++i) {
         nom *= (n - i);
         den *= (i + 1);
     }
-    return nom / den;
+    return div_small(nom, den);
+}
+
+// Start of piece q of [0, len) cut into np near-equal pieces (the first
+// len % np pieces one longer); piece_start(len, np, np) == len.
+__device__ __forceinline__ uint32_t piece_start(uint32_t len, uint32_t q, uint32_t np) {
+    const uint32_t per = len / np, rem = len - per * np;
+    return q * per + min(q, rem);
 }
 
 // Per-wave statistics counters (matches, pairwise intersections, algorithmic
@@ -746,7 +768,8 @@ __device__ __forceinline__ bool gbm_pays(const KParams &P, const WaveLds &w, int
         if ((uint64_t)d <= (uint64_t)SGE_GBM_RATIO * len) probes = d;
     }
     for (int off = WAVE / 2; off > 0; off >>= 1) probes += __shfl_xor(probes, off);
-    const uint64_t est = probes * nsib / max(nsamp, 1u);
+    // probes * nsib / nsamp: nsamp is 64 when nsib >= 64, else nsib
+    const uint64_t est = nsib >= (uint32_t)WAVE ? (probes * nsib) >> 6 : probes;
     return est * (uint64_t)(32 - __clz(len)) >= 4ull * len;
 }
 
@@ -785,15 +808,15 @@ __device__ __forceinline__ bool give_range(const KParams &P, const WaveLds &w, u
             if (lane == 0) P.out_items[slot * ITEM_WORDS] = NONE;
         return false;
     }
-    const uint64_t len = e - b;
+    const uint32_t len = e - b;
     for (uint32_t q = 0; q < np; ++q) {
         uint32_t *it = P.out_items + (base + q) * ITEM_WORDS;
         if (lane < ITEM_WORDS) {
             uint32_t v;
             if (lane == 0) v = plan;
             else if (lane == 1) v = (uint32_t)l;
-            else if (lane == 2) v = b + (uint32_t)(len * q / np);
-            else if (lane == 3) v = b + (uint32_t)(len * (q + 1) / np);
+            else if (lane == 2) v = b + piece_start(len, q, np);
+            else if (lane == 3) v = b + piece_start(len, q + 1, np);
             else v = (lane - 4 < l) ? w.phi[lane - 4] : 0u;
             it[lane] = v;
         }
@@ -1031,7 +1054,10 @@ __device__ __forceinline__ bool count_last(const KParams &P, WaveLds &w, TermLds
                 gave = true;
                 // pieces of about unit / 2 rounds; this wave keeps as much
                 const uint32_t rem = nsib - sb;
-                const uint32_t np = (uint32_t)min<uint64_t>(min<uint64_t>(P.split, rem - 1), max<uint64_t>(1, est / (unit / 2)));
+                // est / (unit / 2) in float: a heuristic piece count
+                const float want = (float)est / (float)(unit / 2);
+                const uint32_t cap = min(P.split, rem - 1);
+                const uint32_t np = want >= (float)cap ? cap : max(1u, (uint32_t)want);
                 const uint32_t keep = min(nb, max(1u, rem / (np + 1)));
                 if (give_range(P, w, plan, s, sibs.b + sb + keep, sibs.e, np)) {
                     // drop the given siblings' entries from this batch
@@ -1230,15 +1256,15 @@ __device__ __forceinline__ void donate(const KParams &P, WaveLds &w, uint32_t pl
     for (int j = l0; j <= d && j < stop; ++j) {
         const uint32_t b = give_from(w, j, d), e = w.iend[j];
         const uint32_t np = pieces_of(P, b, e);
-        const uint64_t len = e - b;
+        const uint32_t len = e - b;
         for (uint32_t q = 0; q < np; ++q, ++slot) {
             uint32_t *it = P.out_items + slot * ITEM_WORDS;
             if (lane < ITEM_WORDS) {
                 uint32_t v;
                 if (lane == 0) v = plan;
                 else if (lane == 1) v = (uint32_t)j;
-                else if (lane == 2) v = b + (uint32_t)(len * q / np);
-                else if (lane == 3) v = b + (uint32_t)(len * (q + 1) / np);
+                else if (lane == 2) v = b + piece_start(len, q, np);
+                else if (lane == 3) v = b + piece_start(len, q + 1, np);
                 else v = (lane - 4 < j) ? w.phi[lane - 4] : 0u;
                 it[lane] = v;
             }
@@ -1431,8 +1457,16 @@ __global__ void __launch_bounds__(256, SGE_WAVES_PER_SIMD) wcoj_kernel(KParams P
             continue;
         }
         // ---- root pass: chunk of subtasks of one plan
-        const uint32_t plan = (uint32_t)(item % P.nplans);
-        const uint64_t ch = item / P.nplans;
+        uint32_t plan;
+        uint64_t ch;
+        if ((item >> 32) == 0) {  // 32-bit division (the usual case)
+            const uint32_t i32 = (uint32_t)item, c32 = i32 / P.nplans;
+            ch = c32;
+            plan = i32 - c32 * P.nplans;
+        } else {
+            ch = item / P.nplans;
+            plan = (uint32_t)(item - ch * P.nplans);
+        }
         if (plan != cur_plan) {
             if (cur_plan != NONE) flush(P, cur_plan, acc);
             cur_plan = plan;
@@ -1499,8 +1533,15 @@ __global__ void __launch_bounds__(256, SGE_WAVES_PER_SIMD) wcoj_kernel(KParams P
             const int stop = ENUM ? (int)pl.k - 2 : (int)pl.sib;  // level whose set feeds the leaf step
             materialize(P, w, wb, 2, acc);
             if (nch > 1) {  // this subtask iterates slice ci of the level-2 candidates
+                // slice boundaries floor(len * ci / nch): when nch > len the
+                // non-empty slices are spread over ci and the highest ci is
+                // never empty.  Which slices are empty
+                // changes the schedule, not the counts: leading non-empty
+                // slices (the first len of ci) made the R-MAT 20 ranges 27%
+                // slower and the R-MAT 23 ranges 40% faster; the same R-MAT 23
+                // gain comes from split_span 8 with these slices.
                 const Src c2 = w.cand[2];
-                uint64_t len = c2.e - c2.b;
+                const uint64_t len = c2.e - c2.b;
                 w.pos[2] = c2.b + (uint32_t)(len * ci / nch);
                 w.iend[2] = c2.b + (uint32_t)(len * (ci + 1) / nch);
             }

@@ -20,7 +20,7 @@ from . import Batch, Graph, Plan, Tuning, default_tuning
 SPACE = {  # first value = the library default, the starting point
     "block_threads": [256, 128, 64],
     "chunk": [4, 2, 8],
-    "split_span": [32, 16, 64],
+    "split_span": [32, 16, 64, 8],
     "donate_budget": [1024, 4096, 256, 16384],
     "donate_split": [64, 16, 256],
     "stage_max": [2, 1, 4, 8],
