@@ -2115,10 +2115,12 @@ int run_launch(sge_graph *g, const DPlan *dplans, uint32_t np, PlanOut *dout, ui
 struct SplitKey {
     uint32_t shape;  // parent mask of level 2 (1, 2, 3), or 0: no level-2 iteration
     uint32_t l1;     // level-1 bound bits: 1 lower, 2 upper
+    uint32_t l2;     // level-2 bounds from levels 0, 1: lower[2] & 3 | (upper[2] & 3) << 2
     uint64_t tb, te, stride;
     uint32_t span;
     bool operator<(const SplitKey &o) const {
-        return std::tie(shape, l1, tb, te, stride, span) < std::tie(o.shape, o.l1, o.tb, o.te, o.stride, o.span);
+        return std::tie(shape, l1, l2, tb, te, stride, span) <
+               std::tie(o.shape, o.l1, o.l2, o.tb, o.te, o.stride, o.span);
     }
 };
 struct SplitArr {
@@ -2146,6 +2148,7 @@ int get_split(sge_graph *g, const sge_plan &p, uint64_t tb, uint64_t te, uint64_
     SplitKey key;
     key.shape = (p.k >= 4 && !(p.mode == SGE_MODE_IEP && p.tail >= 2 && p.k - p.tail <= 2)) ? (p.parent[2] & 3u) : 0u;
     key.l1 = ((p.lower[1] & 1u) ? 1u : 0u) | ((p.upper[1] & 1u) ? 2u : 0u);
+    key.l2 = (p.lower[2] & 3u) | ((p.upper[2] & 3u) << 2);
     key.tb = tb;
     key.te = te;
     key.stride = stride;
@@ -2167,8 +2170,25 @@ int get_split(sge_graph *g, const sge_plan &p, uint64_t tb, uint64_t te, uint64_
             if (((key.l1 & 1u) && !(v1 > v0)) || ((key.l1 & 2u) && !(v1 < v0))) {
                 nch = 0;
             } else {
-                uint32_t d0 = h.deg(v0), d1 = h.deg(v1);
-                uint32_t bound = key.shape == 3 ? std::min(d0, d1) : (key.shape == 1 ? d0 : d1);
+                // bound on the level-2 candidates: each parent row clamped to
+                // level 2's bounds from the root images (ids are degree-ordered,
+                // so a bound like "above the root's image" cuts a hub row to
+                // the few heavier vertices; the degree alone overstates it and
+                // every surplus subtask materialises the level-2 set again)
+                uint64_t lo = 0, hi = UINT64_MAX;  // exclusive
+                const uint32_t img[2] = {v0, v1};
+                for (int j = 0; j < 2; ++j) {
+                    if ((key.l2 >> j) & 1u) lo = std::max<uint64_t>(lo, (uint64_t)img[j] + 1);
+                    if ((key.l2 >> (2 + j)) & 1u) hi = std::min<uint64_t>(hi, img[j]);
+                }
+                uint64_t bound = UINT64_MAX;
+                for (int j = 0; j < 2; ++j) {
+                    if (!((key.shape >> j) & 1u)) continue;
+                    const uint32_t *rb = h.nbr.data() + h.off[img[j]], *re = h.nbr.data() + h.off[img[j] + 1];
+                    const uint32_t *a = lo ? std::lower_bound(rb, re, (uint32_t)std::min<uint64_t>(lo, UINT32_MAX)) : rb;
+                    const uint32_t *z = hi <= UINT32_MAX ? std::lower_bound(a, re, (uint32_t)hi) : re;
+                    bound = std::min<uint64_t>(bound, (uint64_t)(z - a));
+                }
                 nch = std::min<uint64_t>(4096, std::max<uint64_t>(1, (bound + span - 1) / span));
             }
             pre[i + 1] = pre[i] + nch;
