@@ -182,9 +182,25 @@ struct KParams {
     unsigned long long *node_ctr;      // wave steps spent so far, reported in blocks of >= NODE_STEP
 };
 
-// continuation item: plan, level l, absolute candidate range [b, e) of level l,
-// images of levels 0..l-1
-constexpr int ITEM_WORDS = 4 + MAXK;
+// continuation item (64 bytes): plan, level l, absolute candidate range [b, e)
+// of level l, images of levels 0..l-1, then the pass epoch that publishes it
+// (written after the body, so a wave that takes the item within the same pass
+// reads it only once it is whole)
+constexpr int ITEM_BODY = 4 + MAXK;
+constexpr int ITEM_EPOCH = ITEM_BODY;
+constexpr int ITEM_WORDS = 16;
+static_assert(ITEM_BODY < ITEM_WORDS, "the epoch word follows the body");
+
+// control words of a pass, after the root / input queue counter at [0]:
+// tickets for this pass's own output items, waves owning an item, the pass
+// epoch (incremented by reset_pass_kernel before every pass), failed waits
+// (an error the host reports), own-output items finished and waves exited
+constexpr int Q_TAKEN = 4, Q_ACTIVE = 5, Q_EPOCH = 6, Q_ERR = 7, Q_DONE = 8, Q_EXITED = 9, Q_WORDS = 16;
+// polls of a wave waiting for its ticket's item before it reports an error and
+// exits (about a minute at the longest back-off), and iterations of the wait
+// for a claimed item's publication (about a second)
+constexpr uint32_t LIVE_POLLS = 1u << 20;
+constexpr uint32_t ITEM_WAIT_SPINS = 1u << 24;
 
 // list descriptor: [ptr + b, ptr + e)
 typedef const __attribute__((address_space(1))) uint32_t guint;
@@ -794,6 +810,33 @@ __device__ __forceinline__ bool gbm_pays(const KParams &P, const WaveLds &w, int
 // the cap stopped it (its count is then partial).
 constexpr uint32_t SIB_DONATE_AFTER = 16;
 
+// Item words are written and read with device-coherent (agent-scope relaxed
+// atomic) stores and loads: a wave on another XCD may take an item while the
+// pass runs, and per-XCD L2s are not coherent for ordinary accesses.  Fences
+// at agent scope would write back or invalidate a whole L2 instead.
+__device__ __forceinline__ void item_store(uint32_t *p, uint32_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint32_t item_load(const uint32_t *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned long long ctl_load(const unsigned long long *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void ctl_add(unsigned long long *p, unsigned long long v) {
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// every memory operation this wave issued has completed (is device-visible)
+__device__ __forceinline__ void mem_drain() { __builtin_amdgcn_s_waitcnt(0); }
+
+// Publish output slots [base, base + n) whose bodies this wave has written:
+// once the body stores have completed, write the pass epoch into each.
+__device__ __forceinline__ void publish_items(const KParams &P, uint64_t base, uint64_t n) {
+    mem_drain();
+    const uint32_t epoch = (uint32_t)ctl_load(P.queue + Q_EPOCH);
+    for (uint64_t q = lane_id(); q < n; q += WAVE) item_store(P.out_items + (base + q) * ITEM_WORDS + ITEM_EPOCH, epoch);
+}
+
 // Write [b, e) of level l's candidates as np continuation items of near-equal
 // length (plan, level, range, images of levels < l); false (nothing written)
 // when the output buffer is full.
@@ -805,22 +848,24 @@ __device__ __forceinline__ bool give_range(const KParams &P, const WaveLds &w, u
     base = __shfl(base, 0);
     if (base + np > P.item_cap) {
         for (uint64_t slot = base; slot < P.item_cap && slot < base + np; ++slot)
-            if (lane == 0) P.out_items[slot * ITEM_WORDS] = NONE;
+            if (lane == 0) item_store(P.out_items + slot * ITEM_WORDS, NONE);
+        if (base < P.item_cap) publish_items(P, base, min<uint64_t>(np, P.item_cap - base));
         return false;
     }
     const uint32_t len = e - b;
     for (uint32_t q = 0; q < np; ++q) {
         uint32_t *it = P.out_items + (base + q) * ITEM_WORDS;
-        if (lane < ITEM_WORDS) {
+        if (lane < ITEM_BODY) {
             uint32_t v;
             if (lane == 0) v = plan;
             else if (lane == 1) v = (uint32_t)l;
             else if (lane == 2) v = b + piece_start(len, q, np);
             else if (lane == 3) v = b + piece_start(len, q + 1, np);
             else v = (lane - 4 < l) ? w.phi[lane - 4] : 0u;
-            it[lane] = v;
+            item_store(it + lane, v);
         }
     }
+    publish_items(P, base, np);
     return true;
 }
 
@@ -932,7 +977,10 @@ __device__ __forceinline__ bool count_last(const KParams &P, WaveLds &w, TermLds
             }
             terms[j] = t;
         }
-        if (used_g) __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");  // the bits are set before any probe
+        // the bits are set before any probe: the region is this wave's own, so
+        // workgroup scope orders it (an agent-scope fence would write back and
+        // invalidate the XCD's whole L2)
+        if (used_g) __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
         wave_sync();
     }
     PH_SET(w, PH_P1);
@@ -1161,7 +1209,7 @@ __device__ __forceinline__ bool count_last(const KParams &P, WaveLds &w, TermLds
             for (uint32_t q = lane; q < t.F.e - t.F.b; q += WAVE)
                 gb[t.st_off + ((t.F.ptr[t.F.b + q] - t.bm_lo) >> 5)] = 0u;
         }
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");  // this wave's next call sets bits again
     }
     PH_LEAVE(w);
     spent += steps;
@@ -1248,7 +1296,8 @@ __device__ __forceinline__ void donate(const KParams &P, WaveLds &w, uint32_t pl
     if (base + n > P.item_cap) {
         // no room: mark the reserved slots that exist as empty, keep the work
         for (uint64_t slot = base; slot < P.item_cap && slot < base + n; ++slot)
-            if (lane == 0) P.out_items[slot * ITEM_WORDS] = NONE;
+            if (lane == 0) item_store(P.out_items + slot * ITEM_WORDS, NONE);
+        if (base < P.item_cap) publish_items(P, base, min<uint64_t>(n, P.item_cap - base));
         PH_LEAVE(w);
         return;
     }
@@ -1259,19 +1308,103 @@ __device__ __forceinline__ void donate(const KParams &P, WaveLds &w, uint32_t pl
         const uint32_t len = e - b;
         for (uint32_t q = 0; q < np; ++q, ++slot) {
             uint32_t *it = P.out_items + slot * ITEM_WORDS;
-            if (lane < ITEM_WORDS) {
+            if (lane < ITEM_BODY) {
                 uint32_t v;
                 if (lane == 0) v = plan;
                 else if (lane == 1) v = (uint32_t)j;
                 else if (lane == 2) v = b + piece_start(len, q, np);
                 else if (lane == 3) v = b + piece_start(len, q + 1, np);
                 else v = (lane - 4 < j) ? w.phi[lane - 4] : 0u;
-                it[lane] = v;
+                item_store(it + lane, v);
             }
         }
         if (np) w.iend[j] = b;  // this wave keeps [pos, b)
     }
+    publish_items(P, base, n);
     PH_LEAVE(w);
+}
+
+// A wave whose input is exhausted takes items of its own pass's output (the
+// continuation items other waves are writing now) instead of leaving them to
+// the next pass.  A wave counts in queue[Q_ACTIVE] while it owns an item
+// (`holding`; every wave starts as owner).  Here it counts the item it owned as
+// finished (queue[Q_DONE], when it was one of these items), gives up
+// ownership, and takes a ticket t (fetch-add on queue[Q_TAKEN]): its item is
+// output slot t once t is below the output count, and it owns it from then.
+// With no owner left and as many items finished as written, no item can
+// appear any more: the pass is complete and the waiting waves exit (the last
+// wave to exit sets the output count to 0: nothing is left to a next pass).  Every written slot has
+// one ticket holder, so no item is lost; tickets at or above the buffer's
+// capacity never get one and exit at once.  A wait longer than LIVE_POLLS polls
+// is counted as an error (the host fails the call) and ends the wave, as does
+// the work cap.  Returns the slot, or ~0 to exit.
+template <bool CAP>
+__device__ __forceinline__ uint64_t live_claim(const KParams &P, bool &holding, bool was_live) {
+    unsigned long long got = ~0ull;
+    if (lane_id() == 0) {
+        unsigned long long *q = P.queue;
+        // every operation below completes before the next is issued
+        // (mem_drain): each load sees a state no older than the previous one's
+        if (holding) {
+            mem_drain();  // the items this wave wrote are published
+            if (was_live) ctl_add(q + Q_DONE, 1ull);
+            mem_drain();
+            ctl_add(q + Q_ACTIVE, ~0ull);
+        }
+        const unsigned long long t = atomicAdd(q + Q_TAKEN, 1ull);
+        uint32_t reps = 1;
+        for (uint32_t polls = 0; t < P.item_cap; ++polls) {
+            if (CAP && ctl_load(P.node_ctr) >= P.node_cap) break;
+            unsigned long long c = ctl_load(P.out_count);
+            mem_drain();
+            if (t < c) {
+                ctl_add(q + Q_ACTIVE, 1ull);
+                got = t;
+                break;
+            }
+            if (ctl_load(q + Q_ACTIVE) == 0) {
+                mem_drain();
+                const unsigned long long d = ctl_load(q + Q_DONE);
+                mem_drain();
+                c = ctl_load(P.out_count);
+                mem_drain();
+                if (t < c) continue;  // written meanwhile
+                if (d >= (c < P.item_cap ? c : (unsigned long long)P.item_cap)) break;
+            }
+            if (polls >= LIVE_POLLS) {
+                atomicAdd(q + Q_ERR, 1ull);
+                break;
+            }
+            for (uint32_t r = 0; r < reps; ++r) __builtin_amdgcn_s_sleep(127);
+            if (reps < 16) reps *= 2;
+        }
+    }
+    got = __shfl(got, 0);
+    holding = got != ~0ull;
+    return got;
+}
+
+// Word `lane` of claimed output slot j's body once its writer has published
+// it (ok = false, and an error counted, if that takes longer than
+// ITEM_WAIT_SPINS polls).
+__device__ __forceinline__ uint32_t wait_item(const KParams &P, uint64_t j, bool &ok) {
+    const uint32_t *it = P.out_items + j * ITEM_WORDS;
+    uint32_t good = 1;
+    if (lane_id() == 0) {
+        const uint32_t epoch = (uint32_t)ctl_load(P.queue + Q_EPOCH);
+        uint32_t spins = 0;
+        while (item_load(it + ITEM_EPOCH) != epoch) {
+            if (++spins >= ITEM_WAIT_SPINS) {
+                atomicAdd(P.queue + Q_ERR, 1ull);
+                good = 0;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(2);
+        }
+        mem_drain();
+    }
+    ok = __shfl(good, 0) != 0;
+    return (ok && lane_id() < ITEM_BODY) ? item_load(it + lane_id()) : 0u;
 }
 
 // Depth-first search from level l0 whose iteration range [pos, iend) is set,
@@ -1390,6 +1523,8 @@ __global__ void __launch_bounds__(256, SGE_WAVES_PER_SIMD) wcoj_kernel(KParams P
 #ifdef SGE_PHASE_PROFILE
     unsigned long long ph_prev = 0, ph_long = 0, ph_long_start = 0;
 #endif
+    bool holding = true, input_done = false, live = false;  // see live_claim
+    if (lane == 0) atomicAdd(P.queue + Q_ACTIVE, 1ull);
     while (true) {
         {
             // charge the time since the last item started to that item's plan
@@ -1408,23 +1543,42 @@ __global__ void __launch_bounds__(256, SGE_WAVES_PER_SIMD) wcoj_kernel(KParams P
         }
         PH_SET(w, PH_QUEUE);
         unsigned long long item = 0;
-        if (lane == 0) item = atomicAdd(P.queue, 1ull);
-        item = __shfl(item, 0);
+        bool is_cont = false;  // a continuation item (else a root item)
+        uint32_t iw = 0;       // its word `lane` (lanes below ITEM_BODY)
+        if (!input_done) {
+            if (lane == 0) item = atomicAdd(P.queue, 1ull);
+            item = __shfl(item, 0);
+            if (item >= nitems) {
+                input_done = true;
+            } else if (cont) {
+                is_cont = true;
+                if (lane < ITEM_BODY) iw = item_load(P.in_items + item * ITEM_WORDS + lane);
+            }
+        }
+        if (input_done) {
+            // ---- this pass's own output, taken while it is being written
+            if (!P.out_items) break;
+            item = live_claim<CAP>(P, holding, live);
+            live = true;
+            if (item == ~0ull) break;
+            bool ok;
+            iw = wait_item(P, item, ok);
+            if (!ok) continue;
+            is_cont = true;
+        }
 #ifdef SGE_PHASE_PROFILE
         if (lane == 0) w.ph[PH_NUM] += 1;  // fetches
         ph_prev = item;
 #endif
-        if (item >= nitems) break;
         if (CAP && (++fetched & 15u) == 0) {  // every 16th item: has the work cap been reached?
             unsigned long long spent = 0;
             if (lane == 0) spent = __hip_atomic_load(P.node_ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             if (__shfl(spent, 0) >= P.node_cap) break;
         }
         PH_SET(w, PH_ROOT);
-        if (cont) {
+        if (is_cont) {
             // ---- continuation item: rebuild the path, then search its range
-            const uint32_t *it = P.in_items + item * ITEM_WORDS;
-            const uint32_t plan = it[0];
+            const uint32_t plan = __shfl(iw, 0);
             if (plan >= P.nplans) continue;  // empty slot
             if (plan != cur_plan) {
                 if (cur_plan != NONE) flush(P, cur_plan, acc);
@@ -1432,10 +1586,12 @@ __global__ void __launch_bounds__(256, SGE_WAVES_PER_SIMD) wcoj_kernel(KParams P
                 load_plan(P, w, plan);
             }
             const DPlan &pl = w.pl;
-            const int l = (int)it[1];
+            const int l = (int)__shfl(iw, 1);
             const int stop = ENUM ? (int)pl.k - 2 : (int)pl.sib;
             if (l < 2 || l > stop || (ENUM && l == stop)) continue;  // never written by donate() / count_last
-            for (int q = 0; q < l; ++q) w.phi[q] = it[4 + q];
+            const uint32_t rb = __shfl(iw, 2), re = __shfl(iw, 3);
+            if (lane >= 4 && lane < 4 + l) w.phi[lane - 4] = iw;
+            wave_sync();
             if (pl.mode == SGE_MODE_EXPLORE && !ENUM) {
                 explore_after(P, w, wb, 0, acc);
                 explore_after(P, w, wb, 1, acc);
@@ -1450,7 +1606,6 @@ __global__ void __launch_bounds__(256, SGE_WAVES_PER_SIMD) wcoj_kernel(KParams P
                 if (j < l && pl.mode == SGE_MODE_EXPLORE && !ENUM) explore_after(P, w, wb, j, acc);
             }
             if (!live) continue;
-            const uint32_t rb = it[2], re = it[3];
             w.pos[l] = max(rb, w.cand[l].b);
             w.iend[l] = min(re, w.cand[l].e);
             if (run_dfs<ENUM, CAP>(P, w, terms, wb, gb, acc, plan, l, stop)) break;  // work cap reached
@@ -1549,6 +1704,15 @@ __global__ void __launch_bounds__(256, SGE_WAVES_PER_SIMD) wcoj_kernel(KParams P
         }
         if (halt) break;
     }
+    if (holding && lane == 0) atomicAdd(P.queue + Q_ACTIVE, ~0ull);  // the work cap stopped this wave
+    if (lane == 0) {
+        // every item written in this pass has been run in it (unless the work
+        // cap stopped the pass or a wait failed, both reported to the host):
+        // the last wave out leaves no items to a next pass
+        const unsigned long long nw = (unsigned long long)gridDim.x * (blockDim.x / WAVE);
+        if (atomicAdd(P.queue + Q_EXITED, 1ull) + 1 == nw)
+            __hip_atomic_store(P.out_count, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
     if (CAP && since && lane == 0) atomicAdd(P.node_ctr, (unsigned long long)since);
     if (cur_plan != NONE) acc.add(A_CYCLES, __builtin_amdgcn_s_memtime() - t_item);
     if (cur_plan != NONE) flush(P, cur_plan, acc);
@@ -1646,11 +1810,15 @@ int ensure_ctx() {
     HIP_TRY(hipGetDeviceProperties(&prop, c.dev));
     c.num_cu = prop.multiProcessorCount;
     HIP_TRY(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
-    HIP_TRY(hipMalloc(&c.queue, sizeof(unsigned long long) * 4));
+    HIP_TRY(hipMalloc(&c.queue, sizeof(unsigned long long) * Q_WORDS));
+    HIP_TRY(hipMemset(c.queue, 0, sizeof(unsigned long long) * Q_WORDS));
     HIP_TRY(hipEventCreate(&c.ev0));
     HIP_TRY(hipEventCreate(&c.ev1));
     HIP_TRY(hipMalloc(&c.items[0], sizeof(uint32_t) * ITEM_WORDS * ITEM_CAP));
     HIP_TRY(hipMalloc(&c.items[1], sizeof(uint32_t) * ITEM_WORDS * ITEM_CAP));
+    // epoch words start at 0, below every pass epoch (reset_pass_kernel counts from 1)
+    HIP_TRY(hipMemset(c.items[0], 0, sizeof(uint32_t) * ITEM_WORDS * ITEM_CAP));
+    HIP_TRY(hipMemset(c.items[1], 0, sizeof(uint32_t) * ITEM_WORDS * ITEM_CAP));
     HIP_TRY(hipMalloc(&c.icount, sizeof(unsigned long long) * 4));
     c.init = true;
     return SGE_OK;
@@ -1986,11 +2154,30 @@ int ensure_scratch(const sge_graph *g, const LaunchCfg &cfg, KParams &kp) {
     return SGE_OK;
 }
 
+// Before every pass: the input queue counter, the pass's own-output counters
+// and its output count to 0, a new epoch for the items it writes.
 __global__ void reset_pass_kernel(unsigned long long *queue, unsigned long long *count) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         queue[0] = 0;
+        queue[Q_TAKEN] = 0;
+        queue[Q_ACTIVE] = 0;
+        queue[Q_DONE] = 0;
+        queue[Q_EXITED] = 0;
+        queue[Q_EPOCH] += 1;
         *count = 0;
     }
+}
+
+// A continuation item that a wave claimed and never saw published (its wait
+// timed out) means a lost item: the counts of the call are wrong.
+int check_item_waits(const unsigned long long *queue) {
+    unsigned long long err = 0;
+    HIP_TRY(hipMemcpy(&err, queue + Q_ERR, sizeof(err), hipMemcpyDeviceToHost));
+    if (err) {
+        sge::set_error("continuation items: " + std::to_string(err) + " waits for a claimed item timed out");
+        return SGE_EDEVICE;
+    }
+    return SGE_OK;
 }
 
 __global__ void reset_kernel(PlanOut *out, uint32_t n, unsigned long long *queue, unsigned long long *emb_n,
@@ -1999,6 +2186,7 @@ __global__ void reset_kernel(PlanOut *out, uint32_t n, unsigned long long *queue
     if (i < n) out[i] = PlanOut{0, 0, 0, 0, 0, 0, 0, 0};
     if (i == 0) {
         queue[0] = 0;
+        queue[Q_ERR] = 0;
         if (emb_n) *emb_n = 0;
         if (cap_ctr) *cap_ctr = 0;
     }
@@ -2099,6 +2287,7 @@ int run_launch(sge_graph *g, const DPlan *dplans, uint32_t np, PlanOut *dout, ui
     float t = 0;
     HIP_TRY(hipEventElapsedTime(&t, c.ev0, c.ev1));
     if (ms) *ms = t;
+    if (int rc = check_item_waits(c.queue)) return rc;
     if (kp.node_cap) {
         unsigned long long spent = 0;
         HIP_TRY(hipMemcpyAsync(&spent, kp.node_ctr, sizeof(spent), hipMemcpyDeviceToHost, c.stream));
@@ -2718,7 +2907,10 @@ struct sge_batch {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
 
-constexpr int CAPTURED_PASSES = 8;  // continuation passes inside the graph (empty ones exit at once)
+// continuation passes inside the graph: a pass takes the items it writes
+// itself, so one more pass only sees items a full buffer turned away (an empty
+// pass exits at once; sge_batch_run drains anything left)
+constexpr int CAPTURED_PASSES = 1;
 
 // Enqueue one continuation pass reading items[in] and writing items[in ^ 1].
 static void enqueue_cont_pass(sge_batch *b, int in) {
@@ -2800,14 +2992,16 @@ static int batch_create(sge_graph *g, const sge_plan *plans, uint32_t nplans, co
     hipError_t e;
     if ((e = hipMalloc(&b->dplans, sizeof(DPlan) * nplans)) != hipSuccess) return fail(e, "plans");
     if ((e = hipMalloc(&b->dout, sizeof(PlanOut) * nplans)) != hipSuccess) return fail(e, "out");
-    if ((e = hipMalloc(&b->queue, sizeof(unsigned long long) * 2)) != hipSuccess) return fail(e, "queue");
+    if ((e = hipMalloc(&b->queue, sizeof(unsigned long long) * Q_WORDS)) != hipSuccess) return fail(e, "queue");
+    if ((e = hipMemset(b->queue, 0, sizeof(unsigned long long) * Q_WORDS)) != hipSuccess) return fail(e, "queue");
     if ((e = hipMalloc(&b->scratch, waves * kp.wave_stride * sizeof(uint32_t))) != hipSuccess) return fail(e, "scratch");
     kp.gbm_words = gbm_words_for(g);
     if ((e = hipMalloc(&b->gbm, waves * (size_t)kp.gbm_words * sizeof(uint32_t))) != hipSuccess) return fail(e, "bitmaps");
     if ((e = hipMemset(b->gbm, 0, waves * (size_t)kp.gbm_words * sizeof(uint32_t))) != hipSuccess)
         return fail(e, "bitmaps");
     for (int q = 0; q < 2; ++q)
-        if ((e = hipMalloc(&b->items[q], sizeof(uint32_t) * ITEM_WORDS * ITEM_CAP)) != hipSuccess)
+        if ((e = hipMalloc(&b->items[q], sizeof(uint32_t) * ITEM_WORDS * ITEM_CAP)) != hipSuccess ||
+            (e = hipMemset(b->items[q], 0, sizeof(uint32_t) * ITEM_WORDS * ITEM_CAP)) != hipSuccess)
             return fail(e, "items");
     if ((e = hipMalloc(&b->icount, sizeof(unsigned long long) * 4)) != hipSuccess) return fail(e, "item counts");
     if ((e = hipMemcpy(b->dplans, dp.data(), sizeof(DPlan) * nplans, hipMemcpyHostToDevice)) != hipSuccess)
@@ -2923,6 +3117,7 @@ extern "C" int sge_batch_run(sge_batch *b, uint64_t *counts, sge_stats *stats) {
     }
     HIP_TRY(hipEventRecord(b->ev1, b->stream));
     HIP_TRY(hipStreamSynchronize(b->stream));
+    if (int rc = check_item_waits(b->queue)) return rc;
     std::vector<PlanOut> host(n);
     HIP_TRY(hipMemcpy(host.data(), b->dout, sizeof(PlanOut) * n, hipMemcpyDeviceToHost));
     float ms = 0;
@@ -3229,6 +3424,7 @@ extern "C" int sge_count_noalloc(sge_graph *g, const sge_plan *plans, uint32_t n
         HIP_TRY(hipGetLastError());
     }
     HIP_TRY(hipStreamSynchronize(st));
+    if (int rc = check_item_waits(queue)) return rc;
     if (stats) std::memset(stats, 0, sizeof(*stats));
     for (uint32_t i = 0; i < nplans; ++i) {
         PlanOut o;
