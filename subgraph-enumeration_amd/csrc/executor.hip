@@ -69,7 +69,7 @@ constexpr int MAX_TERMS = 7;    // inclusion-exclusion terms of a 3-level indepe
 constexpr int NBUF_MAX = 2 * MAXK + 1 + MAX_TERMS;  // scratch buffers per wave at most (see BufLayout)
 constexpr uint32_t NODE_STEP = 1024;  // work-cap accounting granularity per wavefront (wave steps)
 #ifndef SGE_FSTAGE
-#define SGE_FSTAGE 396
+#define SGE_FSTAGE 395
 #endif
 constexpr int FSTAGE = SGE_FSTAGE;  // LDS words per wave for staging the terms' F sets (bitmaps or copies)
 #ifndef SGE_SEARCH_STAGE
@@ -178,6 +178,7 @@ struct KParams {
     uint32_t split;                    // items per donated range (at most; >= 1)
     uint32_t probe_ratio;              // drive with the sibling's row while |row| <= ratio * |F| (F staged in LDS)
     uint32_t bitmap_ratio;             // the same when F is staged as a bitmap
+    uint32_t own_items;                // 1: idle waves take this pass's own items (see live_claim)
     unsigned long long node_cap;       // stop after about this many wave steps of search work (0 = no cap)
     unsigned long long *node_ctr;      // wave steps spent so far, reported in blocks of >= NODE_STEP
 };
@@ -200,6 +201,9 @@ constexpr int Q_TAKEN = 4, Q_ACTIVE = 5, Q_EPOCH = 6, Q_ERR = 7, Q_DONE = 8, Q_E
 // exits (about a minute at the longest back-off), and iterations of the wait
 // for a claimed item's publication (about a second)
 constexpr uint32_t LIVE_POLLS = 1u << 20;
+// WaveLds::qstate: the wave owns an item (counts in queue[Q_ACTIVE]); its input
+// queue is exhausted; its last item was one of its pass's own
+constexpr uint32_t QS_HOLD = 1, QS_INPUT_DONE = 2, QS_OWN = 4;
 constexpr uint32_t ITEM_WAIT_SPINS = 1u << 24;
 
 // list descriptor: [ptr + b, ptr + e)
@@ -233,6 +237,7 @@ struct WaveLds {
     uint32_t cnt[SIB_BLOCK];   // per-sibling counts
     uint32_t flag[SIB_BLOCK];  // 1: driver is the sibling's CSR row (target F), 0: driver F (target row)
     uint32_t mark[WAVE];       // phase 2: entry starting at each lane's probe slot (index + 1)
+    uint32_t qstate;           // work-queue state bits (QS_*), kept here rather than in registers
     uint32_t fst[FSTAGE];      // staged copies of small F sets (probed through LDS)
 #ifdef SGE_PHASE_PROFILE
     unsigned long long ph[16];  // wavefront clock cycles per phase (PH_*), lane 0 only; [14], [15]: the
@@ -1339,15 +1344,16 @@ __device__ __forceinline__ void donate(const KParams &P, WaveLds &w, uint32_t pl
 // is counted as an error (the host fails the call) and ends the wave, as does
 // the work cap.  Returns the slot, or ~0 to exit.
 template <bool CAP>
-__device__ __forceinline__ uint64_t live_claim(const KParams &P, bool &holding, bool was_live) {
+__device__ __forceinline__ uint64_t live_claim(const KParams &P, WaveLds &w) {
     unsigned long long got = ~0ull;
+    const uint32_t qs = w.qstate;
     if (lane_id() == 0) {
         unsigned long long *q = P.queue;
         // every operation below completes before the next is issued
         // (mem_drain): each load sees a state no older than the previous one's
-        if (holding) {
+        if (qs & QS_HOLD) {
             mem_drain();  // the items this wave wrote are published
-            if (was_live) ctl_add(q + Q_DONE, 1ull);
+            if (qs & QS_OWN) ctl_add(q + Q_DONE, 1ull);
             mem_drain();
             ctl_add(q + Q_ACTIVE, ~0ull);
         }
@@ -1380,7 +1386,7 @@ __device__ __forceinline__ uint64_t live_claim(const KParams &P, bool &holding, 
         }
     }
     got = __shfl(got, 0);
-    holding = got != ~0ull;
+    w.qstate = (qs & QS_INPUT_DONE) | QS_OWN | (got != ~0ull ? QS_HOLD : 0u);
     return got;
 }
 
@@ -1523,7 +1529,7 @@ __global__ void __launch_bounds__(256, SGE_WAVES_PER_SIMD) wcoj_kernel(KParams P
 #ifdef SGE_PHASE_PROFILE
     unsigned long long ph_prev = 0, ph_long = 0, ph_long_start = 0;
 #endif
-    bool holding = true, input_done = false, live = false;  // see live_claim
+    w.qstate = QS_HOLD;  // see live_claim
     if (lane == 0) atomicAdd(P.queue + Q_ACTIVE, 1ull);
     while (true) {
         {
@@ -1545,21 +1551,20 @@ __global__ void __launch_bounds__(256, SGE_WAVES_PER_SIMD) wcoj_kernel(KParams P
         unsigned long long item = 0;
         bool is_cont = false;  // a continuation item (else a root item)
         uint32_t iw = 0;       // its word `lane` (lanes below ITEM_BODY)
-        if (!input_done) {
+        if (!(w.qstate & QS_INPUT_DONE)) {
             if (lane == 0) item = atomicAdd(P.queue, 1ull);
             item = __shfl(item, 0);
             if (item >= nitems) {
-                input_done = true;
+                w.qstate |= QS_INPUT_DONE;
             } else if (cont) {
                 is_cont = true;
                 if (lane < ITEM_BODY) iw = item_load(P.in_items + item * ITEM_WORDS + lane);
             }
         }
-        if (input_done) {
+        if (w.qstate & QS_INPUT_DONE) {
             // ---- this pass's own output, taken while it is being written
-            if (!P.out_items) break;
-            item = live_claim<CAP>(P, holding, live);
-            live = true;
+            if (!P.out_items || !P.own_items) break;
+            item = live_claim<CAP>(P, w);
             if (item == ~0ull) break;
             bool ok;
             iw = wait_item(P, item, ok);
@@ -1704,13 +1709,13 @@ __global__ void __launch_bounds__(256, SGE_WAVES_PER_SIMD) wcoj_kernel(KParams P
         }
         if (halt) break;
     }
-    if (holding && lane == 0) atomicAdd(P.queue + Q_ACTIVE, ~0ull);  // the work cap stopped this wave
+    if ((w.qstate & QS_HOLD) && lane == 0) atomicAdd(P.queue + Q_ACTIVE, ~0ull);  // the work cap stopped this wave
     if (lane == 0) {
         // every item written in this pass has been run in it (unless the work
         // cap stopped the pass or a wait failed, both reported to the host):
         // the last wave out leaves no items to a next pass
         const unsigned long long nw = (unsigned long long)gridDim.x * (blockDim.x / WAVE);
-        if (atomicAdd(P.queue + Q_EXITED, 1ull) + 1 == nw)
+        if (P.own_items && atomicAdd(P.queue + Q_EXITED, 1ull) + 1 == nw)
             __hip_atomic_store(P.out_count, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     if (CAP && since && lane == 0) atomicAdd(P.node_ctr, (unsigned long long)since);
@@ -2038,6 +2043,7 @@ struct LaunchCfg {
     unsigned long long node_cap = 0;
     uint32_t gbm_min = 2048;
     uint32_t split = 1;      // items per donated range (at most)
+    uint32_t own_items = 1;  // idle waves take the pass's own continuation items
 };
 
 
@@ -2063,6 +2069,11 @@ int launch_cfg(const sge_tuning *t, bool enumerate, LaunchCfg &cfg) {
         sge::set_error("tuning: donate_split must be at most " + std::to_string(DONATE_SPLIT_MAX));
         return SGE_EINVAL;
     }
+    if (t->pass_items > 1) {
+        sge::set_error("tuning: pass_items must be 0 or 1");
+        return SGE_EINVAL;
+    }
+    cfg.own_items = t->pass_items == 0;
     cfg.lds = (size_t)(cfg.block / WAVE) * (sizeof(WaveLds) + MAX_TERMS * sizeof(TermLds));
     int per_cu = (int)t->blocks_per_cu;
     if (per_cu <= 0) {
@@ -2251,6 +2262,7 @@ int run_launch(sge_graph *g, const DPlan *dplans, uint32_t np, PlanOut *dout, ui
     kp.item_cap = ITEM_CAP;
     kp.budget = cfg.budget;
     kp.split = cfg.split;
+    kp.own_items = cfg.own_items;
     kp.probe_ratio = cfg.probe_ratio;
     kp.bitmap_ratio = cfg.bitmap_ratio;
     kp.node_cap = cfg.node_cap;
@@ -2907,10 +2919,11 @@ struct sge_batch {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
 
-// continuation passes inside the graph: a pass takes the items it writes
-// itself, so one more pass only sees items a full buffer turned away (an empty
-// pass exits at once; sge_batch_run drains anything left)
-constexpr int CAPTURED_PASSES = 1;
+// continuation passes inside the graph: a pass that takes the items it writes
+// itself leaves nothing to a next one, so one more pass is an empty one;
+// otherwise eight (an empty pass exits at once; sge_batch_run drains anything
+// left)
+constexpr int CAPTURED_PASSES_OWN = 1, CAPTURED_PASSES_NEXT = 8;
 
 // Enqueue one continuation pass reading items[in] and writing items[in ^ 1].
 static void enqueue_cont_pass(sge_batch *b, int in) {
@@ -3032,6 +3045,7 @@ static int batch_create(sge_graph *g, const sge_plan *plans, uint32_t nplans, co
     kp.item_cap = ITEM_CAP;
     kp.budget = cfg.budget;
     kp.split = cfg.split;
+    kp.own_items = cfg.own_items;
     kp.probe_ratio = cfg.probe_ratio;
     kp.bitmap_ratio = cfg.bitmap_ratio;
     kp.gbm_min = cfg.gbm_min;
@@ -3049,7 +3063,8 @@ static int batch_create(sge_graph *g, const sge_plan *plans, uint32_t nplans, co
     if (kp.nitems) {
         launch_wcoj(false, cfg, b->stream, kp);
         int in = 0;
-        for (int pass = 0; pass < CAPTURED_PASSES; ++pass) {
+        const int npass = cfg.own_items ? CAPTURED_PASSES_OWN : CAPTURED_PASSES_NEXT;
+        for (int pass = 0; pass < npass; ++pass) {
             enqueue_cont_pass(b, in);
             in ^= 1;
         }
@@ -3318,6 +3333,11 @@ extern "C" int sge_count_noalloc(sge_graph *g, const sge_plan *plans, uint32_t n
         sge::set_error("tuning: donate_split must be at most " + std::to_string(DONATE_SPLIT_MAX));
         return SGE_EINVAL;
     }
+    if (tune->pass_items > 1) {
+        sge::set_error("tuning: pass_items must be 0 or 1");
+        return SGE_EINVAL;
+    }
+    cfg.own_items = tune->pass_items == 0;
     cfg.probe_ratio = tune->stage_max ? tune->stage_max : 2u;
     cfg.bitmap_ratio = tune->bitmap_ratio ? tune->bitmap_ratio : cfg.probe_ratio;
     cfg.node_cap = (unsigned long long)tune->node_cap_k * NODE_STEP;
@@ -3392,6 +3412,7 @@ extern "C" int sge_count_noalloc(sge_graph *g, const sge_plan *plans, uint32_t n
     kp.item_cap = NOALLOC_ITEMS;
     kp.budget = cfg.budget;
     kp.split = cfg.split;
+    kp.own_items = cfg.own_items;
     kp.probe_ratio = cfg.probe_ratio;
     kp.bitmap_ratio = cfg.bitmap_ratio;
     kp.node_cap = cfg.node_cap;

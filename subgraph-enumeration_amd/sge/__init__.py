@@ -72,7 +72,7 @@ class Tuning(C.Structure):
                 ("blocks_per_cu", C.c_uint32), ("split_span", C.c_uint32), ("donate_budget", C.c_uint32),
                 ("bitmap_ratio", C.c_uint32), ("node_cap_k", C.c_uint32),
                 ("head_items", C.c_uint32), ("global_bitmap_min", C.c_uint32),
-                ("donate_split", C.c_uint32)]
+                ("donate_split", C.c_uint32), ("pass_items", C.c_uint32)]
 
 
 class Stats(C.Structure):

@@ -153,14 +153,23 @@ def test_donation_split_same_counts():
     for mode in (sge.MODE_WCOJ, sge.MODE_EXPLORE, sge.MODE_IEP):
         plans = [sge.build_plan(p, g, mode) for p in pats]
         want, _ = sge.count(g, plans)
-        for split, budget in ((2, 16), (16, 64), (256, 16), (64, 4096)):
+        for split, budget, pass_items in ((2, 16, 0), (16, 64, 0), (256, 16, 0), (64, 4096, 0), (2, 16, 1),
+                                          (64, 4096, 1)):
             t = sge.default_tuning()
-            t.donate_split, t.donate_budget = split, budget
-            say("donation split", mode, split, budget)
-            assert sge.count(g, plans, batched=True, tuning=t)[0] == want, (mode, split, budget)
-            assert sge.count(g, plans, batched=False, tuning=t)[0] == want, (mode, split, budget)
+            t.donate_split, t.donate_budget, t.pass_items = split, budget, pass_items
+            say("donation split", mode, split, budget, "pass items", pass_items)
+            assert sge.count(g, plans, batched=True, tuning=t)[0] == want, (mode, split, budget, pass_items)
+            assert sge.count(g, plans, batched=False, tuning=t)[0] == want, (mode, split, budget, pass_items)
+            if pass_items:  # items left to later passes: the hipGraph batch drains them
+                b = sge.Batch(g, plans, tuning=t)
+                assert b.run()[0] == want
+                b.close()
     t = sge.default_tuning()
     t.donate_split = 257
+    with pytest.raises(sge.SgeError):
+        sge.count(g, plans, tuning=t)
+    t = sge.default_tuning()
+    t.pass_items = 2
     with pytest.raises(sge.SgeError):
         sge.count(g, plans, tuning=t)
 
