@@ -1529,8 +1529,7 @@ __global__ void __launch_bounds__(256, SGE_WAVES_PER_SIMD) wcoj_kernel(KParams P
 #ifdef SGE_PHASE_PROFILE
     unsigned long long ph_prev = 0, ph_long = 0, ph_long_start = 0;
 #endif
-    w.qstate = QS_HOLD;  // see live_claim
-    if (lane == 0) atomicAdd(P.queue + Q_ACTIVE, 1ull);
+    w.qstate = QS_HOLD;  // see live_claim (reset_pass_kernel counted every wavefront as an owner)
     while (true) {
         {
             // charge the time since the last item started to that item's plan
@@ -2166,12 +2165,14 @@ int ensure_scratch(const sge_graph *g, const LaunchCfg &cfg, KParams &kp) {
 }
 
 // Before every pass: the input queue counter, the pass's own-output counters
-// and its output count to 0, a new epoch for the items it writes.
-__global__ void reset_pass_kernel(unsigned long long *queue, unsigned long long *count) {
+// and its output count to 0, a new epoch for the items it writes, and every
+// wavefront of the launch counted as an owner (each starts with its input
+// items and gives that up once, so no wavefront adds itself at its start).
+__global__ void reset_pass_kernel(unsigned long long *queue, unsigned long long *count, unsigned long long nwaves) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         queue[0] = 0;
         queue[Q_TAKEN] = 0;
-        queue[Q_ACTIVE] = 0;
+        queue[Q_ACTIVE] = nwaves;
         queue[Q_DONE] = 0;
         queue[Q_EXITED] = 0;
         queue[Q_EPOCH] += 1;
@@ -2205,6 +2206,8 @@ __global__ void reset_kernel(PlanOut *out, uint32_t n, unsigned long long *queue
 
 // The kernel variant for a launch: enumeration or counting, with or without
 // the work cap's accounting.
+unsigned long long waves_of(const LaunchCfg &cfg) { return (unsigned long long)cfg.grid * (cfg.block / WAVE); }
+
 void launch_wcoj(bool enumerate, const LaunchCfg &cfg, hipStream_t st, const KParams &kp) {
     if (enumerate)
         wcoj_kernel<true, false><<<cfg.grid, cfg.block, cfg.lds, st>>>(kp);
@@ -2269,7 +2272,7 @@ int run_launch(sge_graph *g, const DPlan *dplans, uint32_t np, PlanOut *dout, ui
     kp.gbm_min = cfg.gbm_min;
     kp.node_ctr = c.icount + 2;  // read back with the item counts, in one copy per pass
     reset_kernel<<<(np + 255) / 256, 256, 0, c.stream>>>(dout, np, c.queue, emb_n, kp.node_ctr);
-    reset_pass_kernel<<<1, 64, 0, c.stream>>>(c.queue, c.icount);
+    reset_pass_kernel<<<1, 64, 0, c.stream>>>(c.queue, c.icount, waves_of(cfg));
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(c.ev0, c.stream));
     int cur = 0;
@@ -2286,7 +2289,7 @@ int run_launch(sge_graph *g, const DPlan *dplans, uint32_t np, PlanOut *dout, ui
             kp.in_items = c.items[cur];
             kp.in_count = c.icount + cur;
             cur ^= 1;
-            reset_pass_kernel<<<1, 64, 0, c.stream>>>(c.queue, c.icount + cur);
+            reset_pass_kernel<<<1, 64, 0, c.stream>>>(c.queue, c.icount + cur, waves_of(cfg));
         }
         kp.out_items = c.items[cur];
         kp.out_count = c.icount + cur;
@@ -2920,10 +2923,10 @@ struct sge_batch {
 };
 
 // continuation passes inside the graph: a pass that takes the items it writes
-// itself leaves nothing to a next one, so one more pass is an empty one;
-// otherwise eight (an empty pass exits at once; sge_batch_run drains anything
-// left)
-constexpr int CAPTURED_PASSES_OWN = 1, CAPTURED_PASSES_NEXT = 8;
+// itself leaves nothing to a next one (an empty pass would still cost about a
+// millisecond of queue atomics from every wavefront), otherwise eight (an empty
+// pass exits at once; sge_batch_run drains anything left)
+constexpr int CAPTURED_PASSES_OWN = 0, CAPTURED_PASSES_NEXT = 8;
 
 // Enqueue one continuation pass reading items[in] and writing items[in ^ 1].
 static void enqueue_cont_pass(sge_batch *b, int in) {
@@ -2932,7 +2935,7 @@ static void enqueue_cont_pass(sge_batch *b, int in) {
     kp.in_count = b->icount + in;
     kp.out_items = b->items[in ^ 1];
     kp.out_count = b->icount + (in ^ 1);
-    reset_pass_kernel<<<1, 64, 0, b->stream>>>(b->queue, b->icount + (in ^ 1));
+    reset_pass_kernel<<<1, 64, 0, b->stream>>>(b->queue, b->icount + (in ^ 1), waves_of(b->cfg));
     launch_wcoj(false, b->cfg, b->stream, kp);
 }
 
@@ -3059,7 +3062,7 @@ static int batch_create(sge_graph *g, const sge_plan *plans, uint32_t nplans, co
     // number of continuation passes that read their item counts on the device
     if ((e = hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal)) != hipSuccess) return fail(e, "capture");
     reset_kernel<<<(nplans + 255) / 256, 256, 0, b->stream>>>(b->dout, nplans, b->queue, nullptr, kp.node_ctr);
-    reset_pass_kernel<<<1, 64, 0, b->stream>>>(b->queue, b->icount);
+    reset_pass_kernel<<<1, 64, 0, b->stream>>>(b->queue, b->icount, waves_of(cfg));
     if (kp.nitems) {
         launch_wcoj(false, cfg, b->stream, kp);
         int in = 0;
@@ -3420,7 +3423,7 @@ extern "C" int sge_count_noalloc(sge_graph *g, const sge_plan *plans, uint32_t n
     uint32_t *items[2] = {reinterpret_cast<uint32_t *>(base + L.items0), reinterpret_cast<uint32_t *>(base + L.items1)};
     hipStream_t st = nullptr;  // the null stream: nothing to create
     reset_kernel<<<(nplans + 255) / 256, 256, 0, st>>>(dout, nplans, queue, nullptr, kp.node_ctr);
-    reset_pass_kernel<<<1, 64, 0, st>>>(queue, ctl);
+    reset_pass_kernel<<<1, 64, 0, st>>>(queue, ctl, waves_of(cfg));
     HIP_TRY(hipGetLastError());
     int cur = 0;
     bool limited = false;
@@ -3436,7 +3439,7 @@ extern "C" int sge_count_noalloc(sge_graph *g, const sge_plan *plans, uint32_t n
             kp.in_items = items[cur];
             kp.in_count = ctl + cur;
             cur ^= 1;
-            reset_pass_kernel<<<1, 64, 0, st>>>(queue, ctl + cur);
+            reset_pass_kernel<<<1, 64, 0, st>>>(queue, ctl + cur, waves_of(cfg));
         }
         kp.out_items = items[cur];
         kp.out_count = ctl + cur;
