@@ -802,18 +802,24 @@ __device__ __forceinline__ bool gbm_pays(const KParams &P, const WaveLds &w, int
 // The wave steps it spends (64-lane iterations of the entry setup and the
 // flattened probes), the unit of the donation budget and the work cap, are
 // added to `spent` and `since`; the cap is checked after every sibling batch.
-// A long count hands siblings to the next pass (when donation is on): after the
-// entry set-up of a batch, the probes left in the call are estimated from the
-// batch's (per sibling, times the siblings left); at SIB_DONATE_AFTER donation
-// budgets of 64-lane rounds or more, the call is cut into pieces of about half
-// that many rounds (at most `split`), this wave keeps the first piece's share of
-// siblings (at least one) and writes the rest as continuation items of level
-// sib, once per call (each receiver re-forms the F sets once and may cut its
-// piece again: handing over half of a hub's siblings at every budget was 4.7x
-// slower on the R-MAT 20 ranges).  Single counts of hub siblings had held one
-// wavefront for the whole root pass there (91 M cycles).  Returns true when
-// the cap stopped it (its count is then partial).
+// A long count hands siblings to other wavefronts (when donation is on): after
+// the entry set-up of a batch, the time left in the call is estimated -- at the
+// first batch from its probes (rounds x SIB_CYCLES_PER_STEP clock cycles), at
+// later ones from the clock cycles the batches so far took per sibling, times
+// the siblings left; at SIB_DONATE_AFTER donation budgets of steps' worth of
+// cycles or more, the call is cut into pieces of about half that (at most
+// `split`), this wave keeps the first piece's share of siblings (at least one)
+// and writes the rest as continuation items of level sib, once per call (each
+// receiver re-forms the F sets once and may cut its piece again: handing over
+// half of a hub's siblings at every budget was 4.7x slower on the R-MAT 20
+// ranges).  The clock matters on hub rows: a 64-lane probe round through a
+// 64K-id row costs about 8,000 cycles of dependent loads, eight times the
+// rounds of an L2-resident graph, and an estimate in rounds let one count of
+// 8,784 siblings hold a wavefront for 62 M cycles (29 ms) while the rest of the
+// pass had finished.  Returns true when the cap stopped it (its count is then
+// partial).
 constexpr uint32_t SIB_DONATE_AFTER = 16;
+constexpr float SIB_CYCLES_PER_STEP = 1024.0f;
 
 // Item words are written and read with device-coherent (agent-scope relaxed
 // atomic) stores and loads: a wave on another XCD may take an item while the
@@ -1010,6 +1016,7 @@ __device__ __forceinline__ bool count_last(const KParams &P, WaveLds &w, TermLds
     unsigned long long local = 0;                 // per-lane partial
     uint32_t steps = 1;
     bool stopped = false, gave = false;
+    const unsigned long long t_batches = __builtin_amdgcn_s_memtime();  // start of the sibling batches
     for (uint32_t sb = 0; sb < nsib; sb += per_block) {
         if (CAP) {
             spent += steps;
@@ -1099,16 +1106,22 @@ __device__ __forceinline__ bool count_last(const KParams &P, WaveLds &w, TermLds
         if (lane == 0) w.pre_sum[0] = 0;
         wave_sync();
         if (!gave && s >= 2 && P.budget && P.out_items && nsib - sb >= 2) {
-            // the probes this call has left, estimated from this batch
-            const uint64_t per_sib = max(1u, w.pre_sum[ne] / nb);
-            const uint64_t est = per_sib * (nsib - sb) / WAVE;  // rounds
-            const uint64_t unit = (uint64_t)SIB_DONATE_AFTER * P.budget;
+            // the cycles this call has left: from this batch's probes at the
+            // first batch, from the clock at later ones
+            const uint32_t rem = nsib - sb;
+            float est;
+            if (sb == 0) {
+                const uint32_t per_sib = max(1u, w.pre_sum[ne] / nb);
+                est = (float)per_sib * (float)rem * (SIB_CYCLES_PER_STEP / (float)WAVE);
+            } else {
+                const float took = (float)(__builtin_amdgcn_s_memtime() - t_batches);
+                est = took * (float)rem / (float)sb;
+            }
+            const float unit = (float)SIB_DONATE_AFTER * (float)P.budget * SIB_CYCLES_PER_STEP;
             if (est >= unit) {
                 gave = true;
-                // pieces of about unit / 2 rounds; this wave keeps as much
-                const uint32_t rem = nsib - sb;
-                // est / (unit / 2) in float: a heuristic piece count
-                const float want = (float)est / (float)(unit / 2);
+                // pieces of about unit / 2 cycles; this wave keeps as much
+                const float want = est / (0.5f * unit);
                 const uint32_t cap = min(P.split, rem - 1);
                 const uint32_t np = want >= (float)cap ? cap : max(1u, (uint32_t)want);
                 const uint32_t keep = min(nb, max(1u, rem / (np + 1)));
