@@ -125,6 +125,33 @@ def test_c4_subsets_tuning_variants(c4):
         assert got == want, knobs
 
 
+def test_c4_own_items_and_next_pass_alternating(c4):
+    """Passes that take their own continuation items (tickets on the pass's
+    output, epoch-published slots) and passes that leave them to the next pass
+    alternate in one process, batched and plain, four times: every run returns
+    the fixture counts (an earlier in-pass scheme lost work in the launch after
+    such runs, in 1 of 8 repeats)."""
+    g, doc, plans = c4
+    want = [e["count"] for e in doc["subsets"]]
+    b = [e["begin"] for e in doc["subsets"]]
+    e = [e["end"] for e in doc["subsets"]]
+    for rep in range(4):
+        for pass_items in (0, 1):
+            t = capped()
+            t.pass_items = pass_items
+            say("c4 alternating", rep, "pass_items", pass_items)
+            got, _, _ = sge.count_ranges(g, plans, b, e, batched=True, tuning=t)
+            assert got == want, (rep, pass_items)
+            batch = sge.Batch(g, plans, tuning=t, ranges=(b, e))
+            got, _ = batch.run()
+            batch.close()
+            assert got == want, (rep, pass_items, "graph")
+    t = capped()
+    t.pass_items = 0
+    got, _, _ = sge.count_ranges(g, plans, b, e, batched=False, tuning=t)
+    assert got == want
+
+
 def test_c4_learned_orders_count_the_same_subsets(c4):
     """The learned model, choosing only among orders that keep each cost-model
     order's root edge and symmetry rules, changes orders and leaves every
