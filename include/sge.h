@@ -239,8 +239,9 @@ typedef struct sge_tuning {
                                  1 << 30 = off) */
     uint32_t donate_split;    /* a donated candidate range goes to the next pass as at most this many
                                  items of near-equal length (0 = default 64, at most 256) */
-    uint32_t pass_items;      /* 0 (default): waves whose queue is empty take the continuation items
-                                 their own pass is writing; 1: those items wait for the next pass */
+    uint32_t pass_items;      /* 2: waves whose queue is empty take the continuation items their own
+                                 pass is writing; 1: those items wait for the next pass; 0 (default):
+                                 2 on graphs with a row of 16,384 ids or more, else 1 */
 } sge_tuning;
 
 typedef struct sge_stats {

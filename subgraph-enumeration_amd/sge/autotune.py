@@ -25,7 +25,7 @@ SPACE = {  # first value = the library default, the starting point
     "donate_split": [64, 16, 256],
     "stage_max": [2, 1, 4, 8],
     "bitmap_ratio": [2, 1, 4, 8],
-    "pass_items": [0, 1],
+    "pass_items": [0, 1, 2],
 }
 CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "models", "tuning.json")
 

@@ -153,14 +153,14 @@ def test_donation_split_same_counts():
     for mode in (sge.MODE_WCOJ, sge.MODE_EXPLORE, sge.MODE_IEP):
         plans = [sge.build_plan(p, g, mode) for p in pats]
         want, _ = sge.count(g, plans)
-        for split, budget, pass_items in ((2, 16, 0), (16, 64, 0), (256, 16, 0), (64, 4096, 0), (2, 16, 1),
-                                          (64, 4096, 1)):
+        for split, budget, pass_items in ((2, 16, 2), (16, 64, 2), (256, 16, 2), (64, 4096, 2), (2, 16, 1),
+                                          (64, 4096, 1), (16, 64, 0)):
             t = sge.default_tuning()
             t.donate_split, t.donate_budget, t.pass_items = split, budget, pass_items
             say("donation split", mode, split, budget, "pass items", pass_items)
             assert sge.count(g, plans, batched=True, tuning=t)[0] == want, (mode, split, budget, pass_items)
             assert sge.count(g, plans, batched=False, tuning=t)[0] == want, (mode, split, budget, pass_items)
-            if pass_items:  # items left to later passes: the hipGraph batch drains them
+            if pass_items != 2:  # items left to later passes: the hipGraph batch drains them
                 b = sge.Batch(g, plans, tuning=t)
                 assert b.run()[0] == want
                 b.close()
@@ -169,7 +169,7 @@ def test_donation_split_same_counts():
     with pytest.raises(sge.SgeError):
         sge.count(g, plans, tuning=t)
     t = sge.default_tuning()
-    t.pass_items = 2
+    t.pass_items = 3
     with pytest.raises(sge.SgeError):
         sge.count(g, plans, tuning=t)
 
