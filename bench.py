@@ -427,6 +427,8 @@ def bag_times(args):
 
 PHASES = ["other", "materialize", "explore", "terms", "stage", "probe_setup", "probes", "closed_forms", "queue",
           "root_setup", "dfs_iteration", "donate"]
+ITEM_EVENTS = ["count_calls", "sibling_handovers", "dfs_donations", "materialised_sets", "longest_count_cycles",
+               "longest_f_formation_cycles", "siblings_counted", "count_wave_steps"]
 
 
 def phase_profile(args):
@@ -444,7 +446,22 @@ def phase_profile(args):
         raise SystemExit("phases: load the phase-profile build (SGE_LIB=subgraph-enumeration_amd/libsge_phase.so)")
     L.sge_phase_profile_read.restype = C.c_int
     L.sge_phase_profile_read.argtypes = [C.c_void_p, C.c_int]
+    L.sge_phase_item_read.restype = C.c_int
+    L.sge_phase_item_read.argtypes = [C.c_void_p, C.c_int]
     n = len(PHASES) + 1
+
+    def read_item():
+        """The root pass's longest item: its phase cycles and events."""
+        a = np.zeros(1 + len(PHASES) + len(ITEM_EVENTS) + 4, np.uint64)
+        if L.sge_phase_item_read(a.ctypes.data, 1) != 0:
+            raise SystemExit("phases: sge_phase_item_read failed")
+        r = [int(x) for x in a[1:]]
+        np_, ne = len(PHASES), len(ITEM_EVENTS)
+        return {"wave": int(a[0] & 0xffffffff), "cycles": r[np_ + ne], "queue_index": r[np_ + ne + 1],
+                "plan": r[np_ + ne + 2] & 0xffffffff, "continuation": bool(r[np_ + ne + 2] >> 32),
+                "start_cycles": r[np_ + ne + 3],
+                "phase_cycles": {p: r[i] for i, p in enumerate(PHASES) if r[i]},
+                "events": {e: r[np_ + i] for i, e in enumerate(ITEM_EVENTS)}}
 
     def read():
         a = np.zeros(n + 10, np.uint64)
@@ -458,8 +475,10 @@ def phase_profile(args):
         b = sge.Batch(g, plans, ranges=ranges)
         c, _ = b.run()  # warm-up
         read()
+        read_item()
         c, st = b.run()
         raw = read()
+        item = read_item()
         a = raw.astype(np.float64)
         b.close()
         if want is not None and c != want:
@@ -473,10 +492,12 @@ def phase_profile(args):
                "root_pass_longest_item": {"cycles": int(raw[n + 6] >> 32) << 8, "queue_index": int(raw[n + 6] & 0xffffffff),
                                           "start_cycles": int(raw[n + 7] & 0xffffffff) << 8,
                                           "longest_wave_cycles": int(raw[n + 4])},
-               "longest_f_formation_cycles": int(raw[n + 8]), "longest_count_cycles": int(raw[n + 9])}
+               "longest_f_formation_cycles": int(raw[n + 8]), "longest_count_cycles": int(raw[n + 9]),
+               "root_pass_longest_item_detail": item}
         log(f"phases {tag}: {st.kernel_ms:.1f} ms " + ", ".join(f"{k} {100 * v:.1f}%" for k, v in row['share'].items())
             + f"; root pass busy {100 * row['root_pass_busy']:.1f}%, longest item {row['root_pass_longest_item']}, "
-            f"longest F formation {row['longest_f_formation_cycles']}, longest count {row['longest_count_cycles']}")
+            f"longest F formation {row['longest_f_formation_cycles']}, longest count {row['longest_count_cycles']}; "
+            f"longest item detail {item}")
         out["runs"].append(row)
 
     g2 = W.graph_c2()

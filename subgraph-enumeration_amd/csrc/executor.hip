@@ -68,6 +68,14 @@ static_assert(SIB_BLOCK <= 64, "the probe-window entry mapping covers at most 64
 constexpr int MAX_TERMS = 7;    // inclusion-exclusion terms of a 3-level independent tail
 constexpr int NBUF_MAX = 2 * MAXK + 1 + MAX_TERMS;  // scratch buffers per wave at most (see BufLayout)
 constexpr uint32_t NODE_STEP = 1024;  // work-cap accounting granularity per wavefront (wave steps)
+#ifdef SGE_PHASE_PROFILE
+// per-item record of the phase-profile build: the phase cycles and events of
+// each wavefront's longest item
+constexpr int PH_ITEM_PHASES = 12;  // PH_OTHER .. PH_DONATE
+constexpr int PH_ITEM_EVENTS = 8;   // PHE_*
+constexpr int PH_ITEM_WORDS = PH_ITEM_PHASES + PH_ITEM_EVENTS + 4;
+constexpr int PH_ITEM_WAVES = 8192;  // wavefronts recorded (root passes of 8,192-wave launches)
+#endif
 #ifndef SGE_FSTAGE
 #define SGE_FSTAGE 395
 #endif
@@ -244,10 +252,12 @@ struct WaveLds {
                                 // longest F-set formation and last-level count call
     unsigned long long ph_t;    // clock at the last phase switch
     uint32_t ph_cur;            // current phase
+    unsigned long long ph0[PH_ITEM_PHASES];  // ph[] at the current item's start
+    unsigned long long ev[PH_ITEM_EVENTS];   // the current item's events (PHE_*)
 #endif
 };
 
-// Phase profile (builds with -DSGE_PHASE_PROFILE -DSGE_FSTAGE=348, so that the
+// Phase profile (builds with -DSGE_PHASE_PROFILE -DSGE_FSTAGE=308, so that the
 // counters fit the same LDS): every wavefront clock cycle is charged to the
 // phase the wave is in (phases nest: a function enters its phase and restores
 // the caller's on return), summed over all waves and read back with
@@ -276,10 +286,23 @@ __device__ __forceinline__ uint32_t ph_switch(WaveLds &w, uint32_t i) {
 #define PH_ENTER(w, i) const uint32_t _ph_prev = ph_switch(w, i)
 #define PH_SET(w, i) ph_switch(w, i)
 #define PH_LEAVE(w) ph_switch(w, _ph_prev)
+// per-item events: count calls, sibling hand-overs, DFS donations that wrote
+// items, materialised sets, the longest count call and F formation (cycles),
+// siblings counted, wave steps of the counts
+enum { PHE_COUNTS, PHE_HANDOVERS, PHE_DONATIONS, PHE_MATS, PHE_COUNT_MAX, PHE_F_MAX, PHE_SIBS, PHE_STEPS };
+#define PH_EV(w, k, v) do { if (__lane_id() == 0) (w).ev[k] += (v); } while (0)
+#define PH_EVMAX(w, k, v) do { if (__lane_id() == 0) (w).ev[k] = max((w).ev[k], (unsigned long long)(v)); } while (0)
+// wave index -> its longest item's record (PH_ITEM_WORDS: phase cycles,
+// events, cycles, queue index, plan, start); g_item_key = the longest item of
+// the pass as (cycles >> 8) << 32 | wave index
+__device__ unsigned long long g_item[PH_ITEM_WAVES * PH_ITEM_WORDS];
+__device__ unsigned long long g_item_key;
 #else
 #define PH_ENTER(w, i)
 #define PH_SET(w, i)
 #define PH_LEAVE(w)
+#define PH_EV(w, k, v)
+#define PH_EVMAX(w, k, v)
 #endif
 
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
@@ -676,10 +699,21 @@ __device__ Src form_set(const KParams &P, WaveLds &w, int ns, int nchk, bool may
     return r;
 }
 
+// Wave steps of intersecting w.srcs[0..ns): one per 64-element round of the
+// shortest source for every other source (the searches wave_intersect makes).
+__device__ __forceinline__ uint32_t intersect_steps(const WaveLds &w, int ns) {
+    if (ns < 2) return 0;
+    uint32_t m = 0xffffffffu;
+    for (int i = 0; i < ns; ++i) m = min(m, w.srcs[i].e - w.srcs[i].b);
+    return ((m + WAVE - 1) / WAVE) * (uint32_t)(ns - 1);
+}
+
 // Candidate set of level d (checks of level d applied now when the set is
-// materialised, or when it is read if it aliases a CSR row).
-__device__ __forceinline__ void materialize(const KParams &P, WaveLds &w, uint32_t *wb, int d, Acc &acc) {
+// materialised, or when it is read if it aliases a CSR row).  Returns the wave
+// steps of its intersection (intersect_steps).
+__device__ __forceinline__ uint32_t materialize(const KParams &P, WaveLds &w, uint32_t *wb, int d, Acc &acc) {
     PH_ENTER(w, PH_MAT);
+    PH_EV(w, PHE_MATS, 1);
     const DPlan &pl = w.pl;
     bool has_lo, has_hi;
     uint32_t lo, hi;
@@ -689,12 +723,14 @@ __device__ __forceinline__ void materialize(const KParams &P, WaveLds &w, uint32
     bool use_pre = pl.mode == SGE_MODE_EXPLORE && !((pl.bag >> d) & 1u);
     int ns = gather_sources(P, w, d, pl.parent[d], use_pre ? -1 : pl.reuse[d], use_pre, has_lo, lo, has_hi, hi, acc);
     int nchk = load_checks(w, pl.check[d]);
+    const uint32_t st = intersect_steps(w, ns);
     const Src c = form_set(P, w, ns, nchk, true, buf_ptr(P, wb, d), acc);
     w.cand[d] = c;
     w.pos[d] = c.b;
     w.iend[d] = c.e;
     acc.add(A_NODES, 1);
     PH_LEAVE(w);
+    return st;
 }
 
 // Exploration: form the parent intersection of every level whose last parent
@@ -886,11 +922,16 @@ __device__ __forceinline__ bool count_last(const KParams &P, WaveLds &w, TermLds
     PH_ENTER(w, PH_TERMS);
 #ifdef SGE_PHASE_PROFILE
     const unsigned long long ph_t0 = __builtin_amdgcn_s_memtime();
+    PH_EV(w, PHE_COUNTS, 1);
 #endif
     const DPlan &pl = w.pl;
     const int s = pl.sib;
     const int lane = lane_id();
     const uint32_t K = pl.nterms;
+    // wave steps of the F-set intersections: on hub rows a search whose every
+    // sibling is pruned can spend most of its time here, and an item that
+    // left them uncounted never reached its donation budget
+    uint32_t fsteps = 0;
     // ---- per-term fixed parts
     for (uint32_t j = 0; j < K; ++j) {
         bool has_lo, has_hi;
@@ -915,6 +956,7 @@ __device__ __forceinline__ bool count_last(const KParams &P, WaveLds &w, TermLds
         } else if (from_terms) {
             w.srcs[0] = terms[b1 - 1].F;
             w.srcs[1] = terms[(B ^ b1) - 1].F;
+            fsteps += intersect_steps(w, 2);
             t.F = form_set(P, w, 2, 0, false, buf_ptr(P, wb, P.term_base + j), acc);
             acc.add(A_NODES, 1);
         } else {
@@ -930,6 +972,7 @@ __device__ __forceinline__ bool count_last(const KParams &P, WaveLds &w, TermLds
                                     hi, acc);
             }
             int nchk = load_checks(w, pl.t_check[j]);
+            fsteps += intersect_steps(w, ns);
             t.F = form_set(P, w, ns, nchk, nchk == 0, buf_ptr(P, wb, P.term_base + j), acc);
             acc.add(A_NODES, 1);
         }
@@ -938,6 +981,7 @@ __device__ __forceinline__ bool count_last(const KParams &P, WaveLds &w, TermLds
     wave_sync();
 #ifdef SGE_PHASE_PROFILE
     if (lane_id() == 0) w.ph[14] = max(w.ph[14], __builtin_amdgcn_s_memtime() - ph_t0);
+    PH_EVMAX(w, PHE_F_MAX, __builtin_amdgcn_s_memtime() - ph_t0);
 #endif
     PH_SET(w, PH_STAGE);
     // ---- stage F sets in LDS so per-sibling probes hit LDS: a bitmap over
@@ -1014,7 +1058,7 @@ __device__ __forceinline__ bool count_last(const KParams &P, WaveLds &w, TermLds
     const bool per_entry = (kind != 0) || K > 1;  // counts needed per (sibling, term)
     const uint32_t per_block = SIB_BLOCK / K;     // siblings per batch
     unsigned long long local = 0;                 // per-lane partial
-    uint32_t steps = 1;
+    uint32_t steps = 1 + fsteps;
     bool stopped = false, gave = false;
     const unsigned long long t_batches = __builtin_amdgcn_s_memtime();  // start of the sibling batches
     for (uint32_t sb = 0; sb < nsib; sb += per_block) {
@@ -1126,6 +1170,7 @@ __device__ __forceinline__ bool count_last(const KParams &P, WaveLds &w, TermLds
                 const uint32_t np = want >= (float)cap ? cap : max(1u, (uint32_t)want);
                 const uint32_t keep = min(nb, max(1u, rem / (np + 1)));
                 if (give_range(P, w, plan, s, sibs.b + sb + keep, sibs.e, np)) {
+                    PH_EV(w, PHE_HANDOVERS, 1);
                     // drop the given siblings' entries from this batch
                     nb = keep;
                     ne = keep * K;
@@ -1216,6 +1261,9 @@ __device__ __forceinline__ bool count_last(const KParams &P, WaveLds &w, TermLds
     PH_SET(w, PH_P3);
 #ifdef SGE_PHASE_PROFILE
     if (lane_id() == 0) w.ph[15] = max(w.ph[15], __builtin_amdgcn_s_memtime() - ph_t0);
+    PH_EVMAX(w, PHE_COUNT_MAX, __builtin_amdgcn_s_memtime() - ph_t0);
+    PH_EV(w, PHE_STEPS, steps);
+    PH_EV(w, PHE_SIBS, nsib);
 #endif
     for (int off = WAVE / 2; off > 0; off >>= 1) local += __shfl_xor(local, off);
     acc.add(A_COUNT, local);
@@ -1319,6 +1367,7 @@ __device__ __forceinline__ void donate(const KParams &P, WaveLds &w, uint32_t pl
         PH_LEAVE(w);
         return;
     }
+    PH_EV(w, PHE_DONATIONS, 1);
     uint64_t slot = base;
     for (int j = l0; j <= d && j < stop; ++j) {
         const uint32_t b = give_from(w, j, d), e = w.iend[j];
@@ -1477,9 +1526,10 @@ __device__ __forceinline__ bool run_dfs(const KParams &P, WaveLds &w, TermLds *t
             // valid trie node to descend from, so it is skipped, never read
             if (((pl.bag >> d) & 1u) && !trie_enter_all(w, d, x, p - 1, false)) continue;
             if (pl.mode == SGE_MODE_EXPLORE && !ENUM) explore_after(P, w, wb, d, acc);
-            materialize(P, w, wb, d + 1, acc);
+            const uint32_t ms = materialize(P, w, wb, d + 1, acc);
             ++d;
-            const uint32_t st = 1 + ((w.cand[d].e - w.cand[d].b) >> 6);
+            // the searches of the intersection and the survivors written
+            const uint32_t st = 1 + ms + ((w.cand[d].e - w.cand[d].b) >> 6);
             spent += st;
             since += st;
             if (P.budget && P.out_items && spent >= P.budget) {
@@ -1541,6 +1591,9 @@ __global__ void __launch_bounds__(256, SGE_WAVES_PER_SIMD) wcoj_kernel(KParams P
     uint32_t since = 0;     // wave steps of root tasks counted at the first level (not yet reported)
 #ifdef SGE_PHASE_PROFILE
     unsigned long long ph_prev = 0, ph_long = 0, ph_long_start = 0;
+    unsigned long long ph_wlong = 0;  // this wave's longest item (cycles >> 8)
+    bool ph_cont = false;             // the current item is a continuation item
+    const uint32_t ph_wid = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
 #endif
     w.qstate = QS_HOLD;  // see live_claim (reset_pass_kernel counted every wavefront as an owner)
     while (true) {
@@ -1555,6 +1608,25 @@ __global__ void __launch_bounds__(256, SGE_WAVES_PER_SIMD) wcoj_kernel(KParams P
                     ph_long = d << 32 | (ph_prev & 0xffffffffull);
                     ph_long_start = d << 32 | (((t_item - t_kernel) >> 8) & 0xffffffffull);
                 }
+                if (!cont && d > ph_wlong && ph_wid < (uint32_t)PH_ITEM_WAVES) {
+                    // this wave's longest item so far: its phase cycles and events
+                    ph_wlong = d;
+                    ph_switch(w, w.ph_cur);  // charge the open phase
+                    if (lane == 0) {
+                        unsigned long long *r = g_item + (size_t)ph_wid * PH_ITEM_WORDS;
+                        for (int k = 0; k < PH_ITEM_PHASES; ++k) r[k] = w.ph[k] - w.ph0[k];
+                        for (int k = 0; k < PH_ITEM_EVENTS; ++k) r[PH_ITEM_PHASES + k] = w.ev[k];
+                        r[PH_ITEM_PHASES + PH_ITEM_EVENTS] = now - t_item;
+                        r[PH_ITEM_PHASES + PH_ITEM_EVENTS + 1] = ph_prev;
+                        r[PH_ITEM_PHASES + PH_ITEM_EVENTS + 2] = cur_plan | (unsigned long long)ph_cont << 32;
+                        r[PH_ITEM_PHASES + PH_ITEM_EVENTS + 3] = t_item - t_kernel;
+                    }
+                }
+            }
+            ph_switch(w, w.ph_cur);
+            if (lane == 0) {
+                for (int k = 0; k < PH_ITEM_PHASES; ++k) w.ph0[k] = w.ph[k];
+                for (int k = 0; k < PH_ITEM_EVENTS; ++k) w.ev[k] = 0;
             }
 #endif
             t_item = now;
@@ -1586,6 +1658,7 @@ __global__ void __launch_bounds__(256, SGE_WAVES_PER_SIMD) wcoj_kernel(KParams P
 #ifdef SGE_PHASE_PROFILE
         if (lane == 0) w.ph[PH_NUM] += 1;  // fetches
         ph_prev = item;
+        ph_cont = is_cont;
 #endif
         if (CAP && (++fetched & 15u) == 0) {  // every 16th item: has the work cap been reached?
             unsigned long long spent = 0;
@@ -1746,6 +1819,7 @@ __global__ void __launch_bounds__(256, SGE_WAVES_PER_SIMD) wcoj_kernel(KParams P
             atomicAdd(&g_phase[PH_NUM + 5], 1ull);
             atomicMax(&g_phase[PH_NUM + 6], ph_long);
             atomicMax(&g_phase[PH_NUM + 7], ph_long_start);
+            if (ph_wlong && ph_wid < (uint32_t)PH_ITEM_WAVES) atomicMax(&g_item_key, ph_wlong << 32 | ph_wid);
         }
         atomicMax(&g_phase[PH_NUM + 8], w.ph[14]);
         atomicMax(&g_phase[PH_NUM + 9], w.ph[15]);
@@ -1766,6 +1840,24 @@ __global__ void __launch_bounds__(256, SGE_WAVES_PER_SIMD) wcoj_kernel(KParams P
 // the root passes' longest item (cycles >> 8 : queue index, cycles >> 8 : start
 // within its wavefront), out[PH_NUM + 8 .. 9] = the longest F-set formation and
 // the longest last-level count call; reset != 0 zeroes the counters.
+// The longest item of the last root passes: out[0] = (cycles >> 8) << 32 |
+// wave index, out[1 ..] = that wave's record (PH_ITEM_PHASES phase cycles,
+// PH_ITEM_EVENTS events, item cycles, queue index, plan | continuation << 32,
+// start since the wave started); reset != 0 clears the key.
+extern "C" int sge_phase_item_read(unsigned long long *out, int reset) {
+    if (hipMemcpyFromSymbol(out, HIP_SYMBOL(g_item_key), sizeof(unsigned long long)) != hipSuccess) return -1;
+    const uint32_t wid = (uint32_t)(out[0] & 0xffffffffull);
+    if (out[0] && wid < (uint32_t)PH_ITEM_WAVES &&
+        hipMemcpyFromSymbol(out + 1, HIP_SYMBOL(g_item), sizeof(unsigned long long) * PH_ITEM_WORDS,
+                            sizeof(unsigned long long) * PH_ITEM_WORDS * wid) != hipSuccess)
+        return -1;
+    if (reset) {
+        static const unsigned long long zero = 0;
+        if (hipMemcpyToSymbol(HIP_SYMBOL(g_item_key), &zero, sizeof(zero)) != hipSuccess) return -1;
+    }
+    return 0;
+}
+
 extern "C" int sge_phase_profile_read(unsigned long long *out, int reset) {
     if (hipMemcpyFromSymbol(out, HIP_SYMBOL(g_phase), sizeof(unsigned long long) * (PH_NUM + 10)) != hipSuccess)
         return -1;
