@@ -219,10 +219,13 @@ typedef struct sge_tuning {
     uint32_t blocks_per_cu;   /* 0 = occupancy-derived */
     uint32_t split_span;      /* a root edge whose level-2 set may exceed this is split into
                                  ceil(bound / split_span) subtasks (0 = default 32, 1 << 30 = off) */
-    uint32_t donate_budget;   /* wave steps (one per search-tree node plus one per 64 candidates it
-                                 forms, plus the 64-lane iterations of each last-level count) one
-                                 work item may spend before its open candidate ranges go to the
-                                 next pass (0 = default 1024, 1 << 30 = off) */
+    uint32_t donate_budget;   /* wave steps (one per search-tree node, one per 64 candidates it
+                                 forms, one per 64 elements of the shortest list per other list of
+                                 every intersection, plus the 64-lane iterations of each last-level
+                                 count) one work item may spend before its open candidate ranges
+                                 are handed out as continuation items (0 = default 1024, scaled to
+                                 1024 * 65536 / max degree, at least 256, on graphs with rows of
+                                 more than 65,536 ids; 1 << 30 = off) */
     uint32_t bitmap_ratio;    /* the same ratio when F is staged as an LDS bitmap (0 = stage_max) */
     uint32_t node_cap_k;      /* stop a counting call (plain or batched) after about node_cap_k * 1024
                                  wave steps (as donate_budget counts them) and fail with SGE_ELIMIT

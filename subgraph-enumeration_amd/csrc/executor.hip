@@ -2148,6 +2148,7 @@ struct LaunchCfg {
     uint32_t gbm_min = 2048;
     uint32_t split = 1;      // items per donated range (at most)
     uint32_t pass_mode = 0;  // sge_tuning::pass_items (0 by the graph, 1 next pass, 2 own pass)
+    bool budget_auto = true; // sge_tuning::donate_budget 0: the budget follows the graph (budget_for)
 };
 
 // Whether idle wavefronts take their own pass's continuation items.  By default
@@ -2159,6 +2160,21 @@ struct LaunchCfg {
 constexpr uint32_t OWN_ITEMS_MIN_DEGREE = 1u << 14;
 bool own_items_for(const LaunchCfg &cfg, const sge_graph *g) {
     return cfg.pass_mode == 2 || (cfg.pass_mode == 0 && g->h.max_degree >= OWN_ITEMS_MIN_DEGREE);
+}
+
+// The default donation budget on graphs with rows of more than
+// BUDGET_FULL_DEGREE ids: a wave step costs more cycles there (its searches run
+// through longer rows), so the budget in steps shrinks with the largest row,
+// DONATE_BUDGET_DEFAULT * BUDGET_FULL_DEGREE / max degree, at least
+// BUDGET_MIN_AUTO.  R-MAT scale 23 root ranges (largest row 258,497): 17.0 ms
+// at 1,024 steps, 10.4-10.9 ms at 256 or 512; the R-MAT scale 20 ranges
+// (64,810) keep 1,024, their best (256: 58.5 ms, 512: 41.6, 1,024: 38.7).
+constexpr uint32_t BUDGET_FULL_DEGREE = 1u << 16;
+constexpr uint32_t BUDGET_MIN_AUTO = 256;
+uint32_t budget_for(const LaunchCfg &cfg, const sge_graph *g) {
+    if (!cfg.budget_auto || g->h.max_degree <= BUDGET_FULL_DEGREE) return cfg.budget;
+    const uint64_t b = (uint64_t)DONATE_BUDGET_DEFAULT * BUDGET_FULL_DEGREE / g->h.max_degree;
+    return (uint32_t)std::max<uint64_t>(b, BUDGET_MIN_AUTO);
 }
 
 
@@ -2175,6 +2191,7 @@ int launch_cfg(const sge_tuning *t, bool enumerate, LaunchCfg &cfg) {
     cfg.chunk = t->chunk ? t->chunk : 4;
     cfg.head = t->head_items;
     cfg.budget = t->donate_budget ? (t->donate_budget >= (1u << 30) ? 0u : t->donate_budget) : DONATE_BUDGET_DEFAULT;
+    cfg.budget_auto = t->donate_budget == 0;
     cfg.probe_ratio = t->stage_max ? t->stage_max : 2u;
     cfg.bitmap_ratio = t->bitmap_ratio ? t->bitmap_ratio : cfg.probe_ratio;
     cfg.node_cap = (unsigned long long)t->node_cap_k * NODE_STEP;
@@ -2379,7 +2396,7 @@ int run_launch(sge_graph *g, const DPlan *dplans, uint32_t np, PlanOut *dout, ui
     int rc = ensure_scratch(g, cfg, kp);
     if (rc) return rc;
     kp.item_cap = ITEM_CAP;
-    kp.budget = cfg.budget;
+    kp.budget = budget_for(cfg, g);
     kp.split = cfg.split;
     kp.own_items = own_items_for(cfg, g);
     kp.probe_ratio = cfg.probe_ratio;
@@ -3162,7 +3179,7 @@ static int batch_create(sge_graph *g, const sge_plan *plans, uint32_t nplans, co
     kp.gbm = b->gbm;
     kp.out = b->dout;
     kp.item_cap = ITEM_CAP;
-    kp.budget = cfg.budget;
+    kp.budget = budget_for(cfg, g);
     kp.split = cfg.split;
     kp.own_items = own_items_for(cfg, g);
     kp.probe_ratio = cfg.probe_ratio;
@@ -3447,6 +3464,7 @@ extern "C" int sge_count_noalloc(sge_graph *g, const sge_plan *plans, uint32_t n
     }
     cfg.chunk = tune->chunk ? tune->chunk : 4;
     cfg.budget = tune->donate_budget ? (tune->donate_budget >= (1u << 30) ? 0u : tune->donate_budget) : DONATE_BUDGET_DEFAULT;
+    cfg.budget_auto = tune->donate_budget == 0;
     cfg.split = tune->donate_split ? tune->donate_split : DONATE_SPLIT_DEFAULT;
     if (cfg.split > DONATE_SPLIT_MAX) {
         sge::set_error("tuning: donate_split must be at most " + std::to_string(DONATE_SPLIT_MAX));
@@ -3529,7 +3547,7 @@ extern "C" int sge_count_noalloc(sge_graph *g, const sge_plan *plans, uint32_t n
     kp.gbm = nullptr;  // large F sets are probed by search instead of a global bitmap
     kp.out = dout;
     kp.item_cap = NOALLOC_ITEMS;
-    kp.budget = cfg.budget;
+    kp.budget = budget_for(cfg, g);
     kp.split = cfg.split;
     kp.own_items = own_items_for(cfg, g);
     kp.probe_ratio = cfg.probe_ratio;
